@@ -1,0 +1,201 @@
+// Shared device helpers and the C-ABI error state of libmsc_hip.
+//
+// Everything here is written for gfx950 (CDNA4): wave64, 64-bit lane masks, LDS-backed
+// cross-lane moves.  The library is built with -ffp-contract=off so every float operation
+// in a kernel is separately rounded unless a builtin asks otherwise; the exact-distance and
+// fixed-point parts of the neighbor-graph path rely on that.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+namespace msc {
+
+constexpr int kWave = 64;
+
+// ---- error state (thread-local message, int status codes) --------------------------------
+enum Status : int {
+  kOk = 0,
+  kInvalidArgument = 1,
+  kHipError = 2,
+  kWorkspaceTooSmall = 3,
+  kUnsupported = 4,
+  kInternal = 5,
+};
+
+void set_error(const char* fmt, ...);
+const char* last_error();
+
+#define MSC_HIP(call)                                                                   \
+  do {                                                                                  \
+    hipError_t e_ = (call);                                                             \
+    if (e_ != hipSuccess) {                                                             \
+      ::msc::set_error("%s:%d %s: %s", __FILE__, __LINE__, #call, hipGetErrorString(e_)); \
+      return ::msc::kHipError;                                                          \
+    }                                                                                   \
+  } while (0)
+
+#define MSC_LAUNCH_CHECK()                                                              \
+  do {                                                                                  \
+    hipError_t e_ = hipGetLastError();                                                  \
+    if (e_ != hipSuccess) {                                                             \
+      ::msc::set_error("%s:%d kernel launch: %s", __FILE__, __LINE__, hipGetErrorString(e_)); \
+      return ::msc::kHipError;                                                          \
+    }                                                                                   \
+  } while (0)
+
+#define MSC_REQUIRE(cond, ...)                 \
+  do {                                         \
+    if (!(cond)) {                             \
+      ::msc::set_error(__VA_ARGS__);           \
+      return ::msc::kInvalidArgument;          \
+    }                                          \
+  } while (0)
+
+#define MSC_TRY(expr)              \
+  do {                             \
+    int rc_ = (expr);              \
+    if (rc_ != ::msc::kOk) return rc_; \
+  } while (0)
+
+// ---- workspace carving -------------------------------------------------------------------
+// A bump allocator over one caller-provided device buffer; every sub-buffer 256-B aligned.
+struct Carve {
+  char* base;
+  size_t cap;
+  size_t used = 0;
+  bool sizing;  // base == nullptr: only count bytes
+  Carve(void* p, size_t c) : base(static_cast<char*>(p)), cap(c), sizing(p == nullptr) {}
+  template <typename T>
+  T* take(size_t count) {
+    size_t off = (used + 255) & ~size_t(255);
+    used = off + count * sizeof(T);
+    if (sizing) return nullptr;
+    return used <= cap ? reinterpret_cast<T*>(base + off) : nullptr;
+  }
+  bool ok() const { return sizing || used <= cap; }
+};
+
+__host__ __device__ inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// ---- device helpers ----------------------------------------------------------------------
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t x, int m) {
+  uint32_t lo = static_cast<uint32_t>(x), hi = static_cast<uint32_t>(x >> 32);
+  lo = __shfl_xor(lo, m, kWave);
+  hi = __shfl_xor(hi, m, kWave);
+  return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+
+__device__ __forceinline__ uint64_t shfl_u64(uint64_t x, int src) {
+  uint32_t lo = static_cast<uint32_t>(x), hi = static_cast<uint32_t>(x >> 32);
+  lo = __shfl(lo, src, kWave);
+  hi = __shfl(hi, src, kWave);
+  return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+
+__device__ __forceinline__ uint64_t umin64(uint64_t a, uint64_t b) { return a < b ? a : b; }
+__device__ __forceinline__ uint64_t umax64(uint64_t a, uint64_t b) { return a < b ? b : a; }
+
+// Bitonic sort of one 64-bit key per lane, ascending across the 64 lanes of a wave.
+__device__ __forceinline__ uint64_t wave_sort_u64(uint64_t x) {
+  const int l = lane_id();
+#pragma unroll
+  for (int k = 2; k <= kWave; k <<= 1) {
+#pragma unroll
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      uint64_t y = shfl_xor_u64(x, j);
+      bool up = (l & k) == 0;
+      bool lower = (l & j) == 0;
+      x = (lower == up) ? umin64(x, y) : umax64(x, y);
+    }
+  }
+  return x;
+}
+
+// Merge: `cur` and `add` ascending across lanes; returns the 64 smallest of both, ascending.
+__device__ __forceinline__ uint64_t wave_merge_min_u64(uint64_t cur, uint64_t add_sorted) {
+  const int l = lane_id();
+  uint64_t rev = shfl_u64(add_sorted, kWave - 1 - l);
+  uint64_t x = umin64(cur, rev);  // bitonic
+#pragma unroll
+  for (int j = kWave >> 1; j > 0; j >>= 1) {
+    uint64_t y = shfl_xor_u64(x, j);
+    x = ((l & j) == 0) ? umin64(x, y) : umax64(x, y);
+  }
+  return x;
+}
+
+// Order-preserving map float -> uint32 (total order; -0 < +0 is irrelevant here).
+__device__ __forceinline__ uint32_t float_ord(float f) {
+  uint32_t u = __float_as_uint(f);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float ord_float(uint32_t u) {
+  return __uint_as_float((u & 0x80000000u) ? (u & 0x7FFFFFFFu) : ~u);
+}
+
+// Wave-wide sums (deterministic butterfly order).
+__device__ __forceinline__ double wave_sum_f64(double v) {
+#pragma unroll
+  for (int m = 1; m < kWave; m <<= 1) v += __shfl_xor(v, m, kWave);
+  return v;
+}
+__device__ __forceinline__ int64_t wave_sum_i64(int64_t v) {
+#pragma unroll
+  for (int m = 1; m < kWave; m <<= 1) v += static_cast<int64_t>(shfl_xor_u64(static_cast<uint64_t>(v), m));
+  return v;
+}
+__device__ __forceinline__ float wave_max_f32(float v) {
+#pragma unroll
+  for (int m = 1; m < kWave; m <<= 1) v = fmaxf(v, __shfl_xor(v, m, kWave));
+  return v;
+}
+
+// ---- numerics shared bit-for-bit with the CPU oracle ---------------------------------------
+// exp(x) in float32: Cody-Waite reduction x = n ln2 + r with the fdlibm two-part ln2, the
+// degree-7 Taylor polynomial of e^r by Horner's rule (separately rounded mul/add), scaled
+// by 2^n.  x < -87 gives 0, x > 88 gives +inf.
+__device__ __forceinline__ float expf32(float x) {
+  if (x != x) return x;
+  if (x < -87.0f) return 0.0f;
+  if (x > 88.0f) return __uint_as_float(0x7F800000u);
+  const float nf = rintf(__fmul_rn(x, 1.44269504088896341f));
+  float r = __fsub_rn(x, __fmul_rn(nf, 6.93145751953125e-01f));
+  r = __fsub_rn(r, __fmul_rn(nf, 1.42860682030941723e-06f));
+  float p = static_cast<float>(1.0 / 5040.0);
+  p = __fadd_rn(__fmul_rn(p, r), static_cast<float>(1.0 / 720.0));
+  p = __fadd_rn(__fmul_rn(p, r), static_cast<float>(1.0 / 120.0));
+  p = __fadd_rn(__fmul_rn(p, r), static_cast<float>(1.0 / 24.0));
+  p = __fadd_rn(__fmul_rn(p, r), static_cast<float>(1.0 / 6.0));
+  p = __fadd_rn(__fmul_rn(p, r), 0.5f);
+  p = __fadd_rn(__fmul_rn(p, r), 1.0f);
+  p = __fadd_rn(__fmul_rn(p, r), 1.0f);
+  return ldexpf(p, static_cast<int>(nf));
+}
+
+// splitmix64 finaliser and the per-(seed, phase, pass, vertex) coin stream.
+__host__ __device__ __forceinline__ uint64_t mix64(uint64_t x) {
+  x ^= x >> 30;
+  x *= 0xBF58476D1CE4E5B9ull;
+  x ^= x >> 27;
+  x *= 0x94D049BB133111EBull;
+  x ^= x >> 31;
+  return x;
+}
+__host__ __device__ __forceinline__ uint64_t stream_key(int64_t seed, uint64_t phase) {
+  uint64_t s = static_cast<uint64_t>(seed & 0x7FFFFFFFFFFFFFFFll);
+  return mix64(s * 0x9E3779B97F4A7C15ull + phase);
+}
+__host__ __device__ __forceinline__ uint64_t pass_key(uint64_t key, int pass) {
+  return mix64(key + static_cast<uint64_t>(pass + 1) * 0xD1B54A32D192ED03ull);
+}
+__device__ __forceinline__ uint32_t coin_u32(uint64_t pkey, uint64_t v) {
+  return static_cast<uint32_t>(mix64(pkey ^ (v * 0x9E3779B97F4A7C15ull)) >> 32);
+}
+
+}  // namespace msc

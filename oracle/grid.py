@@ -1,0 +1,84 @@
+"""Exact spatial kNN oracle (2-D / 3-D) — TEST INFRASTRUCTURE ONLY.
+
+Restates what ``gr.spatial_neighbors_knn`` builds (src/metalsinglecell/gr.py:174-225 on
+``neighbors._knn_grid``, neighbors.py:333-417): a directed graph with ``n_neighs`` edges per
+observation, self excluded, neighbours ordered by ``(float32 distance, index)``
+(gr.py:190-193).  Connectivities are 1.0 in float32 with columns in that order
+(gr.py:218-220); distances are float64 edge lengths from float64 coordinates (gr.py:32-34,
+214-217).
+
+The float32 distance is ``sqrt(fl(fl(dx*dx) + fl(dy*dy)))`` (plus ``fl(dz*dz)`` in 3-D),
+``dx = fl(x_j - x_i)``, every operation separately rounded.  Direct differences do not
+cancel at large coordinates — the failure the reference pins in
+tests/test_spatial_neighbors.py:377-386 (two spots 273 apart at offsets up to 2e5).
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+
+def spatial_knn(coords, k, block=512):
+    """Brute force: ``(idx int64 (n, k), dist float32 (n, k))`` with self excluded."""
+    X = np.ascontiguousarray(coords, dtype=np.float32)
+    n, d = X.shape
+    if not 1 <= k < n:
+        raise ValueError("need 1 <= k < n")
+    out_i = np.empty((n, k), np.int64)
+    out_d = np.empty((n, k), np.float32)
+    cols = np.arange(n)
+    for s in range(0, n, block):
+        Q = X[s:s + block]
+        acc = None
+        for t in range(d):
+            diff = (X[None, :, t] - Q[:, t, None]).astype(np.float32)
+            sq = (diff * diff).astype(np.float32)
+            acc = sq if acc is None else (acc + sq).astype(np.float32)
+        dist = np.sqrt(acc).astype(np.float32)
+        for r in range(Q.shape[0]):
+            i = s + r
+            drow = dist[r].copy()
+            drow[i] = np.inf
+            kth = np.partition(drow, k - 1)[k - 1]
+            cand = np.flatnonzero(drow <= kth)
+            cand = cand[cand != i]
+            o = np.lexsort((cols[cand], drow[cand]))[:k]
+            out_i[i] = cand[o]
+            out_d[i] = drow[cand[o]]
+    return out_i, out_d
+
+
+def spatial_knn_c(coords, k, threads=0):
+    """Same definition in C (knn_ref.c ``oracle_spatial_knn``)."""
+    import ctypes
+    from .knn import _lib
+    X = np.ascontiguousarray(coords, dtype=np.float32)
+    n, d = X.shape
+    lib = _lib()
+    lib.oracle_spatial_knn.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int,
+                                       ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                       ctypes.c_int]
+    out_i = np.empty((n, k), np.int64)
+    out_d = np.empty((n, k), np.float32)
+    rc = lib.oracle_spatial_knn(X.ctypes.data, n, d, k, out_i.ctypes.data,
+                                out_d.ctypes.data, int(threads))
+    if rc != 0:
+        raise RuntimeError(f"oracle_spatial_knn failed ({rc})")
+    return out_i, out_d
+
+
+def knn_graph(coords, n_neighs=6, exact=None):
+    """``(connectivities, distances)`` CSR pair as written by ``spatial_neighbors_knn``."""
+    import scipy.sparse as sp
+    X = np.asarray(coords)
+    n = X.shape[0]
+    idx, _ = spatial_knn(X, n_neighs) if n <= 20_000 else spatial_knn_c(X, n_neighs)
+    ex = np.asarray(X if exact is None else exact, dtype=np.float64)
+    rows = np.repeat(np.arange(n), n_neighs)
+    cols = idx.ravel()
+    diff = ex[rows] - ex[cols]
+    dist = np.sqrt((diff * diff).sum(1))
+    indptr = np.arange(n + 1, dtype=np.int64) * n_neighs
+    A = sp.csr_matrix((np.ones(dist.size, np.float32), cols, indptr), shape=(n, n))
+    D = sp.csr_matrix((dist, cols, indptr), shape=(n, n))
+    return A, D
