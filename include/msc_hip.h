@@ -70,6 +70,92 @@ int msc_knn_distances_csr(const int32_t* idx, const float* dist, int64_t n, int 
                           int64_t* out_indptr, int32_t* out_indices, float* out_data,
                           void* stream);
 
+/* ---------------------------------------------------------------------------------------
+ * (c) UMAP fuzzy-simplicial-set connectivities.
+ * Replaces the call of umap-learn's fuzzy_simplicial_set in neighbors.neighbors
+ * (src/metalsinglecell/neighbors.py:742-753) with set_op_mix_ratio=1, local_connectivity=1.
+ * knn_idx/knn_dist: n x k self-inclusive kNN (rows ascending, distinct indices per row).
+ * Output: symmetric canonical CSR (columns ascending, no zeros, no duplicates) in
+ * out_indptr (n+1 int64), out_indices / out_data (capacity entries; 2*n*(k-1) always
+ * suffices), per-row sigma and rho (n float32 each), and *out_nnz (host).  Synchronises. */
+int msc_fuzzy_workspace_size(int64_t n, int k, size_t* bytes /*host*/);
+int msc_fuzzy_connectivities(const int32_t* knn_idx, const float* knn_dist, int64_t n, int k,
+                             int64_t* out_indptr, int32_t* out_indices, float* out_data,
+                             int64_t capacity, float* out_sigma, float* out_rho,
+                             int64_t* out_nnz /*host*/, void* workspace, size_t workspace_bytes,
+                             void* stream);
+
+/* ---------------------------------------------------------------------------------------
+ * (d) Synchronous parallel Leiden (modularity, resolution gamma).
+ * Replaces cluster.leiden(backend="gpu") -> graph.leiden.leiden
+ * (src/metalsinglecell/cluster.py:36-54, graph/leiden.py:352-455) with its move / refine
+ * kernels (graph/louvain.py:88-131, graph/leiden.py:100-144), pass loop
+ * (louvain.py:282-394, leiden.py:251-349), contraction (louvain.py:440-453) and modularity
+ * (graph/primitives.py:94-111).
+ * Input: symmetric canonical CSR (indptr n+1 int64, indices nnz int32 columns ascending,
+ * weights nnz float32 >= 0), both directions stored, self-loops allowed.
+ * init_labels: n int32 labels to start the first pass from, or NULL (singletons).
+ * out_labels: n int32 dense labels 0..C-1.  *out_modularity (host, may be NULL): Q of the
+ * result.  Weights are quantised once to int64 fixed point (exponent reported in info), so
+ * every reduction is an exact integer sum; labels are deterministic.  Synchronises. */
+#define MSC_LEIDEN_MAX_RECORDS 64
+typedef struct {
+  int64_t n, n_edges;
+  int32_t move_passes, refine_passes;
+  int64_t n_comms, n_refined;
+  double modularity;
+} msc_leiden_level;
+
+typedef struct {
+  int n_levels;                 /* records written (all iterations, in order) */
+  int scale_exponent;           /* s: weights quantised as rint(w * 2^s) */
+  int64_t twom_fixed;           /* total directed weight in fixed point */
+  msc_leiden_level levels[MSC_LEIDEN_MAX_RECORDS];
+} msc_leiden_info;
+
+int msc_leiden_workspace_size(int64_t n, int64_t nnz, size_t* bytes /*host*/);
+int msc_leiden(const int64_t* indptr, const int32_t* indices, const float* weights, int64_t n,
+               int64_t nnz, double resolution, int64_t seed, int n_iterations,
+               double commit_prob, const int32_t* init_labels, int32_t* out_labels,
+               double* out_modularity /*host*/, msc_leiden_info* info /*host, may be NULL*/,
+               void* workspace, size_t workspace_bytes, void* stream);
+
+/* ---------------------------------------------------------------------------------------
+ * (b) Exact spatial kNN on a uniform grid, self excluded (2-D or 3-D coordinates).
+ * Replaces neighbors._knn_grid and its grid_knn kernel (src/metalsinglecell/neighbors.py:
+ * 333-417, 143-196) as used by gr._knn_edges / gr.spatial_neighbors_knn (gr.py:171-225).
+ * coords: n x dim float32.  out_idx / out_dist: n x k, ordered by (float32 distance, index).
+ * coords64 / out_len64 (optional, together): n x dim float64 coordinates -> n x k float64
+ * edge lengths (obsp['spatial_distances'] values, gr.py:32-34).  1 <= k <= 32, k < n. */
+typedef struct {
+  int32_t max_shell;     /* out: largest Chebyshev cell shell any query needed */
+  int32_t grid_x, grid_y, grid_z;
+  float cell;
+} msc_spatial_info;
+
+int msc_spatial_knn_workspace_size(int64_t n, int dim, int k, size_t* bytes /*host*/);
+int msc_spatial_knn(const float* coords, int64_t n, int dim, int k, int32_t* out_idx,
+                    float* out_dist, const double* coords64, double* out_len64,
+                    msc_spatial_info* info /*host, may be NULL*/, void* workspace,
+                    size_t workspace_bytes, void* stream);
+
+/* ---------------------------------------------------------------------------------------
+ * (e) Moran's I with a permutation null (float64 reductions).
+ * Replaces the Moran branch of spatial.spatial_autocorr (src/metalsinglecell/spatial.py:
+ * 357-406) including row normalisation (spatial.py:268-279) when transformation != 0.
+ * X: n x n_genes float32 row-major (cells x genes).  W: CSR (w_indptr n+1 int64, w_indices
+ * nnz int32, w_data nnz float64).  n_perms permutations come from `perms` (n_perms x n
+ * int32 device array, e.g. numpy default_rng(seed).permutation(n) drawn in sequence) or,
+ * when perms is NULL, from a device Feistel bijection keyed by `seed`.
+ * out_I: n_genes float64; out_sims: n_perms x n_genes float64; *out_S0 (host, may be NULL):
+ * the weight sum after normalisation. */
+int msc_moran_workspace_size(int64_t n, int64_t n_genes, int64_t nnz, int n_perms,
+                             size_t* bytes /*host*/);
+int msc_moran(const float* X, int64_t n, int64_t n_genes, const int64_t* w_indptr,
+              const int32_t* w_indices, const double* w_data, int64_t nnz, int transformation,
+              int n_perms, const int32_t* perms, int64_t seed, double* out_I, double* out_sims,
+              double* out_S0 /*host*/, void* workspace, size_t workspace_bytes, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,317 @@
+// UMAP fuzzy-simplicial-set connectivities on gfx950.
+//
+// Replaces the reference's CPU call of umap-learn's fuzzy_simplicial_set
+// (src/metalsinglecell/neighbors.py:742-753; set_op_mix_ratio=1, local_connectivity=1,
+// precomputed kNN).  Stages, all deterministic and free of global atomics:
+//   1. mean of all kNN distances (float64, fixed reduction order)
+//   2. per row: rho (first positive distance), sigma (<= 64 float32 bisection steps toward
+//      log2 k), the 1e-3 * mean floors, membership strengths v_ij
+//   3. transpose: stable radix sort of the nonzero off-diagonal edges by destination
+//   4. per row: union of the out-list (sorted by column) and the in-list (sources ascending),
+//      P_ij = (a + b) - a*b in float32; count, scan, fill -> canonical CSR
+// exp is the project's float32 expf32 (common.h), identical to the oracle's.
+#include "common.h"
+#include "msc_hip.h"
+#include "prims.h"
+
+namespace msc {
+namespace fuzzy {
+
+constexpr int kThreads = 256;
+
+__global__ void k_dist_partial(const float* __restrict__ dist, int64_t m, int64_t per,
+                               double* __restrict__ partial) {
+  const int64_t b0 = blockIdx.x * per, b1 = min(m, b0 + per);
+  double s = 0.0;
+  for (int64_t i = b0 + threadIdx.x; i < b1; i += blockDim.x) s += static_cast<double>(dist[i]);
+  s = wave_sum_f64(s);
+  __shared__ double sh[kThreads / kWave];
+  if (lane_id() == 0) sh[threadIdx.x / kWave] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < kThreads / kWave; ++w) t += sh[w];
+    partial[blockIdx.x] = t;
+  }
+}
+
+__global__ void k_dist_mean(const double* __restrict__ partial, int nb, int64_t m, double* __restrict__ mean) {
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int b = 0; b < nb; ++b) t += partial[b];
+    *mean = t / static_cast<double>(m);
+  }
+}
+
+// one thread per row; the row's distances staged in LDS (k <= 64)
+__global__ __launch_bounds__(64) void k_smooth_membership(
+    const int32_t* __restrict__ idx, const float* __restrict__ dist, int64_t n, int k,
+    double target, const double* __restrict__ mean_all, float* __restrict__ sigma_out,
+    float* __restrict__ rho_out, float* __restrict__ vals) {
+  __shared__ float sd[64 * 65];
+  const int64_t r0 = static_cast<int64_t>(blockIdx.x) * 64;
+  const int rows = static_cast<int>(min<int64_t>(64, n - r0));
+  for (int e = threadIdx.x; e < rows * k; e += 64) sd[(e / k) * 65 + (e % k)] = dist[r0 * k + e];
+  __syncthreads();
+  const int t = threadIdx.x;
+  if (t >= rows) return;
+  const int64_t i = r0 + t;
+  const float* D = &sd[t * 65];
+
+  float rho = 0.0f;
+  for (int j = 0; j < k; ++j)
+    if (D[j] > 0.0f) { rho = D[j]; break; }
+  double rowsum = 0.0;
+  for (int j = 0; j < k; ++j) rowsum += static_cast<double>(D[j]);
+
+  float lo = 0.0f, hi = __uint_as_float(0x7F800000u), mid = 1.0f;
+  for (int it = 0; it < 64; ++it) {
+    float psum = 0.0f;
+    for (int j = 1; j < k; ++j) {
+      const float dd = __fsub_rn(D[j], rho);
+      const float e = dd > 0.0f ? expf32(-(dd / mid)) : 1.0f;
+      psum = __fadd_rn(psum, e);
+    }
+    const double ps = static_cast<double>(psum);
+    if (fabs(ps - target) < 1e-5) break;
+    if (ps > target) {
+      hi = mid;
+      mid = __fadd_rn(lo, hi) / 2.0f;
+    } else {
+      lo = mid;
+      mid = isinf(hi) ? mid * 2.0f : __fadd_rn(lo, hi) / 2.0f;
+    }
+  }
+  float sigma = mid;
+  const double floor_v = (rho > 0.0f) ? 1e-3 * (rowsum / static_cast<double>(k)) : 1e-3 * (*mean_all);
+  if (static_cast<double>(sigma) < floor_v) sigma = static_cast<float>(floor_v);
+  sigma_out[i] = sigma;
+  rho_out[i] = rho;
+
+  for (int j = 0; j < k; ++j) {
+    const int32_t c = idx[i * k + j];
+    float v;
+    if (c == static_cast<int32_t>(i)) v = 0.0f;
+    else {
+      const float dd = __fsub_rn(D[j], rho);
+      v = (dd <= 0.0f || sigma == 0.0f) ? 1.0f : expf32(-(dd / sigma));
+    }
+    vals[i * k + j] = v;
+  }
+}
+
+// transposed edge list: key = destination for kept edges (off-diagonal, v != 0), n otherwise
+__global__ void k_edge_keys(const int32_t* __restrict__ idx, const float* __restrict__ vals, int64_t n,
+                            int k, uint32_t* __restrict__ key, uint32_t* __restrict__ eid) {
+  const int64_t e = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (e >= n * k) return;
+  const int64_t i = e / k;
+  const int32_t c = idx[e];
+  const bool keep = c >= 0 && c != static_cast<int32_t>(i) && vals[e] != 0.0f;
+  key[e] = keep ? static_cast<uint32_t>(c) : static_cast<uint32_t>(n);
+  eid[e] = static_cast<uint32_t>(e);
+}
+
+// lower_bound of x in the source ids of the in-list [lo, hi) (sources ascending)
+__device__ __forceinline__ int64_t in_lower_bound(const uint32_t* __restrict__ in_eid, int64_t lo,
+                                                  int64_t hi, int k, int64_t x) {
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (static_cast<int64_t>(in_eid[mid] / k) < x) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// one wave per row; FILL=false counts entries, FILL=true writes them
+template <bool FILL>
+__global__ __launch_bounds__(kThreads) void k_union(
+    const int32_t* __restrict__ idx, const float* __restrict__ vals, int64_t n, int k,
+    const int64_t* __restrict__ in_start, const uint32_t* __restrict__ in_eid,
+    int64_t* __restrict__ row_nnz, const int64_t* __restrict__ indptr,
+    int32_t* __restrict__ out_idx, float* __restrict__ out_val) {
+  __shared__ int64_t fcol[kThreads / kWave][64];
+  __shared__ int32_t fpre[kThreads / kWave][65];
+  const int w = threadIdx.x / kWave, l = lane_id();
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * (kThreads / kWave) + w;
+  if (i >= n) return;
+
+  // out-list F: kept entries of row i sorted by column (key = col << 32 | slot)
+  uint64_t key = ~0ull;
+  if (l < k) {
+    const int32_t c = idx[i * k + l];
+    if (c >= 0 && c != static_cast<int32_t>(i) && vals[i * k + l] != 0.0f)
+      key = (static_cast<uint64_t>(static_cast<uint32_t>(c)) << 32) | static_cast<uint32_t>(l);
+  }
+  key = wave_sort_u64(key);
+  const bool inF = key != ~0ull;
+  const int nF = __popcll(__ballot(inF));
+  const int64_t col = inF ? static_cast<int64_t>(key >> 32) : INT64_MAX;
+  const int slot = static_cast<int>(key & 0xFFFFFFFFu);
+  const int64_t s0 = in_start[i], s1 = in_start[i + 1];
+  const int64_t nS = s1 - s0;
+
+  // membership of each F column in the in-list
+  int64_t lb = s0;
+  bool found = false;
+  if (inF) {
+    lb = in_lower_bound(in_eid, s0, s1, k, col);
+    found = lb < s1 && static_cast<int64_t>(in_eid[lb] / k) == col;
+  }
+  const uint64_t fmask = __ballot(found);
+  const int nBoth = __popcll(fmask);
+  if (!FILL) {
+    if (l == 0) row_nnz[i] = nF + nS - nBoth;
+    if (i == 0 && l == 0) row_nnz[n] = 0;
+    return;
+  }
+  const int64_t base = indptr[i];
+  fcol[w][l] = col;
+  // fpre[j] = number of found flags among F[0..j)
+  const int pre = __popcll(fmask & ((1ull << l) - 1ull));
+  fpre[w][l] = pre;
+  if (l == kWave - 1) fpre[w][kWave] = nBoth;
+  // fcol/fpre are private to this wave: LDS ops of one wave complete in order, so only a
+  // compiler-level ordering point is needed (no block barrier; other waves may have exited)
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+
+  if (inF) {
+    const float a = vals[i * k + slot];
+    float v = a;
+    if (found) {
+      const float b = vals[in_eid[lb]];
+      v = __fsub_rn(__fadd_rn(a, b), __fmul_rn(a, b));
+    }
+    const int64_t pos = base + l + (lb - s0) - pre;
+    out_idx[pos] = static_cast<int32_t>(col);
+    out_val[pos] = v;
+  }
+  // in-only entries
+  for (int64_t p = s0 + l; p < s1; p += kWave) {
+    const uint32_t e = in_eid[p];
+    const int64_t s = static_cast<int64_t>(e / k);
+    // lower_bound of s in F
+    int lo = 0, hi = nF;
+    while (lo < hi) {
+      const int m = (lo + hi) >> 1;
+      if (fcol[w][m] < s) lo = m + 1;
+      else hi = m;
+    }
+    const bool dup = lo < nF && fcol[w][lo] == s;
+    if (!dup) {
+      const int64_t pos = base + lo + ((p - s0) - fpre[w][lo]);
+      out_idx[pos] = static_cast<int32_t>(s);
+      out_val[pos] = vals[e];
+    }
+  }
+}
+
+struct Bufs {
+  double* partial;
+  double* mean;
+  float* vals;
+  uint32_t* key;
+  uint32_t* eid;
+  uint32_t* skey;
+  uint32_t* seid;
+  int64_t* in_start;
+  int64_t* row_nnz;
+  void* sort_ws;
+  size_t sort_ws_bytes;
+  void* scan_ws;
+  size_t scan_ws_bytes;
+};
+
+constexpr int kMeanBlocks = 256;
+
+static Bufs carve(Carve& cv, int64_t n, int k) {
+  Bufs b{};
+  const int64_t m = n * k;
+  b.partial = cv.take<double>(kMeanBlocks);
+  b.mean = cv.take<double>(1);
+  b.vals = cv.take<float>(m);
+  b.key = cv.take<uint32_t>(m);
+  b.eid = cv.take<uint32_t>(m);
+  b.skey = cv.take<uint32_t>(m);
+  b.seid = cv.take<uint32_t>(m);
+  b.in_start = cv.take<int64_t>(n + 1);
+  b.row_nnz = cv.take<int64_t>(n + 1);
+  b.sort_ws_bytes = prims::radix_workspace(m);
+  b.sort_ws = cv.take<char>(b.sort_ws_bytes);
+  b.scan_ws_bytes = prims::scan_workspace<int64_t>(n + 1);
+  b.scan_ws = cv.take<char>(b.scan_ws_bytes);
+  return b;
+}
+
+static int validate(int64_t n, int k) {
+  MSC_REQUIRE(n >= 1 && k >= 1 && k <= 64, "fuzzy: need n >= 1 and 1 <= k <= 64");
+  MSC_REQUIRE(n * k < (int64_t(1) << 31), "fuzzy: n*k must be < 2^31");
+  return kOk;
+}
+
+}  // namespace fuzzy
+}  // namespace msc
+
+using namespace msc;
+using namespace msc::fuzzy;
+
+extern "C" int msc_fuzzy_workspace_size(int64_t n, int k, size_t* bytes) {
+  MSC_TRY(validate(n, k));
+  MSC_REQUIRE(bytes, "fuzzy: bytes is null");
+  Carve cv(nullptr, 0);
+  carve(cv, n, k);
+  *bytes = cv.used + 256;
+  return kOk;
+}
+
+extern "C" int msc_fuzzy_connectivities(const int32_t* knn_idx, const float* knn_dist, int64_t n,
+                                        int k, int64_t* out_indptr, int32_t* out_indices,
+                                        float* out_data, int64_t capacity, float* out_sigma,
+                                        float* out_rho, int64_t* out_nnz, void* workspace,
+                                        size_t workspace_bytes, void* stream) {
+  MSC_TRY(validate(n, k));
+  MSC_REQUIRE(knn_idx && knn_dist && out_indptr && out_indices && out_data && out_sigma &&
+                  out_rho && out_nnz && workspace,
+              "fuzzy: null pointer argument");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  Carve cv(workspace, workspace_bytes);
+  Bufs b = carve(cv, n, k);
+  if (!cv.ok()) {
+    set_error("fuzzy: workspace too small (%zu < %zu)", workspace_bytes, cv.used);
+    return kWorkspaceTooSmall;
+  }
+  const int64_t m = n * k;
+  const int64_t per = ceil_div(m, kMeanBlocks);
+  k_dist_partial<<<kMeanBlocks, kThreads, 0, st>>>(knn_dist, m, per, b.partial);
+  k_dist_mean<<<1, 64, 0, st>>>(b.partial, kMeanBlocks, m, b.mean);
+  MSC_LAUNCH_CHECK();
+  const double target = std::log2(static_cast<double>(k));
+  k_smooth_membership<<<static_cast<unsigned>(ceil_div(n, 64)), 64, 0, st>>>(
+      knn_idx, knn_dist, n, k, target, b.mean, out_sigma, out_rho, b.vals);
+  MSC_LAUNCH_CHECK();
+  k_edge_keys<<<static_cast<unsigned>(ceil_div(m, kThreads)), kThreads, 0, st>>>(knn_idx, b.vals, n, k,
+                                                                                 b.key, b.eid);
+  MSC_LAUNCH_CHECK();
+  MSC_TRY(prims::radix_sort_pairs(b.key, b.eid, b.skey, b.seid, m, prims::bits_for(n), b.sort_ws,
+                                  b.sort_ws_bytes, st));
+  MSC_TRY(prims::segment_starts(b.skey, m, n, b.in_start, st));
+  const unsigned wb = static_cast<unsigned>(ceil_div(n, kThreads / kWave));
+  k_union<false><<<wb, kThreads, 0, st>>>(knn_idx, b.vals, n, k, b.in_start, b.seid, b.row_nnz,
+                                          nullptr, nullptr, nullptr);
+  MSC_LAUNCH_CHECK();
+  MSC_TRY(prims::exclusive_scan<int64_t>(b.row_nnz, out_indptr, n + 1, b.scan_ws, b.scan_ws_bytes, st));
+  int64_t nnz = 0;
+  MSC_HIP(hipMemcpyAsync(&nnz, out_indptr + n, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  MSC_HIP(hipStreamSynchronize(st));
+  if (nnz > capacity) {
+    set_error("fuzzy: output capacity %lld < nnz %lld", (long long)capacity, (long long)nnz);
+    return kInvalidArgument;
+  }
+  k_union<true><<<wb, kThreads, 0, st>>>(knn_idx, b.vals, n, k, b.in_start, b.seid, nullptr,
+                                         out_indptr, out_indices, out_data);
+  MSC_LAUNCH_CHECK();
+  *out_nnz = nnz;
+  return kOk;
+}

@@ -1,0 +1,953 @@
+// Synchronous parallel Leiden on gfx950, with exact fixed-point weights.
+//
+// Replaces graph.leiden.leiden / _leiden_pass (src/metalsinglecell/graph/leiden.py:352-455)
+// with the move and refine kernels (graph/louvain.py:88-131, graph/leiden.py:100-144), the
+// synchronous pass loop (louvain.py:282-394, leiden.py:251-349), contraction
+// (louvain.py:440-453) and modularity (graph/primitives.py:94-111).
+//
+// Arithmetic (identical to oracle/leiden.py): weights quantised once to int64 fixed point
+// (wq = rint(w * 2^s), s = 52 - ceil(log2 max w) - ceil(log2 E)), so degrees, sigma_tot,
+// community weights and contracted weights are exact integer sums; gains are one fixed
+// float64 expression; the coin is the counter RNG of common.h.  Every reduction is a
+// sort-based segmented sum (stable radix sort + int64 prefix differences) or a per-vertex
+// wave sum: no atomics anywhere, so labels are bit-identical to the oracle and independent
+// of scheduling.
+//
+// Per-vertex best move: vertices of degree <= 64 take one wave (one neighbour per lane,
+// distinct communities found by ballot); heavier vertices go through a sort path over
+// their edges keyed by (vertex, community).
+#include <cmath>
+#include <vector>
+
+#include "common.h"
+#include "msc_hip.h"
+#include "prims.h"
+
+namespace msc {
+namespace leiden {
+
+constexpr int kThreads = 256;
+constexpr int kWavesPerBlock = kThreads / kWave;
+constexpr int kLightDeg = 64;
+constexpr int kDegreeCap = 1024;  // pruning is disabled above this degree (louvain.py:350-352)
+constexpr int kMaxLevels = 20;
+constexpr int kMaxPasses = 200;
+constexpr int kSyncEvery = 4;
+constexpr double kTol = 1e-9;
+
+struct Graph {
+  int64_t n = 0, E = 0;
+  int64_t* indptr = nullptr;
+  int32_t* indices = nullptr;
+  int64_t* wq = nullptr;
+  int64_t* k = nullptr;       // weighted degree (n)
+  int64_t twom = 0;
+  int64_t max_deg = 0;
+};
+
+struct GainParams {
+  double inv_s;   // 2^-s
+  double inv;     // 1 / (twom * 2^-s)
+  double gamma;
+};
+
+__device__ __forceinline__ double gain_of(int64_t wc, int64_t sig, int64_t kv, const GainParams& gp) {
+  const double W = static_cast<double>(wc) * gp.inv_s;
+  const double S = static_cast<double>(sig) * gp.inv_s;
+  const double K = static_cast<double>(kv) * gp.inv_s;
+  return W - ((gp.gamma * S) * K) * gp.inv;
+}
+
+// better(g1, c1) than (g2, c2): larger gain, ties to the smaller community
+__device__ __forceinline__ bool better(double g1, int32_t c1, double g2, int32_t c2) {
+  return g1 > g2 || (g1 == g2 && c1 < c2);
+}
+
+// ------------------------------------------------------------------------------------------
+// simple element-wise kernels
+// ------------------------------------------------------------------------------------------
+__global__ void k_fill_i32(int32_t* p, int64_t n, int32_t v) {
+  const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+__global__ void k_iota_i32(int32_t* p, int64_t n) {
+  const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (i < n) p[i] = static_cast<int32_t>(i);
+}
+__global__ void k_fill_u8(uint8_t* p, int64_t n, uint8_t v) {
+  const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+__global__ void k_gather_i32(const int32_t* __restrict__ src, const int32_t* __restrict__ idx, int32_t* __restrict__ dst, int64_t n) {
+  const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (i < n) dst[i] = src[idx[i]];
+}
+__global__ void k_copy_i32(const int32_t* __restrict__ src, int32_t* __restrict__ dst, int64_t n) {
+  const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (i < n) dst[i] = src[i];
+}
+
+static unsigned grid_for(int64_t n) { return static_cast<unsigned>(std::max<int64_t>(1, ceil_div(n, kThreads))); }
+
+// ------------------------------------------------------------------------------------------
+// graph set-up
+// ------------------------------------------------------------------------------------------
+__global__ void k_max_weight(const float* __restrict__ w, int64_t E, float* __restrict__ out) {
+  float m = 0.0f;
+  for (int64_t i = threadIdx.x; i < E; i += blockDim.x) m = fmaxf(m, w[i]);
+  m = wave_max_f32(m);
+  __shared__ float sh[kWavesPerBlock];
+  if (lane_id() == 0) sh[threadIdx.x / kWave] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int i = 1; i < kWavesPerBlock; ++i) m = fmaxf(m, sh[i]);
+    *out = fmaxf(m, sh[0]);
+  }
+}
+
+__global__ void k_quantize(const float* __restrict__ w, int64_t E, int s, int64_t* __restrict__ wq) {
+  const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (i < E) wq[i] = static_cast<int64_t>(rint(ldexp(static_cast<double>(w[i]), s)));
+}
+
+// weighted degree per vertex (wave per vertex, exact) and the degree
+__global__ __launch_bounds__(kThreads) void k_degrees(const int64_t* __restrict__ indptr, const int64_t* __restrict__ wq,
+                                                      int64_t n, int64_t* __restrict__ k) {
+  const int64_t v = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + threadIdx.x / kWave;
+  if (v >= n) return;
+  int64_t s = 0;
+  for (int64_t e = indptr[v] + lane_id(); e < indptr[v + 1]; e += kWave) s += wq[e];
+  s = wave_sum_i64(s);
+  if (lane_id() == 0) k[v] = s;
+}
+
+// block partial sums (int64, exact) and max degree
+__global__ void k_sum_i64_max_deg(const int64_t* __restrict__ k, const int64_t* __restrict__ indptr, int64_t n,
+                                  int64_t* __restrict__ out /*[0]=sum,[1]=maxdeg*/) {
+  int64_t s = 0, md = 0;
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+    s += k[i];
+    md = max(md, indptr[i + 1] - indptr[i]);
+  }
+  s = wave_sum_i64(s);
+#pragma unroll
+  for (int m = 1; m < kWave; m <<= 1) md = max(md, static_cast<int64_t>(shfl_xor_u64(static_cast<uint64_t>(md), m)));
+  __shared__ int64_t ss[kWavesPerBlock], sm[kWavesPerBlock];
+  if (lane_id() == 0) { ss[threadIdx.x / kWave] = s; sm[threadIdx.x / kWave] = md; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int64_t a = 0, b = 0;
+    for (int i = 0; i < kWavesPerBlock; ++i) { a += ss[i]; b = max(b, sm[i]); }
+    out[0] = a;
+    out[1] = b;
+  }
+}
+
+// ------------------------------------------------------------------------------------------
+// grouped sums by label (sort-based, exact): out[c] = sum_{v: lab[v]=c} val[v]
+// ------------------------------------------------------------------------------------------
+struct GroupScratch {
+  uint32_t *keys, *vals, *skeys, *svals;
+  int64_t* gathered;  // n + 1
+  int64_t* prefix;    // n + 1
+  int64_t* starts;    // C + 1
+  void* sort_ws;
+  size_t sort_ws_bytes;
+  void* scan_ws;
+  size_t scan_ws_bytes;
+};
+
+__global__ void k_label_keys(const int32_t* __restrict__ lab, int64_t n, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+  const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (i < n) {
+    keys[i] = static_cast<uint32_t>(lab[i]);
+    vals[i] = static_cast<uint32_t>(i);
+  }
+}
+__global__ void k_gather_i64_perm(const int64_t* __restrict__ src, const uint32_t* __restrict__ perm, int64_t n,
+                                  int64_t* __restrict__ dst) {
+  const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (i < n) dst[i] = src[perm[i]];
+  if (i == n) dst[n] = 0;
+}
+__global__ void k_seg_diff(const int64_t* __restrict__ prefix, const int64_t* __restrict__ starts, int64_t C,
+                           int64_t* __restrict__ out) {
+  const int64_t c = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (c < C) out[c] = prefix[starts[c + 1]] - prefix[starts[c]];
+}
+
+// sort (label, vertex) once; then any number of value arrays can be grouped
+static int group_sort(const int32_t* lab, int64_t n, int64_t C, GroupScratch& g, hipStream_t st) {
+  k_label_keys<<<grid_for(n), kThreads, 0, st>>>(lab, n, g.keys, g.vals);
+  MSC_LAUNCH_CHECK();
+  MSC_TRY(prims::radix_sort_pairs(g.keys, g.vals, g.skeys, g.svals, n, prims::bits_for(std::max<int64_t>(C - 1, 1)),
+                                  g.sort_ws, g.sort_ws_bytes, st));
+  MSC_TRY(prims::segment_starts(g.skeys, n, C, g.starts, st));
+  return kOk;
+}
+static int group_sum(const int64_t* val, int64_t n, int64_t C, GroupScratch& g, int64_t* out, hipStream_t st) {
+  k_gather_i64_perm<<<grid_for(n + 1), kThreads, 0, st>>>(val, g.svals, n, g.gathered);
+  MSC_LAUNCH_CHECK();
+  MSC_TRY(prims::exclusive_scan<int64_t>(g.gathered, g.prefix, n + 1, g.scan_ws, g.scan_ws_bytes, st));
+  k_seg_diff<<<grid_for(C), kThreads, 0, st>>>(g.prefix, g.starts, C, out);
+  MSC_LAUNCH_CHECK();
+  return kOk;
+}
+
+// ------------------------------------------------------------------------------------------
+// best move, light vertices (degree <= 64): one wave per vertex
+// ------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(kThreads) void k_move_light(
+    const int64_t* __restrict__ indptr, const int32_t* __restrict__ indices, const int64_t* __restrict__ wq,
+    const int64_t* __restrict__ kdeg, int64_t n, const int32_t* __restrict__ comm,
+    const int32_t* __restrict__ part, const int64_t* __restrict__ sigtot, const uint8_t* __restrict__ active,
+    GainParams gp, int32_t* __restrict__ target) {
+  const int64_t v = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + threadIdx.x / kWave;
+  if (v >= n) return;
+  const int l = lane_id();
+  const int64_t e0 = indptr[v], deg = indptr[v + 1] - e0;
+  const int32_t curr = comm[v];
+  if (deg > kLightDeg) return;  // heavy path writes target[v]
+  if (!active[v]) {
+    if (l == 0) target[v] = curr;
+    return;
+  }
+  const int32_t pv = part ? part[v] : 0;
+  bool ok = false;
+  int32_t c = -1;
+  int64_t w = 0;
+  if (l < deg) {
+    const int32_t u = indices[e0 + l];
+    ok = u != static_cast<int32_t>(v) && (!part || part[u] == pv);
+    if (ok) {
+      c = comm[u];
+      w = wq[e0 + l];
+    }
+  }
+  const int64_t kv = kdeg[v];
+  uint64_t pending = __ballot(ok);
+  double best_g = -1e300;
+  int32_t best_c = curr;
+  double stay = 0.0;
+  while (pending) {
+    const int leader = __ffsll(static_cast<long long>(pending)) - 1;
+    const int32_t cc = __shfl(c, leader, kWave);
+    const uint64_t match = __ballot(ok && c == cc);
+    const int64_t wc = wave_sum_i64((ok && c == cc) ? w : 0);
+    const int64_t sig = sigtot[cc] - (cc == curr ? kv : 0);
+    const double g = gain_of(wc, sig, kv, gp);
+    if (cc == curr) stay = g;
+    if (better(g, cc, best_g, best_c)) {
+      best_g = g;
+      best_c = cc;
+    }
+    pending &= ~match;
+  }
+  if (l == 0) target[v] = (best_g > stay + kTol) ? best_c : curr;
+}
+
+// ------------------------------------------------------------------------------------------
+// best move, heavy vertices (degree > 64): sort their eligible edges by (vertex, community)
+// ------------------------------------------------------------------------------------------
+struct Heavy {
+  int64_t nh = 0, eh = 0;
+  int32_t* verts = nullptr;   // heavy vertex ids, ascending (nh)
+  int64_t* eoff = nullptr;    // edge offsets per heavy vertex (nh + 1)
+  int32_t* eh_h = nullptr;    // heavy-edge -> local heavy index (eh)
+  int64_t* eh_e = nullptr;    // heavy-edge -> CSR edge id (eh)
+  // per-pass scratch (eh + 1)
+  uint32_t *kc, *kv, *skc, *skv, *kh, *skh, *skv2;
+  int64_t* wsorted;
+  int64_t* prefix;
+  int32_t* flag;
+  int32_t* segid;
+  int64_t* segstart;   // nseg + 1 (<= eh + 1)
+  uint32_t* seg_h;     // per segment heavy index
+  int64_t* hseg;       // nh + 1: first segment per heavy vertex
+};
+
+__global__ void k_heavy_flags(const int64_t* __restrict__ indptr, int64_t n, int32_t* __restrict__ flag,
+                              int64_t* __restrict__ deg_h) {
+  const int64_t v = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (v < n) {
+    const int64_t d = indptr[v + 1] - indptr[v];
+    flag[v] = d > kLightDeg ? 1 : 0;
+    deg_h[v] = d > kLightDeg ? d : 0;
+  }
+  if (v == n) {
+    flag[n] = 0;
+    deg_h[n] = 0;
+  }
+}
+__global__ void k_heavy_compact(const int32_t* __restrict__ flag, const int32_t* __restrict__ pos, int64_t n,
+                                int32_t* __restrict__ verts) {
+  const int64_t v = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (v < n && flag[v]) verts[pos[v]] = static_cast<int32_t>(v);
+}
+// one wave per heavy vertex: write its edge ids and local index
+__global__ void k_heavy_edges(const int64_t* __restrict__ indptr, const int32_t* __restrict__ verts,
+                              const int64_t* __restrict__ degoff_all, int64_t nh, int64_t* __restrict__ eoff,
+                              int32_t* __restrict__ eh_h, int64_t* __restrict__ eh_e, int64_t eh) {
+  const int64_t h = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + threadIdx.x / kWave;
+  if (h > nh) return;
+  if (h == nh) {
+    if (lane_id() == 0) eoff[nh] = eh;
+    return;
+  }
+  const int32_t v = verts[h];
+  const int64_t base = degoff_all[v];
+  if (lane_id() == 0) eoff[h] = base;
+  const int64_t e0 = indptr[v], d = indptr[v + 1] - e0;
+  for (int64_t j = lane_id(); j < d; j += kWave) {
+    eh_h[base + j] = static_cast<int32_t>(h);
+    eh_e[base + j] = e0 + j;
+  }
+}
+
+__global__ void k_heavy_keys(const int32_t* __restrict__ eh_h, const int64_t* __restrict__ eh_e, int64_t eh,
+                             const int32_t* __restrict__ verts, const int32_t* __restrict__ indices,
+                             const int32_t* __restrict__ comm, const int32_t* __restrict__ part,
+                             const uint8_t* __restrict__ active, int64_t n, uint32_t* __restrict__ kc,
+                             uint32_t* __restrict__ kvv) {
+  const int64_t p = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (p >= eh) return;
+  const int32_t v = verts[eh_h[p]];
+  const int32_t u = indices[eh_e[p]];
+  const bool ok = active[v] && u != v && (!part || part[u] == part[v]);
+  kc[p] = ok ? static_cast<uint32_t>(comm[u]) : static_cast<uint32_t>(n);
+  kvv[p] = static_cast<uint32_t>(p);
+}
+__global__ void k_heavy_hkeys(const uint32_t* __restrict__ perm, const int32_t* __restrict__ eh_h, int64_t eh,
+                              uint32_t* __restrict__ kh) {
+  const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (i < eh) kh[i] = static_cast<uint32_t>(eh_h[perm[i]]);
+}
+// after both sorts: perm2[i] = heavy position p in (h, comm) order
+__global__ void k_heavy_seg_flags(const uint32_t* __restrict__ perm2, const uint32_t* __restrict__ skh, int64_t eh,
+                                  const int64_t* __restrict__ eh_e, const int32_t* __restrict__ indices,
+                                  const int32_t* __restrict__ comm, const uint32_t* __restrict__ kc_unsorted,
+                                  const int64_t* __restrict__ wq, int64_t n, int32_t* __restrict__ flag,
+                                  int64_t* __restrict__ wsorted) {
+  const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (i > eh) return;
+  if (i == eh) {
+    wsorted[eh] = 0;
+    flag[eh] = 0;
+    return;
+  }
+  const uint32_t p = perm2[i];
+  const uint32_t c = kc_unsorted[p];
+  const bool ok = c != static_cast<uint32_t>(n);
+  wsorted[i] = ok ? wq[eh_e[p]] : 0;
+  bool start = ok;
+  if (ok && i > 0) {
+    const uint32_t pp = perm2[i - 1];
+    start = !(skh[i - 1] == skh[i] && kc_unsorted[pp] == c);
+  }
+  flag[i] = start ? 1 : 0;
+}
+__global__ void k_heavy_seg_starts(const int32_t* __restrict__ flag, const int32_t* __restrict__ pos, int64_t eh,
+                                   const uint32_t* __restrict__ skh, int64_t* __restrict__ segstart,
+                                   uint32_t* __restrict__ seg_h) {
+  const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (i < eh && flag[i]) {
+    segstart[pos[i]] = i;
+    seg_h[pos[i]] = skh[i];
+  }
+  if (i == eh) segstart[pos[eh]] = eh;
+}
+// one wave per heavy vertex over its segments
+__global__ __launch_bounds__(kThreads) void k_heavy_argmax(
+    const int32_t* __restrict__ verts, int64_t nh, const int64_t* __restrict__ hseg,
+    const int64_t* __restrict__ segstart, const int64_t* __restrict__ prefix, const uint32_t* __restrict__ perm2,
+    const uint32_t* __restrict__ kc_unsorted, const int64_t* __restrict__ kdeg, const int32_t* __restrict__ comm,
+    const int64_t* __restrict__ sigtot, GainParams gp, int32_t* __restrict__ target) {
+  const int64_t h = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + threadIdx.x / kWave;
+  if (h >= nh) return;
+  const int32_t v = verts[h];
+  const int32_t curr = comm[v];
+  const int64_t kv = kdeg[v];
+  double best_g = -1e300, stay = 0.0;
+  int32_t best_c = curr;
+  bool has_stay = false;
+  for (int64_t s = hseg[h] + lane_id(); s < hseg[h + 1]; s += kWave) {
+    const int64_t a = segstart[s], b = segstart[s + 1];
+    const int32_t c = static_cast<int32_t>(kc_unsorted[perm2[a]]);
+    const int64_t wc = prefix[b] - prefix[a];
+    const int64_t sig = sigtot[c] - (c == curr ? kv : 0);
+    const double g = gain_of(wc, sig, kv, gp);
+    if (c == curr) { stay = g; has_stay = true; }
+    if (better(g, c, best_g, best_c)) { best_g = g; best_c = c; }
+  }
+#pragma unroll
+  for (int m = 1; m < kWave; m <<= 1) {
+    const double og = __shfl_xor(best_g, m, kWave);
+    const int32_t oc = __shfl_xor(best_c, m, kWave);
+    if (better(og, oc, best_g, best_c)) { best_g = og; best_c = oc; }
+    const double os = __shfl_xor(stay, m, kWave);
+    const int ohs = __shfl_xor(static_cast<int>(has_stay), m, kWave);
+    if (ohs && !has_stay) { stay = os; has_stay = true; }
+  }
+  if (lane_id() == 0) target[v] = (best_g > stay + kTol) ? best_c : curr;
+}
+
+// ------------------------------------------------------------------------------------------
+// commit, frontier, relabel, modularity, contraction
+// ------------------------------------------------------------------------------------------
+__global__ void k_commit(int32_t* __restrict__ comm, const int32_t* __restrict__ target, int64_t n, uint64_t pkey,
+                         uint64_t thr, uint8_t* __restrict__ commit, uint8_t* __restrict__ held,
+                         int32_t* __restrict__ wants_flag) {
+  const int64_t v = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (v >= n) return;
+  const int32_t t = target[v], c = comm[v];
+  const bool wants = t != c;
+  const bool coin = static_cast<uint64_t>(coin_u32(pkey, static_cast<uint64_t>(v))) < thr;
+  const bool cm = wants && coin;
+  if (cm) comm[v] = t;
+  commit[v] = cm;
+  held[v] = wants && !coin;
+  if (wants) *wants_flag = 1;  // every writer stores the same value
+}
+
+__global__ __launch_bounds__(kThreads) void k_frontier(const int64_t* __restrict__ indptr, const int32_t* __restrict__ indices,
+                                                       int64_t n, const uint8_t* __restrict__ commit,
+                                                       const uint8_t* __restrict__ held, uint8_t* __restrict__ active) {
+  const int64_t v = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + threadIdx.x / kWave;
+  if (v >= n) return;
+  bool a = false;
+  for (int64_t e = indptr[v] + lane_id(); e < indptr[v + 1]; e += kWave) a |= commit[indices[e]] != 0;
+  a = __any(a);
+  if (lane_id() == 0) active[v] = (a || commit[v] || held[v]) ? 1 : 0;
+}
+
+__global__ void k_mark_used(const int32_t* __restrict__ comm, int64_t n, int32_t* __restrict__ used) {
+  const int64_t v = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (v < n) used[comm[v]] = 1;  // every writer stores the same value
+}
+__global__ void k_apply_map(const int32_t* __restrict__ comm, const int32_t* __restrict__ map, int64_t n,
+                            int32_t* __restrict__ dense) {
+  const int64_t v = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (v < n) dense[v] = map[comm[v]];
+}
+
+// internal weight per vertex: sum of w over neighbours (self-loops included) in the same label
+__global__ __launch_bounds__(kThreads) void k_internal(const int64_t* __restrict__ indptr, const int32_t* __restrict__ indices,
+                                                       const int64_t* __restrict__ wq, const int32_t* __restrict__ lab,
+                                                       int64_t n, int64_t* __restrict__ iw) {
+  const int64_t v = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + threadIdx.x / kWave;
+  if (v >= n) return;
+  const int32_t lv = lab[v];
+  int64_t s = 0;
+  for (int64_t e = indptr[v] + lane_id(); e < indptr[v + 1]; e += kWave)
+    if (lab[indices[e]] == lv) s += wq[e];
+  s = wave_sum_i64(s);
+  if (lane_id() == 0) iw[v] = s;
+}
+__global__ void k_q_terms(const int64_t* __restrict__ tot, const int64_t* __restrict__ in, int64_t C, double inv_s,
+                          double twom_real, double gamma, double* __restrict__ terms) {
+  const int64_t c = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (c < C) {
+    const double a = (static_cast<double>(tot[c]) * inv_s) / twom_real;
+    terms[c] = (static_cast<double>(in[c]) * inv_s) / twom_real - gamma * (a * a);
+  }
+}
+
+// contraction keys
+__global__ void k_edge_src_dst_keys(const int64_t* __restrict__ indptr, const int32_t* __restrict__ indices,
+                                    const int32_t* __restrict__ R, int64_t n, uint32_t* __restrict__ kdst,
+                                    uint32_t* __restrict__ eid) {
+  // one wave per source row
+  const int64_t v = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + threadIdx.x / kWave;
+  if (v >= n) return;
+  for (int64_t e = indptr[v] + lane_id(); e < indptr[v + 1]; e += kWave) {
+    kdst[e] = static_cast<uint32_t>(R[indices[e]]);
+    eid[e] = static_cast<uint32_t>(e);
+  }
+}
+__global__ __launch_bounds__(kThreads) void k_edge_src_of(const int64_t* __restrict__ indptr, const int32_t* __restrict__ R,
+                                                          int64_t n, int32_t* __restrict__ esrc) {
+  const int64_t v = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + threadIdx.x / kWave;
+  if (v >= n) return;
+  const int32_t r = R[v];
+  for (int64_t e = indptr[v] + lane_id(); e < indptr[v + 1]; e += kWave) esrc[e] = r;
+}
+__global__ void k_gather_src_keys(const uint32_t* __restrict__ perm, const int32_t* __restrict__ esrc, int64_t E,
+                                  uint32_t* __restrict__ ksrc) {
+  const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (i < E) ksrc[i] = static_cast<uint32_t>(esrc[perm[i]]);
+}
+// sorted by (src, dst): flags of segment starts, weights in sorted order
+__global__ void k_contract_flags(const uint32_t* __restrict__ perm2, const uint32_t* __restrict__ ssrc,
+                                 const uint32_t* __restrict__ kdst_unsorted, const int64_t* __restrict__ wq, int64_t E,
+                                 int32_t* __restrict__ flag, int64_t* __restrict__ wsorted) {
+  const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (i > E) return;
+  if (i == E) {
+    flag[E] = 0;
+    wsorted[E] = 0;
+    return;
+  }
+  const uint32_t e = perm2[i];
+  wsorted[i] = wq[e];
+  bool start = true;
+  if (i > 0) start = !(ssrc[i - 1] == ssrc[i] && kdst_unsorted[perm2[i - 1]] == kdst_unsorted[e]);
+  flag[i] = start ? 1 : 0;
+}
+__global__ void k_contract_emit(const int32_t* __restrict__ flag, const int32_t* __restrict__ pos, int64_t E,
+                                const uint32_t* __restrict__ perm2, const uint32_t* __restrict__ ssrc,
+                                const uint32_t* __restrict__ kdst_unsorted, const int64_t* __restrict__ prefix,
+                                int32_t* __restrict__ out_idx, uint32_t* __restrict__ out_src,
+                                int64_t* __restrict__ segstart) {
+  const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (i < E && flag[i]) {
+    const int32_t s = pos[i];
+    out_idx[s] = static_cast<int32_t>(kdst_unsorted[perm2[i]]);
+    out_src[s] = ssrc[i];
+    segstart[s] = i;
+  }
+  if (i == E) segstart[pos[E]] = E;
+}
+__global__ void k_contract_weights(const int64_t* __restrict__ segstart, const int64_t* __restrict__ prefix, int64_t m,
+                                   int64_t* __restrict__ out_w) {
+  const int64_t s = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (s < m) out_w[s] = prefix[segstart[s + 1]] - prefix[segstart[s]];
+}
+__global__ void k_scatter_pagg(const int32_t* __restrict__ R, const int32_t* __restrict__ P, int64_t n,
+                               int32_t* __restrict__ Pagg) {
+  const int64_t v = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (v < n) Pagg[R[v]] = P[v];  // refined communities lie inside one P community: same value
+}
+
+// ------------------------------------------------------------------------------------------
+// the driver
+// ------------------------------------------------------------------------------------------
+struct Ctx {
+  hipStream_t st;
+  int64_t n0, E0;
+  double gamma;
+  uint64_t thr;
+  int s;           // fixed-point exponent
+  double inv_s;
+  // per-vertex (n0)
+  int32_t *comm, *target, *P, *R, *Pagg, *orig2node, *final_lab, *prev_lab, *dense, *used, *map, *tmp_lab;
+  uint8_t *active, *commit, *held;
+  int64_t *sigtot, *iw, *tot, *inw, *kbuf[2];
+  double* terms;
+  int32_t* wants_flags;     // kMaxPasses
+  int64_t* red;             // small reduction outputs
+  float* fred;
+  // graph slots
+  int64_t* gindptr[2];
+  int32_t* gindices[2];
+  int64_t* gwq[2];
+  GroupScratch grp;
+  // contraction scratch (E0 + 1)
+  uint32_t *ckd, *ceid, *cskd, *cseid, *cks, *csks, *cseid2;
+  int32_t *cesrc, *cflag, *cpos;
+  int64_t *cw, *cprefix, *cseg;
+  uint32_t* csrc_out;
+  void* sort_ws;
+  size_t sort_ws_bytes;
+  void* scan_ws;
+  size_t scan_ws_bytes;
+  Heavy hv;
+  int32_t *hflag, *hpos;
+  int64_t *hdeg, *hdegoff;
+  msc_leiden_info* info;
+};
+
+static void carve_ctx(Carve& cv, Ctx& c, int64_t n, int64_t E) {
+  const int64_t n1 = n + 1, E1 = E + 1;
+  c.comm = cv.take<int32_t>(n1); c.target = cv.take<int32_t>(n1); c.P = cv.take<int32_t>(n1);
+  c.R = cv.take<int32_t>(n1); c.Pagg = cv.take<int32_t>(n1); c.orig2node = cv.take<int32_t>(n1);
+  c.final_lab = cv.take<int32_t>(n1); c.prev_lab = cv.take<int32_t>(n1); c.dense = cv.take<int32_t>(n1);
+  c.used = cv.take<int32_t>(n1); c.map = cv.take<int32_t>(n1); c.tmp_lab = cv.take<int32_t>(n1);
+  c.active = cv.take<uint8_t>(n1); c.commit = cv.take<uint8_t>(n1); c.held = cv.take<uint8_t>(n1);
+  c.sigtot = cv.take<int64_t>(n1); c.iw = cv.take<int64_t>(n1); c.tot = cv.take<int64_t>(n1);
+  c.inw = cv.take<int64_t>(n1); c.kbuf[0] = cv.take<int64_t>(n1); c.kbuf[1] = cv.take<int64_t>(n1);
+  c.terms = cv.take<double>(n1);
+  c.wants_flags = cv.take<int32_t>(kMaxPasses);
+  c.red = cv.take<int64_t>(8);
+  c.fred = cv.take<float>(8);
+  for (int i = 0; i < 2; ++i) {
+    c.gindptr[i] = cv.take<int64_t>(n1);
+    c.gindices[i] = cv.take<int32_t>(E1);
+    c.gwq[i] = cv.take<int64_t>(E1);
+  }
+  GroupScratch& g = c.grp;
+  g.keys = cv.take<uint32_t>(n1); g.vals = cv.take<uint32_t>(n1);
+  g.skeys = cv.take<uint32_t>(n1); g.svals = cv.take<uint32_t>(n1);
+  g.gathered = cv.take<int64_t>(n1); g.prefix = cv.take<int64_t>(n1); g.starts = cv.take<int64_t>(n1 + 1);
+  c.ckd = cv.take<uint32_t>(E1); c.ceid = cv.take<uint32_t>(E1); c.cskd = cv.take<uint32_t>(E1);
+  c.cseid = cv.take<uint32_t>(E1); c.cks = cv.take<uint32_t>(E1); c.csks = cv.take<uint32_t>(E1);
+  c.cseid2 = cv.take<uint32_t>(E1);
+  c.cesrc = cv.take<int32_t>(E1); c.cflag = cv.take<int32_t>(E1); c.cpos = cv.take<int32_t>(E1);
+  c.cw = cv.take<int64_t>(E1); c.cprefix = cv.take<int64_t>(E1); c.cseg = cv.take<int64_t>(E1 + 1);
+  c.csrc_out = cv.take<uint32_t>(E1);
+  c.sort_ws_bytes = prims::radix_workspace(std::max(E1, n1));
+  c.sort_ws = cv.take<char>(c.sort_ws_bytes);
+  c.scan_ws_bytes = std::max(prims::scan_workspace<int64_t>(E1 + 1), prims::scan_workspace<int32_t>(E1 + 1));
+  c.scan_ws = cv.take<char>(c.scan_ws_bytes);
+  g.sort_ws = c.sort_ws; g.sort_ws_bytes = c.sort_ws_bytes;
+  g.scan_ws = c.scan_ws; g.scan_ws_bytes = c.scan_ws_bytes;
+  Heavy& h = c.hv;
+  h.verts = cv.take<int32_t>(n1); h.eoff = cv.take<int64_t>(n1 + 1);
+  h.eh_h = cv.take<int32_t>(E1); h.eh_e = cv.take<int64_t>(E1);
+  h.kc = cv.take<uint32_t>(E1); h.kv = cv.take<uint32_t>(E1); h.skc = cv.take<uint32_t>(E1);
+  h.skv = cv.take<uint32_t>(E1); h.kh = cv.take<uint32_t>(E1); h.skh = cv.take<uint32_t>(E1);
+  h.skv2 = cv.take<uint32_t>(E1);
+  h.wsorted = cv.take<int64_t>(E1); h.prefix = cv.take<int64_t>(E1);
+  h.flag = cv.take<int32_t>(E1); h.segid = cv.take<int32_t>(E1);
+  h.segstart = cv.take<int64_t>(E1 + 1); h.seg_h = cv.take<uint32_t>(E1); h.hseg = cv.take<int64_t>(n1 + 1);
+  c.hflag = cv.take<int32_t>(n1); c.hpos = cv.take<int32_t>(n1);
+  c.hdeg = cv.take<int64_t>(n1); c.hdegoff = cv.take<int64_t>(n1);
+}
+
+template <typename T>
+static int read_scalar(const T* dev, T* host, hipStream_t st) {
+  MSC_HIP(hipMemcpyAsync(host, dev, sizeof(T), hipMemcpyDeviceToHost, st));
+  MSC_HIP(hipStreamSynchronize(st));
+  return kOk;
+}
+
+// graph derived quantities: k, twom, max degree, heavy structure
+static int graph_setup(Ctx& c, Graph& g) {
+  hipStream_t st = c.st;
+  k_degrees<<<grid_for(g.n * kWave), kThreads, 0, st>>>(g.indptr, g.wq, g.n, g.k);
+  k_sum_i64_max_deg<<<1, kThreads, 0, st>>>(g.k, g.indptr, g.n, c.red);
+  MSC_LAUNCH_CHECK();
+  int64_t hostv[2];
+  MSC_HIP(hipMemcpyAsync(hostv, c.red, 2 * sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  MSC_HIP(hipStreamSynchronize(st));
+  g.twom = hostv[0];
+  g.max_deg = hostv[1];
+  // heavy vertices
+  Heavy& h = c.hv;
+  h.nh = 0;
+  h.eh = 0;
+  if (g.max_deg > kLightDeg) {
+    k_heavy_flags<<<grid_for(g.n + 1), kThreads, 0, st>>>(g.indptr, g.n, c.hflag, c.hdeg);
+    MSC_LAUNCH_CHECK();
+    MSC_TRY(prims::exclusive_scan<int32_t>(c.hflag, c.hpos, g.n + 1, c.scan_ws, c.scan_ws_bytes, st));
+    MSC_TRY(prims::exclusive_scan<int64_t>(c.hdeg, c.hdegoff, g.n + 1, c.scan_ws, c.scan_ws_bytes, st));
+    int32_t nh32;
+    int64_t eh;
+    MSC_TRY(read_scalar(c.hpos + g.n, &nh32, st));
+    MSC_TRY(read_scalar(c.hdegoff + g.n, &eh, st));
+    h.nh = nh32;
+    h.eh = eh;
+    k_heavy_compact<<<grid_for(g.n), kThreads, 0, st>>>(c.hflag, c.hpos, g.n, h.verts);
+    k_heavy_edges<<<grid_for((h.nh + 1) * kWave), kThreads, 0, st>>>(g.indptr, h.verts, c.hdegoff, h.nh, h.eoff,
+                                                                     h.eh_h, h.eh_e, h.eh);
+    MSC_LAUNCH_CHECK();
+  }
+  return kOk;
+}
+
+static int sigma_tot(Ctx& c, const Graph& g, const int32_t* comm) {
+  MSC_TRY(group_sort(comm, g.n, g.n, c.grp, c.st));
+  MSC_TRY(group_sum(g.k, g.n, g.n, c.grp, c.sigtot, c.st));
+  return kOk;
+}
+
+static int heavy_moves(Ctx& c, const Graph& g, const int32_t* comm, const int32_t* part, const GainParams& gp) {
+  Heavy& h = c.hv;
+  if (h.nh == 0) return kOk;
+  hipStream_t st = c.st;
+  const int64_t eh = h.eh;
+  k_heavy_keys<<<grid_for(eh), kThreads, 0, st>>>(h.eh_h, h.eh_e, eh, h.verts, g.indices, comm, part, c.active,
+                                                  g.n, h.kc, h.kv);
+  MSC_LAUNCH_CHECK();
+  MSC_TRY(prims::radix_sort_pairs(h.kc, h.kv, h.skc, h.skv, eh, prims::bits_for(g.n), c.sort_ws, c.sort_ws_bytes, st));
+  k_heavy_hkeys<<<grid_for(eh), kThreads, 0, st>>>(h.skv, h.eh_h, eh, h.kh);
+  MSC_LAUNCH_CHECK();
+  MSC_TRY(prims::radix_sort_pairs(h.kh, h.skv, h.skh, h.skv2, eh, prims::bits_for(std::max<int64_t>(h.nh - 1, 1)),
+                                  c.sort_ws, c.sort_ws_bytes, st));
+  k_heavy_seg_flags<<<grid_for(eh + 1), kThreads, 0, st>>>(h.skv2, h.skh, eh, h.eh_e, g.indices, comm, h.kc, g.wq,
+                                                           g.n, h.flag, h.wsorted);
+  MSC_LAUNCH_CHECK();
+  MSC_TRY(prims::exclusive_scan<int32_t>(h.flag, h.segid, eh + 1, c.scan_ws, c.scan_ws_bytes, st));
+  MSC_TRY(prims::exclusive_scan<int64_t>(h.wsorted, h.prefix, eh + 1, c.scan_ws, c.scan_ws_bytes, st));
+  int32_t nseg;
+  MSC_TRY(read_scalar(h.segid + eh, &nseg, st));
+  k_heavy_seg_starts<<<grid_for(eh + 1), kThreads, 0, st>>>(h.flag, h.segid, eh, h.skh, h.segstart, h.seg_h);
+  MSC_LAUNCH_CHECK();
+  MSC_TRY(prims::segment_starts(h.seg_h, nseg, h.nh, h.hseg, st));
+  k_heavy_argmax<<<grid_for(h.nh * kWave), kThreads, 0, st>>>(h.verts, h.nh, h.hseg, h.segstart, h.prefix, h.skv2,
+                                                              h.kc, g.k, comm, c.sigtot, gp, c.target);
+  MSC_LAUNCH_CHECK();
+  return kOk;
+}
+
+// synchronous local moving (part == nullptr) or refinement; comm updated in place
+static int local_moving(Ctx& c, const Graph& g, int64_t seed, const int32_t* part, int* passes_out) {
+  hipStream_t st = c.st;
+  GainParams gp;
+  gp.inv_s = c.inv_s;
+  gp.inv = 1.0 / (static_cast<double>(g.twom) * c.inv_s);
+  gp.gamma = c.gamma;
+  const uint64_t key = stream_key(seed, part ? 1 : 0);
+  const bool prune_on = g.max_deg <= kDegreeCap;
+  k_fill_u8<<<grid_for(g.n), kThreads, 0, st>>>(c.active, g.n, 1);
+  k_fill_i32<<<grid_for(kMaxPasses), kThreads, 0, st>>>(c.wants_flags, kMaxPasses, 0);
+  MSC_LAUNCH_CHECK();
+  int p = 0;
+  for (; p < kMaxPasses; ++p) {
+    MSC_TRY(sigma_tot(c, g, c.comm));
+    k_move_light<<<grid_for(g.n * kWave), kThreads, 0, st>>>(g.indptr, g.indices, g.wq, g.k, g.n, c.comm, part,
+                                                            c.sigtot, c.active, gp, c.target);
+    MSC_LAUNCH_CHECK();
+    MSC_TRY(heavy_moves(c, g, c.comm, part, gp));
+    k_commit<<<grid_for(g.n), kThreads, 0, st>>>(c.comm, c.target, g.n, pass_key(key, p), c.thr, c.commit, c.held,
+                                                 c.wants_flags + p);
+    MSC_LAUNCH_CHECK();
+    if (prune_on) {
+      k_frontier<<<grid_for(g.n * kWave), kThreads, 0, st>>>(g.indptr, g.indices, g.n, c.commit, c.held, c.active);
+      MSC_LAUNCH_CHECK();
+    }
+    if ((p % kSyncEvery == kSyncEvery - 1) || p == kMaxPasses - 1) {
+      int32_t any;
+      MSC_TRY(read_scalar(c.wants_flags + p, &any, st));
+      if (!any) {
+        ++p;
+        break;
+      }
+    }
+  }
+  *passes_out = std::min(p, kMaxPasses);
+  return kOk;
+}
+
+// dense relabel of lab (n) into out (may alias); returns number of labels
+static int dense_relabel(Ctx& c, const int32_t* lab, int64_t n, int32_t* out, int64_t* C) {
+  hipStream_t st = c.st;
+  k_fill_i32<<<grid_for(n + 1), kThreads, 0, st>>>(c.used, n + 1, 0);
+  k_mark_used<<<grid_for(n), kThreads, 0, st>>>(lab, n, c.used);
+  MSC_LAUNCH_CHECK();
+  MSC_TRY(prims::exclusive_scan<int32_t>(c.used, c.map, n + 1, c.scan_ws, c.scan_ws_bytes, st));
+  k_apply_map<<<grid_for(n), kThreads, 0, st>>>(lab, c.map, n, out);
+  MSC_LAUNCH_CHECK();
+  int32_t nc;
+  MSC_TRY(read_scalar(c.map + n, &nc, st));
+  *C = nc;
+  return kOk;
+}
+
+static int modularity(Ctx& c, const Graph& g, const int32_t* lab, int64_t C, double* q) {
+  hipStream_t st = c.st;
+  k_internal<<<grid_for(g.n * kWave), kThreads, 0, st>>>(g.indptr, g.indices, g.wq, lab, g.n, c.iw);
+  MSC_LAUNCH_CHECK();
+  MSC_TRY(group_sort(lab, g.n, C, c.grp, st));
+  MSC_TRY(group_sum(g.k, g.n, C, c.grp, c.tot, st));
+  MSC_TRY(group_sum(c.iw, g.n, C, c.grp, c.inw, st));
+  const double twom_real = static_cast<double>(g.twom) * c.inv_s;
+  k_q_terms<<<grid_for(C), kThreads, 0, st>>>(c.tot, c.inw, C, c.inv_s, twom_real, c.gamma, c.terms);
+  MSC_LAUNCH_CHECK();
+  std::vector<double> terms(static_cast<size_t>(C));
+  MSC_HIP(hipMemcpyAsync(terms.data(), c.terms, C * sizeof(double), hipMemcpyDeviceToHost, st));
+  MSC_HIP(hipStreamSynchronize(st));
+  double s = 0.0;
+  for (double t : terms) s += t;  // sequential, label order
+  *q = s;
+  return kOk;
+}
+
+// contract g on dense labels R (nR) into slot `dst`
+static int contract(Ctx& c, const Graph& g, const int32_t* R, int64_t nR, int dst, Graph* out) {
+  hipStream_t st = c.st;
+  const int64_t E = g.E;
+  k_edge_src_dst_keys<<<grid_for(g.n * kWave), kThreads, 0, st>>>(g.indptr, g.indices, R, g.n, c.ckd, c.ceid);
+  k_edge_src_of<<<grid_for(g.n * kWave), kThreads, 0, st>>>(g.indptr, R, g.n, c.cesrc);
+  MSC_LAUNCH_CHECK();
+  const int bits = prims::bits_for(std::max<int64_t>(nR - 1, 1));
+  MSC_TRY(prims::radix_sort_pairs(c.ckd, c.ceid, c.cskd, c.cseid, E, bits, c.sort_ws, c.sort_ws_bytes, st));
+  k_gather_src_keys<<<grid_for(E), kThreads, 0, st>>>(c.cseid, c.cesrc, E, c.cks);
+  MSC_LAUNCH_CHECK();
+  MSC_TRY(prims::radix_sort_pairs(c.cks, c.cseid, c.csks, c.cseid2, E, bits, c.sort_ws, c.sort_ws_bytes, st));
+  k_contract_flags<<<grid_for(E + 1), kThreads, 0, st>>>(c.cseid2, c.csks, c.ckd, g.wq, E, c.cflag, c.cw);
+  MSC_LAUNCH_CHECK();
+  MSC_TRY(prims::exclusive_scan<int32_t>(c.cflag, c.cpos, E + 1, c.scan_ws, c.scan_ws_bytes, st));
+  MSC_TRY(prims::exclusive_scan<int64_t>(c.cw, c.cprefix, E + 1, c.scan_ws, c.scan_ws_bytes, st));
+  int32_t m;
+  MSC_TRY(read_scalar(c.cpos + E, &m, st));
+  k_contract_emit<<<grid_for(E + 1), kThreads, 0, st>>>(c.cflag, c.cpos, E, c.cseid2, c.csks, c.ckd, c.cprefix,
+                                                        c.gindices[dst], c.csrc_out, c.cseg);
+  MSC_LAUNCH_CHECK();
+  k_contract_weights<<<grid_for(m), kThreads, 0, st>>>(c.cseg, c.cprefix, m, c.gwq[dst]);
+  MSC_LAUNCH_CHECK();
+  MSC_TRY(prims::segment_starts(c.csrc_out, m, nR, c.gindptr[dst], st));
+  out->n = nR;
+  out->E = m;
+  out->indptr = c.gindptr[dst];
+  out->indices = c.gindices[dst];
+  out->wq = c.gwq[dst];
+  out->k = c.kbuf[dst];
+  return graph_setup(c, *out);
+}
+
+// one multilevel pass (_leiden_pass); init (n0) may be null; result dense labels in c.final_lab
+static int leiden_pass(Ctx& c, Graph& g0, int64_t seed, const int32_t* init) {
+  hipStream_t st = c.st;
+  const int64_t n0 = g0.n;
+  Graph g = g0;
+  int slot = 0;
+  k_iota_i32<<<grid_for(n0), kThreads, 0, st>>>(c.orig2node, n0);
+  if (init) k_copy_i32<<<grid_for(n0), kThreads, 0, st>>>(init, c.P, n0);
+  else k_iota_i32<<<grid_for(n0), kThreads, 0, st>>>(c.P, n0);
+  k_copy_i32<<<grid_for(n0), kThreads, 0, st>>>(c.P, c.final_lab, n0);  // P[orig2node] with identity map
+  MSC_LAUNCH_CHECK();
+  double q_prev = -1.0;
+  for (int level = 0; level < kMaxLevels; ++level) {
+    k_gather_i32<<<grid_for(n0), kThreads, 0, st>>>(c.P, c.orig2node, c.prev_lab, n0);
+    k_copy_i32<<<grid_for(g.n), kThreads, 0, st>>>(c.P, c.comm, g.n);
+    MSC_LAUNCH_CHECK();
+    int mp = 0, rp = 0;
+    MSC_TRY(local_moving(c, g, seed + level, nullptr, &mp));
+    int64_t ncomm;
+    MSC_TRY(dense_relabel(c, c.comm, g.n, c.P, &ncomm));
+    double q;
+    MSC_TRY(modularity(c, g, c.P, ncomm, &q));
+    msc_leiden_level* rec = nullptr;
+    if (c.info && c.info->n_levels < MSC_LEIDEN_MAX_RECORDS) {
+      rec = &c.info->levels[c.info->n_levels++];
+      rec->n = g.n; rec->n_edges = g.E; rec->move_passes = mp; rec->refine_passes = 0;
+      rec->n_comms = ncomm; rec->n_refined = 0; rec->modularity = q;
+    }
+    if (q <= q_prev + kTol) {
+      k_copy_i32<<<grid_for(n0), kThreads, 0, st>>>(c.prev_lab, c.final_lab, n0);
+      MSC_LAUNCH_CHECK();
+      break;
+    }
+    q_prev = q;
+    k_gather_i32<<<grid_for(n0), kThreads, 0, st>>>(c.P, c.orig2node, c.final_lab, n0);
+    MSC_LAUNCH_CHECK();
+    if (ncomm == g.n) break;
+    k_iota_i32<<<grid_for(g.n), kThreads, 0, st>>>(c.comm, g.n);
+    MSC_LAUNCH_CHECK();
+    MSC_TRY(local_moving(c, g, seed + level, c.P, &rp));
+    int64_t nR;
+    MSC_TRY(dense_relabel(c, c.comm, g.n, c.R, &nR));
+    if (rec) { rec->refine_passes = rp; rec->n_refined = nR; }
+    if (nR == g.n) break;
+    k_scatter_pagg<<<grid_for(g.n), kThreads, 0, st>>>(c.R, c.P, g.n, c.Pagg);
+    k_gather_i32<<<grid_for(n0), kThreads, 0, st>>>(c.R, c.orig2node, c.tmp_lab, n0);
+    k_copy_i32<<<grid_for(n0), kThreads, 0, st>>>(c.tmp_lab, c.orig2node, n0);
+    MSC_LAUNCH_CHECK();
+    Graph ng;
+    MSC_TRY(contract(c, g, c.R, nR, slot, &ng));
+    slot ^= 1;
+    g = ng;
+    k_copy_i32<<<grid_for(nR), kThreads, 0, st>>>(c.Pagg, c.P, nR);
+    MSC_LAUNCH_CHECK();
+  }
+  int64_t nf;
+  MSC_TRY(dense_relabel(c, c.final_lab, n0, c.final_lab, &nf));
+  return kOk;
+}
+
+static int validate(int64_t n, int64_t nnz) {
+  MSC_REQUIRE(n >= 1 && n < (int64_t(1) << 31) - 1, "leiden: n out of range");
+  MSC_REQUIRE(nnz >= 0 && nnz < (int64_t(1) << 31) - 1, "leiden: nnz out of range (< 2^31)");
+  return kOk;
+}
+
+static int ceil_log2(double x) {
+  int e;
+  const double m = std::frexp(x, &e);
+  return m == 0.5 ? e - 1 : e;
+}
+
+}  // namespace leiden
+}  // namespace msc
+
+using namespace msc;
+using namespace msc::leiden;
+
+extern "C" int msc_leiden_workspace_size(int64_t n, int64_t nnz, size_t* bytes) {
+  MSC_TRY(validate(n, nnz));
+  MSC_REQUIRE(bytes, "leiden: bytes is null");
+  Carve cv(nullptr, 0);
+  Ctx c{};
+  carve_ctx(cv, c, n, nnz);
+  cv.take<int64_t>(nnz + 1);  // level-0 quantised weights
+  cv.take<int64_t>(n + 1);    // level-0 degrees
+  *bytes = cv.used + 256;
+  return kOk;
+}
+
+extern "C" int msc_leiden(const int64_t* indptr, const int32_t* indices, const float* weights, int64_t n,
+                          int64_t nnz, double resolution, int64_t seed, int n_iterations, double commit_prob,
+                          const int32_t* init_labels, int32_t* out_labels, double* out_modularity,
+                          msc_leiden_info* info, void* workspace, size_t workspace_bytes, void* stream) {
+  MSC_TRY(validate(n, nnz));
+  MSC_REQUIRE(indptr && out_labels && workspace && (nnz == 0 || (indices && weights)),
+              "leiden: null pointer argument");
+  MSC_REQUIRE(commit_prob > 0.0 && commit_prob <= 1.0, "leiden: commit_prob must be in (0, 1]");
+  Ctx c{};
+  c.st = static_cast<hipStream_t>(stream);
+  c.n0 = n;
+  c.E0 = nnz;
+  c.gamma = resolution;
+  c.thr = static_cast<uint64_t>(std::min(std::floor(commit_prob * 4294967296.0), 4294967296.0));
+  c.info = info;
+  if (info) info->n_levels = 0;
+  Carve cv(workspace, workspace_bytes);
+  carve_ctx(cv, c, n, nnz);
+  int64_t* wq0 = cv.take<int64_t>(nnz + 1);
+  int64_t* k0 = cv.take<int64_t>(n + 1);
+  if (!cv.ok()) {
+    set_error("leiden: workspace too small (%zu < %zu)", workspace_bytes, cv.used);
+    return kWorkspaceTooSmall;
+  }
+  hipStream_t st = c.st;
+  // fixed-point exponent from max weight and edge count (both exact)
+  float wmax = 0.0f;
+  if (nnz > 0) {
+    k_max_weight<<<1, kThreads, 0, st>>>(weights, nnz, c.fred);
+    MSC_LAUNCH_CHECK();
+    MSC_TRY(read_scalar(c.fred, &wmax, st));
+  }
+  MSC_REQUIRE(std::isfinite(wmax), "leiden: weights must be finite");
+  c.s = (nnz > 0 && wmax > 0.0f) ? 52 - ceil_log2(wmax) - ceil_log2(static_cast<double>(nnz)) : 0;
+  c.inv_s = std::ldexp(1.0, -c.s);
+  if (nnz > 0) {
+    k_quantize<<<grid_for(nnz), kThreads, 0, st>>>(weights, nnz, c.s, wq0);
+    MSC_LAUNCH_CHECK();
+  }
+  Graph g0;
+  g0.n = n;
+  g0.E = nnz;
+  g0.indptr = const_cast<int64_t*>(indptr);
+  g0.indices = const_cast<int32_t*>(indices);
+  g0.wq = wq0;
+  g0.k = k0;
+  MSC_TRY(graph_setup(c, g0));
+  MSC_REQUIRE(g0.twom > 0, "leiden: graph has no positive edge weight");
+  const int iters = std::max(1, n_iterations);
+  for (int it = 0; it < iters; ++it) {
+    const int32_t* init = nullptr;
+    if (it > 0 || init_labels) {
+      if (it == 0) {
+        k_copy_i32<<<grid_for(n), kThreads, 0, st>>>(init_labels, out_labels, n);
+        MSC_LAUNCH_CHECK();
+      }
+      init = out_labels;
+    }
+    // leiden_pass reads init before overwriting anything it aliases (init copied into P first)
+    MSC_TRY(graph_setup(c, g0));
+    MSC_TRY(leiden_pass(c, g0, seed + 17 * it, init));
+    k_copy_i32<<<grid_for(n), kThreads, 0, st>>>(c.final_lab, out_labels, n);
+    MSC_LAUNCH_CHECK();
+  }
+  if (out_modularity) {
+    int64_t C;
+    MSC_TRY(dense_relabel(c, out_labels, n, c.tmp_lab, &C));
+    MSC_TRY(modularity(c, g0, c.tmp_lab, C, out_modularity));
+  }
+  if (info) {
+    info->scale_exponent = c.s;
+    info->twom_fixed = g0.twom;
+  }
+  MSC_HIP(hipStreamSynchronize(st));
+  return kOk;
+}

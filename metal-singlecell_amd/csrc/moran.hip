@@ -1,0 +1,308 @@
+// Moran's I with a permutation null on gfx950, float64 reductions.
+//
+// Replaces spatial.spatial_autocorr's Moran branch (src/metalsinglecell/spatial.py:357-406):
+// I_g = (n/S0) sum_i Xc_ig (W Xc)_ig / sum_i Xc_ig^2, and for each permutation pi the
+// reference's permuted-source scatter (:401-405), which equals
+// sim_pg = (n/S0) sum_r Xc[pi(r), g] Y[r, g] / denom_g with Y = W Xc.
+//
+// Layout: X dense cells x genes float32 row-major (as the reference densifies, gr.py:452);
+// W CSR with float64 values; Y stored float32 (one read per permutation batch).  Column
+// means, denominators, numerators and the permuted sums accumulate in float64 in a fixed
+// order (block partials over row chunks, then a sequential sum over chunks), so results
+// are deterministic.
+//
+// Permutations: either caller-supplied (P x n int32, e.g. numpy default_rng(seed)
+// .permutation(n) in sequence, the reference stream) or generated on the device by a
+// keyed 4-round Feistel bijection with cycle walking (no storage, no host transfer).
+#include <cmath>
+
+#include "common.h"
+#include "msc_hip.h"
+
+namespace msc {
+namespace moran {
+
+constexpr int kThreads = 256;
+constexpr int kPermBatch = 8;
+
+// ---- column statistics --------------------------------------------------------------------
+// partial[b][g] over rows [b*per, (b+1)*per); thread per gene (strided), rows sequential
+__global__ void k_col_partial(const float* __restrict__ X, int64_t n, int64_t G, int64_t per,
+                              const double* __restrict__ mean, double* __restrict__ partial) {
+  const int64_t r0 = blockIdx.x * per, r1 = min(n, r0 + per);
+  for (int64_t g = threadIdx.x + static_cast<int64_t>(blockIdx.y) * blockDim.x; g < G;
+       g += static_cast<int64_t>(gridDim.y) * blockDim.x) {
+    double s = 0.0;
+    if (mean) {
+      const double m = mean[g];
+      for (int64_t r = r0; r < r1; ++r) {
+        const double d = static_cast<double>(X[r * G + g]) - m;
+        s += d * d;
+      }
+    } else {
+      for (int64_t r = r0; r < r1; ++r) s += static_cast<double>(X[r * G + g]);
+    }
+    partial[blockIdx.x * G + g] = s;
+  }
+}
+__global__ void k_col_final(const double* __restrict__ partial, int nb, int64_t G, double scale,
+                            double* __restrict__ out) {
+  const int64_t g = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (g >= G) return;
+  double s = 0.0;
+  for (int b = 0; b < nb; ++b) s += partial[static_cast<int64_t>(b) * G + g];
+  out[g] = s * scale;
+}
+
+// ---- weights --------------------------------------------------------------------------------
+// row-normalise (spatial.py:268-279): w / sum|row|, zero rows stay zero; also row sums
+__global__ void k_row_normalize(const int64_t* __restrict__ indptr, const double* __restrict__ w_in, int64_t n,
+                                int transform, double* __restrict__ w_out) {
+  const int64_t r = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (r >= n) return;
+  double s = 0.0;
+  for (int64_t e = indptr[r]; e < indptr[r + 1]; ++e) s += fabs(w_in[e]);
+  const double inv = (transform && s != 0.0) ? 1.0 / s : 1.0;
+  for (int64_t e = indptr[r]; e < indptr[r + 1]; ++e) w_out[e] = (transform ? (s != 0.0 ? w_in[e] * inv : 0.0) : w_in[e]);
+}
+__global__ void k_sum_f64(const double* __restrict__ v, int64_t m, double* __restrict__ out) {
+  // single block, fixed order: per-thread strided sums, then lanes/waves in order
+  double s = 0.0;
+  for (int64_t i = threadIdx.x; i < m; i += blockDim.x) s += v[i];
+  __shared__ double sh[kThreads];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int i = 0; i < kThreads; ++i) t += sh[i];
+    *out = t;
+  }
+}
+
+// Y[r, :] = sum_e w_e (X[j_e, :] - mean), float64 accumulate, stored float32; block per row
+__global__ __launch_bounds__(kThreads) void k_spmm(const float* __restrict__ X, int64_t n, int64_t G,
+                                                   const int64_t* __restrict__ indptr, const int32_t* __restrict__ indices,
+                                                   const double* __restrict__ w, const double* __restrict__ mean,
+                                                   float* __restrict__ Y) {
+  const int64_t r = blockIdx.x;
+  const int64_t e0 = indptr[r], e1 = indptr[r + 1];
+  for (int64_t g = threadIdx.x; g < G; g += kThreads) {
+    const double m = mean[g];
+    double s = 0.0;
+    for (int64_t e = e0; e < e1; ++e) s += w[e] * (static_cast<double>(X[static_cast<int64_t>(indices[e]) * G + g]) - m);
+    Y[r * G + g] = static_cast<float>(s);
+  }
+}
+
+// ---- permutations -----------------------------------------------------------------------------
+struct Feistel {
+  uint64_t key;
+  int half;       // bits per half; domain 2^(2*half) >= n
+  uint64_t mask;
+};
+
+__device__ __forceinline__ uint64_t feistel_once(uint64_t x, uint64_t key, int half, uint64_t mask) {
+  uint64_t L = x >> half, R = x & mask;
+#pragma unroll
+  for (int round = 0; round < 4; ++round) {
+    const uint64_t F = mix64(R ^ (key + static_cast<uint64_t>(round) * 0x9E3779B97F4A7C15ull)) & mask;
+    const uint64_t nL = R;
+    R = L ^ F;
+    L = nL;
+  }
+  return (L << half) | R;
+}
+__device__ __forceinline__ int64_t feistel_perm(int64_t r, int64_t n, const Feistel& f, uint64_t pkey) {
+  uint64_t x = static_cast<uint64_t>(r);
+  do {
+    x = feistel_once(x, pkey, f.half, f.mask);
+  } while (x >= static_cast<uint64_t>(n));
+  return static_cast<int64_t>(x);
+}
+
+// ---- observed numerator and permuted numerators -----------------------------------------------
+// grid: (row chunks, perm batches, gene tiles); partial[chunk][p][g]
+__global__ __launch_bounds__(kThreads) void k_perm_partial(
+    const float* __restrict__ X, const float* __restrict__ Y, int64_t n, int64_t G, const double* __restrict__ mean,
+    int64_t per, int P, const int32_t* __restrict__ perms, Feistel fst, int observed, double* __restrict__ partial) {
+  const int64_t r0 = blockIdx.x * per, r1 = min(n, r0 + per);
+  const int p0 = blockIdx.y * kPermBatch;
+  const int64_t g = static_cast<int64_t>(blockIdx.z) * kThreads + threadIdx.x;
+  const bool gok = g < G;
+  const double m = gok ? mean[g] : 0.0;
+  double acc[kPermBatch];
+  uint64_t pk[kPermBatch];
+#pragma unroll
+  for (int b = 0; b < kPermBatch; ++b) {
+    acc[b] = 0.0;
+    pk[b] = mix64(fst.key + static_cast<uint64_t>(p0 + b + 1) * 0xD1B54A32D192ED03ull);
+  }
+  for (int64_t r = r0; r < r1; ++r) {
+    const double y = gok ? static_cast<double>(Y[r * G + g]) : 0.0;
+    if (observed) {
+      const double x = gok ? static_cast<double>(X[r * G + g]) - m : 0.0;
+      acc[0] += x * y;
+      continue;
+    }
+#pragma unroll
+    for (int b = 0; b < kPermBatch; ++b) {
+      const int p = p0 + b;
+      if (p < P) {
+        const int64_t j = perms ? static_cast<int64_t>(perms[static_cast<int64_t>(p) * n + r])
+                                : feistel_perm(r, n, fst, pk[b]);
+        const double x = gok ? static_cast<double>(X[j * G + g]) - m : 0.0;
+        acc[b] += x * y;
+      }
+    }
+  }
+  if (!gok) return;
+  if (observed) {
+    partial[blockIdx.x * G + g] = acc[0];
+    return;
+  }
+#pragma unroll
+  for (int b = 0; b < kPermBatch; ++b) {
+    const int p = p0 + b;
+    if (p < P) partial[(static_cast<int64_t>(blockIdx.x) * P + p) * G + g] = acc[b];
+  }
+}
+
+// out[p][g] = (n/S0) * sum_chunks partial / denom
+__global__ void k_finish(const double* __restrict__ partial, int nchunks, int P, int64_t G, double n,
+                         const double* __restrict__ S0, const double* __restrict__ denom, double* __restrict__ out) {
+  const int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (t >= static_cast<int64_t>(P) * G) return;
+  const int64_t p = t / G, g = t % G;
+  double s = 0.0;
+  for (int c = 0; c < nchunks; ++c) s += partial[(static_cast<int64_t>(c) * P + p) * G + g];
+  out[t] = (n / *S0) * s / denom[g];
+}
+
+struct Plan {
+  int64_t n, G, E;
+  int P;
+  int nchunks;
+  int64_t per;
+  int gtiles;
+};
+
+static Plan make_plan(int64_t n, int64_t G, int64_t E, int P) {
+  Plan p{};
+  p.n = n;
+  p.G = G;
+  p.E = E;
+  p.P = P;
+  p.gtiles = static_cast<int>(ceil_div(G, kThreads));
+  const int pb = std::max(1, static_cast<int>(ceil_div(std::max(P, 1), kPermBatch)));
+  // about 2048 blocks for the permutation kernel, chunks of >= 256 rows
+  int64_t chunks = ceil_div(2048, static_cast<int64_t>(pb) * p.gtiles);
+  chunks = std::max<int64_t>(1, std::min<int64_t>(chunks, ceil_div(n, 256)));
+  chunks = std::min<int64_t>(chunks, 1024);
+  p.per = ceil_div(n, chunks);
+  p.nchunks = static_cast<int>(ceil_div(n, p.per));
+  return p;
+}
+
+struct Bufs {
+  double *mean, *denom, *w, *S0, *partial, *cpartial;
+  float* Y;
+};
+
+constexpr int kColBlocks = 128;
+
+static Bufs carve(Carve& cv, const Plan& p) {
+  Bufs b{};
+  b.mean = cv.take<double>(p.G);
+  b.denom = cv.take<double>(p.G);
+  b.w = cv.take<double>(std::max<int64_t>(p.E, 1));
+  b.S0 = cv.take<double>(1);
+  b.cpartial = cv.take<double>(static_cast<size_t>(kColBlocks) * p.G);
+  b.partial = cv.take<double>(static_cast<size_t>(p.nchunks) * std::max(p.P, 1) * p.G);
+  b.Y = cv.take<float>(static_cast<size_t>(p.n) * p.G);
+  return b;
+}
+
+static int validate(int64_t n, int64_t G, int64_t E, int P) {
+  MSC_REQUIRE(n >= 2 && G >= 1 && E >= 0 && P >= 0, "moran: need n >= 2, G >= 1, nnz >= 0, n_perms >= 0");
+  MSC_REQUIRE(n < (int64_t(1) << 31), "moran: n too large");
+  return kOk;
+}
+
+}  // namespace moran
+}  // namespace msc
+
+using namespace msc;
+using namespace msc::moran;
+
+extern "C" int msc_moran_workspace_size(int64_t n, int64_t n_genes, int64_t nnz, int n_perms, size_t* bytes) {
+  MSC_TRY(validate(n, n_genes, nnz, n_perms));
+  MSC_REQUIRE(bytes, "moran: bytes is null");
+  Plan p = make_plan(n, n_genes, nnz, n_perms);
+  Carve cv(nullptr, 0);
+  carve(cv, p);
+  *bytes = cv.used + 256;
+  return kOk;
+}
+
+extern "C" int msc_moran(const float* X, int64_t n, int64_t n_genes, const int64_t* w_indptr,
+                         const int32_t* w_indices, const double* w_data, int64_t nnz, int transformation,
+                         int n_perms, const int32_t* perms, int64_t seed, double* out_I, double* out_sims,
+                         double* out_S0, void* workspace, size_t workspace_bytes, void* stream) {
+  MSC_TRY(validate(n, n_genes, nnz, n_perms));
+  MSC_REQUIRE(X && w_indptr && out_I && workspace && (nnz == 0 || (w_indices && w_data)),
+              "moran: null pointer argument");
+  MSC_REQUIRE(n_perms == 0 || out_sims, "moran: out_sims required when n_perms > 0");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const int64_t G = n_genes;
+  Plan p = make_plan(n, G, nnz, n_perms);
+  Carve cv(workspace, workspace_bytes);
+  Bufs b = carve(cv, p);
+  if (!cv.ok()) {
+    set_error("moran: workspace too small (%zu < %zu)", workspace_bytes, cv.used);
+    return kWorkspaceTooSmall;
+  }
+  const int64_t per = ceil_div(n, kColBlocks);
+  const dim3 cg(kColBlocks, static_cast<unsigned>(std::min<int64_t>(ceil_div(G, kThreads), 64)));
+  k_col_partial<<<cg, kThreads, 0, st>>>(X, n, G, per, nullptr, b.cpartial);
+  k_col_final<<<static_cast<unsigned>(ceil_div(G, kThreads)), kThreads, 0, st>>>(b.cpartial, kColBlocks, G,
+                                                                                   1.0 / static_cast<double>(n), b.mean);
+  k_col_partial<<<cg, kThreads, 0, st>>>(X, n, G, per, b.mean, b.cpartial);
+  k_col_final<<<static_cast<unsigned>(ceil_div(G, kThreads)), kThreads, 0, st>>>(b.cpartial, kColBlocks, G, 1.0,
+                                                                                   b.denom);
+  MSC_LAUNCH_CHECK();
+  k_row_normalize<<<static_cast<unsigned>(ceil_div(n, kThreads)), kThreads, 0, st>>>(w_indptr, w_data, n,
+                                                                                       transformation, b.w);
+  k_sum_f64<<<1, kThreads, 0, st>>>(b.w, nnz, b.S0);
+  MSC_LAUNCH_CHECK();
+  k_spmm<<<static_cast<unsigned>(n), kThreads, 0, st>>>(X, n, G, w_indptr, w_indices, b.w, b.mean, b.Y);
+  MSC_LAUNCH_CHECK();
+  Feistel fst{};
+  {
+    int bits = 2;
+    while ((int64_t(1) << bits) < n) bits += 2;
+    fst.half = bits / 2;
+    fst.mask = (uint64_t(1) << fst.half) - 1;
+    fst.key = stream_key(seed, 7);
+  }
+  // observed statistic
+  {
+    const dim3 grid(static_cast<unsigned>(p.nchunks), 1, static_cast<unsigned>(p.gtiles));
+    k_perm_partial<<<grid, kThreads, 0, st>>>(X, b.Y, n, G, b.mean, p.per, 1, nullptr, fst, 1, b.partial);
+    k_finish<<<static_cast<unsigned>(ceil_div(G, kThreads)), kThreads, 0, st>>>(
+        b.partial, p.nchunks, 1, G, static_cast<double>(n), b.S0, b.denom, out_I);
+    MSC_LAUNCH_CHECK();
+  }
+  if (n_perms > 0) {
+    const dim3 grid(static_cast<unsigned>(p.nchunks), static_cast<unsigned>(ceil_div(n_perms, kPermBatch)),
+                    static_cast<unsigned>(p.gtiles));
+    k_perm_partial<<<grid, kThreads, 0, st>>>(X, b.Y, n, G, b.mean, p.per, n_perms, perms, fst, 0, b.partial);
+    k_finish<<<static_cast<unsigned>(ceil_div(static_cast<int64_t>(n_perms) * G, kThreads)), kThreads, 0, st>>>(
+        b.partial, p.nchunks, n_perms, G, static_cast<double>(n), b.S0, b.denom, out_sims);
+    MSC_LAUNCH_CHECK();
+  }
+  if (out_S0) {
+    MSC_HIP(hipMemcpyAsync(out_S0, b.S0, sizeof(double), hipMemcpyDeviceToHost, st));
+    MSC_HIP(hipStreamSynchronize(st));
+  }
+  return kOk;
+}

@@ -1,0 +1,282 @@
+"""ctypes binding of libmsc_hip (the gfx950 neighbor-graph kernels).
+
+PyTorch supplies device memory and the current stream; every computation runs in the HIP
+library.  There is no CPU fallback: without a visible ROCm device or without the built
+library, every entry point raises.
+
+``torch`` is imported before the library is loaded so that the library binds to the same
+HIP runtime instance as torch (both resolve ``libamdhip64.so.7``); device pointers and
+streams are then shared.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+
+import numpy as np
+
+_LIB = None
+_LOCK = threading.Lock()
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_lib", "libmsc_hip.so")
+
+PRECISION = {"fp32": 0, "bf16": 1, "bf16_rerank": 1}
+
+c_i64, c_int, c_vp, c_sz, c_dbl = ctypes.c_int64, ctypes.c_int, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_double
+
+
+class LeidenLevel(ctypes.Structure):
+    _fields_ = [("n", c_i64), ("n_edges", c_i64), ("move_passes", ctypes.c_int32),
+                ("refine_passes", ctypes.c_int32), ("n_comms", c_i64), ("n_refined", c_i64),
+                ("modularity", c_dbl)]
+
+
+class LeidenInfo(ctypes.Structure):
+    _fields_ = [("n_levels", c_int), ("scale_exponent", c_int), ("twom_fixed", c_i64),
+                ("levels", LeidenLevel * 64)]
+
+
+class SpatialInfo(ctypes.Structure):
+    _fields_ = [("max_shell", ctypes.c_int32), ("grid_x", ctypes.c_int32), ("grid_y", ctypes.c_int32),
+                ("grid_z", ctypes.c_int32), ("cell", ctypes.c_float)]
+
+
+class KnnInfo(ctypes.Structure):
+    _fields_ = [("timing", c_int), ("n_splits", c_int), ("list_len", c_int),
+                ("n_fallback", c_i64), ("prep_ms", c_dbl), ("select_ms", c_dbl),
+                ("finalize_ms", c_dbl), ("fallback_ms", c_dbl)]
+
+
+# name -> (restype, argtypes); the declarations of include/msc_hip.h
+SIGNATURES = {
+    "msc_version": (c_int, []),
+    "msc_last_error": (ctypes.c_char_p, []),
+    "msc_knn_workspace_size": (c_int, [c_i64, c_int, c_int, c_int, ctypes.POINTER(c_sz)]),
+    "msc_knn_exact": (c_int, [c_vp, c_i64, c_int, c_int, c_int, c_vp, c_vp, c_vp, c_sz, c_vp,
+                              ctypes.POINTER(KnnInfo)]),
+    "msc_knn_distances_csr": (c_int, [c_vp, c_vp, c_i64, c_int, c_vp, c_vp, c_vp, c_vp]),
+    "msc_fuzzy_workspace_size": (c_int, [c_i64, c_int, ctypes.POINTER(c_sz)]),
+    "msc_fuzzy_connectivities": (c_int, [c_vp, c_vp, c_i64, c_int, c_vp, c_vp, c_vp, c_i64, c_vp,
+                                         c_vp, ctypes.POINTER(c_i64), c_vp, c_sz, c_vp]),
+    "msc_spatial_knn_workspace_size": (c_int, [c_i64, c_int, c_int, ctypes.POINTER(c_sz)]),
+    "msc_spatial_knn": (c_int, [c_vp, c_i64, c_int, c_int, c_vp, c_vp, c_vp, c_vp,
+                                ctypes.POINTER(SpatialInfo), c_vp, c_sz, c_vp]),
+    "msc_moran_workspace_size": (c_int, [c_i64, c_i64, c_i64, c_int, ctypes.POINTER(c_sz)]),
+    "msc_moran": (c_int, [c_vp, c_i64, c_i64, c_vp, c_vp, c_vp, c_i64, c_int, c_int, c_vp, c_i64,
+                          c_vp, c_vp, ctypes.POINTER(c_dbl), c_vp, c_sz, c_vp]),
+    "msc_leiden_workspace_size": (c_int, [c_i64, c_i64, ctypes.POINTER(c_sz)]),
+    "msc_leiden": (c_int, [c_vp, c_vp, c_vp, c_i64, c_i64, c_dbl, c_i64, c_int, c_dbl, c_vp, c_vp,
+                           ctypes.POINTER(c_dbl), ctypes.POINTER(LeidenInfo), c_vp, c_sz, c_vp]),
+}
+
+
+def load_library(path=LIB_PATH):
+    """Load and type the library (no device needed; used by the symbol-export tests)."""
+    global _LIB
+    with _LOCK:
+        if _LIB is not None:
+            return _LIB
+        if not os.path.exists(path):
+            raise ImportError(f"libmsc_hip.so not found at {path}: build it with "
+                              "`python metal-singlecell_amd/build_lib.py`")
+        import torch  # noqa: F401  (bind to torch's HIP runtime)
+        lib = ctypes.CDLL(path)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = lib
+        return lib
+
+
+def check(rc):
+    if rc != 0:
+        msg = load_library().msc_last_error()
+        raise RuntimeError(f"libmsc_hip error {rc}: {msg.decode() if msg else ''}")
+
+
+def device():
+    """The ROCm device every call runs on; raises when none is visible."""
+    import torch
+    if not torch.cuda.is_available():
+        raise RuntimeError("metalsinglecell's neighbor-graph path needs a ROCm GPU "
+                           "(torch.cuda.is_available() is False); there is no CPU fallback")
+    load_library()
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def stream_ptr():
+    import torch
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def ptr(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def to_device(a, dtype):
+    """numpy / torch -> contiguous device tensor of ``dtype``."""
+    import torch
+    dev = device()
+    if isinstance(a, torch.Tensor):
+        return a.to(device=dev, dtype=dtype).contiguous()
+    return torch.from_numpy(np.ascontiguousarray(a)).to(device=dev, dtype=dtype).contiguous()
+
+
+def workspace(nbytes):
+    import torch
+    return torch.empty(max(int(nbytes), 256), dtype=torch.uint8, device=device())
+
+
+# ----------------------------------------------------------------------------------------
+# (a) exact kNN
+# ----------------------------------------------------------------------------------------
+def knn_exact(X, k, precision="fp32", timing=False, return_info=False):
+    """Exact self-inclusive kNN on the device: ``(idx int32 (n,k), dist float32 (n,k))``."""
+    import torch
+    lib = load_library()
+    Xd = to_device(X, torch.float32)
+    if Xd.dim() != 2:
+        raise ValueError("X must be 2-D (cells x dims)")
+    n, d = Xd.shape
+    p = PRECISION[precision]
+    nbytes = c_sz(0)
+    check(lib.msc_knn_workspace_size(n, d, int(k), p, ctypes.byref(nbytes)))
+    ws = workspace(nbytes.value)
+    idx = torch.empty((n, k), dtype=torch.int32, device=Xd.device)
+    dist = torch.empty((n, k), dtype=torch.float32, device=Xd.device)
+    info = KnnInfo()
+    info.timing = int(bool(timing))
+    check(lib.msc_knn_exact(ptr(Xd), n, d, int(k), p, ptr(idx), ptr(dist), ptr(ws),
+                            ws.numel(), stream_ptr(), ctypes.byref(info) if (timing or return_info) else None))
+    if return_info or timing:
+        return idx, dist, {f: getattr(info, f) for f, _ in KnnInfo._fields_}
+    return idx, dist
+
+
+def knn_distances_csr(idx, dist):
+    """Device CSR arrays ``(indptr int64, indices int32, data float32)`` of obsp['distances']."""
+    import torch
+    lib = load_library()
+    n, k = idx.shape
+    indptr = torch.empty(n + 1, dtype=torch.int64, device=idx.device)
+    m = n * (k - 1)
+    indices = torch.empty(max(m, 1), dtype=torch.int32, device=idx.device)
+    data = torch.empty(max(m, 1), dtype=torch.float32, device=idx.device)
+    check(lib.msc_knn_distances_csr(ptr(idx), ptr(dist), n, k, ptr(indptr), ptr(indices),
+                                    ptr(data), stream_ptr()))
+    return indptr, indices[:m], data[:m]
+
+
+# ----------------------------------------------------------------------------------------
+# (c) fuzzy simplicial set connectivities
+# ----------------------------------------------------------------------------------------
+def fuzzy_connectivities(idx, dist):
+    """Device CSR ``(indptr int64, indices int32, data float32)`` plus ``(sigma, rho)``."""
+    import torch
+    lib = load_library()
+    n, k = idx.shape
+    nbytes = c_sz(0)
+    check(lib.msc_fuzzy_workspace_size(n, k, ctypes.byref(nbytes)))
+    ws = workspace(nbytes.value)
+    cap = max(2 * n * (k - 1), 1)
+    dev = idx.device
+    indptr = torch.empty(n + 1, dtype=torch.int64, device=dev)
+    indices = torch.empty(cap, dtype=torch.int32, device=dev)
+    data = torch.empty(cap, dtype=torch.float32, device=dev)
+    sigma = torch.empty(n, dtype=torch.float32, device=dev)
+    rho = torch.empty(n, dtype=torch.float32, device=dev)
+    nnz = c_i64(0)
+    check(lib.msc_fuzzy_connectivities(ptr(idx), ptr(dist), n, k, ptr(indptr), ptr(indices),
+                                       ptr(data), cap, ptr(sigma), ptr(rho), ctypes.byref(nnz),
+                                       ptr(ws), ws.numel(), stream_ptr()))
+    m = nnz.value
+    return (indptr, indices[:m], data[:m]), (sigma, rho)
+
+
+# ----------------------------------------------------------------------------------------
+# (d) Leiden
+# ----------------------------------------------------------------------------------------
+def leiden(indptr, indices, data, n, resolution=1.0, random_state=0, n_iterations=2,
+           commit_prob=0.9, init_labels=None):
+    """Leiden on a device CSR: ``(labels int32 device (n,), modularity, info dict)``."""
+    import torch
+    lib = load_library()
+    indptr = to_device(indptr, torch.int64)
+    indices = to_device(indices, torch.int32)
+    data = to_device(data, torch.float32)
+    nnz = int(indices.numel())
+    nbytes = c_sz(0)
+    check(lib.msc_leiden_workspace_size(n, nnz, ctypes.byref(nbytes)))
+    ws = workspace(nbytes.value)
+    labels = torch.empty(n, dtype=torch.int32, device=indptr.device)
+    init = None if init_labels is None else to_device(init_labels, torch.int32)
+    q = c_dbl(0.0)
+    info = LeidenInfo()
+    check(lib.msc_leiden(ptr(indptr), ptr(indices), ptr(data), n, nnz, float(resolution),
+                         int(random_state), int(n_iterations), float(commit_prob),
+                         None if init is None else ptr(init), ptr(labels), ctypes.byref(q),
+                         ctypes.byref(info), ptr(ws), ws.numel(), stream_ptr()))
+    levels = [{f: getattr(info.levels[i], f) for f, _ in LeidenLevel._fields_}
+              for i in range(info.n_levels)]
+    return labels, q.value, {"scale_exponent": info.scale_exponent,
+                             "twom_fixed": info.twom_fixed, "levels": levels}
+
+
+# ----------------------------------------------------------------------------------------
+# (b) spatial grid kNN
+# ----------------------------------------------------------------------------------------
+def spatial_knn(coords, k, coords64=None, return_info=False):
+    """Exact spatial kNN (self excluded): ``(idx int32, dist float32[, len64 float64])``."""
+    import torch
+    lib = load_library()
+    X = to_device(coords, torch.float32)
+    n, dim = X.shape
+    nbytes = c_sz(0)
+    check(lib.msc_spatial_knn_workspace_size(n, dim, int(k), ctypes.byref(nbytes)))
+    ws = workspace(nbytes.value)
+    idx = torch.empty((n, k), dtype=torch.int32, device=X.device)
+    dist = torch.empty((n, k), dtype=torch.float32, device=X.device)
+    X64 = None if coords64 is None else to_device(coords64, torch.float64)
+    L64 = None if coords64 is None else torch.empty((n, k), dtype=torch.float64, device=X.device)
+    info = SpatialInfo()
+    check(lib.msc_spatial_knn(ptr(X), n, dim, int(k), ptr(idx), ptr(dist),
+                              None if X64 is None else ptr(X64), None if L64 is None else ptr(L64),
+                              ctypes.byref(info) if return_info else None, ptr(ws), ws.numel(),
+                              stream_ptr()))
+    out = (idx, dist) if L64 is None else (idx, dist, L64)
+    if return_info:
+        return out + ({f: getattr(info, f) for f, _ in SpatialInfo._fields_},)
+    return out
+
+
+# ----------------------------------------------------------------------------------------
+# (e) Moran's I
+# ----------------------------------------------------------------------------------------
+def moran(X, w_indptr, w_indices, w_data, n_perms=0, perms=None, seed=0, transformation=True):
+    """Moran's I on the device: ``(I float64 (G,), sims float64 (P, G) or None, S0)``."""
+    import torch
+    lib = load_library()
+    Xd = to_device(X, torch.float32)
+    n, G = Xd.shape
+    ip = to_device(w_indptr, torch.int64)
+    ind = to_device(w_indices, torch.int32)
+    dat = to_device(w_data, torch.float64)
+    nnz = int(ind.numel())
+    P = int(n_perms or 0)
+    pr = None
+    if P and perms is not None:
+        pr = to_device(perms, torch.int32)
+        if tuple(pr.shape) != (P, n):
+            raise ValueError(f"perms must have shape ({P}, {n})")
+    nbytes = c_sz(0)
+    check(lib.msc_moran_workspace_size(n, G, nnz, P, ctypes.byref(nbytes)))
+    ws = workspace(nbytes.value)
+    I = torch.empty(G, dtype=torch.float64, device=Xd.device)
+    sims = torch.empty((max(P, 1), G), dtype=torch.float64, device=Xd.device)
+    S0 = c_dbl(0.0)
+    check(lib.msc_moran(ptr(Xd), n, G, ptr(ip), ptr(ind), ptr(dat), nnz, int(bool(transformation)), P,
+                        None if pr is None else ptr(pr), int(seed), ptr(I), ptr(sims),
+                        ctypes.byref(S0), ptr(ws), ws.numel(), stream_ptr()))
+    return I, (sims[:P] if P else None), S0.value

@@ -1,0 +1,96 @@
+"""Neighbour graphs on the PCA embedding and on spatial coordinates (MI355X HIP kernels).
+
+Mirrors the reference's ``metalsinglecell.neighbors`` seams:
+
+* ``_knn(X, n_neighbors, ...)`` -> ``(indices, dists)`` (src/metalsinglecell/neighbors.py:698-724),
+  self-inclusive, self in column 0, ascending.  Here it is always EXACT (the reference is
+  exact below 30k cells and approximate NNDescent above): tiled MFMA distances with a fused
+  top-k, an exact float32 re-rank and a per-row certificate (csrc/knn.hip).
+* ``neighbors(X, n_neighbors, random_state)`` -> ``(distances, connectivities)`` scipy CSR
+  (neighbors.py:727-760), connectivities from the fuzzy simplicial set (csrc/fuzzy.hip).
+* ``_knn_grid(coords, k)`` -> ``(indices, dists)`` self-inclusive exact spatial kNN
+  (neighbors.py:333-417) on the device grid (csrc/grid.hip).
+
+All computation is on the GPU; without one these functions raise.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+from . import _hip
+
+DEFAULT_PRECISION = "bf16"
+
+
+def _csr_to_scipy(indptr, indices, data, n):
+    import scipy.sparse as sp
+    return sp.csr_matrix((data.cpu().numpy(), indices.cpu().numpy(), indptr.cpu().numpy()),
+                         shape=(n, n))
+
+
+def knn_device(X, n_neighbors, precision=DEFAULT_PRECISION):
+    """Device tensors ``(idx int32 (n,k), dist float32 (n,k))`` — no host copies."""
+    n = X.shape[0]
+    if not 1 <= n_neighbors <= n:
+        raise ValueError(f"n_neighbors={n_neighbors} must be in [1, n_obs={n}]")
+    return _hip.knn_exact(X, int(n_neighbors), precision=precision)
+
+
+def _knn(X_pca, n_neighbors, random_state=0, approx=None, backend="exact",
+         precision=DEFAULT_PRECISION):
+    """Exact kNN ``(indices int64 (n,k), dists float32 (n,k))``, self first.
+
+    ``approx`` and ``backend`` are accepted for signature compatibility with the reference;
+    ``approx=True`` and the NNDescent / IVF / pynndescent engines do not exist here — every
+    size takes the exact path.  ``random_state`` does not affect exact results.
+    """
+    if approx:
+        raise NotImplementedError("approximate kNN is not provided: this path is exact at every size")
+    if backend not in ("exact", "nndescent"):
+        raise ValueError(f"unknown kNN backend {backend!r} (exact)")
+    idx, dist = knn_device(X_pca, n_neighbors, precision)
+    return idx.cpu().numpy().astype(np.int64), dist.cpu().numpy()
+
+
+def neighbors_device(X, n_neighbors=15, precision=DEFAULT_PRECISION):
+    """kNN, distances CSR and connectivities CSR, all as device tensors.
+
+    Returns a dict with ``idx``, ``dist`` (n x k), ``distances`` and ``connectivities`` as
+    ``(indptr int64, indices int32, data float32)`` tuples, and ``sigma``/``rho``.
+    """
+    idx, dist = knn_device(X, n_neighbors, precision)
+    dcsr = _hip.knn_distances_csr(idx, dist)
+    conn, (sigma, rho) = _hip.fuzzy_connectivities(idx, dist)
+    return {"idx": idx, "dist": dist, "distances": dcsr, "connectivities": conn,
+            "sigma": sigma, "rho": rho}
+
+
+def neighbors(X_pca, n_neighbors=15, random_state=0, approx=None, precision=DEFAULT_PRECISION):
+    """``(distances, connectivities)`` scipy CSR matrices (neighbors.py:727-760 contract).
+
+    ``distances`` holds k-1 entries per row (self dropped), columns ascending, explicit zeros
+    kept for coincident points; ``connectivities`` is the symmetric fuzzy simplicial set.
+    """
+    if approx:
+        raise NotImplementedError("approximate kNN is not provided: this path is exact at every size")
+    X = np.asarray(X_pca, dtype=np.float32) if not hasattr(X_pca, "device") else X_pca
+    n = X.shape[0]
+    out = neighbors_device(X, n_neighbors, precision)
+    return _csr_to_scipy(*out["distances"], n), _csr_to_scipy(*out["connectivities"], n)
+
+
+def _knn_grid(coords, k, min_n=None):
+    """Exact self-inclusive spatial kNN ``(indices int64 (n,k), dists float32 (n,k))``.
+
+    Column 0 is the point itself at distance 0; the rest are the k-1 nearest other points
+    ordered by (float32 distance, index).  ``min_n`` is accepted for signature compatibility.
+    """
+    X = np.ascontiguousarray(coords, dtype=np.float32)
+    n = X.shape[0]
+    if k == 1:
+        return np.arange(n)[:, None].astype(np.int64), np.zeros((n, 1), np.float32)
+    idx, dist = _hip.spatial_knn(X, int(k) - 1)
+    idx = np.c_[np.arange(n), idx.cpu().numpy()].astype(np.int64)
+    dist = np.c_[np.zeros(n, np.float32), dist.cpu().numpy()]
+    return idx, dist

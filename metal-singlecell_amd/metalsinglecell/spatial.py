@@ -1,0 +1,120 @@
+"""Spatial autocorrelation (squidpy ``gr.spatial_autocorr`` compute), Moran's I on the MI355X.
+
+Mirrors the reference's ``metalsinglecell.spatial`` (src/metalsinglecell/spatial.py):
+``spatial_autocorr(X, connectivity, mode, n_perms, random_state) -> {mode: I, "sims": ...}``
+(:357-406) with the statistic and the permutation null computed by csrc/moran.hip, and the
+small host-side statistics over the (genes,) / (perms, genes) results:
+``row_normalize`` (:268-279), ``graph_moments`` and ``analytic_pval`` (:282-313, Cliff & Ord
+1981 / pysal), ``permutation_pvals`` (:316-331) and ``multiple_testing`` (:334-354).
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+from . import _hip
+
+
+def row_normalize(g):
+    """L1-normalise every row (squidpy ``transformation=True``); zero rows stay zero."""
+    import scipy.sparse as sp
+    m = sp.csr_matrix(g, dtype=np.float64, copy=True)
+    s = np.asarray(np.abs(m).sum(axis=1)).ravel()
+    scale = np.zeros_like(s)
+    np.divide(1.0, s, out=scale, where=s != 0)
+    return sp.csr_matrix(sp.diags(scale) @ m)
+
+
+def graph_moments(g):
+    """pysal's weight moments ``(s0, s1, s2)``: total, half the squared symmetric sum, and
+    the summed squared (row + column) totals."""
+    import scipy.sparse as sp
+    m = sp.csr_matrix(g, dtype=np.float64)
+    sym = m + m.T
+    s0 = float(m.sum())
+    s1 = 0.5 * float(sym.multiply(sym).sum())
+    rc = np.asarray(m.sum(axis=1)).ravel() + np.asarray(m.sum(axis=0)).ravel()
+    s2 = float(np.dot(rc, rc))
+    return s0, s1, s2
+
+
+def _normal_tail(z):
+    """One-sided normal tail toward the sign of z, as ``1 - Phi(z)`` / ``Phi(z)``."""
+    from scipy import stats
+    return np.where(z > 0, 1.0 - stats.norm.cdf(z), stats.norm.cdf(z))
+
+
+def analytic_pval(score, g, mode="moran", two_tailed=False):
+    """Normality-assumption p-value and variance of Moran's I (randomisation-free form)."""
+    if mode != "moran":
+        raise NotImplementedError(f"mode={mode!r}: only Moran's I is implemented on the MI355X")
+    s0, s1, s2 = graph_moments(g)
+    n = g.shape[0]
+    e_i = -1.0 / (n - 1)
+    var = (n * n * s1 - n * s2 + 3.0 * s0 * s0) / ((n * n - 1.0) * s0 * s0) - e_i * e_i
+    with np.errstate(divide="ignore", invalid="ignore"):
+        p = _normal_tail((np.asarray(score, np.float64) - e_i) / np.sqrt(var))
+    return (2.0 * p if two_tailed else p), var
+
+
+def permutation_pvals(score, sims):
+    """Folded two-sided permutation p-value, its z-score p-value and the null variance."""
+    P = sims.shape[0]
+    ge = (sims >= score[None, :]).sum(axis=0)
+    folded = np.minimum(ge, P - ge)
+    p_sim = (folded + 1.0) / (P + 1.0)
+    mu = sims.mean(axis=0)
+    sd = sims.std(axis=0)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        p_z = _normal_tail((score - mu) / sd)
+    return p_sim, p_z, sims.var(axis=0)
+
+
+def multiple_testing(pvals, method="fdr_bh"):
+    """Benjamini-Hochberg step-up (``fdr_bh``) or Bonferroni correction, capped at 1."""
+    p = np.asarray(pvals, dtype=np.float64)
+    m = p.size
+    if method == "bonferroni":
+        return np.minimum(p * m, 1.0)
+    if method != "fdr_bh":
+        raise NotImplementedError(f"corr_method={method!r} is not implemented "
+                                  f"(expected 'fdr_bh', 'bonferroni' or None)")
+    if m == 0:
+        return p.copy()
+    order = np.argsort(p, kind="mergesort")
+    scaled = p[order] * m / np.arange(1, m + 1)
+    adj = np.minimum(np.minimum.accumulate(scaled[::-1])[::-1], 1.0)
+    out = np.empty(m)
+    out[order] = adj
+    return out
+
+
+def permutations_numpy(n, n_perms, random_state):
+    """The reference's permutation stream: ``default_rng(seed).permutation(n)`` in sequence."""
+    rng = np.random.default_rng(random_state)
+    return np.stack([rng.permutation(n) for _ in range(n_perms)]).astype(np.int32)
+
+
+def spatial_autocorr(X, connectivity, mode="moran", n_perms=None, random_state=0,
+                     permutations="numpy", transformation=False):
+    """Per-gene Moran's I and (optionally) its ``(n_perms, genes)`` permutation null.
+
+    ``connectivity`` is used as given (the caller row-normalises, as in the reference),
+    unless ``transformation=True``, which normalises on the device.  ``permutations`` picks
+    the null's source: ``"numpy"`` draws the reference's ``default_rng(random_state)`` stream
+    on the host; ``"device"`` uses a keyed Feistel bijection generated in the kernel.
+    """
+    import scipy.sparse as sp
+    if mode != "moran":
+        raise NotImplementedError(f"mode={mode!r}: only Moran's I is implemented on the MI355X")
+    W = sp.csr_matrix(connectivity)
+    n = W.shape[0]
+    P = int(n_perms or 0)
+    perms = None
+    if P and permutations == "numpy":
+        perms = permutations_numpy(n, P, random_state)
+    elif P and permutations != "device":
+        raise ValueError("permutations must be 'numpy' or 'device'")
+    I, sims, _ = _hip.moran(X, W.indptr, W.indices, W.data.astype(np.float64), n_perms=P,
+                            perms=perms, seed=random_state, transformation=transformation)
+    return {mode: I.cpu().numpy(), "sims": None if sims is None else sims.cpu().numpy()}

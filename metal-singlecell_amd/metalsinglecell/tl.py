@@ -1,0 +1,49 @@
+"""``tl`` namespace — the Leiden drop-in of ``scanpy.tl.leiden``.
+
+Mirrors the reference's ``metalsinglecell.tl.leiden`` (src/metalsinglecell/tl.py:55-73) with
+``_conn`` (:14-21) and ``_categorical`` (:35-52): the connectivity graph is resolved through
+``uns[neighbors_key]``, labels are written to ``obs[key_added]`` as a Categorical of string
+labels with natural-sorted categories, and ``uns[key_added]['params']`` records resolution,
+random_state and n_iterations.  ``backend="gpu"`` runs csrc/leiden.hip on the MI355X.
+"""
+
+from __future__ import annotations
+
+import re
+
+import numpy as np
+
+
+def _conn(adata, neighbors_key="neighbors"):
+    key = "connectivities"
+    if neighbors_key in adata.uns:
+        key = adata.uns[neighbors_key].get("connectivities_key", key)
+    if key not in adata.obsp:
+        raise ValueError(f"run msc.pp.neighbors first (no adata.obsp[{key!r}])")
+    return adata.obsp[key]
+
+
+def _natkey(s):
+    """Natural-order key: digit runs compare as integers."""
+    return [int(t) if t.isdigit() else t for t in re.split(r"(\d+)", str(s))]
+
+
+def _categorical(labels):
+    import pandas as pd
+    vals = np.asarray([str(x) for x in labels])
+    return pd.Categorical(values=vals, categories=sorted(set(vals.tolist()), key=_natkey))
+
+
+def leiden(adata, resolution: float = 1.0, key_added: str = "leiden", random_state: int = 0,
+           n_iterations: int = 2, backend: str = "igraph", variant: str = "sync",
+           commit_prob: float = 0.9, neighbors_key: str = "neighbors", copy: bool = False):
+    """Leiden clustering; writes ``obs[key_added]`` (categorical) and ``uns[key_added]``."""
+    from .cluster import leiden as _leiden
+    adata = adata.copy() if copy else adata
+    lab = _leiden(_conn(adata, neighbors_key), resolution=resolution, random_state=random_state,
+                  n_iterations=n_iterations, backend=backend, variant=variant,
+                  commit_prob=commit_prob)
+    adata.obs[key_added] = _categorical(lab)
+    adata.uns[key_added] = {"params": {"resolution": resolution, "random_state": random_state,
+                                       "n_iterations": n_iterations}}
+    return adata if copy else None

@@ -65,7 +65,10 @@ def smooth_knn_dist(dist, k, n_iter=64):
             break
     sigma = mid.copy()
     mean_all = float(np.sum(D, dtype=np.float64)) / D.size
-    mean_row = np.sum(D, axis=1, dtype=np.float64) / D.shape[1]
+    mean_row = np.zeros(n)
+    for j in range(D.shape[1]):          # sequential float64 row sums, column order
+        mean_row = mean_row + D[:, j].astype(np.float64)
+    mean_row = mean_row / D.shape[1]
     floor = np.where(rho > 0, MIN_K_DIST_SCALE * mean_row, MIN_K_DIST_SCALE * mean_all)
     sigma = np.where(sigma.astype(np.float64) < floor, floor.astype(np.float32), sigma)
     return sigma.astype(np.float32), rho

@@ -25,8 +25,10 @@ summed) with P lifted; at most 20 levels.  ``n_iterations`` repeats the pass wit
 Deliberate definitions shared bit-for-bit with the HIP kernels (documented deviations from
 the reference's MLX arithmetic):
 * edge weights are quantised once to 64-bit fixed point, ``wq = rint(w * 2^s)`` with
-  ``s = 52 - ceil(log2(sum w))``, so every degree, sigma_tot and w_{v->c} is an exact,
-  order-independent integer sum; gains are then evaluated in float64 by a fixed expression;
+  ``s = 52 - ceil(log2(max w)) - ceil(log2(E))`` (both exact, order-independent), so every
+  degree, sigma_tot and w_{v->c} is an exact integer sum below 2^52; gains are then
+  evaluated in float64 by a fixed expression and the modularity sum runs sequentially over
+  communities in label order;
 * the coin comes from the counter RNG in oracle/numerics.py keyed by (seed, phase, pass, v)
   instead of MLX's key/split stream;
 * vertices of degree > 1024 follow the same synchronous rule as all others (the reference
@@ -64,11 +66,18 @@ class QGraph:
         return float(np.ldexp(1.0, -self.s))
 
 
-def weight_scale_exponent(total):
-    """``s`` such that ``total * 2^s`` < 2^52 (0 for an empty graph)."""
-    if total <= 0:
+def ceil_log2(x):
+    """Exact ceil(log2(x)) for x > 0 (from the binary exponent)."""
+    import math
+    m, e = math.frexp(float(x))
+    return e - 1 if m == 0.5 else e
+
+
+def weight_scale_exponent(max_w, n_edges):
+    """``s`` with ``max_w * n_edges * 2^s <= 2^52`` (0 for an empty or all-zero graph)."""
+    if n_edges <= 0 or max_w <= 0:
         return 0
-    return int(52 - int(np.ceil(np.log2(total))))
+    return int(52 - ceil_log2(max_w) - ceil_log2(n_edges))
 
 
 def quantize(w, s):
@@ -82,7 +91,7 @@ def from_scipy(mat):
     csr.sum_duplicates()
     csr.sort_indices()
     w = csr.data.astype(np.float32)
-    s = weight_scale_exponent(float(np.sum(w, dtype=np.float64)))
+    s = weight_scale_exponent(float(w.max()) if w.size else 0.0, w.size)
     return QGraph(csr.indptr, csr.indices, quantize(w, s), s)
 
 
@@ -179,7 +188,12 @@ def modularity(g, comm, gamma):
     twom = float(g.twom) * g.inv_scale
     tot = tot * g.inv_scale
     inw = inw * g.inv_scale
-    return float(np.sum(inw / twom - gamma * (tot / twom) ** 2))
+    a = tot / twom
+    terms = inw / twom - gamma * (a * a)
+    q = 0.0
+    for t in terms.tolist():          # sequential, label order (the GPU sums the same way)
+        q += t
+    return q
 
 
 def contract(g, R, nR):

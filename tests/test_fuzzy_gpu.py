@@ -1,0 +1,56 @@
+"""HIP fuzzy-simplicial-set connectivities against the CPU oracle."""
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+pytestmark = pytest.mark.gpu
+
+
+def _gpu_conn(X, k):
+    from metalsinglecell import _hip
+    idx, dist = _hip.knn_exact(X, k)
+    (ip, ind, dat), (sig, rho) = _hip.fuzzy_connectivities(idx, dist)
+    n = X.shape[0]
+    P = sp.csr_matrix((dat.cpu().numpy(), ind.cpu().numpy(), ip.cpu().numpy()), shape=(n, n))
+    return idx.cpu().numpy(), dist.cpu().numpy(), P, sig.cpu().numpy(), rho.cpu().numpy()
+
+
+def _check(X, k):
+    from oracle import fuzzy
+    idx, dist, P, sig, rho = _gpu_conn(X, k)
+    R, rsig, rrho = fuzzy.fuzzy_simplicial_set(idx, dist)
+    assert np.array_equal(rho, rrho)
+    assert np.array_equal(sig, rsig)
+    assert np.array_equal(P.indptr, R.indptr) and np.array_equal(P.indices, R.indices)
+    assert np.array_equal(P.data, R.data)          # bit-identical (tolerance 1e-5 is the spec)
+    assert P.has_canonical_format
+    assert (abs(P - P.T) > 0).nnz == 0
+
+
+def test_fuzzy_mixture_3000(c1_data):
+    _check(c1_data[0], 15)
+
+
+@pytest.mark.parametrize("k", [2, 5, 30])
+def test_fuzzy_gaussian_k(k):
+    X = np.random.default_rng(1).standard_normal((700, 12)).astype(np.float32)
+    _check(X, k)
+
+
+def test_fuzzy_duplicates_no_nan():
+    """Reference tests/test_postfix_fixes.py:197-207: duplicate points give no NaN."""
+    rng = np.random.default_rng(0)
+    X = rng.standard_normal((60, 10)).astype(np.float32)
+    Xd = np.vstack([X, X[:5], X[:5], X[:5]]).astype(np.float32)
+    _check(Xd, 10)
+    _, _, P, _, _ = _gpu_conn(Xd, 10)
+    assert not np.isnan(P.data).any()
+
+
+def test_fuzzy_hub_in_degree():
+    """A hub (one point near everything) has in-degree far above k: exercises long in-lists."""
+    rng = np.random.default_rng(3)
+    X = rng.standard_normal((3000, 3)).astype(np.float32) * 10
+    X[:200] = rng.standard_normal((200, 3)).astype(np.float32) * 0.01
+    _check(X, 15)

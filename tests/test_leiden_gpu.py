@@ -1,0 +1,84 @@
+"""HIP Leiden against the CPU oracle: identical labels and modularity."""
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+pytestmark = pytest.mark.gpu
+
+
+def _gpu_leiden(P, **kw):
+    from metalsinglecell import _hip
+    P = sp.csr_matrix(P, dtype=np.float32)
+    P.sum_duplicates()
+    P.sort_indices()
+    lab, q, info = _hip.leiden(P.indptr, P.indices, P.data, P.shape[0], **kw)
+    return lab.cpu().numpy().astype(np.int64), q, info
+
+
+def _check(P, **kw):
+    from oracle import leiden
+    lab, q, info = _gpu_leiden(P, **kw)
+    stats = []
+    ref = leiden.leiden(P, resolution=kw.get("resolution", 1.0),
+                        random_state=kw.get("random_state", 0),
+                        n_iterations=kw.get("n_iterations", 2), stats=stats)
+    rq = leiden.modularity_of_labels(P, ref, kw.get("resolution", 1.0))
+    assert np.array_equal(lab, ref), (info["levels"], stats)
+    assert q == rq
+    return lab, q
+
+
+def _knn_graph(X, k):
+    from metalsinglecell import _hip
+    idx, dist = _hip.knn_exact(X, k)
+    (ip, ind, dat), _ = _hip.fuzzy_connectivities(idx, dist)
+    n = X.shape[0]
+    return sp.csr_matrix((dat.cpu().numpy(), ind.cpu().numpy(), ip.cpu().numpy()), shape=(n, n))
+
+
+def test_leiden_mixture_3000(c1_data):
+    from sklearn.metrics import adjusted_rand_score
+    X, lab_true = c1_data
+    P = _knn_graph(X, 15)
+    lab, q = _check(P)
+    assert adjusted_rand_score(lab_true, lab) > 0.95
+
+
+@pytest.mark.parametrize("res", [0.5, 2.0, 8.0])
+def test_leiden_resolutions(c1_data, res):
+    P = _knn_graph(c1_data[0][:1500], 15)
+    _check(P, resolution=res, random_state=3)
+
+
+def test_leiden_heavy_vertices():
+    """Hubs with degree > 64 (and > 1024, which turns frontier pruning off)."""
+    rng = np.random.default_rng(5)
+    n = 3000
+    rows, cols = [], []
+    for hub, deg in [(0, 1500), (1, 300), (2, 90)]:
+        nb = rng.choice(np.arange(3, n), deg, replace=False)
+        rows += [hub] * deg
+        cols += nb.tolist()
+    # planted blocks of 100
+    for b in range(30):
+        m = np.arange(b * 100, (b + 1) * 100)
+        r = rng.choice(m, 800)
+        c = rng.choice(m, 800)
+        rows += r.tolist()
+        cols += c.tolist()
+    w = rng.uniform(0.1, 1.0, len(rows)).astype(np.float32)
+    A = sp.coo_matrix((w, (rows, cols)), shape=(n, n)).tocsr()
+    A = (A + A.T).tocsr()
+    A.setdiag(0)
+    A.eliminate_zeros()
+    _check(A, random_state=1)
+
+
+def test_leiden_init_labels_and_single_iteration(c1_data):
+    from metalsinglecell import _hip
+    from oracle import leiden
+    P = _knn_graph(c1_data[0][:2000], 15)
+    lab1, _, _ = _gpu_leiden(P, n_iterations=1)
+    ref1 = leiden.leiden(P, n_iterations=1)
+    assert np.array_equal(lab1, ref1)
