@@ -1,0 +1,237 @@
+"""Neighbor-graph benchmark: cells/sec for exact kNN + connectivities + Leiden on one MI355X.
+
+Workload (default): 1,000,000 cells x 50 PCs float32, 200-cluster Gaussian mixture with
+Zipf(1.5) cluster sizes (seed 2), k = 30, Leiden resolution 1.0, seed 0, n_iterations 2.
+One step = one full pass from the device-resident embedding to device labels:
+
+    exact kNN (BF16 MFMA ranking, 64 admitted candidates, exact FP32 re-rank)
+    -> UMAP fuzzy connectivities -> synchronous parallel Leiden.
+
+``value`` = cells / (mean step time) for that BF16+re-rank form.  The plain FP32 GPU form
+(FP32 MFMA ranking) is timed beside it on the same input and reported in ``fp32_form``;
+both forms return identical graphs and labels.  ``roofline`` covers the dominant MFMA
+kernel (the kNN select kernel): algorithmic flops 2 n^2 d per launch over its HIP-event
+duration, against the 2.5 PFLOP/s dense BF16 peak.  ``cpu_baseline`` times the project's
+CPU restatement (oracle/) of the same three stages on bounded samples on this host.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N] [--no-cpu]
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "metal-singlecell_amd"))
+sys.path.insert(0, ROOT)
+
+PEAK = {"bf16": 2500.0, "fp32": 157.3}   # dense MFMA TFLOP/s, MI355X datasheet
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def gpu_step(Xd, k, precision, resolution=1.0, seed=0, n_iterations=2):
+    from metalsinglecell import _hip
+    import torch
+    t0 = time.perf_counter()
+    idx, dist, kinfo = _hip.knn_exact(Xd, k, precision=precision, timing=True)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    (ip, ind, dat), _ = _hip.fuzzy_connectivities(idx, dist)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    labels, q, linfo = _hip.leiden(ip, ind, dat, Xd.shape[0], resolution=resolution,
+                                   random_state=seed, n_iterations=n_iterations)
+    torch.cuda.synchronize()
+    t3 = time.perf_counter()
+    return labels, {"knn_s": t1 - t0, "conn_s": t2 - t1, "leiden_s": t3 - t2, "q": q,
+                    "nnz": int(ind.numel()), "knn": kinfo, "leiden_levels": len(linfo["levels"]),
+                    "n_clusters": int(labels.max().item()) + 1}
+
+
+def timed(Xd, k, precision, steps, warmup):
+    import torch
+    for _ in range(warmup):
+        gpu_step(Xd, k, precision)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    recs = []
+    labels = None
+    for _ in range(steps):
+        labels, r = gpu_step(Xd, k, precision)
+        recs.append(r)
+    torch.cuda.synchronize()
+    total = time.perf_counter() - t0
+    return total / steps, recs, labels
+
+
+def cpu_baseline(X, k, threads):
+    """CPU restatement timed on bounded samples of the same workload (extrapolated)."""
+    import numpy as np
+    from metalsinglecell import synthetic  # noqa: F401
+    from oracle import fuzzy, knn, leiden
+    n = X.shape[0]
+    rng = np.random.default_rng(0)
+    # kNN: queries are independent -> time q queries against all n candidates
+    q = 1000 if n >= 200_000 else min(n, 4000)
+    rows = rng.choice(n, q, replace=False)
+    t = time.perf_counter()
+    knn.exact_knn_c(X, k, rows=rows, threads=threads)
+    t_knn = (time.perf_counter() - t) * n / q
+    # connectivities + Leiden on a subsample of m cells (cost ~ linear in cells at fixed k)
+    m = min(n, 20_000)
+    sub = np.ascontiguousarray(X[rng.choice(n, m, replace=False)])
+    si, _, sd = knn.exact_knn_c(sub, k, threads=threads)
+    t = time.perf_counter()
+    P, _, _ = fuzzy.fuzzy_simplicial_set(si, sd)
+    t_conn = (time.perf_counter() - t) * n / m
+    t = time.perf_counter()
+    leiden.leiden(P, resolution=1.0, random_state=0, n_iterations=2)
+    t_leiden = (time.perf_counter() - t) * n / m
+    total = t_knn + t_conn + t_leiden
+    return {"value": n / total, "unit": "cells/s", "cores": threads, "kind": "port",
+            "sample": (f"kNN: {q} query rows x all {n} candidates (C oracle, {threads} threads), "
+                       f"x{n / q:.0f}; connectivities + Leiden (NumPy oracle, n_iterations=2) on a "
+                       f"{m}-cell subsample, x{n / m:.0f}; all extrapolated linearly to {n} cells"),
+            "stage_s": {"knn": t_knn, "connectivities": t_conn, "leiden": t_leiden}}
+
+
+def serving(duration_s, rate=20.0, genes=500, n_perms=100, max_slots=6, seed=4):
+    """Poisson stream of spatial jobs (grid kNN k=6 + Moran's I, P permutations) through the
+    continuous-batching server; arrival -> completion latency percentiles."""
+    import threading
+    import numpy as np
+    import torch
+    from metalsinglecell import synthetic
+    from metalsinglecell.serve import Server, SpatialJob
+
+    n_jobs = max(1, int(duration_s * rate))
+    arrivals, sizes = synthetic.poisson_job_stream(n_jobs, rate=rate, seed=seed)
+    t = time.perf_counter()
+    pool = synthetic.nb_counts(int(sizes.max()), genes, rng=np.random.default_rng(seed))
+    rng = np.random.default_rng(seed + 1)
+    jobs = []
+    for i, n in enumerate(sizes):
+        xy = synthetic.tissue_coordinates(int(n), rng=rng)
+        jobs.append(SpatialJob(coords=xy, counts=pool[: int(n)], n_perms=n_perms, seed=i, job_id=i))
+    log(f"serving: {n_jobs} jobs prepared in {time.perf_counter() - t:.1f}s "
+        f"(sizes {int(sizes.min())}..{int(sizes.max())}, mean {int(sizes.mean())})")
+    srv = Server(max_slots=max_slots)
+    futs = []
+    t0 = time.perf_counter()
+
+    def feed():
+        for j, at in zip(jobs, arrivals):
+            delay = t0 + at - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            j.arrival = time.perf_counter()
+            futs.append(srv.submit(j))
+
+    th = threading.Thread(target=feed)
+    th.start()
+    th.join()
+    errors = 0
+    for f in futs:
+        try:
+            f.wait(600)
+        except Exception:
+            errors += 1
+    t1 = time.perf_counter()
+    srv.close()
+    st = srv.stats(t0, t1)
+    st.update({"offered_rate_jobs_per_s": rate, "errors": errors, "genes": genes,
+               "n_perms": n_perms, "n_neighs": 6, "max_slots": max_slots,
+               "mean_cells": float(sizes.mean()), "hbm_budget_gib": srv.hbm_budget / 2**30,
+               "stream_s": float(arrivals[-1])})
+    return st
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=1_000_000)
+    ap.add_argument("--k", type=int, default=30)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--serve-seconds", type=float, default=10.0)
+    a = ap.parse_args()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from metalsinglecell import synthetic
+
+    torch.cuda.set_device(0)
+    t = time.perf_counter()
+    X, _ = synthetic.gaussian_mixture_zipf(n=a.n, seed=2)
+    log(f"data {X.shape} in {time.perf_counter() - t:.1f}s")
+    Xd = torch.from_numpy(X).cuda()
+    torch.cuda.synchronize()
+    n, d = X.shape
+
+    ms_bf, recs_bf, lab_bf = timed(Xd, a.k, "bf16", a.steps, a.warmup)
+    log(f"bf16 form: {ms_bf * 1e3:.1f} ms/step", recs_bf[-1])
+    fp32 = None
+    if not a.no_fp32:
+        ms_fp, recs_fp, lab_fp = timed(Xd, a.k, "fp32", 1, 0)
+        same = bool(torch.equal(lab_bf, lab_fp))
+        fp32 = {"value": n / ms_fp, "ms_per_step": ms_fp * 1e3, "identical_labels": same,
+                "knn_s": recs_fp[-1]["knn_s"], "select_ms": recs_fp[-1]["knn"]["select_ms"],
+                "speedup_bf16_vs_fp32": ms_fp / ms_bf}
+        log("fp32 form:", fp32)
+
+    sel = [r["knn"]["select_ms"] for r in recs_bf]
+    sel_ms = float(np.mean(sel))
+    flops = 2.0 * n * n * d
+    achieved = flops / (sel_ms * 1e-3) / 1e12
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "knn_select_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("bytes_per_launch")
+    roof = {"bound": "mfma", "kernel": "select_kernel<bf16>", "achieved": achieved,
+            "peak": PEAK["bf16"], "unit": "TFLOP/s", "frac": achieved / PEAK["bf16"],
+            "traffic": traffic, "flops_per_launch": flops, "avg_launch_ms": sel_ms}
+
+    cpu = None
+    if not a.no_cpu:
+        threads = len(os.sched_getaffinity(0))
+        t = time.perf_counter()
+        cpu = cpu_baseline(X, a.k, threads)
+        log(f"cpu baseline in {time.perf_counter() - t:.1f}s", cpu)
+
+    serve = None
+    if a.serve_seconds > 0:
+        t = time.perf_counter()
+        serve = serving(a.serve_seconds)
+        log(f"serving in {time.perf_counter() - t:.1f}s", serve)
+
+    last = recs_bf[-1]
+    out = {
+        "metric": "cells/sec kNN+Leiden at 1M×50 k=30 (BF16+rerank vs FP32); serve p50/p99",
+        "value": n / ms_bf, "unit": "cells/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": ms_bf * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "bf16", "data": "synthetic",
+        "config": {"workload": f"gaussian_mixture_zipf n={n} d={d} k={a.k} k'=64 "
+                               "BF16 MFMA + exact FP32 re-rank; Leiden res=1.0 seed=0 n_iter=2",
+                   "cells": n, "dims": d, "k": a.k, "parallelism": "single GPU"},
+        "stages_s": {"knn": last["knn_s"], "connectivities": last["conn_s"], "leiden": last["leiden_s"]},
+        "knn_info": last["knn"], "modularity": last["q"], "n_clusters": last["n_clusters"],
+        "fp32_form": fp32, "serving": serve, "roofline": roof, "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

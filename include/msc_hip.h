@@ -156,6 +156,18 @@ int msc_moran(const float* X, int64_t n, int64_t n_genes, const int64_t* w_indpt
               int n_perms, const int32_t* perms, int64_t seed, double* out_I, double* out_sims,
               double* out_S0 /*host*/, void* workspace, size_t workspace_bytes, void* stream);
 
+/* Dense n x n_cols float32 row-major copy of a CSR matrix (duplicates: last write wins; a
+ * canonical CSR has none).  Moran densifies the selected genes (gr.py:452). */
+int msc_csr_to_dense(const int64_t* indptr, const int32_t* indices, const float* data, int64_t n,
+                     int64_t n_cols, float* out, void* stream);
+
+/* pysal weight moments of W (spatial.py:282-290): out_s[0] = s0 = sum w,
+ * out_s[1] = s1 = 1/2 sum (W + W^T)^2, out_s[2] = s2 = sum_i (row_i + col_i)^2 (host). */
+int msc_graph_moments_workspace_size(int64_t n, int64_t nnz, size_t* bytes /*host*/);
+int msc_graph_moments(const int64_t* indptr, const int32_t* indices, const double* w, int64_t n,
+                      int64_t nnz, double* out_s /*host, 3*/, void* workspace,
+                      size_t workspace_bytes, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -30,12 +30,17 @@ struct Geo {
 
 __global__ void k_bbox(const float* __restrict__ X, int64_t n, int dim, float* __restrict__ out) {
   float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+  int bad = 0;
   for (int64_t i = threadIdx.x; i < n; i += blockDim.x)
     for (int t = 0; t < dim; ++t) {
       const float v = X[i * dim + t];
+      bad |= !isfinite(v);
       mn[t] = fminf(mn[t], v);
       mx[t] = fmaxf(mx[t], v);
     }
+  bad = __any(bad);
+  __shared__ int sbad[kThreads / kWave];
+  if (lane_id() == 0) sbad[threadIdx.x / kWave] = bad;
   __shared__ float smn[3][kThreads / kWave], smx[3][kThreads / kWave];
   for (int t = 0; t < 3; ++t) {
     float a = mn[t], b = mx[t];
@@ -50,6 +55,11 @@ __global__ void k_bbox(const float* __restrict__ X, int64_t n, int dim, float* _
     }
   }
   __syncthreads();
+  if (threadIdx.x == 0) {
+    int any_bad = 0;
+    for (int w = 0; w < kThreads / kWave; ++w) any_bad |= sbad[w];
+    out[6] = any_bad ? 1.0f : 0.0f;
+  }
   if (threadIdx.x == 0)
     for (int t = 0; t < 3; ++t) {
       float a = smn[t][0], b = smx[t][0];
@@ -301,8 +311,9 @@ extern "C" int msc_spatial_knn(const float* coords, int64_t n, int dim, int k, i
   k_bbox<<<1, kThreads, 0, st>>>(coords, n, dim, b.bbox);
   MSC_LAUNCH_CHECK();
   float bb[8];
-  MSC_HIP(hipMemcpyAsync(bb, b.bbox, 6 * sizeof(float), hipMemcpyDeviceToHost, st));
+  MSC_HIP(hipMemcpyAsync(bb, b.bbox, 7 * sizeof(float), hipMemcpyDeviceToHost, st));
   MSC_HIP(hipStreamSynchronize(st));
+  MSC_REQUIRE(bb[6] == 0.0f, "spatial_knn: coordinates must be finite");
   for (int t = 0; t < dim; ++t)
     MSC_REQUIRE(std::isfinite(bb[t]) && std::isfinite(bb[3 + t]), "spatial_knn: coordinates must be finite");
   Geo geo;

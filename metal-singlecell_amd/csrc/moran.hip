@@ -18,6 +18,7 @@
 
 #include "common.h"
 #include "msc_hip.h"
+#include "prims.h"
 
 namespace msc {
 namespace moran {
@@ -304,5 +305,140 @@ extern "C" int msc_moran(const float* X, int64_t n, int64_t n_genes, const int64
     MSC_HIP(hipMemcpyAsync(out_S0, b.S0, sizeof(double), hipMemcpyDeviceToHost, st));
     MSC_HIP(hipStreamSynchronize(st));
   }
+  return kOk;
+}
+
+// ------------------------------------------------------------------------------------------
+// CSR (cells x genes) -> dense row-major float32 on the device (gr.py:452 densifies on the host)
+// ------------------------------------------------------------------------------------------
+namespace msc {
+namespace moran {
+__global__ void k_fill_f32(float* __restrict__ p, int64_t m) {
+  const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (i < m) p[i] = 0.0f;
+}
+__global__ __launch_bounds__(kThreads) void k_scatter_rows(const int64_t* __restrict__ indptr,
+                                                           const int32_t* __restrict__ indices,
+                                                           const float* __restrict__ data, int64_t n,
+                                                           int64_t G, float* __restrict__ out) {
+  const int64_t r = static_cast<int64_t>(blockIdx.x) * (kThreads / kWave) + threadIdx.x / kWave;
+  if (r >= n) return;
+  for (int64_t e = indptr[r] + lane_id(); e < indptr[r + 1]; e += kWave) out[r * G + indices[e]] = data[e];
+}
+}  // namespace moran
+}  // namespace msc
+
+extern "C" int msc_csr_to_dense(const int64_t* indptr, const int32_t* indices, const float* data, int64_t n,
+                                int64_t n_cols, float* out, void* stream) {
+  MSC_REQUIRE(indptr && out && n >= 0 && n_cols >= 1, "csr_to_dense: bad arguments");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const int64_t m = n * n_cols;
+  if (m > 0) {
+    k_fill_f32<<<static_cast<unsigned>(ceil_div(m, kThreads)), kThreads, 0, st>>>(out, m);
+    k_scatter_rows<<<static_cast<unsigned>(ceil_div(n, kThreads / kWave)), kThreads, 0, st>>>(indptr, indices, data,
+                                                                                             n, n_cols, out);
+    MSC_LAUNCH_CHECK();
+  }
+  return kOk;
+}
+
+// ------------------------------------------------------------------------------------------
+// pysal weight moments (spatial.py:282-290) on the device:
+//   s0 = sum w,  s1 = 1/2 sum_ij (w_ij + w_ji)^2,  s2 = sum_i (row_i + col_i)^2
+// Reverse entries are found by scanning the (short) row of the destination; column sums
+// come from a per-destination sequential sum after a stable sort by destination.
+// ------------------------------------------------------------------------------------------
+namespace msc {
+namespace moran {
+// per directed edge: contribution to 2*s1 (pairs counted from both ends)
+__global__ void k_s1_terms(const int64_t* __restrict__ indptr, const int32_t* __restrict__ indices,
+                           const double* __restrict__ w, int64_t n, double* __restrict__ rowterm) {
+  const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (i >= n) return;
+  double acc = 0.0;
+  for (int64_t e = indptr[i]; e < indptr[i + 1]; ++e) {
+    const int64_t j = indices[e];
+    double rev = 0.0;
+    bool has = false;
+    for (int64_t f = indptr[j]; f < indptr[j + 1]; ++f)
+      if (indices[f] == i) { rev += w[f]; has = true; }
+    const double a = w[e];
+    // the (i,j) entry of W + W^T is a + rev; when j->i is absent the (j,i) entry is a too
+    acc += has ? (a + rev) * (a + rev) : 2.0 * a * a;
+  }
+  rowterm[i] = acc;
+}
+__global__ void k_edge_dst(const int64_t* __restrict__ indptr, const int32_t* __restrict__ indices, int64_t n,
+                           uint32_t* __restrict__ key, uint32_t* __restrict__ val) {
+  const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (i >= n) return;
+  for (int64_t e = indptr[i]; e < indptr[i + 1]; ++e) {
+    key[e] = static_cast<uint32_t>(indices[e]);
+    val[e] = static_cast<uint32_t>(e);
+  }
+}
+__global__ void k_s2_terms(const int64_t* __restrict__ indptr, const double* __restrict__ w,
+                           const int64_t* __restrict__ cstart, const uint32_t* __restrict__ sorted_e, int64_t n,
+                           double* __restrict__ out) {
+  const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (i >= n) return;
+  double r = 0.0, c = 0.0;
+  for (int64_t e = indptr[i]; e < indptr[i + 1]; ++e) r += w[e];
+  for (int64_t p = cstart[i]; p < cstart[i + 1]; ++p) c += w[sorted_e[p]];
+  out[i] = (r + c) * (r + c);
+}
+}  // namespace moran
+}  // namespace msc
+
+extern "C" int msc_graph_moments_workspace_size(int64_t n, int64_t nnz, size_t* bytes) {
+  MSC_REQUIRE(bytes && n >= 1 && nnz >= 0, "graph_moments: bad arguments");
+  Carve cv(nullptr, 0);
+  cv.take<double>(n + 1);
+  cv.take<double>(4);
+  cv.take<uint32_t>(nnz + 1); cv.take<uint32_t>(nnz + 1); cv.take<uint32_t>(nnz + 1); cv.take<uint32_t>(nnz + 1);
+  cv.take<int64_t>(n + 2);
+  cv.take<char>(msc::prims::radix_workspace(nnz + 1));
+  *bytes = cv.used + 256;
+  return kOk;
+}
+
+extern "C" int msc_graph_moments(const int64_t* indptr, const int32_t* indices, const double* w, int64_t n,
+                                 int64_t nnz, double* out_s /*host: s0, s1, s2*/, void* workspace,
+                                 size_t workspace_bytes, void* stream) {
+  MSC_REQUIRE(indptr && out_s && workspace && n >= 1 && (nnz == 0 || (indices && w)),
+              "graph_moments: bad arguments");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  Carve cv(workspace, workspace_bytes);
+  double* terms = cv.take<double>(n + 1);
+  double* red = cv.take<double>(4);
+  uint32_t* key = cv.take<uint32_t>(nnz + 1);
+  uint32_t* val = cv.take<uint32_t>(nnz + 1);
+  uint32_t* skey = cv.take<uint32_t>(nnz + 1);
+  uint32_t* sval = cv.take<uint32_t>(nnz + 1);
+  int64_t* cstart = cv.take<int64_t>(n + 2);
+  const size_t rsb = msc::prims::radix_workspace(nnz + 1);
+  void* rws = cv.take<char>(rsb);
+  if (!cv.ok()) {
+    set_error("graph_moments: workspace too small");
+    return kWorkspaceTooSmall;
+  }
+  const unsigned g = static_cast<unsigned>(ceil_div(n, kThreads));
+  double hs[3];
+  k_sum_f64<<<1, kThreads, 0, st>>>(w, nnz, red);
+  k_s1_terms<<<g, kThreads, 0, st>>>(indptr, indices, w, n, terms);
+  k_sum_f64<<<1, kThreads, 0, st>>>(terms, n, red + 1);
+  MSC_LAUNCH_CHECK();
+  k_edge_dst<<<g, kThreads, 0, st>>>(indptr, indices, n, key, val);
+  MSC_LAUNCH_CHECK();
+  MSC_TRY(msc::prims::radix_sort_pairs(key, val, skey, sval, nnz, msc::prims::bits_for(n), rws, rsb, st));
+  MSC_TRY(msc::prims::segment_starts(skey, nnz, n, cstart, st));
+  k_s2_terms<<<g, kThreads, 0, st>>>(indptr, w, cstart, sval, n, terms);
+  k_sum_f64<<<1, kThreads, 0, st>>>(terms, n, red + 2);
+  MSC_LAUNCH_CHECK();
+  MSC_HIP(hipMemcpyAsync(hs, red, 3 * sizeof(double), hipMemcpyDeviceToHost, st));
+  MSC_HIP(hipStreamSynchronize(st));
+  out_s[0] = hs[0];
+  out_s[1] = 0.5 * hs[1];
+  out_s[2] = hs[2];
   return kOk;
 }

@@ -65,6 +65,9 @@ SIGNATURES = {
     "msc_moran_workspace_size": (c_int, [c_i64, c_i64, c_i64, c_int, ctypes.POINTER(c_sz)]),
     "msc_moran": (c_int, [c_vp, c_i64, c_i64, c_vp, c_vp, c_vp, c_i64, c_int, c_int, c_vp, c_i64,
                           c_vp, c_vp, ctypes.POINTER(c_dbl), c_vp, c_sz, c_vp]),
+    "msc_csr_to_dense": (c_int, [c_vp, c_vp, c_vp, c_i64, c_i64, c_vp, c_vp]),
+    "msc_graph_moments_workspace_size": (c_int, [c_i64, c_i64, ctypes.POINTER(c_sz)]),
+    "msc_graph_moments": (c_int, [c_vp, c_vp, c_vp, c_i64, c_i64, c_vp, c_vp, c_sz, c_vp]),
     "msc_leiden_workspace_size": (c_int, [c_i64, c_i64, ctypes.POINTER(c_sz)]),
     "msc_leiden": (c_int, [c_vp, c_vp, c_vp, c_i64, c_i64, c_dbl, c_i64, c_int, c_dbl, c_vp, c_vp,
                            ctypes.POINTER(c_dbl), ctypes.POINTER(LeidenInfo), c_vp, c_sz, c_vp]),
@@ -280,3 +283,34 @@ def moran(X, w_indptr, w_indices, w_data, n_perms=0, perms=None, seed=0, transfo
                         None if pr is None else ptr(pr), int(seed), ptr(I), ptr(sims),
                         ctypes.byref(S0), ptr(ws), ws.numel(), stream_ptr()))
     return I, (sims[:P] if P else None), S0.value
+
+
+def csr_to_dense(indptr, indices, data, n_rows, n_cols):
+    """Dense float32 device matrix from a (host or device) CSR."""
+    import torch
+    lib = load_library()
+    ip = to_device(indptr, torch.int64)
+    ind = to_device(indices, torch.int32)
+    dat = to_device(data, torch.float32)
+    out = torch.empty((n_rows, n_cols), dtype=torch.float32, device=ip.device)
+    check(lib.msc_csr_to_dense(ptr(ip), ptr(ind), ptr(dat), n_rows, n_cols, ptr(out), stream_ptr()))
+    return out
+
+
+def graph_moments(w_indptr, w_indices, w_data):
+    """``(s0, s1, s2)`` of a (device or host) CSR weight matrix, computed on the device."""
+    import torch
+    lib = load_library()
+    ip = to_device(w_indptr, torch.int64)
+    ind = to_device(w_indices, torch.int32)
+    dat = to_device(w_data, torch.float64)
+    n = ip.numel() - 1
+    nnz = int(ind.numel())
+    nbytes = c_sz(0)
+    check(lib.msc_graph_moments_workspace_size(n, nnz, ctypes.byref(nbytes)))
+    ws = workspace(nbytes.value)
+    out = (c_dbl * 3)()
+    check(lib.msc_graph_moments(ptr(ip), ptr(ind), ptr(dat), n, nnz, ctypes.cast(out, c_vp),
+                                ptr(ws), ws.numel(),
+                                stream_ptr()))
+    return float(out[0]), float(out[1]), float(out[2])

@@ -114,9 +114,36 @@ def nb_counts(n, n_genes=500, density=0.05, theta=2.0, rng=None, seed=4):
         mid = np.sqrt(lo * hi)
         lo, hi = (mid, hi) if nz(mid) < density else (lo, mid)
     mu = base * np.sqrt(lo * hi)
-    p = theta / (theta + mu)
-    counts = rng.negative_binomial(theta, p[None, :], size=(n, n_genes)).astype(np.float32)
-    return sp.csr_matrix(counts, dtype=np.float32)
+    q = theta / (theta + mu)                       # NB "success" probability per gene
+    p_nz = 1.0 - q ** theta
+    rows_all, cols_all, vals_all = [], [], []
+    for g in range(n_genes):
+        # nonzero positions: geometric gaps (Bernoulli(p_nz) per cell), O(nnz)
+        est = int(n * p_nz[g] * 1.2) + 64
+        pos = np.cumsum(rng.geometric(p_nz[g], size=est)) - 1
+        while pos.size and pos[-1] < n:
+            more = np.cumsum(rng.geometric(p_nz[g], size=est)) + pos[-1]
+            pos = np.concatenate([pos, more])
+        pos = pos[pos < n]
+        # zero-truncated NB values by inverse CDF: P(k) = C(k+theta-1, k) q^theta (1-q)^k
+        kk = np.arange(1, 400)
+        logp = (np.log(q[g]) * theta + kk * np.log1p(-q[g])
+                + np.cumsum(np.log((kk + theta - 1) / kk)))
+        pk = np.exp(logp) / p_nz[g]
+        cdf = np.cumsum(pk)
+        cdf[-1] = max(cdf[-1], 1.0)
+        vals = 1 + np.searchsorted(cdf, rng.uniform(0.0, cdf[-1], pos.size))
+        rows_all.append(pos)
+        cols_all.append(np.full(pos.size, g, np.int32))
+        vals_all.append(vals.astype(np.float32))
+    rows = np.concatenate(rows_all)
+    cols = np.concatenate(cols_all)
+    vals = np.concatenate(vals_all)
+    out = sp.csr_matrix((vals, (rows, cols)), shape=(n, n_genes), dtype=np.float32)
+    out.sort_indices()
+    out.indices = out.indices.astype(np.int32)
+    out.indptr = out.indptr.astype(np.int64)
+    return out
 
 
 def spatial_job(n, n_genes=500, seed=4):
