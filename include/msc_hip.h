@@ -46,22 +46,35 @@ const char* msc_last_error(void);
  * out_idx n x k int32, out_dist n x k float32 (= sqrt of that squared distance).
  * precision: MSC_PRECISION_FP32 ranks candidates with FP32 MFMA; MSC_PRECISION_BF16_RERANK
  * ranks them with BF16 MFMA (FP32 accumulation).  Both re-rank 64 admitted candidates
- * exactly and certify each row; uncertified rows are recomputed by exact brute force, so
- * the two precisions return identical results.  Limits: 1 <= k <= 64, k <= n < 2^31,
+ * exactly and certify each row; BF16 rows that fail their certificate are re-selected with
+ * FP32 MFMA, and rows no pass certifies are recomputed by exact brute force, so the two
+ * precisions return identical results.  Limits: 1 <= k <= 64, k <= n < 2^31,
  * d <= 127 (FP32) / 125 (BF16).
  * ------------------------------------------------------------------------------------- */
 typedef struct {
   int timing;           /* in: nonzero -> record per-stage times (adds one stream sync) */
   int n_splits;         /* out: candidate splits used by the select kernel */
   int list_len;         /* out: admitted candidates per query */
+  int64_t n_second_pass;/* out: BF16 rows re-selected with FP32 MFMA (certificate failed) */
   int64_t n_fallback;   /* out: rows recomputed by exact brute force */
-  double prep_ms, select_ms, finalize_ms, fallback_ms; /* out (timing only) */
+  double prep_ms, select_ms, finalize_ms, second_pass_ms, fallback_ms; /* out (timing only) */
 } msc_knn_info;
 
 int msc_knn_workspace_size(int64_t n, int d, int k, int precision, size_t* bytes /*host*/);
 int msc_knn_exact(const float* X, int64_t n, int d, int k, int precision, int32_t* out_idx,
                   float* out_dist, void* workspace, size_t workspace_bytes, void* stream,
                   msc_knn_info* info /*host, may be NULL*/);
+
+/* Batched exact kNN over many independent samples in one select launch (each query scans
+ * only its own sample).  X: total x d rows, samples contiguous; offsets (host): n_samples+1
+ * row offsets.  out_idx / out_dist: total x k, column ids GLOBAL (offset of the sample
+ * included), so the result is the block-diagonal kNN of the concatenation.  Every sample
+ * needs >= k cells; d <= 125.  Replaces per-sample calls of neighbors._knn. */
+int msc_knn_batched_workspace_size(const int64_t* offsets /*host*/, int n_samples, int d, int k,
+                                   int precision, size_t* bytes /*host*/);
+int msc_knn_batched(const float* X, const int64_t* offsets /*host*/, int n_samples, int d, int k,
+                    int precision, int32_t* out_idx, float* out_dist, void* workspace,
+                    size_t workspace_bytes, void* stream, msc_knn_info* info /*host, may be NULL*/);
 
 /* obsp['distances'] as CSR from a self-inclusive kNN (neighbors.py:755-760): k-1 entries per
  * row (self column dropped), columns ascending, explicit zeros kept.

@@ -11,21 +11,30 @@
 //
 // Pipeline (all on one stream):
 //   1. colsum/colmean  float64 column means (deterministic two-level sum)
-//   2. prep            centred rows in the MFMA operand format, |x~| and the rounding
-//                      residual r = |x~ - (x - mu)| per point (float64 math)
+//   2. prep            centred rows in the MFMA operand format, norms and rounding
+//                      residuals per point (float64 math)
 //   3. select          per 128 queries: stream every candidate tile through LDS, MFMA
-//                      acc = |c~|^2 - 2 q~.c~ (the |c~|^2 term rides in the K padding),
+//                      acc = |c~|^2 - 2 q^.c~ (the |c~|^2 term rides in the K padding),
 //                      keep each lane's L smallest (acc, j) keys in registers
 //   4. finalize        merge the lane lists (and candidate splits), exact d2 for the 64
 //                      admitted candidates, sort by the exact key, certify that no
 //                      rejected candidate can beat the k-th, write idx/dist
-//   5. fallback        exact brute force over all n for uncertified rows
+//   5. second pass     rows that fail the certificate are selected again with a wide
+//                      list (16 queries per wave, 8 lists of L per query) and re-certified
+//   6. fallback        exact brute force over all n for rows still uncertified
+//
+// Representations.  q^ is the query side of the dot product, c~ the candidate side:
+//   FP32 form:  q^ = c~ = float32(x - mu)
+//   BF16 form:  q^ = hi(x - mu), c~ = hi + lo, where hi = bf16(x - mu) and
+//               lo = bf16((x - mu) - hi); the candidate is carried to ~16 bits, so the
+//               ranking error is set by the query's own rounding r_q = |q^ - (q - mu)|.
 //
 // Certificate: a rejected candidate c has acc(q,c) >= tau (tau = the smallest list-tail
-// value over the lanes that saw q).  With E bounding the float32 accumulation error of the
-// MFMA dot product, |q~ - c~|^2 >= tau + |q~|^2 - E, and by the triangle inequality
-// |q - c| >= sqrt(tau + |q~|^2 - E) - r_q - r_max.  A row is certified when its k-th exact
-// d2 is below that bound squared (less the float32 rounding of the exact sum).
+// value over the lists that saw q).  With E bounding the float32 accumulation error of the
+// MFMA dot product, |q^ - c~|^2 >= tau + |q^|^2 - E, and by the triangle inequality
+// |q - c| >= sqrt(tau + |q^|^2 - E) - r_q - r_c,max with r_c = |c~ - (c - mu)|.  A row is
+// certified when its k-th exact d2 is below that bound squared (less the float32 rounding
+// of the exact sum).
 #include <hip/hip_bf16.h>
 
 #include <cmath>
@@ -33,6 +42,7 @@
 
 #include "common.h"
 #include "msc_hip.h"
+#include "prims.h"
 
 namespace msc {
 namespace knn {
@@ -41,22 +51,24 @@ typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
 typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4;
 
 constexpr int kThreads = 256;       // 4 waves
-constexpr int kQueriesPerWave = 32; // 2 N-blocks of 16
-constexpr int kQueriesPerBlock = 128;
+constexpr int kQueriesPerBlock = 128; // 4 waves x 2 N-blocks of 16 queries (the wide form: 64)
 constexpr int kTile = 64;           // candidates per LDS tile = 4 M-blocks of 16
 constexpr int kListStride = 64;     // keys written per query per split
+constexpr int kHitCapMax = 12;      // per-lane LDS hit-buffer slots per list (at most)
+constexpr int kLdsHalfCU = 81920;   // LDS bytes per block that still lets two blocks share a CU
 constexpr uint64_t kEmpty = ~0ull;
 
 // ------------------------------------------------------------------------------------------
 // operand formats
 // ------------------------------------------------------------------------------------------
-// BF16: row of KB = 32*KS bf16: [x~_0 .. x~_{d-1}, cn_hi, cn_mid, cn_lo, 0...]
-//       cn = |x~|^2 split into three bf16 terms; padding rows carry cn_hi = +inf.
+// BF16: row of KB = 32*KS bf16: [hi_0 .. hi_{d-1}, lo_0 .. lo_{d-1}, cn_hi, cn_mid, cn_lo, 0...]
+//       cn = |hi + lo|^2 split into three bf16 terms; padding rows carry cn_hi = +inf.
+//       The query operand of the same row is [-2 hi, -2 hi, 1, 1, 1, 0...].
 // FP32: row of KF = 8*KS floats in fragment order: logical k = 4s + g (MFMA step s, lane
 //       group g) sits at position 8*(s/2) + 2g + (s%2), so a lane reads its two steps with
 //       one 8-byte load.  Logical layout [x~_0 .. x~_{d-1}, cn, 0...].
 
-__host__ __device__ inline int bf16_ks(int d) { return (d + 3 + 31) / 32; }
+__host__ __device__ inline int bf16_ks(int d) { return (2 * d + 3 + 31) / 32; }
 __host__ __device__ inline int f32_ks(int d) { return (d + 1 + 7) / 8; }
 __host__ __device__ inline int f32_pos(int k) {
   int s = k >> 2, g = k & 3;
@@ -99,39 +111,51 @@ __global__ void colmean_final(const double* __restrict__ partial, int nb, int d,
 // ------------------------------------------------------------------------------------------
 // 2. operand preparation
 // ------------------------------------------------------------------------------------------
+// nrm2/res: candidate side |c~|^2 and |c~ - x_c| (for the global maxima);
+// qn2/qres: query side |hi|^2 and |hi - x_c| (per-row certificate terms)
 __global__ void prep_bf16(const float* __restrict__ X, int64_t n, int64_t n_pad, int d, int KB,
                           const double* __restrict__ mean, uint16_t* __restrict__ Xb,
-                          float* __restrict__ nrm2, float* __restrict__ res) {
+                          float* __restrict__ nrm2, float* __restrict__ res, float* __restrict__ qn2,
+                          float* __restrict__ qres) {
   const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
   if (i >= n_pad) return;
   uint16_t* row = Xb + i * KB;
   if (i >= n) {
     for (int t = 0; t < KB; ++t) row[t] = 0;
-    row[d] = 0x7F80u;  // +inf: padding candidates never beat a threshold
+    row[2 * d] = 0x7F80u;  // +inf: padding candidates never beat a threshold
     nrm2[i] = 0.0f;
     res[i] = 0.0f;
+    qn2[i] = 0.0f;
+    qres[i] = 0.0f;
     return;
   }
-  double cn = 0.0, r2 = 0.0;
+  double cn = 0.0, r2 = 0.0, hn = 0.0, hr2 = 0.0;
   for (int t = 0; t < d; ++t) {
     const double xc = static_cast<double>(X[i * d + t]) - mean[t];
     const uint16_t h = f32_to_bf16_rne(static_cast<float>(xc));
-    const double xb = static_cast<double>(bf16_to_f32(h));
-    cn += xb * xb;
-    r2 += (xb - xc) * (xb - xc);
+    const double xh = static_cast<double>(bf16_to_f32(h));
+    const uint16_t lo = f32_to_bf16_rne(static_cast<float>(xc - xh));
+    const double xt = xh + static_cast<double>(bf16_to_f32(lo));
+    cn += xt * xt;
+    r2 += (xt - xc) * (xt - xc);
+    hn += xh * xh;
+    hr2 += (xh - xc) * (xh - xc);
     row[t] = h;
+    row[d + t] = lo;
   }
   const uint16_t hi = f32_to_bf16_rne(static_cast<float>(cn));
   const double rem1 = cn - static_cast<double>(bf16_to_f32(hi));
   const uint16_t mid = f32_to_bf16_rne(static_cast<float>(rem1));
   const double rem2 = rem1 - static_cast<double>(bf16_to_f32(mid));
-  const uint16_t lo = f32_to_bf16_rne(static_cast<float>(rem2));
-  row[d] = hi;
-  row[d + 1] = mid;
-  row[d + 2] = lo;
-  for (int t = d + 3; t < KB; ++t) row[t] = 0;
+  const uint16_t lo3 = f32_to_bf16_rne(static_cast<float>(rem2));
+  row[2 * d] = hi;
+  row[2 * d + 1] = mid;
+  row[2 * d + 2] = lo3;
+  for (int t = 2 * d + 3; t < KB; ++t) row[t] = 0;
   nrm2[i] = static_cast<float>(cn);
   res[i] = static_cast<float>(sqrt(r2));
+  qn2[i] = static_cast<float>(hn);
+  qres[i] = static_cast<float>(sqrt(hr2));
 }
 
 __global__ void prep_f32(const float* __restrict__ X, int64_t n, int64_t n_pad, int d, int KF,
@@ -219,40 +243,58 @@ struct SelCfg {
   static constexpr int NB_FRAG = BF16 ? KS : KS;             // B fragments per N-block
 };
 
-template <bool BF16, int KS, int L>
+// WIDE = false: each wave holds 32 queries (2 N-blocks), one list of L keys per lane and
+//   query; a query's 64 candidates of every tile spread over 4 lanes (lists) -> 4L keys.
+// WIDE = true: each wave holds 16 queries (1 N-block); a lane keeps two lists for its
+//   query, one for M-blocks 0-1 and one for 2-3 -> 8 lists of L per query, written as two
+//   virtual splits (2 * split + list), so the finalize step merges them like splits.
+template <bool BF16, int KS, int L, bool WIDE>
 __global__ __launch_bounds__(kThreads, 2) void select_kernel(
-    const void* __restrict__ Xp, int64_t n, int64_t n_pad, int d, int64_t split_len,
+    const void* __restrict__ Xp, int64_t n, int64_t n_slots, int d, int64_t split_len,
+    const int32_t* __restrict__ qrows, const int64_t* __restrict__ blk_range,
     uint64_t* __restrict__ keys_out, float* __restrict__ tau_out) {
   using C = SelCfg<BF16, KS>;
+  constexpr int NB = WIDE ? 1 : 2;         // N-blocks (of 16 queries) per wave
+  constexpr int QPB = 4 * 16 * NB;         // queries per block
+  // hit-buffer slots per lane and list: as many as keep two blocks per CU (>= 8), else the max
+  constexpr int kFit = (kLdsHalfCU - 2 * kTile * C::STRIDE) / ((kThreads / kWave) * 2 * kWave * 8);
+  constexpr int kHitCap = (kFit >= 8) ? (kFit < kHitCapMax ? kFit : kHitCapMax) : kHitCapMax;
   __shared__ __attribute__((aligned(16))) char tile[2][kTile * C::STRIDE];
+  // per-lane hit buffers: [wave][nb][slot][lane], lane-minor (conflict-free 8-byte access)
+  __shared__ uint64_t hitbuf[kThreads / kWave][2][kHitCap][kWave];
 
   const int l = lane_id();
   const int wv = threadIdx.x / kWave;
   const int g = l >> 4;     // lane group 0..3
   const int lm = l & 15;    // row within an MFMA block
   const int split = blockIdx.y;
-  const int64_t c_begin = static_cast<int64_t>(split) * split_len;
-  const int64_t c_end = min(n, c_begin + split_len);
-  const int64_t qbase = static_cast<int64_t>(blockIdx.x) * kQueriesPerBlock + wv * kQueriesPerWave;
+  // candidate range: one split of [0, n), or the block's own range (batched samples)
+  const int64_t c_begin = blk_range ? blk_range[2 * blockIdx.x] : static_cast<int64_t>(split) * split_len;
+  const int64_t c_end = blk_range ? blk_range[2 * blockIdx.x + 1] : min(n, c_begin + split_len);
+  // query slots: slot s is row s, or row qrows[s] when a row list is given
+  const int64_t sbase = static_cast<int64_t>(blockIdx.x) * QPB + wv * (16 * NB);
   const char* Xb = static_cast<const char*>(Xp);
 
   // ---- query operand fragments, resident in registers for the whole stream ----
-  bf16x8 qb[2][BF16 ? KS : 1];
-  float qf[2][BF16 ? 1 : 2 * KS];
+  bf16x8 qb[NB][BF16 ? KS : 1];
+  float qf[NB][BF16 ? 1 : 2 * KS];
 #pragma unroll
-  for (int nb = 0; nb < 2; ++nb) {
-    const int64_t q = qbase + 16 * nb + lm;
+  for (int nb = 0; nb < NB; ++nb) {
+    const int64_t slot = sbase + 16 * nb + lm;
+    const int64_t q = qrows ? static_cast<int64_t>(qrows[slot]) : slot;
     const char* qrow = Xb + q * C::ROWB;
     if constexpr (BF16) {
+      // [-2 hi, -2 hi, 1, 1, 1, 0 ...] against the candidate row [hi, lo, cn_hi, cn_mid, cn_lo, 0 ...]
+      const uint16_t* src = reinterpret_cast<const uint16_t*>(qrow);
 #pragma unroll
       for (int s = 0; s < KS; ++s) {
-        const uint16_t* src = reinterpret_cast<const uint16_t*>(qrow) + 32 * s + 8 * g;
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
           const int k = 32 * s + 8 * g + e;
           float v;
-          if (k < d) v = -2.0f * bf16_to_f32(src[e]);
-          else if (k < d + 3) v = 1.0f;
+          if (k < d) v = -2.0f * bf16_to_f32(src[k]);
+          else if (k < 2 * d) v = -2.0f * bf16_to_f32(src[k - d]);
+          else if (k < 2 * d + 3) v = 1.0f;
           else v = 0.0f;
           qb[nb][s][e] = static_cast<__bf16>(v);
         }
@@ -273,6 +315,7 @@ __global__ __launch_bounds__(kThreads, 2) void select_kernel(
 
   uint64_t lst[2][L];
   float tau[2];
+  int cnt[2] = {0, 0};
 #pragma unroll
   for (int nb = 0; nb < 2; ++nb) {
 #pragma unroll
@@ -282,51 +325,63 @@ __global__ __launch_bounds__(kThreads, 2) void select_kernel(
 
   const int64_t ntiles = (c_end > c_begin) ? ceil_div(c_end - c_begin, kTile) : 0;
 
-  // prefetch registers for one tile
-  typedef typename std::conditional<BF16, uint4, uint2>::type chunk_t;
-  chunk_t pf[C::PER_THREAD];
-  auto load_tile = [&](int64_t t) {
-    const char* base = Xb + (c_begin + t * kTile) * C::ROWB;
-#pragma unroll
-    for (int p = 0; p < C::PER_THREAD; ++p) {
-      const int c = threadIdx.x + p * kThreads;
-      if (c < C::NCHUNK) pf[p] = *reinterpret_cast<const chunk_t*>(base + static_cast<int64_t>(c) * C::CHUNK);
-    }
-  };
-  auto store_tile = [&](int buf) {
-#pragma unroll
-    for (int p = 0; p < C::PER_THREAD; ++p) {
-      const int c = threadIdx.x + p * kThreads;
-      if (c < C::NCHUNK) {
-        const int row = c / (C::ROWB / C::CHUNK);
-        const int col = c % (C::ROWB / C::CHUNK);
-        *reinterpret_cast<chunk_t*>(&tile[buf][row * C::STRIDE + col * C::CHUNK]) = pf[p];
-      }
-    }
-  };
+  // Prefetch registers for one tile: up to eight named 16-byte registers chosen at compile
+  // time (an array here gets demoted to LDS or scratch by the compiler).  Global reads are
+  // 16-byte chunks for both precisions; FP32 rows are stored to LDS as two 8-byte halves
+  // because their padded LDS stride is only 8-byte aligned.
+  constexpr int kRowChunks = C::ROWB / 16;
+  constexpr int kNChunk = kTile * kRowChunks;
+  constexpr int kPer = (kNChunk + kThreads - 1) / kThreads;
+  static_assert(kPer <= 8, "tile prefetch needs more than 8 registers");
+  uint4 pf0{}, pf1{}, pf2{}, pf3{}, pf4{}, pf5{}, pf6{}, pf7{};
+#define MSC_PF(i) (i == 0 ? pf0 : i == 1 ? pf1 : i == 2 ? pf2 : i == 3 ? pf3 : i == 4 ? pf4 : i == 5 ? pf5 : i == 6 ? pf6 : pf7)
+#define MSC_LOAD1(i, T)                                                                         \
+  if constexpr (kPer > i) {                                                                     \
+    const int c_ = threadIdx.x + i * kThreads;                                                  \
+    if (kNChunk % kThreads == 0 || c_ < kNChunk)                                                \
+      MSC_PF(i) = *reinterpret_cast<const uint4*>(Xb + (c_begin + (T) * kTile) * C::ROWB + static_cast<int64_t>(c_) * 16); \
+  }
+#define MSC_STORE1(i, B)                                                                        \
+  if constexpr (kPer > i) {                                                                     \
+    const int c_ = threadIdx.x + i * kThreads;                                                  \
+    if (kNChunk % kThreads == 0 || c_ < kNChunk) {                                              \
+      char* dst_ = &tile[(B)][(c_ / kRowChunks) * C::STRIDE + (c_ % kRowChunks) * 16];           \
+      const uint4 v_ = MSC_PF(i);                                                               \
+      if constexpr (BF16) {                                                                     \
+        *reinterpret_cast<uint4*>(dst_) = v_;                                                   \
+      } else {                                                                                  \
+        reinterpret_cast<uint2*>(dst_)[0] = make_uint2(v_.x, v_.y);                             \
+        reinterpret_cast<uint2*>(dst_)[1] = make_uint2(v_.z, v_.w);                             \
+      }                                                                                         \
+    }                                                                                           \
+  }
+#define MSC_LOAD_TILE(T) { MSC_LOAD1(0, T) MSC_LOAD1(1, T) MSC_LOAD1(2, T) MSC_LOAD1(3, T) \
+                           MSC_LOAD1(4, T) MSC_LOAD1(5, T) MSC_LOAD1(6, T) MSC_LOAD1(7, T) }
+#define MSC_STORE_TILE(B) { MSC_STORE1(0, B) MSC_STORE1(1, B) MSC_STORE1(2, B) MSC_STORE1(3, B) \
+                            MSC_STORE1(4, B) MSC_STORE1(5, B) MSC_STORE1(6, B) MSC_STORE1(7, B) }
 
   if (ntiles > 0) {
-    load_tile(0);
-    store_tile(0);
+    MSC_LOAD_TILE(0);
+    MSC_STORE_TILE(0);
   }
   __syncthreads();
 
   for (int64_t t = 0; t < ntiles; ++t) {
     const int buf = static_cast<int>(t & 1);
-    if (t + 1 < ntiles) load_tile(t + 1);
+    if (t + 1 < ntiles) MSC_LOAD_TILE(t + 1);
 
-    f32x4 acc[4][2];
+    f32x4 acc[4][NB];
 #pragma unroll
     for (int mb = 0; mb < 4; ++mb) {
 #pragma unroll
-      for (int nb = 0; nb < 2; ++nb) acc[mb][nb] = f32x4{0.f, 0.f, 0.f, 0.f};
+      for (int nb = 0; nb < NB; ++nb) acc[mb][nb] = f32x4{0.f, 0.f, 0.f, 0.f};
       const char* arow = &tile[buf][(16 * mb + lm) * C::STRIDE];
       if constexpr (BF16) {
 #pragma unroll
         for (int s = 0; s < KS; ++s) {
           const bf16x8 a = *reinterpret_cast<const bf16x8*>(arow + 64 * s + 16 * g);
 #pragma unroll
-          for (int nb = 0; nb < 2; ++nb)
+          for (int nb = 0; nb < NB; ++nb)
             acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, qb[nb][s], acc[mb][nb], 0, 0, 0);
         }
       } else {
@@ -334,7 +389,7 @@ __global__ __launch_bounds__(kThreads, 2) void select_kernel(
         for (int h = 0; h < KS; ++h) {
           const float2 a2 = *reinterpret_cast<const float2*>(arow + 32 * h + 8 * g);
 #pragma unroll
-          for (int nb = 0; nb < 2; ++nb) {
+          for (int nb = 0; nb < NB; ++nb) {
             acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a2.x, qf[nb][2 * h], acc[mb][nb], 0, 0, 0);
             acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a2.y, qf[nb][2 * h + 1], acc[mb][nb], 0, 0, 0);
           }
@@ -343,53 +398,81 @@ __global__ __launch_bounds__(kThreads, 2) void select_kernel(
     }
 
     // ---- fused selection ----
-    bool hit = false;
+    // Hits (acc <= the lane's list tail) are appended to the lane's LDS buffer; buffers are
+    // merged into the sorted register lists when any lane's buffer could overflow on the
+    // next M-block, so a rare hit costs one LDS store instead of a wave-wide insertion.
+    // Each (M-block, N-block) group is gated by a wave-uniform test: after the first few
+    // hundred tiles almost every group has no hit and costs four compares.
+    const int64_t c0 = c_begin + t * kTile;
+    const bool tail_tile = (t + 1 == ntiles);
 #pragma unroll
-    for (int mb = 0; mb < 4; ++mb)
+    for (int nb = 0; nb < NB; ++nb) {
 #pragma unroll
-      for (int nb = 0; nb < 2; ++nb)
+      for (int mb = 0; mb < 4; ++mb) {
+        const int li = WIDE ? (mb >> 1) : nb;  // list receiving this group
+        bool h = false;
 #pragma unroll
-        for (int i = 0; i < 4; ++i) hit |= acc[mb][nb][i] <= tau[nb];
-    if (__any(hit)) {
-      const int64_t c0 = c_begin + t * kTile;
+        for (int i = 0; i < 4; ++i) h |= acc[mb][nb][i] <= tau[li];
+        if (__builtin_amdgcn_ballot_w64(h) == 0ull) continue;
 #pragma unroll
-      for (int mb = 0; mb < 4; ++mb)
-#pragma unroll
-        for (int nb = 0; nb < 2; ++nb)
-#pragma unroll
-          for (int i = 0; i < 4; ++i) {
-            const float v = acc[mb][nb][i];
-            if (v <= tau[nb]) {
-              const int64_t c = c0 + 16 * mb + 4 * g + i;
-              const uint64_t key = (static_cast<uint64_t>(float_ord(v)) << 32) | static_cast<uint64_t>(c);
-              if (c < c_end && key < lst[nb][L - 1]) {
-                list_insert<L>(lst[nb], key);
-                tau[nb] = tail_value(lst[nb][L - 1]);
-              }
-            }
+        for (int i = 0; i < 4; ++i) {
+          const float v = acc[mb][nb][i];
+          const int64_t c = c0 + 16 * mb + 4 * g + i;
+          if (v <= tau[li] && (!tail_tile || c < c_end)) {
+            hitbuf[wv][li][cnt[li]][l] = (static_cast<uint64_t>(float_ord(v)) << 32) | static_cast<uint64_t>(c);
+            ++cnt[li];
           }
+        }
+        // an M-block adds at most 4 hits per lane: flush before a buffer could overflow
+        if (__builtin_amdgcn_ballot_w64(cnt[li] > kHitCap - 4) != 0ull) {
+          for (int j = 0; j < cnt[li]; ++j) {
+            const uint64_t key = hitbuf[wv][li][j][l];
+            if (key < lst[li][L - 1]) list_insert<L>(lst[li], key);
+          }
+          cnt[li] = 0;
+          tau[li] = tail_value(lst[li][L - 1]);
+        }
+      }
     }
 
+    // Compiler fence: keep the LDS store of the prefetched tile here, after this tile's
+    // compute, so the global-load latency of tile t+1 hides behind it (without the fence the
+    // store is hoisted next to its load and every tile waits on memory).
+    asm volatile("" ::: "memory");
     // buf^1 was last read in iteration t-1, which every wave finished before the barrier
     // that ended it; one barrier per tile publishes the new tile.
-    if (t + 1 < ntiles) store_tile(buf ^ 1);
+    if (t + 1 < ntiles) MSC_STORE_TILE(buf ^ 1);
     __syncthreads();
   }
+#undef MSC_LOAD_TILE
+#undef MSC_STORE_TILE
+#undef MSC_LOAD1
+#undef MSC_STORE1
+#undef MSC_PF
 
-  // ---- write the lane lists and the per-query threshold ----
+  // ---- drain the hit buffers, write the lane lists and the per-query threshold ----
 #pragma unroll
   for (int nb = 0; nb < 2; ++nb) {
-    const int64_t q = qbase + 16 * nb + lm;
-    uint64_t* out = keys_out + (static_cast<int64_t>(split) * n_pad + q) * kListStride;
+    for (int j = 0; j < cnt[nb]; ++j) {
+      const uint64_t key = hitbuf[wv][nb][j][l];
+      if (key < lst[nb][L - 1]) list_insert<L>(lst[nb], key);
+    }
+    tau[nb] = tail_value(lst[nb][L - 1]);
+  }
 #pragma unroll
-    for (int i = 0; i < L; ++i) out[g * L + i] = lst[nb][i];
+  for (int li = 0; li < 2; ++li) {
+    const int64_t slot = sbase + (WIDE ? 0 : 16 * li) + lm;
+    const int64_t vsplit = WIDE ? 2 * static_cast<int64_t>(split) + li : split;
+    uint64_t* out = keys_out + (vsplit * n_slots + slot) * kListStride;
+#pragma unroll
+    for (int i = 0; i < L; ++i) out[g * L + i] = lst[li][i];
     if (4 * L < kListStride) {
       for (int i = 4 * L + g; i < kListStride; i += 4) out[i] = kEmpty;
     }
-    float tv = tau[nb];
+    float tv = tau[li];
     tv = fminf(tv, __shfl_xor(tv, 16, kWave));
     tv = fminf(tv, __shfl_xor(tv, 32, kWave));
-    if (g == 0) tau_out[static_cast<int64_t>(split) * n_pad + q] = tv;
+    if (g == 0) tau_out[vsplit * n_slots + slot] = tv;
   }
 }
 
@@ -425,21 +508,26 @@ struct Bound {
 };
 
 __global__ __launch_bounds__(kThreads) void finalize_kernel(
-    const float* __restrict__ X, int64_t n, int d, int k, int S, int64_t n_pad,
-    const uint64_t* __restrict__ keys, const float* __restrict__ taus,
+    const float* __restrict__ X, int64_t nq, int d, int k, int S, int64_t n_slots,
+    const int32_t* __restrict__ qrows, const uint64_t* __restrict__ keys, const float* __restrict__ taus,
     const float* __restrict__ nrm2, const float* __restrict__ res, const float* __restrict__ gstat,
-    Bound bnd, int32_t* __restrict__ out_idx, float* __restrict__ out_dist,
-    int32_t* __restrict__ flags) {
+    Bound bnd, const int64_t* __restrict__ row_range, int32_t* __restrict__ out_idx,
+    float* __restrict__ out_dist, int32_t* __restrict__ flags) {
   const int l = lane_id();
-  const int64_t q = static_cast<int64_t>(blockIdx.x) * (kThreads / kWave) + threadIdx.x / kWave;
-  if (q >= n) return;
+  const int64_t slot = static_cast<int64_t>(blockIdx.x) * (kThreads / kWave) + threadIdx.x / kWave;
+  if (slot >= nq) return;
+  const int64_t q = qrows ? static_cast<int64_t>(qrows[slot]) : slot;
+  if (row_range && row_range[2 * q + 1] == 0) {  // padding row of a batched layout
+    if (l == 0) flags[q] = 0;
+    return;
+  }
 
-  uint64_t cur = wave_sort_u64(keys[q * kListStride + l]);
-  float tau = taus[q];
+  uint64_t cur = wave_sort_u64(keys[slot * kListStride + l]);
+  float tau = taus[slot];
   for (int s = 1; s < S; ++s) {
-    const uint64_t nk = wave_sort_u64(keys[(static_cast<int64_t>(s) * n_pad + q) * kListStride + l]);
+    const uint64_t nk = wave_sort_u64(keys[(static_cast<int64_t>(s) * n_slots + slot) * kListStride + l]);
     cur = wave_merge_min_u64(cur, nk);
-    tau = fminf(tau, taus[static_cast<int64_t>(s) * n_pad + q]);
+    tau = fminf(tau, taus[static_cast<int64_t>(s) * n_slots + slot]);
   }
   if (S > 1) tau = fminf(tau, tail_value(shfl_u64(cur, kWave - 1)));
 
@@ -459,7 +547,8 @@ __global__ __launch_bounds__(kThreads) void finalize_kernel(
     const double nq2 = static_cast<double>(nrm2[q]);
     const double nmax2 = static_cast<double>(gstat[0]);
     const double nq = sqrt(nq2), nmax = sqrt(nmax2);
-    const double E = bnd.gam * (2.0 * nq * nmax + nmax2) + bnd.split_rel * nmax2 +
+    // sum |terms| <= |c~|^2 + 2 |q^| (|hi_c| + |lo_c|) <= nmax^2 + 2.02 |q^| nmax
+    const double E = bnd.gam * (2.02 * nq * nmax + nmax2) + bnd.split_rel * nmax2 +
                      4.0e-7 * (nq2 + nmax2);
     const double lb2 = static_cast<double>(tau) + nq2 - E;
     const double lb = (lb2 > 0.0 ? sqrt(lb2) : 0.0) -
@@ -484,14 +573,16 @@ __global__ __launch_bounds__(kThreads) void finalize_kernel(
 
 __global__ __launch_bounds__(kThreads) void fallback_kernel(
     const float* __restrict__ X, int64_t n, int d, int k, const int32_t* __restrict__ flags,
-    int32_t* __restrict__ out_idx, float* __restrict__ out_dist) {
+    const int64_t* __restrict__ row_range, int32_t* __restrict__ out_idx, float* __restrict__ out_dist) {
   const int l = lane_id();
   const int64_t q = static_cast<int64_t>(blockIdx.x) * (kThreads / kWave) + threadIdx.x / kWave;
   if (q >= n || flags[q] == 0) return;
+  const int64_t cb = row_range ? row_range[2 * q] : 0;
+  const int64_t ce = row_range ? row_range[2 * q + 1] : n;
   uint64_t cur = kEmpty, tail = kEmpty;
-  for (int64_t base = 0; base < n; base += kWave) {
+  for (int64_t base = cb; base < ce; base += kWave) {
     const int64_t j = base + l;
-    const uint64_t ek = (j < n) ? exact_key(q, j, exact_d2(X, d, q, j)) : kEmpty;
+    const uint64_t ek = (j < ce) ? exact_key(q, j, exact_d2(X, d, q, j)) : kEmpty;
     if (__any(ek < tail)) {
       cur = wave_merge_min_u64(cur, wave_sort_u64(ek));
       tail = shfl_u64(cur, kWave - 1);
@@ -545,10 +636,24 @@ __global__ __launch_bounds__(kThreads) void distances_csr_kernel(
 // ------------------------------------------------------------------------------------------
 // host orchestration
 // ------------------------------------------------------------------------------------------
+constexpr int kL = 16;              // per-lane list length: 4 lanes x 16 = 64 admitted candidates
+
+// compact flagged rows into a list (deterministic order: ascending row)
+__global__ void k_flag_rows(const int32_t* __restrict__ flags, const int32_t* __restrict__ pos, int64_t n,
+                            int32_t* __restrict__ rows) {
+  const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (i < n && flags[i]) rows[pos[i]] = static_cast<int32_t>(i);
+}
+// pad the row list to a whole number of query blocks by repeating its last row
+__global__ void k_pad_rows(int32_t* __restrict__ rows, int64_t m, int64_t m_pad) {
+  const int64_t i = m + blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (i < m_pad) rows[i] = rows[m - 1];
+}
+
 struct Plan {
   int64_t n, n_pad, split_len;
-  int d, k, S, KS, L, precision;
-  size_t elem_bytes_row;
+  int d, k, S, precision;
+  int ks_bf16, ks_f32;
 };
 
 static Plan make_plan(int64_t n, int d, int k, int precision) {
@@ -558,9 +663,8 @@ static Plan make_plan(int64_t n, int d, int k, int precision) {
   p.k = k;
   p.precision = precision;
   p.n_pad = ceil_div(n, kQueriesPerBlock) * kQueriesPerBlock;
-  p.L = (k <= 15) ? 8 : 16;
-  p.KS = precision == MSC_PRECISION_BF16_RERANK ? bf16_ks(d) : f32_ks(d);
-  p.elem_bytes_row = precision == MSC_PRECISION_BF16_RERANK ? 64 * p.KS : 32 * p.KS;
+  p.ks_bf16 = bf16_ks(d);
+  p.ks_f32 = f32_ks(d);
   const int64_t qblocks = p.n_pad / kQueriesPerBlock;
   // split candidates so that the grid holds >= ~4 blocks per CU, each split >= 4096 rows
   int64_t S = ceil_div(1024, qblocks);
@@ -574,78 +678,126 @@ static Plan make_plan(int64_t n, int d, int k, int precision) {
 struct Buffers {
   double* partial;
   double* mean;
-  void* Xp;
-  float* nrm2;
-  float* res;
-  float* gstat;
+  void* Xb;        // BF16 operands (BF16 form only)
+  void* Xf;        // FP32 operands
+  float* nrm2b;    // BF16 candidate side |c~|^2, |c~ - x_c|
+  float* resb;
+  float* qn2b;     // BF16 query side |hi|^2, |hi - x_c|
+  float* qresb;
+  float* nrm2f;
+  float* resf;
+  float* gstatb;
+  float* gstatf;
   uint64_t* keys;
   float* taus;
   int32_t* flags;
+  int32_t* pos;
+  int32_t* rows;
   int64_t* nfall;
+  void* scan_ws;
+  size_t scan_ws_bytes;
 };
 
 static constexpr int kColBlocks = 256;
 
+static size_t second_pass_slots(int64_t n_pad) {
+  return 2 * static_cast<size_t>(n_pad) + 2048 * static_cast<size_t>(kQueriesPerBlock);
+}
+
 static Buffers carve(Carve& cv, const Plan& p) {
   Buffers b{};
+  const bool bf = p.precision == MSC_PRECISION_BF16_RERANK;
   b.partial = cv.take<double>(static_cast<size_t>(kColBlocks) * p.d);
   b.mean = cv.take<double>(p.d);
-  b.Xp = cv.take<char>(static_cast<size_t>(p.n_pad) * p.elem_bytes_row);
-  b.nrm2 = cv.take<float>(p.n_pad);
-  b.res = cv.take<float>(p.n_pad);
-  b.gstat = cv.take<float>(4);
-  b.keys = cv.take<uint64_t>(static_cast<size_t>(p.S) * p.n_pad * kListStride);
-  b.taus = cv.take<float>(static_cast<size_t>(p.S) * p.n_pad);
-  b.flags = cv.take<int32_t>(p.n_pad);
+  b.Xb = bf ? cv.take<char>(static_cast<size_t>(p.n_pad) * 64 * p.ks_bf16) : nullptr;
+  b.Xf = bf ? nullptr : cv.take<char>(static_cast<size_t>(p.n_pad) * 32 * p.ks_f32);
+  b.nrm2b = cv.take<float>(p.n_pad);
+  b.resb = cv.take<float>(p.n_pad);
+  b.qn2b = cv.take<float>(p.n_pad);
+  b.qresb = cv.take<float>(p.n_pad);
+  b.nrm2f = cv.take<float>(p.n_pad);
+  b.resf = cv.take<float>(p.n_pad);
+  b.gstatb = cv.take<float>(4);
+  b.gstatf = cv.take<float>(4);
+  // the wide second pass over m rows writes 2 * S2 * m_pad <= second_pass_slots(n_pad) slots
+  const size_t slots = std::max<size_t>(static_cast<size_t>(p.S) * p.n_pad, second_pass_slots(p.n_pad));
+  b.keys = cv.take<uint64_t>(slots * kListStride);
+  b.taus = cv.take<float>(slots);
+  b.flags = cv.take<int32_t>(p.n_pad + 1);
+  b.pos = cv.take<int32_t>(p.n_pad + 1);
+  b.rows = cv.take<int32_t>(p.n_pad + kQueriesPerBlock);
   b.nfall = cv.take<int64_t>(1);
+  b.scan_ws_bytes = prims::scan_workspace<int32_t>(p.n_pad + 1);
+  b.scan_ws = cv.take<char>(b.scan_ws_bytes);
   return b;
 }
 
-template <bool BF16, int KS>
-static void launch_select_L(const Plan& p, const Buffers& b, hipStream_t st) {
-  dim3 grid(static_cast<unsigned>(p.n_pad / kQueriesPerBlock), static_cast<unsigned>(p.S));
-  if (p.L == 8)
-    select_kernel<BF16, KS, 8><<<grid, kThreads, 0, st>>>(b.Xp, p.n, p.n_pad, p.d, p.split_len, b.keys, b.taus);
-  else
-    select_kernel<BF16, KS, 16><<<grid, kThreads, 0, st>>>(b.Xp, p.n, p.n_pad, p.d, p.split_len, b.keys, b.taus);
+template <bool BF16, int KS, bool WIDE>
+static void launch_select_t(const void* Xp, int64_t n, int64_t n_slots, int d, int64_t split_len, int S,
+                            const int32_t* qrows, const int64_t* blk_range, uint64_t* keys, float* taus,
+                            hipStream_t st) {
+  const int64_t qpb = WIDE ? kQueriesPerBlock / 2 : kQueriesPerBlock;
+  dim3 grid(static_cast<unsigned>(n_slots / qpb), static_cast<unsigned>(S));
+  select_kernel<BF16, KS, kL, WIDE><<<grid, kThreads, 0, st>>>(Xp, n, n_slots, d, split_len, qrows, blk_range, keys,
+                                                               taus);
 }
 
-static int launch_select(const Plan& p, const Buffers& b, hipStream_t st) {
-  if (p.precision == MSC_PRECISION_BF16_RERANK) {
-    switch (p.KS) {
-      case 1: launch_select_L<true, 1>(p, b, st); break;
-      case 2: launch_select_L<true, 2>(p, b, st); break;
-      case 3: launch_select_L<true, 3>(p, b, st); break;
-      case 4: launch_select_L<true, 4>(p, b, st); break;
-      default: set_error("knn: d=%d too large for the BF16 form (max 125)", p.d); return kUnsupported;
+// n_slots must be a multiple of kQueriesPerBlock; the wide form writes 2 * S key sets
+static int launch_select(bool bf16, bool wide, int ks, const void* Xp, int64_t n, int64_t n_slots, int d,
+                         int64_t split_len, int S, const int32_t* qrows, const int64_t* blk_range, uint64_t* keys,
+                         float* taus, hipStream_t st) {
+#define MSC_SEL(B, K)                                                                                         \
+  (wide ? launch_select_t<B, K, true>(Xp, n, n_slots, d, split_len, S, qrows, blk_range, keys, taus, st)     \
+        : launch_select_t<B, K, false>(Xp, n, n_slots, d, split_len, S, qrows, blk_range, keys, taus, st))
+  if (bf16) {
+    switch (ks) {
+      case 1: MSC_SEL(true, 1); break;
+      case 2: MSC_SEL(true, 2); break;
+      case 3: MSC_SEL(true, 3); break;
+      case 4: MSC_SEL(true, 4); break;
+      case 5: MSC_SEL(true, 5); break;
+      case 6: MSC_SEL(true, 6); break;
+      case 7: MSC_SEL(true, 7); break;
+      case 8: MSC_SEL(true, 8); break;
+      default: set_error("knn: d=%d too large for the BF16 form (max 125)", d); return kUnsupported;
     }
   } else {
-    switch (p.KS) {
-      case 1: launch_select_L<false, 1>(p, b, st); break;
-      case 2: launch_select_L<false, 2>(p, b, st); break;
-      case 3: launch_select_L<false, 3>(p, b, st); break;
-      case 4: launch_select_L<false, 4>(p, b, st); break;
-      case 5: launch_select_L<false, 5>(p, b, st); break;
-      case 6: launch_select_L<false, 6>(p, b, st); break;
-      case 7: launch_select_L<false, 7>(p, b, st); break;
-      case 8: launch_select_L<false, 8>(p, b, st); break;
-      case 9: launch_select_L<false, 9>(p, b, st); break;
-      case 10: launch_select_L<false, 10>(p, b, st); break;
-      case 11: launch_select_L<false, 11>(p, b, st); break;
-      case 12: launch_select_L<false, 12>(p, b, st); break;
-      case 13: launch_select_L<false, 13>(p, b, st); break;
-      case 14: launch_select_L<false, 14>(p, b, st); break;
-      case 15: launch_select_L<false, 15>(p, b, st); break;
-      case 16: launch_select_L<false, 16>(p, b, st); break;
-      default: set_error("knn: d=%d too large for the FP32 form (max 127)", p.d); return kUnsupported;
+    switch (ks) {
+      case 1: MSC_SEL(false, 1); break;
+      case 2: MSC_SEL(false, 2); break;
+      case 3: MSC_SEL(false, 3); break;
+      case 4: MSC_SEL(false, 4); break;
+      case 5: MSC_SEL(false, 5); break;
+      case 6: MSC_SEL(false, 6); break;
+      case 7: MSC_SEL(false, 7); break;
+      case 8: MSC_SEL(false, 8); break;
+      case 9: MSC_SEL(false, 9); break;
+      case 10: MSC_SEL(false, 10); break;
+      case 11: MSC_SEL(false, 11); break;
+      case 12: MSC_SEL(false, 12); break;
+      case 13: MSC_SEL(false, 13); break;
+      case 14: MSC_SEL(false, 14); break;
+      case 15: MSC_SEL(false, 15); break;
+      case 16: MSC_SEL(false, 16); break;
+      default: set_error("knn: d=%d too large for the FP32 form (max 127)", d); return kUnsupported;
     }
   }
+#undef MSC_SEL
   MSC_LAUNCH_CHECK();
   return kOk;
 }
 
+static Bound make_bound(bool bf16, int ks) {
+  Bound bnd;
+  const double u = 5.9604644775390625e-08;  // 2^-24
+  const int terms = bf16 ? 32 * ks : 2 * 8 * ks;
+  bnd.gam = terms * u / (1.0 - terms * u);
+  bnd.split_rel = bf16 ? 1.0 / (1 << 26) : 2.0 * u;
+  return bnd;
+}
+
 static int validate(int64_t n, int d, int k, int precision) {
-  MSC_REQUIRE(n >= 1 && n < (int64_t(1) << 31), "knn: n=%lld out of range", (long long)n);
+  MSC_REQUIRE(n >= 1 && n < (int64_t(1) << 31) - kQueriesPerBlock, "knn: n=%lld out of range", (long long)n);
   MSC_REQUIRE(d >= 1, "knn: d must be >= 1");
   MSC_REQUIRE(k >= 1 && k <= 64, "knn: k=%d must be in [1, 64]", k);
   MSC_REQUIRE(k <= n, "knn: k=%d exceeds n=%lld", k, (long long)n);
@@ -654,6 +806,54 @@ static int validate(int64_t n, int d, int k, int precision) {
   MSC_REQUIRE(precision == MSC_PRECISION_FP32 ? d <= 127 : d <= 125,
               "knn: d=%d unsupported for this precision", d);
   return kOk;
+}
+
+// flags -> compacted row list padded to whole query blocks; returns the row count
+static int compact_flags(const Plan& p, const Buffers& b, int64_t* m_out, hipStream_t st) {
+  MSC_TRY(prims::exclusive_scan<int32_t>(b.flags, b.pos, p.n + 1, b.scan_ws, b.scan_ws_bytes, st));
+  int32_t m = 0;
+  MSC_HIP(hipMemcpyAsync(&m, b.pos + p.n, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  MSC_HIP(hipStreamSynchronize(st));
+  if (m > 0) {
+    k_flag_rows<<<static_cast<unsigned>(ceil_div(p.n, kThreads)), kThreads, 0, st>>>(b.flags, b.pos, p.n, b.rows);
+    const int64_t m_pad = ceil_div(m, kQueriesPerBlock) * kQueriesPerBlock;
+    if (m_pad > m)
+      k_pad_rows<<<static_cast<unsigned>(ceil_div(m_pad - m, kThreads)), kThreads, 0, st>>>(b.rows, m, m_pad);
+    MSC_LAUNCH_CHECK();
+  }
+  *m_out = m;
+  return kOk;
+}
+
+// column means, operand rows, certificate terms for one precision
+struct Operands {
+  const void* X;
+  const float* qn2;
+  const float* qres;
+  const float* gstat;
+  int ks;
+  Bound bnd;
+};
+
+static Operands prepare(const Plan& p, const Buffers& b, const float* X, int64_t n, bool bf16, hipStream_t st) {
+  const int d = p.d;
+  const int64_t rows_per = ceil_div(n, kColBlocks);
+  colsum_partial<<<kColBlocks, 64, 0, st>>>(X, n, d, rows_per, b.partial);
+  colmean_final<<<1, 64, 0, st>>>(b.partial, kColBlocks, d, n, b.mean);
+  const unsigned pb = static_cast<unsigned>(ceil_div(p.n_pad, kThreads));
+  Operands op{};
+  if (bf16) {
+    prep_bf16<<<pb, kThreads, 0, st>>>(X, n, p.n_pad, d, 32 * p.ks_bf16, b.mean, static_cast<uint16_t*>(b.Xb),
+                                       b.nrm2b, b.resb, b.qn2b, b.qresb);
+    point_max<<<1, kThreads, 0, st>>>(b.nrm2b, b.resb, n, b.gstatb);
+    op = Operands{b.Xb, b.qn2b, b.qresb, b.gstatb, p.ks_bf16, make_bound(true, p.ks_bf16)};
+  } else {
+    prep_f32<<<pb, kThreads, 0, st>>>(X, n, p.n_pad, d, 8 * p.ks_f32, b.mean, static_cast<float*>(b.Xf), b.nrm2f,
+                                      b.resf);
+    point_max<<<1, kThreads, 0, st>>>(b.nrm2f, b.resf, n, b.gstatf);
+    op = Operands{b.Xf, b.nrm2f, b.resf, b.gstatf, p.ks_f32, make_bound(false, p.ks_f32)};
+  }
+  return op;
 }
 
 }  // namespace knn
@@ -685,58 +885,71 @@ extern "C" int msc_knn_exact(const float* X, int64_t n, int d, int k, int precis
     set_error("knn: workspace too small (%zu < %zu)", workspace_bytes, cv.used);
     return kWorkspaceTooSmall;
   }
+  const bool bf16 = precision == MSC_PRECISION_BF16_RERANK;
   const bool timing = info && info->timing;
-  hipEvent_t ev[6];
+  hipEvent_t ev[7];
   if (timing)
     for (auto& e : ev) MSC_HIP(hipEventCreate(&e));
   if (timing) MSC_HIP(hipEventRecord(ev[0], st));
 
-  const int64_t rows_per = ceil_div(n, kColBlocks);
-  colsum_partial<<<kColBlocks, 64, 0, st>>>(X, n, d, rows_per, b.partial);
-  colmean_final<<<1, 64, 0, st>>>(b.partial, kColBlocks, d, n, b.mean);
-  const unsigned pb = static_cast<unsigned>(ceil_div(p.n_pad, kThreads));
-  if (precision == MSC_PRECISION_BF16_RERANK)
-    prep_bf16<<<pb, kThreads, 0, st>>>(X, n, p.n_pad, d, 32 * p.KS, b.mean,
-                                       static_cast<uint16_t*>(b.Xp), b.nrm2, b.res);
-  else
-    prep_f32<<<pb, kThreads, 0, st>>>(X, n, p.n_pad, d, 8 * p.KS, b.mean,
-                                      static_cast<float*>(b.Xp), b.nrm2, b.res);
-  point_max<<<1, kThreads, 0, st>>>(b.nrm2, b.res, n, b.gstat);
+  Operands op = prepare(p, b, X, n, bf16, st);
   MSC_LAUNCH_CHECK();
   if (timing) MSC_HIP(hipEventRecord(ev[1], st));
 
-  MSC_TRY(launch_select(p, b, st));
+  MSC_TRY(launch_select(bf16, false, op.ks, op.X, n, p.n_pad, d, p.split_len, p.S, nullptr, nullptr, b.keys, b.taus,
+                        st));
   if (timing) MSC_HIP(hipEventRecord(ev[2], st));
 
-  Bound bnd;
-  const double u = 5.9604644775390625e-08;  // 2^-24
-  const int terms = precision == MSC_PRECISION_BF16_RERANK ? 32 * p.KS : 2 * 8 * p.KS;
-  bnd.gam = terms * u / (1.0 - terms * u);
-  bnd.split_rel = precision == MSC_PRECISION_BF16_RERANK ? 1.0 / (1 << 26) : 2.0 * u;
   const unsigned wb = static_cast<unsigned>(ceil_div(n, kThreads / kWave));
-  finalize_kernel<<<wb, kThreads, 0, st>>>(X, n, d, k, p.S, p.n_pad, b.keys, b.taus, b.nrm2, b.res,
-                                           b.gstat, bnd, out_idx, out_dist, b.flags);
+  finalize_kernel<<<wb, kThreads, 0, st>>>(X, n, d, k, p.S, p.n_pad, nullptr, b.keys, b.taus, op.qn2, op.qres,
+                                           op.gstat, op.bnd, nullptr, out_idx, out_dist, b.flags);
   MSC_LAUNCH_CHECK();
   if (timing) MSC_HIP(hipEventRecord(ev[3], st));
-  fallback_kernel<<<wb, kThreads, 0, st>>>(X, n, d, k, b.flags, out_idx, out_dist);
-  MSC_LAUNCH_CHECK();
-  if (timing) MSC_HIP(hipEventRecord(ev[4], st));
-  if (info) {
-    count_flags<<<1, kThreads, 0, st>>>(b.flags, n, b.nfall);
+
+  // second pass: rows the certificate could not clear are selected again with the wide list
+  int64_t m1 = 0, m2 = 0;
+  MSC_TRY(compact_flags(p, b, &m1, st));
+  if (m1 > 0) {
+    const int64_t m_pad = ceil_div(m1, kQueriesPerBlock) * kQueriesPerBlock;
+    const int64_t wblocks = m_pad / (kQueriesPerBlock / 2);
+    // split the candidates of the few rows finely enough that the grid still fills the device
+    int64_t S2 = std::max<int64_t>(1, std::min<int64_t>(ceil_div(2048, wblocks), 16));
+    S2 = std::min<int64_t>(S2, std::max<int64_t>(1, n / 4096));
+    S2 = std::max<int64_t>(1, std::min<int64_t>(S2, static_cast<int64_t>(second_pass_slots(p.n_pad)) / (2 * m_pad)));
+    const int64_t split2 = ceil_div(ceil_div(n, S2), kTile) * kTile;
+    S2 = ceil_div(n, split2);
+    MSC_REQUIRE(2 * S2 * m_pad <= static_cast<int64_t>(second_pass_slots(p.n_pad)), "knn: internal second-pass sizing");
+    MSC_TRY(launch_select(bf16, true, op.ks, op.X, n, m_pad, d, split2, static_cast<int>(S2), b.rows, nullptr, b.keys,
+                          b.taus, st));
+    finalize_kernel<<<static_cast<unsigned>(ceil_div(m1, kThreads / kWave)), kThreads, 0, st>>>(
+        X, m1, d, k, static_cast<int>(2 * S2), m_pad, b.rows, b.keys, b.taus, op.qn2, op.qres, op.gstat, op.bnd,
+        nullptr, out_idx, out_dist, b.flags);
     MSC_LAUNCH_CHECK();
-    MSC_HIP(hipMemcpyAsync(&info->n_fallback, b.nfall, sizeof(int64_t), hipMemcpyDeviceToHost, st));
-    MSC_HIP(hipStreamSynchronize(st));
+  }
+  if (timing) MSC_HIP(hipEventRecord(ev[4], st));
+  if (m1 > 0) {
+    fallback_kernel<<<wb, kThreads, 0, st>>>(X, n, d, k, b.flags, nullptr, out_idx, out_dist);
+    MSC_LAUNCH_CHECK();
+  }
+  if (timing) MSC_HIP(hipEventRecord(ev[5], st));
+  if (info) {
+    if (m1 > 0) MSC_TRY(compact_flags(p, b, &m2, st));
+    info->n_fallback = m2;
+    info->n_second_pass = m1;
     info->n_splits = p.S;
-    info->list_len = 4 * p.L;
+    info->list_len = 4 * kL;
     if (timing) {
+      MSC_HIP(hipEventSynchronize(ev[5]));
       float ms;
       MSC_HIP(hipEventElapsedTime(&ms, ev[0], ev[1])); info->prep_ms = ms;
       MSC_HIP(hipEventElapsedTime(&ms, ev[1], ev[2])); info->select_ms = ms;
       MSC_HIP(hipEventElapsedTime(&ms, ev[2], ev[3])); info->finalize_ms = ms;
-      MSC_HIP(hipEventElapsedTime(&ms, ev[3], ev[4])); info->fallback_ms = ms;
-      for (auto& e : ev) MSC_HIP(hipEventDestroy(e));
+      MSC_HIP(hipEventElapsedTime(&ms, ev[3], ev[4])); info->second_pass_ms = ms;
+      MSC_HIP(hipEventElapsedTime(&ms, ev[4], ev[5])); info->fallback_ms = ms;
     }
   }
+  if (timing)
+    for (auto& e : ev) MSC_HIP(hipEventDestroy(e));
   return kOk;
 }
 
@@ -750,5 +963,164 @@ extern "C" int msc_knn_distances_csr(const int32_t* idx, const float* dist, int6
   const unsigned wb = static_cast<unsigned>(ceil_div(n + 1, kThreads / kWave));
   distances_csr_kernel<<<wb, kThreads, 0, st>>>(idx, dist, n, k, out_indptr, out_indices, out_data);
   MSC_LAUNCH_CHECK();
+  return kOk;
+}
+
+// ------------------------------------------------------------------------------------------
+// batched samples: one launch over many independent samples
+// ------------------------------------------------------------------------------------------
+namespace msc {
+namespace knn {
+// padded layout: sample s occupies rows [P_s, P_s + n_s) of an n_tot x d copy, P_s % 128 == 0
+__global__ void k_pad_copy(const float* __restrict__ X, const int32_t* __restrict__ prow2row, int64_t n_tot, int d,
+                           float* __restrict__ Xpad) {
+  const int64_t e = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (e >= n_tot * d) return;
+  const int64_t r = e / d;
+  const int32_t src = prow2row[r];
+  Xpad[e] = src >= 0 ? X[static_cast<int64_t>(src) * d + (e % d)] : 0.0f;
+}
+// results back to input rows with unpadded global column ids
+__global__ void k_unpad(const int32_t* __restrict__ idx_pad, const float* __restrict__ dist_pad,
+                        const int32_t* __restrict__ prow2row, const int64_t* __restrict__ row_range,
+                        const int64_t* __restrict__ in_off, const int32_t* __restrict__ prow_sample, int64_t n_tot, int k,
+                        int32_t* __restrict__ out_idx, float* __restrict__ out_dist) {
+  const int64_t e = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (e >= n_tot * k) return;
+  const int64_t r = e / k;
+  const int32_t dst = prow2row[r];
+  if (dst < 0) return;
+  const int64_t t = e % k;
+  const int32_t s = prow_sample[r];
+  out_idx[static_cast<int64_t>(dst) * k + t] = static_cast<int32_t>(idx_pad[e] - row_range[2 * r] + in_off[s]);
+  out_dist[static_cast<int64_t>(dst) * k + t] = dist_pad[e];
+}
+}  // namespace knn
+}  // namespace msc
+
+namespace {
+struct BatchLayout {
+  int64_t n_tot = 0, nblk = 0;
+  std::vector<int32_t> prow2row, prow_sample;
+  std::vector<int64_t> row_range, blk_range;
+};
+
+int make_layout(const int64_t* offsets, int n_samples, int k, BatchLayout* L) {
+  int64_t P = 0;
+  for (int s = 0; s < n_samples; ++s) {
+    const int64_t ns = offsets[s + 1] - offsets[s];
+    MSC_REQUIRE(ns >= k, "knn_batched: sample %d has %lld cells < k=%d", s, (long long)ns, k);
+    P += msc::ceil_div(ns, msc::knn::kQueriesPerBlock) * msc::knn::kQueriesPerBlock;
+  }
+  MSC_REQUIRE(P < (int64_t(1) << 31) - 256, "knn_batched: total padded rows too large");
+  L->n_tot = P;
+  L->nblk = P / msc::knn::kQueriesPerBlock;
+  L->prow2row.assign(P, -1);
+  L->prow_sample.assign(P, 0);
+  L->row_range.assign(2 * P, 0);
+  L->blk_range.assign(2 * L->nblk, 0);
+  int64_t base = 0;
+  for (int s = 0; s < n_samples; ++s) {
+    const int64_t ns = offsets[s + 1] - offsets[s];
+    const int64_t padded = msc::ceil_div(ns, msc::knn::kQueriesPerBlock) * msc::knn::kQueriesPerBlock;
+    for (int64_t i = 0; i < padded; ++i) {
+      const int64_t r = base + i;
+      L->prow_sample[r] = s;
+      if (i < ns) {
+        L->prow2row[r] = static_cast<int32_t>(offsets[s] + i);
+        L->row_range[2 * r] = base;
+        L->row_range[2 * r + 1] = base + ns;
+      }
+    }
+    for (int64_t b = base / msc::knn::kQueriesPerBlock; b < (base + padded) / msc::knn::kQueriesPerBlock; ++b) {
+      L->blk_range[2 * b] = base;
+      L->blk_range[2 * b + 1] = base + ns;
+    }
+    base += padded;
+  }
+  return msc::kOk;
+}
+
+size_t batched_bytes(int64_t n_in, int n_samples, const BatchLayout& L, int d, int k, int precision) {
+  using namespace msc::knn;
+  Plan p = make_plan(L.n_tot, d, k, precision);
+  msc::Carve cv(nullptr, 0);
+  carve(cv, p);
+  cv.take<float>(static_cast<size_t>(L.n_tot) * d);  // padded X
+  cv.take<int32_t>(L.n_tot);                          // prow2row
+  cv.take<int32_t>(L.n_tot);                          // prow_sample
+  cv.take<int64_t>(2 * L.n_tot);                      // row_range
+  cv.take<int64_t>(2 * L.nblk);                       // blk_range
+  cv.take<int64_t>(n_samples + 1);                    // input offsets
+  cv.take<int32_t>(static_cast<size_t>(L.n_tot) * k); // padded idx
+  cv.take<float>(static_cast<size_t>(L.n_tot) * k);   // padded dist
+  (void)n_in;
+  return cv.used + 256;
+}
+}  // namespace
+
+extern "C" int msc_knn_batched_workspace_size(const int64_t* offsets, int n_samples, int d, int k, int precision,
+                                              size_t* bytes) {
+  MSC_REQUIRE(offsets && bytes && n_samples >= 1, "knn_batched: bad arguments");
+  MSC_REQUIRE(k >= 1 && k <= 64 && d >= 1, "knn_batched: bad d/k");
+  BatchLayout L;
+  MSC_TRY(make_layout(offsets, n_samples, k, &L));
+  *bytes = batched_bytes(offsets[n_samples], n_samples, L, d, k, precision);
+  return kOk;
+}
+
+extern "C" int msc_knn_batched(const float* X, const int64_t* offsets, int n_samples, int d, int k, int precision,
+                               int32_t* out_idx, float* out_dist, void* workspace, size_t workspace_bytes,
+                               void* stream, msc_knn_info* info) {
+  MSC_REQUIRE(X && offsets && out_idx && out_dist && workspace && n_samples >= 1, "knn_batched: bad arguments");
+  MSC_REQUIRE(precision == MSC_PRECISION_FP32 || precision == MSC_PRECISION_BF16_RERANK, "knn_batched: precision");
+  MSC_REQUIRE(k >= 1 && k <= 64 && d >= 1 && d <= 125, "knn_batched: bad d/k");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  BatchLayout L;
+  MSC_TRY(make_layout(offsets, n_samples, k, &L));
+  if (workspace_bytes < batched_bytes(offsets[n_samples], n_samples, L, d, k, precision)) {
+    set_error("knn_batched: workspace too small");
+    return kWorkspaceTooSmall;
+  }
+  Plan p = make_plan(L.n_tot, d, k, precision);
+  Carve cv(workspace, workspace_bytes);
+  Buffers b = carve(cv, p);
+  float* Xpad = cv.take<float>(static_cast<size_t>(L.n_tot) * d);
+  int32_t* prow2row = cv.take<int32_t>(L.n_tot);
+  int32_t* prow_sample = cv.take<int32_t>(L.n_tot);
+  int64_t* row_range = cv.take<int64_t>(2 * L.n_tot);
+  int64_t* blk_range = cv.take<int64_t>(2 * L.nblk);
+  int64_t* in_off = cv.take<int64_t>(n_samples + 1);
+  int32_t* idx_pad = cv.take<int32_t>(static_cast<size_t>(L.n_tot) * k);
+  float* dist_pad = cv.take<float>(static_cast<size_t>(L.n_tot) * k);
+  MSC_HIP(hipMemcpyAsync(prow2row, L.prow2row.data(), L.n_tot * sizeof(int32_t), hipMemcpyHostToDevice, st));
+  MSC_HIP(hipMemcpyAsync(prow_sample, L.prow_sample.data(), L.n_tot * sizeof(int32_t), hipMemcpyHostToDevice, st));
+  MSC_HIP(hipMemcpyAsync(row_range, L.row_range.data(), 2 * L.n_tot * sizeof(int64_t), hipMemcpyHostToDevice, st));
+  MSC_HIP(hipMemcpyAsync(blk_range, L.blk_range.data(), 2 * L.nblk * sizeof(int64_t), hipMemcpyHostToDevice, st));
+  MSC_HIP(hipMemcpyAsync(in_off, offsets, (n_samples + 1) * sizeof(int64_t), hipMemcpyHostToDevice, st));
+  MSC_HIP(hipStreamSynchronize(st));  // the host tables are pageable and local to this call
+  const int64_t n = L.n_tot;
+  k_pad_copy<<<static_cast<unsigned>(ceil_div(n * d, kThreads)), kThreads, 0, st>>>(X, prow2row, n, d, Xpad);
+  MSC_LAUNCH_CHECK();
+  const bool bf16 = precision == MSC_PRECISION_BF16_RERANK;
+  Operands op = prepare(p, b, Xpad, n, bf16, st);
+  MSC_LAUNCH_CHECK();
+  MSC_TRY(launch_select(bf16, false, op.ks, op.X, n, n, d, n, 1, nullptr, blk_range, b.keys, b.taus, st));
+  const unsigned wb = static_cast<unsigned>(ceil_div(n, kThreads / kWave));
+  finalize_kernel<<<wb, kThreads, 0, st>>>(Xpad, n, d, k, 1, n, nullptr, b.keys, b.taus, op.qn2, op.qres, op.gstat,
+                                           op.bnd, row_range, idx_pad, dist_pad, b.flags);
+  MSC_LAUNCH_CHECK();
+  fallback_kernel<<<wb, kThreads, 0, st>>>(Xpad, n, d, k, b.flags, row_range, idx_pad, dist_pad);
+  k_unpad<<<static_cast<unsigned>(ceil_div(n * k, kThreads)), kThreads, 0, st>>>(
+      idx_pad, dist_pad, prow2row, row_range, in_off, prow_sample, n, k, out_idx, out_dist);
+  MSC_LAUNCH_CHECK();
+  if (info) {
+    int64_t m = 0;
+    MSC_TRY(compact_flags(p, b, &m, st));
+    info->n_fallback = m;
+    info->n_second_pass = 0;
+    info->n_splits = 1;
+    info->list_len = 4 * kL;
+  }
   return kOk;
 }

@@ -264,6 +264,7 @@ struct Heavy {
   int64_t* segstart;   // nseg + 1 (<= eh + 1)
   uint32_t* seg_h;     // per segment heavy index
   int64_t* hseg;       // nh + 1: first segment per heavy vertex
+  int64_t* hstart;     // nh + 1: first sorted heavy edge per heavy vertex
 };
 
 __global__ void k_heavy_flags(const int64_t* __restrict__ indptr, int64_t n, int32_t* __restrict__ flag,
@@ -355,6 +356,11 @@ __global__ void k_heavy_seg_starts(const int32_t* __restrict__ flag, const int32
     seg_h[pos[i]] = skh[i];
   }
   if (i == eh) segstart[pos[eh]] = eh;
+}
+__global__ void k_heavy_hseg(const int64_t* __restrict__ hstart, const int32_t* __restrict__ segid, int64_t nh,
+                             int64_t* __restrict__ hseg) {
+  const int64_t h = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (h <= nh) hseg[h] = segid[hstart[h]];
 }
 // one wave per heavy vertex over its segments
 __global__ __launch_bounds__(kThreads) void k_heavy_argmax(
@@ -599,6 +605,7 @@ static void carve_ctx(Carve& cv, Ctx& c, int64_t n, int64_t E) {
   h.wsorted = cv.take<int64_t>(E1); h.prefix = cv.take<int64_t>(E1);
   h.flag = cv.take<int32_t>(E1); h.segid = cv.take<int32_t>(E1);
   h.segstart = cv.take<int64_t>(E1 + 1); h.seg_h = cv.take<uint32_t>(E1); h.hseg = cv.take<int64_t>(n1 + 1);
+  h.hstart = cv.take<int64_t>(n1 + 1);
   c.hflag = cv.take<int32_t>(n1); c.hpos = cv.take<int32_t>(n1);
   c.hdeg = cv.take<int64_t>(n1); c.hdegoff = cv.take<int64_t>(n1);
 }
@@ -668,11 +675,12 @@ static int heavy_moves(Ctx& c, const Graph& g, const int32_t* comm, const int32_
   MSC_LAUNCH_CHECK();
   MSC_TRY(prims::exclusive_scan<int32_t>(h.flag, h.segid, eh + 1, c.scan_ws, c.scan_ws_bytes, st));
   MSC_TRY(prims::exclusive_scan<int64_t>(h.wsorted, h.prefix, eh + 1, c.scan_ws, c.scan_ws_bytes, st));
-  int32_t nseg;
-  MSC_TRY(read_scalar(h.segid + eh, &nseg, st));
   k_heavy_seg_starts<<<grid_for(eh + 1), kThreads, 0, st>>>(h.flag, h.segid, eh, h.skh, h.segstart, h.seg_h);
   MSC_LAUNCH_CHECK();
-  MSC_TRY(prims::segment_starts(h.seg_h, nseg, h.nh, h.hseg, st));
+  // first segment of each heavy vertex = segment id at its first sorted edge (no host sync)
+  MSC_TRY(prims::segment_starts(h.skh, eh, h.nh, h.hstart, st));
+  k_heavy_hseg<<<grid_for(h.nh + 1), kThreads, 0, st>>>(h.hstart, h.segid, h.nh, h.hseg);
+  MSC_LAUNCH_CHECK();
   k_heavy_argmax<<<grid_for(h.nh * kWave), kThreads, 0, st>>>(h.verts, h.nh, h.hseg, h.segstart, h.prefix, h.skv2,
                                                               h.kc, g.k, comm, c.sigtot, gp, c.target);
   MSC_LAUNCH_CHECK();

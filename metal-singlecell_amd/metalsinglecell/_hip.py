@@ -44,8 +44,9 @@ class SpatialInfo(ctypes.Structure):
 
 class KnnInfo(ctypes.Structure):
     _fields_ = [("timing", c_int), ("n_splits", c_int), ("list_len", c_int),
-                ("n_fallback", c_i64), ("prep_ms", c_dbl), ("select_ms", c_dbl),
-                ("finalize_ms", c_dbl), ("fallback_ms", c_dbl)]
+                ("n_second_pass", c_i64), ("n_fallback", c_i64), ("prep_ms", c_dbl),
+                ("select_ms", c_dbl), ("finalize_ms", c_dbl), ("second_pass_ms", c_dbl),
+                ("fallback_ms", c_dbl)]
 
 
 # name -> (restype, argtypes); the declarations of include/msc_hip.h
@@ -55,6 +56,9 @@ SIGNATURES = {
     "msc_knn_workspace_size": (c_int, [c_i64, c_int, c_int, c_int, ctypes.POINTER(c_sz)]),
     "msc_knn_exact": (c_int, [c_vp, c_i64, c_int, c_int, c_int, c_vp, c_vp, c_vp, c_sz, c_vp,
                               ctypes.POINTER(KnnInfo)]),
+    "msc_knn_batched_workspace_size": (c_int, [c_vp, c_int, c_int, c_int, c_int, ctypes.POINTER(c_sz)]),
+    "msc_knn_batched": (c_int, [c_vp, c_vp, c_int, c_int, c_int, c_int, c_vp, c_vp, c_vp, c_sz, c_vp,
+                                ctypes.POINTER(KnnInfo)]),
     "msc_knn_distances_csr": (c_int, [c_vp, c_vp, c_i64, c_int, c_vp, c_vp, c_vp, c_vp]),
     "msc_fuzzy_workspace_size": (c_int, [c_i64, c_int, ctypes.POINTER(c_sz)]),
     "msc_fuzzy_connectivities": (c_int, [c_vp, c_vp, c_i64, c_int, c_vp, c_vp, c_vp, c_i64, c_vp,
@@ -154,6 +158,29 @@ def knn_exact(X, k, precision="fp32", timing=False, return_info=False):
     check(lib.msc_knn_exact(ptr(Xd), n, d, int(k), p, ptr(idx), ptr(dist), ptr(ws),
                             ws.numel(), stream_ptr(), ctypes.byref(info) if (timing or return_info) else None))
     if return_info or timing:
+        return idx, dist, {f: getattr(info, f) for f, _ in KnnInfo._fields_}
+    return idx, dist
+
+
+def knn_batched(X, offsets, k, precision="bf16", return_info=False):
+    """Block-diagonal exact kNN of concatenated samples: ``(idx int32 global, dist float32)``."""
+    import torch
+    lib = load_library()
+    Xd = to_device(X, torch.float32)
+    off = np.ascontiguousarray(offsets, dtype=np.int64)
+    S = off.size - 1
+    d = Xd.shape[1]
+    p = PRECISION[precision]
+    nbytes = c_sz(0)
+    check(lib.msc_knn_batched_workspace_size(off.ctypes.data, S, d, int(k), p, ctypes.byref(nbytes)))
+    ws = workspace(nbytes.value)
+    n = Xd.shape[0]
+    idx = torch.empty((n, k), dtype=torch.int32, device=Xd.device)
+    dist = torch.empty((n, k), dtype=torch.float32, device=Xd.device)
+    info = KnnInfo()
+    check(lib.msc_knn_batched(ptr(Xd), off.ctypes.data, S, d, int(k), p, ptr(idx), ptr(dist), ptr(ws),
+                              ws.numel(), stream_ptr(), ctypes.byref(info)))
+    if return_info:
         return idx, dist, {f: getattr(info, f) for f, _ in KnnInfo._fields_}
     return idx, dist
 

@@ -90,3 +90,57 @@ def test_knn_distances_csr_matches_scipy(c1_data):
     R = knn.distances_csr(i.cpu().numpy(), d.cpu().numpy(), n)
     assert np.array_equal(M.indptr, R.indptr) and np.array_equal(M.indices, R.indices)
     assert np.array_equal(M.data, R.data)
+
+
+def test_knn_certificate_rescue_paths():
+    """Widely separated clusters make the query's BF16 rounding residual (~|x| 2^-9) exceed
+    the neighbour gaps: the certificate fails in both select passes and those rows go to
+    brute force.  Huge offsets with tiny spreads defeat FP32 ranking the same way.  Either
+    way the result must equal the oracle's exactly."""
+    from oracle import knn
+    rng = np.random.default_rng(5)
+    centers = rng.normal(0, 2000.0, (20, 50))
+    X = (centers[rng.integers(0, 20, 3000)] + rng.normal(0, 1.0, (3000, 50))).astype(np.float32)
+    idx, dist, info = _run(X, 15, "bf16")
+    ri, _, rd = knn.exact_knn_c(X, 15)
+    assert info["n_second_pass"] > 0
+    assert np.array_equal(idx, ri) and np.array_equal(dist, rd)
+    centers = rng.normal(0, 1e6, (10, 8))
+    X = (centers[rng.integers(0, 10, 1500)] + rng.normal(0, 1e-2, (1500, 8))).astype(np.float32)
+    for prec in ("bf16", "fp32"):
+        idx, dist, info = _run(X, 10, prec)
+        ri, _, rd = knn.exact_knn_c(X, 10)
+        assert info["n_fallback"] > 0
+        assert np.array_equal(idx, ri) and np.array_equal(dist, rd)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16"])
+def test_knn_lane_partition_rescued_by_wide_pass(precision):
+    """Candidate j reaches list (j mod 16) div 4 of its query in the first select pass, so
+    a query whose 20 nearest neighbours all have j mod 16 < 4 overflows one list of 16 and
+    fails the certificate.  The wide pass splits those candidates again by (j mod 64) div 32
+    (about 10 per list) and must certify every such row without brute force.
+
+    Every point sits in a tight cluster of 21 (its 20 neighbours are its cluster), so no
+    near-ties exist across clusters: 32 clusters use only indices with j mod 16 < 4, the
+    other 96 only the remaining indices."""
+    from oracle import knn
+    rng = np.random.default_rng(11)
+    n, dim, k = 2688, 8, 20
+    j = np.arange(n)
+    groups = [j[j % 16 < 4], j[j % 16 >= 4]]
+    n_cl = [len(g) // 21 for g in groups]
+    assert n_cl == [32, 96]
+    centers = rng.normal(0.0, 100.0, (sum(n_cl), dim))
+    X = np.empty((n, dim))
+    c = 0
+    for g, m in zip(groups, n_cl):
+        for t in range(m):
+            X[g[21 * t: 21 * (t + 1)]] = centers[c] + rng.normal(0.0, 1.0, (21, dim))
+            c += 1
+    X = X.astype(np.float32)
+    idx, dist, info = _run(X, k, precision)
+    ri, _, rd = knn.exact_knn_c(X, k)
+    assert info["n_second_pass"] >= 21 * n_cl[0]
+    assert info["n_fallback"] == 0
+    assert np.array_equal(idx, ri) and np.array_equal(dist, rd)
