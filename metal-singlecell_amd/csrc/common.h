@@ -98,6 +98,13 @@ __device__ __forceinline__ uint64_t shfl_u64(uint64_t x, int src) {
   return (static_cast<uint64_t>(hi) << 32) | lo;
 }
 
+__device__ __forceinline__ uint64_t shfl_up_u64(uint64_t x, int d) {
+  uint32_t lo = static_cast<uint32_t>(x), hi = static_cast<uint32_t>(x >> 32);
+  lo = __shfl_up(lo, d, kWave);
+  hi = __shfl_up(hi, d, kWave);
+  return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+
 __device__ __forceinline__ uint64_t umin64(uint64_t a, uint64_t b) { return a < b ? a : b; }
 __device__ __forceinline__ uint64_t umax64(uint64_t a, uint64_t b) { return a < b ? b : a; }
 

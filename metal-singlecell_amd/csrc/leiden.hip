@@ -13,9 +13,11 @@
 // wave sum: no atomics anywhere, so labels are bit-identical to the oracle and independent
 // of scheduling.
 //
-// Per-vertex best move: vertices of degree <= 64 take one wave (one neighbour per lane,
-// distinct communities found by ballot); heavier vertices go through a sort path over
-// their edges keyed by (vertex, community).
+// Per-vertex best move: vertices of degree <= 256 take one wave (up to four neighbours per
+// lane); degree 257..1024 take one wave each from a list (up to sixteen per lane); both sort
+// the neighbours' communities inside the wave.  Heavier vertices (and every vertex above 256
+// when the graph has 2^22 or more vertices) go through a sort path over their edges keyed by
+// (vertex, community).
 #include <cmath>
 #include <vector>
 
@@ -28,8 +30,8 @@ namespace leiden {
 
 constexpr int kThreads = 256;
 constexpr int kWavesPerBlock = kThreads / kWave;
-constexpr int kLightDeg = 64;
-constexpr int kDegreeCap = 1024;  // pruning is disabled above this degree (louvain.py:350-352)
+constexpr int kLightItems = 4;                  // neighbours per lane on the one-wave path
+constexpr int kLightDeg = kWave * kLightItems;  // 256
 constexpr int kMaxLevels = 20;
 constexpr int kMaxPasses = 200;
 constexpr int kSyncEvery = 4;
@@ -92,16 +94,25 @@ static unsigned grid_for(int64_t n) { return static_cast<unsigned>(std::max<int6
 // ------------------------------------------------------------------------------------------
 // graph set-up
 // ------------------------------------------------------------------------------------------
+// max weight: per-block maxima (grid-stride) into out[block], then one block over them;
+// a maximum does not depend on the order of evaluation.  Non-finite and negative weights
+// map to +inf so the caller's finiteness check rejects them.
+constexpr int kMaxBlocks = 1024;
 __global__ void k_max_weight(const float* __restrict__ w, int64_t E, float* __restrict__ out) {
   float m = 0.0f;
-  for (int64_t i = threadIdx.x; i < E; i += blockDim.x) m = fmaxf(m, w[i]);
+  for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < E;
+       i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+  {
+    const float x = w[i];
+    m = fmaxf(m, (isfinite(x) && x >= 0.0f) ? x : __uint_as_float(0x7F800000u));
+  }
   m = wave_max_f32(m);
   __shared__ float sh[kWavesPerBlock];
   if (lane_id() == 0) sh[threadIdx.x / kWave] = m;
   __syncthreads();
   if (threadIdx.x == 0) {
     for (int i = 1; i < kWavesPerBlock; ++i) m = fmaxf(m, sh[i]);
-    *out = fmaxf(m, sh[0]);
+    out[blockIdx.x] = m;
   }
 }
 
@@ -195,59 +206,307 @@ static int group_sum(const int64_t* val, int64_t n, int64_t C, GroupScratch& g, 
 }
 
 // ------------------------------------------------------------------------------------------
-// best move, light vertices (degree <= 64): one wave per vertex
+// best move, light vertices (degree <= 256): one wave per vertex
 // ------------------------------------------------------------------------------------------
-__global__ __launch_bounds__(kThreads) void k_move_light(
-    const int64_t* __restrict__ indptr, const int32_t* __restrict__ indices, const int64_t* __restrict__ wq,
-    const int64_t* __restrict__ kdeg, int64_t n, const int32_t* __restrict__ comm,
-    const int32_t* __restrict__ part, const int64_t* __restrict__ sigtot, const uint8_t* __restrict__ active,
-    GainParams gp, int32_t* __restrict__ target) {
-  const int64_t v = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + threadIdx.x / kWave;
-  if (v >= n) return;
+// Lane l holds neighbours l, l + 64, ... (NI of them) with their community, edge weight and
+// that community's sigma_tot, all gathered up front.  Each round takes the lowest pending
+// (item, lane), sums the weights of every neighbour in its community (exact int64 wave sum)
+// and scores the move; rounds = distinct neighbour communities.
+template <int NI>
+__device__ __forceinline__ void move_one(int64_t v, int64_t e0, int64_t deg, int32_t curr, int64_t kv,
+                                         const int32_t* __restrict__ indices, const int64_t* __restrict__ wq,
+                                         const int32_t* __restrict__ comm, const int32_t* __restrict__ part,
+                                         const int64_t* __restrict__ sigtot, const GainParams& gp,
+                                         int32_t* __restrict__ target) {
   const int l = lane_id();
-  const int64_t e0 = indptr[v], deg = indptr[v + 1] - e0;
-  const int32_t curr = comm[v];
-  if (deg > kLightDeg) return;  // heavy path writes target[v]
-  if (!active[v]) {
-    if (l == 0) target[v] = curr;
-    return;
-  }
   const int32_t pv = part ? part[v] : 0;
-  bool ok = false;
-  int32_t c = -1;
-  int64_t w = 0;
-  if (l < deg) {
-    const int32_t u = indices[e0 + l];
-    ok = u != static_cast<int32_t>(v) && (!part || part[u] == pv);
-    if (ok) {
-      c = comm[u];
-      w = wq[e0 + l];
+  bool ok[NI];
+  int32_t c[NI];
+  int64_t w[NI], sg[NI];
+  uint64_t pend[NI];
+#pragma unroll
+  for (int i = 0; i < NI; ++i) {
+    ok[i] = false;
+    c[i] = -1;
+    w[i] = 0;
+    sg[i] = 0;
+    const int64_t j = l + kWave * i;
+    if (j < deg) {
+      const int32_t u = indices[e0 + j];
+      ok[i] = u != static_cast<int32_t>(v) && (!part || part[u] == pv);
+      if (ok[i]) {
+        c[i] = comm[u];
+        w[i] = wq[e0 + j];
+      }
     }
   }
-  const int64_t kv = kdeg[v];
-  uint64_t pending = __ballot(ok);
+#pragma unroll
+  for (int i = 0; i < NI; ++i) {
+    if (ok[i]) sg[i] = sigtot[c[i]];
+    pend[i] = __ballot(ok[i]);
+  }
   double best_g = -1e300;
   int32_t best_c = curr;
   double stay = 0.0;
-  while (pending) {
-    const int leader = __ffsll(static_cast<long long>(pending)) - 1;
-    const int32_t cc = __shfl(c, leader, kWave);
-    const uint64_t match = __ballot(ok && c == cc);
-    const int64_t wc = wave_sum_i64((ok && c == cc) ? w : 0);
-    const int64_t sig = sigtot[cc] - (cc == curr ? kv : 0);
+  while (true) {
+    int it = -1;
+#pragma unroll
+    for (int i = NI - 1; i >= 0; --i)
+      if (pend[i]) it = i;
+    if (it < 0) break;
+    int32_t cc = 0;
+    int64_t sig = 0;
+#pragma unroll
+    for (int i = 0; i < NI; ++i) {
+      if (i == it) {
+        const int leader = __ffsll(static_cast<long long>(pend[i])) - 1;
+        cc = __shfl(c[i], leader, kWave);
+        sig = static_cast<int64_t>(shfl_u64(static_cast<uint64_t>(sg[i]), leader));
+      }
+    }
+    int64_t part_w = 0;
+#pragma unroll
+    for (int i = 0; i < NI; ++i) {
+      const bool m = ok[i] && c[i] == cc;
+      part_w += m ? w[i] : 0;
+      pend[i] &= ~__ballot(m);
+    }
+    const int64_t wc = wave_sum_i64(part_w);
+    sig -= (cc == curr ? kv : 0);
     const double g = gain_of(wc, sig, kv, gp);
     if (cc == curr) stay = g;
     if (better(g, cc, best_g, best_c)) {
       best_g = g;
       best_c = cc;
     }
-    pending &= ~match;
   }
   if (l == 0) target[v] = (best_g > stay + kTol) ? best_c : curr;
 }
 
+// Same decision as move_one, at a cost independent of the number of distinct communities:
+// the wave sorts keys (community << IB | slot) of its up to 64*NI eligible neighbours
+// (bitonic network, slots = item * 64 + lane), sums the weights of each run of equal
+// community with an int64 prefix sum (staged in the wave's LDS), scores each community
+// once at the end of its run and takes the wave argmax (ties to the smaller community).
+// Requires community ids < 2^(32 - IB) so that a key fits 32 bits.
+constexpr int64_t kSortedMaxN = int64_t(1) << 24;  // NI <= 4
+constexpr int kMidItems = 16;
+constexpr int kMidDeg = kWave * kMidItems;           // 1024
+constexpr int64_t kMidMaxN = int64_t(1) << 22;      // NI = 16
+constexpr int kMoveGroup = 8;  // consecutive vertices per wave in k_move_light / k_frontier_push
+constexpr int kMidGroup = 2;   // listed vertices per single-wave block in k_move_mid
+
+template <int NI>
+__device__ __forceinline__ void move_sorted(int64_t v, int64_t e0, int64_t deg, int32_t curr, int64_t kv,
+                                            const int32_t* __restrict__ indices, const int64_t* __restrict__ wq,
+                                            const int32_t* __restrict__ comm, const int32_t* __restrict__ part,
+                                            const int64_t* __restrict__ sigtot, const GainParams& gp,
+                                            int32_t* __restrict__ target, int64_t* __restrict__ lw,
+                                            int64_t* __restrict__ lsg, int64_t* __restrict__ lp) {
+  constexpr int N = NI * kWave;
+  constexpr int IB = NI == 1 ? 6 : (NI == 2 ? 7 : (NI == 4 ? 8 : (NI == 8 ? 9 : 10)));  // slot bits
+  static_assert(NI == 1 || NI == 2 || NI == 4 || NI == 8 || NI == 16, "NI");
+  constexpr uint32_t kNone = 0xFFFFFFFFu;
+  const int l = lane_id();
+  const int32_t pv = part ? part[v] : 0;
+  uint32_t key[NI];
+  int nvalid = 0;
+#pragma unroll
+  for (int i = 0; i < NI; ++i) {
+    const int64_t j = l + kWave * i;
+    key[i] = kNone;
+    int64_t w = 0;
+    if (j < deg) {
+      const int32_t u = indices[e0 + j];
+      if (u != static_cast<int32_t>(v) && (!part || part[u] == pv)) {
+        const int32_t c = comm[u];
+        w = wq[e0 + j];
+        key[i] = (static_cast<uint32_t>(c) << IB) | static_cast<uint32_t>(i * kWave + l);
+      }
+    }
+    lw[i * kWave + l] = w;
+    nvalid += __popcll(__ballot(key[i] != kNone));
+  }
+#pragma unroll
+  for (int i = 0; i < NI; ++i)
+    lsg[i * kWave + l] = key[i] != kNone ? sigtot[key[i] >> IB] : 0;
+
+  // bitonic sort, ascending in slot order e = i * 64 + l
+#pragma unroll
+  for (int k = 2; k <= N; k <<= 1) {
+#pragma unroll
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      if (j >= kWave) {
+#pragma unroll
+        for (int i = 0; i < NI; ++i) {
+          const int ip = i ^ (j / kWave);
+          if (ip > i) {
+            const bool asc = ((i * kWave + l) & k) == 0;
+            const uint32_t a = key[i], b = key[ip];
+            const uint32_t lo = a < b ? a : b, hi = a < b ? b : a;
+            key[i] = asc ? lo : hi;
+            key[ip] = asc ? hi : lo;
+          }
+        }
+      } else {
+#pragma unroll
+        for (int i = 0; i < NI; ++i) {
+          const uint32_t p = static_cast<uint32_t>(__shfl_xor(static_cast<int>(key[i]), j, kWave));
+          const bool asc = ((i * kWave + l) & k) == 0;
+          const bool lower = (l & j) == 0;
+          const uint32_t lo = key[i] < p ? key[i] : p, hi = key[i] < p ? p : key[i];
+          key[i] = (lower == asc) ? lo : hi;
+        }
+      }
+    }
+  }
+
+  // per slot: community, weight prefix, run heads
+  int32_t c[NI];
+  int64_t sg[NI];
+  uint64_t hmask[NI];
+  int64_t carry = 0;
+  int32_t c_last = -1;  // community of slot (i * 64 - 1)
+#pragma unroll
+  for (int i = 0; i < NI; ++i) {
+    const int e = i * kWave + l;
+    const bool valid = e < nvalid;
+    const uint32_t src = key[i] & ((1u << IB) - 1u);
+    c[i] = valid ? static_cast<int32_t>(key[i] >> IB) : -1;
+    const int64_t w = valid ? lw[src] : 0;
+    sg[i] = valid ? lsg[src] : 0;
+    // inclusive prefix of w over the slots
+    int64_t x = w;
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+      const int64_t o = static_cast<int64_t>(shfl_up_u64(static_cast<uint64_t>(x), d));
+      if (l >= d) x += o;
+    }
+    x += carry;
+    carry = static_cast<int64_t>(shfl_u64(static_cast<uint64_t>(x), kWave - 1));
+    lp[e] = x;
+    int32_t cprev = __shfl_up(c[i], 1, kWave);
+    if (l == 0) cprev = c_last;
+    hmask[i] = __ballot(valid && (e == 0 || cprev != c[i]));
+    c_last = __shfl(c[i], kWave - 1, kWave);
+  }
+
+  double best_g = -1e300, stay = 0.0;
+  int32_t best_c = 0x7FFFFFFF;
+  bool has_stay = false;
+  int64_t head_carry = 0;  // last run head before item i
+#pragma unroll
+  for (int i = 0; i < NI; ++i) {
+    const int e = i * kWave + l;
+    const bool valid = e < nvalid;
+    const bool next_head = (l < kWave - 1) ? ((hmask[i] >> (l + 1)) & 1ull) != 0
+                                           : (i + 1 < NI ? (hmask[i + 1 < NI ? i + 1 : i] & 1ull) != 0 : false);
+    const bool end = valid && (e + 1 == nvalid || next_head);
+    const uint64_t upto = (l == kWave - 1) ? ~0ull : ((2ull << l) - 1ull);
+    const uint64_t hm = hmask[i] & upto;
+    const int64_t start = hm ? static_cast<int64_t>(i * kWave + 63 - __clzll(static_cast<long long>(hm))) : head_carry;
+    if (hmask[i]) head_carry = i * kWave + 63 - __clzll(static_cast<long long>(hmask[i]));
+    if (end) {
+      const int64_t wc = lp[e] - (start > 0 ? lp[start - 1] : 0);
+      const int64_t sig = sg[i] - (c[i] == curr ? kv : 0);
+      const double g = gain_of(wc, sig, kv, gp);
+      if (c[i] == curr) { stay = g; has_stay = true; }
+      if (better(g, c[i], best_g, best_c)) { best_g = g; best_c = c[i]; }
+    }
+  }
+#pragma unroll
+  for (int m = 1; m < kWave; m <<= 1) {
+    const double og = __shfl_xor(best_g, m, kWave);
+    const int32_t oc = __shfl_xor(best_c, m, kWave);
+    if (better(og, oc, best_g, best_c)) { best_g = og; best_c = oc; }
+  }
+  const uint64_t sm = __ballot(has_stay);
+  if (sm) stay = __shfl(stay, __ffsll(static_cast<long long>(sm)) - 1, kWave);
+  if (l == 0) target[v] = (best_g > stay + kTol) ? best_c : curr;
+}
+
+__device__ __forceinline__ void move_vertex(int64_t v, const int64_t* __restrict__ indptr,
+                                            const int32_t* __restrict__ indices, const int64_t* __restrict__ wq,
+                                            const int64_t* __restrict__ kdeg, int64_t n,
+                                            const int32_t* __restrict__ comm, const int32_t* __restrict__ part,
+                                            const int64_t* __restrict__ sigtot, const GainParams& gp,
+                                            int32_t* __restrict__ target, int64_t (*lds)[kLightDeg]) {
+  const int64_t e0 = indptr[v], deg = indptr[v + 1] - e0;
+  const int32_t curr = comm[v];
+  const int64_t kv = kdeg[v];
+  if (n < kSortedMaxN) {
+    int64_t* lw = lds[0];
+    int64_t* lsg = lds[1];
+    int64_t* lp = lds[2];
+    if (deg <= kWave) move_sorted<1>(v, e0, deg, curr, kv, indices, wq, comm, part, sigtot, gp, target, lw, lsg, lp);
+    else if (deg <= 2 * kWave)
+      move_sorted<2>(v, e0, deg, curr, kv, indices, wq, comm, part, sigtot, gp, target, lw, lsg, lp);
+    else move_sorted<4>(v, e0, deg, curr, kv, indices, wq, comm, part, sigtot, gp, target, lw, lsg, lp);
+  } else {
+    if (deg <= kWave) move_one<1>(v, e0, deg, curr, kv, indices, wq, comm, part, sigtot, gp, target);
+    else if (deg <= 2 * kWave) move_one<2>(v, e0, deg, curr, kv, indices, wq, comm, part, sigtot, gp, target);
+    else move_one<4>(v, e0, deg, curr, kv, indices, wq, comm, part, sigtot, gp, target);
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void k_move_light(
+    const int64_t* __restrict__ indptr, const int32_t* __restrict__ indices, const int64_t* __restrict__ wq,
+    const int64_t* __restrict__ kdeg, int64_t n, const int32_t* __restrict__ comm,
+    const int32_t* __restrict__ part, const int64_t* __restrict__ sigtot, const uint8_t* __restrict__ active,
+    GainParams gp, int32_t* __restrict__ target) {
+  __shared__ int64_t lds[kWavesPerBlock][3][kLightDeg];
+  // each wave owns kMoveGroup consecutive vertices: inactive ones keep their community
+  // (one coalesced store), active light ones are moved one after another
+  const int l = lane_id();
+  const int64_t v0 = (static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + threadIdx.x / kWave) * kMoveGroup;
+  bool todo = false;
+  if (l < kMoveGroup && v0 + l < n) {
+    const int64_t vv = v0 + l;
+    const bool light = indptr[vv + 1] - indptr[vv] <= kLightDeg;  // heavy path writes target[v]
+    if (light && !active[vv]) target[vv] = comm[vv];
+    todo = light && active[vv];
+  }
+  uint64_t pending = __ballot(todo);
+  while (pending) {
+    const int64_t v = v0 + __ffsll(static_cast<long long>(pending)) - 1;
+    pending &= pending - 1;
+    move_vertex(v, indptr, indices, wq, kdeg, n, comm, part, sigtot, gp, target, lds[threadIdx.x / kWave]);
+  }
+}
+
+// degree 257..1024 (graphs below 2^22 vertices): one single-wave block per kMidGroup listed
+// vertices, active ones moved one after another
+__global__ __launch_bounds__(kWave) void k_move_mid(
+    const int32_t* __restrict__ verts, int64_t nm, const int64_t* __restrict__ indptr, const int32_t* __restrict__ indices,
+    const int64_t* __restrict__ wq, const int64_t* __restrict__ kdeg, const int32_t* __restrict__ comm,
+    const int32_t* __restrict__ part, const int64_t* __restrict__ sigtot, const uint8_t* __restrict__ active,
+    GainParams gp, int32_t* __restrict__ target) {
+  __shared__ int64_t lds[3][kMidDeg];
+  const int l = lane_id();
+  const int64_t i0 = static_cast<int64_t>(blockIdx.x) * kMidGroup;
+  bool todo = false;
+  if (l < kMidGroup && i0 + l < nm) {
+    const int64_t vv = verts[i0 + l];
+    if (!active[vv]) target[vv] = comm[vv];
+    else todo = true;
+  }
+  uint64_t pending = __ballot(todo);
+  while (pending) {
+    const int64_t v = verts[i0 + __ffsll(static_cast<long long>(pending)) - 1];
+    pending &= pending - 1;
+    const int32_t curr = comm[v];
+    const int64_t e0 = indptr[v], deg = indptr[v + 1] - e0;
+    const int64_t kv = kdeg[v];
+    if (deg <= 8 * kWave)
+      move_sorted<8>(v, e0, deg, curr, kv, indices, wq, comm, part, sigtot, gp, target, lds[0], lds[1], lds[2]);
+    else
+      move_sorted<16>(v, e0, deg, curr, kv, indices, wq, comm, part, sigtot, gp, target, lds[0], lds[1], lds[2]);
+  }
+}
+
 // ------------------------------------------------------------------------------------------
-// best move, heavy vertices (degree > 64): sort their eligible edges by (vertex, community)
+// best move, heavy vertices (degree above the one-wave paths): sort their eligible edges by
+// (vertex, community)
 // ------------------------------------------------------------------------------------------
 struct Heavy {
   int64_t nh = 0, eh = 0;
@@ -267,17 +526,21 @@ struct Heavy {
   int64_t* hstart;     // nh + 1: first sorted heavy edge per heavy vertex
 };
 
-__global__ void k_heavy_flags(const int64_t* __restrict__ indptr, int64_t n, int32_t* __restrict__ flag,
-                              int64_t* __restrict__ deg_h) {
+// heavy: degree > heavy_min; mid: kLightDeg < degree <= heavy_min
+__global__ void k_heavy_flags(const int64_t* __restrict__ indptr, int64_t n, int64_t heavy_min,
+                              int32_t* __restrict__ flag, int64_t* __restrict__ deg_h,
+                              int32_t* __restrict__ mflag) {
   const int64_t v = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
   if (v < n) {
     const int64_t d = indptr[v + 1] - indptr[v];
-    flag[v] = d > kLightDeg ? 1 : 0;
-    deg_h[v] = d > kLightDeg ? d : 0;
+    flag[v] = d > heavy_min ? 1 : 0;
+    deg_h[v] = d > heavy_min ? d : 0;
+    mflag[v] = (d > kLightDeg && d <= heavy_min) ? 1 : 0;
   }
   if (v == n) {
     flag[n] = 0;
     deg_h[n] = 0;
+    mflag[n] = 0;
   }
 }
 __global__ void k_heavy_compact(const int32_t* __restrict__ flag, const int32_t* __restrict__ pos, int64_t n,
@@ -415,15 +678,27 @@ __global__ void k_commit(int32_t* __restrict__ comm, const int32_t* __restrict__
   if (wants) *wants_flag = 1;  // every writer stores the same value
 }
 
-__global__ __launch_bounds__(kThreads) void k_frontier(const int64_t* __restrict__ indptr, const int32_t* __restrict__ indices,
-                                                       int64_t n, const uint8_t* __restrict__ commit,
-                                                       const uint8_t* __restrict__ held, uint8_t* __restrict__ active) {
-  const int64_t v = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + threadIdx.x / kWave;
-  if (v >= n) return;
-  bool a = false;
-  for (int64_t e = indptr[v] + lane_id(); e < indptr[v + 1]; e += kWave) a |= commit[indices[e]] != 0;
-  a = __any(a);
-  if (lane_id() == 0) active[v] = (a || commit[v] || held[v]) ? 1 : 0;
+// next active set = commit | held | N(commit) (louvain.py:364-372); the graph stores both
+// directions, so marking the neighbours of each committed vertex equals testing every
+// vertex's neighbours.  Concurrent marks all store 1.
+__global__ void k_frontier_self(int64_t n, const uint8_t* __restrict__ commit, const uint8_t* __restrict__ held,
+                                uint8_t* __restrict__ active) {
+  const int64_t v = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (v < n) active[v] = (commit[v] || held[v]) ? 1 : 0;
+}
+// one wave per kMoveGroup consecutive vertices; only committed ones touch their edges
+__global__ __launch_bounds__(kThreads) void k_frontier_push(const int64_t* __restrict__ indptr,
+                                                            const int32_t* __restrict__ indices, int64_t n,
+                                                            const uint8_t* __restrict__ commit,
+                                                            uint8_t* __restrict__ active) {
+  const int l = lane_id();
+  const int64_t v0 = (static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + threadIdx.x / kWave) * kMoveGroup;
+  uint64_t pending = __ballot(l < kMoveGroup && v0 + l < n && commit[v0 + l]);
+  while (pending) {
+    const int64_t v = v0 + __ffsll(static_cast<long long>(pending)) - 1;
+    pending &= pending - 1;
+    for (int64_t e = indptr[v] + l; e < indptr[v + 1]; e += kWave) active[indices[e]] = 1;
+  }
 }
 
 __global__ void k_mark_used(const int32_t* __restrict__ comm, int64_t n, int32_t* __restrict__ used) {
@@ -559,6 +834,8 @@ struct Ctx {
   Heavy hv;
   int32_t *hflag, *hpos;
   int64_t *hdeg, *hdegoff;
+  int32_t *mflag, *mpos, *mverts;  // mid-degree vertices (one-wave sorted path)
+  int64_t nm = 0;
   msc_leiden_info* info;
 };
 
@@ -574,7 +851,7 @@ static void carve_ctx(Carve& cv, Ctx& c, int64_t n, int64_t E) {
   c.terms = cv.take<double>(n1);
   c.wants_flags = cv.take<int32_t>(kMaxPasses);
   c.red = cv.take<int64_t>(8);
-  c.fred = cv.take<float>(8);
+  c.fred = cv.take<float>(kMaxBlocks + 8);
   for (int i = 0; i < 2; ++i) {
     c.gindptr[i] = cv.take<int64_t>(n1);
     c.gindices[i] = cv.take<int32_t>(E1);
@@ -608,6 +885,7 @@ static void carve_ctx(Carve& cv, Ctx& c, int64_t n, int64_t E) {
   h.hstart = cv.take<int64_t>(n1 + 1);
   c.hflag = cv.take<int32_t>(n1); c.hpos = cv.take<int32_t>(n1);
   c.hdeg = cv.take<int64_t>(n1); c.hdegoff = cv.take<int64_t>(n1);
+  c.mflag = cv.take<int32_t>(n1); c.mpos = cv.take<int32_t>(n1); c.mverts = cv.take<int32_t>(n1);
 }
 
 template <typename T>
@@ -628,13 +906,23 @@ static int graph_setup(Ctx& c, Graph& g) {
   MSC_HIP(hipStreamSynchronize(st));
   g.twom = hostv[0];
   g.max_deg = hostv[1];
-  // heavy vertices
+  // mid and heavy vertices
   Heavy& h = c.hv;
   h.nh = 0;
   h.eh = 0;
+  c.nm = 0;
   if (g.max_deg > kLightDeg) {
-    k_heavy_flags<<<grid_for(g.n + 1), kThreads, 0, st>>>(g.indptr, g.n, c.hflag, c.hdeg);
+    const int64_t heavy_min = g.n < kMidMaxN ? kMidDeg : kLightDeg;
+    k_heavy_flags<<<grid_for(g.n + 1), kThreads, 0, st>>>(g.indptr, g.n, heavy_min, c.hflag, c.hdeg, c.mflag);
     MSC_LAUNCH_CHECK();
+    MSC_TRY(prims::exclusive_scan<int32_t>(c.mflag, c.mpos, g.n + 1, c.scan_ws, c.scan_ws_bytes, st));
+    int32_t nm32;
+    MSC_TRY(read_scalar(c.mpos + g.n, &nm32, st));
+    c.nm = nm32;
+    if (c.nm > 0) {
+      k_heavy_compact<<<grid_for(g.n), kThreads, 0, st>>>(c.mflag, c.mpos, g.n, c.mverts);
+      MSC_LAUNCH_CHECK();
+    }
     MSC_TRY(prims::exclusive_scan<int32_t>(c.hflag, c.hpos, g.n + 1, c.scan_ws, c.scan_ws_bytes, st));
     MSC_TRY(prims::exclusive_scan<int64_t>(c.hdeg, c.hdegoff, g.n + 1, c.scan_ws, c.scan_ws_bytes, st));
     int32_t nh32;
@@ -695,24 +983,28 @@ static int local_moving(Ctx& c, const Graph& g, int64_t seed, const int32_t* par
   gp.inv = 1.0 / (static_cast<double>(g.twom) * c.inv_s);
   gp.gamma = c.gamma;
   const uint64_t key = stream_key(seed, part ? 1 : 0);
-  const bool prune_on = g.max_deg <= kDegreeCap;
   k_fill_u8<<<grid_for(g.n), kThreads, 0, st>>>(c.active, g.n, 1);
   k_fill_i32<<<grid_for(kMaxPasses), kThreads, 0, st>>>(c.wants_flags, kMaxPasses, 0);
   MSC_LAUNCH_CHECK();
   int p = 0;
   for (; p < kMaxPasses; ++p) {
     MSC_TRY(sigma_tot(c, g, c.comm));
-    k_move_light<<<grid_for(g.n * kWave), kThreads, 0, st>>>(g.indptr, g.indices, g.wq, g.k, g.n, c.comm, part,
+    k_move_light<<<grid_for(ceil_div(g.n, kMoveGroup) * kWave), kThreads, 0, st>>>(g.indptr, g.indices, g.wq, g.k, g.n, c.comm, part,
                                                             c.sigtot, c.active, gp, c.target);
     MSC_LAUNCH_CHECK();
+    if (c.nm > 0) {
+      k_move_mid<<<static_cast<unsigned>(ceil_div(c.nm, kMidGroup)), kWave, 0, st>>>(
+          c.mverts, c.nm, g.indptr, g.indices, g.wq, g.k, c.comm, part, c.sigtot, c.active, gp, c.target);
+      MSC_LAUNCH_CHECK();
+    }
     MSC_TRY(heavy_moves(c, g, c.comm, part, gp));
     k_commit<<<grid_for(g.n), kThreads, 0, st>>>(c.comm, c.target, g.n, pass_key(key, p), c.thr, c.commit, c.held,
                                                  c.wants_flags + p);
     MSC_LAUNCH_CHECK();
-    if (prune_on) {
-      k_frontier<<<grid_for(g.n * kWave), kThreads, 0, st>>>(g.indptr, g.indices, g.n, c.commit, c.held, c.active);
-      MSC_LAUNCH_CHECK();
-    }
+    k_frontier_self<<<grid_for(g.n), kThreads, 0, st>>>(g.n, c.commit, c.held, c.active);
+    k_frontier_push<<<grid_for(ceil_div(g.n, kMoveGroup) * kWave), kThreads, 0, st>>>(g.indptr, g.indices, g.n,
+                                                                                     c.commit, c.active);
+    MSC_LAUNCH_CHECK();
     if ((p % kSyncEvery == kSyncEvery - 1) || p == kMaxPasses - 1) {
       int32_t any;
       MSC_TRY(read_scalar(c.wants_flags + p, &any, st));
@@ -911,11 +1203,13 @@ extern "C" int msc_leiden(const int64_t* indptr, const int32_t* indices, const f
   // fixed-point exponent from max weight and edge count (both exact)
   float wmax = 0.0f;
   if (nnz > 0) {
-    k_max_weight<<<1, kThreads, 0, st>>>(weights, nnz, c.fred);
+    const int nb = static_cast<int>(std::min<int64_t>(kMaxBlocks, ceil_div(nnz, kThreads)));
+    k_max_weight<<<nb, kThreads, 0, st>>>(weights, nnz, c.fred);
+    k_max_weight<<<1, kThreads, 0, st>>>(c.fred, nb, c.fred + kMaxBlocks);
     MSC_LAUNCH_CHECK();
-    MSC_TRY(read_scalar(c.fred, &wmax, st));
+    MSC_TRY(read_scalar(c.fred + kMaxBlocks, &wmax, st));
   }
-  MSC_REQUIRE(std::isfinite(wmax), "leiden: weights must be finite");
+  MSC_REQUIRE(std::isfinite(wmax), "leiden: weights must be finite and >= 0");
   c.s = (nnz > 0 && wmax > 0.0f) ? 52 - ceil_log2(wmax) - ceil_log2(static_cast<double>(nnz)) : 0;
   c.inv_s = std::ldexp(1.0, -c.s);
   if (nnz > 0) {

@@ -7,6 +7,7 @@ namespace prims {
 constexpr int kScanThreads = 256;
 constexpr int kScanItems = 8;
 constexpr int kScanTile = kScanThreads * kScanItems;
+constexpr int64_t kScanLookback = 2048;  // tiles up to which a block sums its predecessors
 
 template <typename T>
 __device__ __forceinline__ T shfl_up_t(T v, int d) {
@@ -64,9 +65,20 @@ __global__ __launch_bounds__(kScanThreads) void k_block_reduce(const T* __restri
   if (threadIdx.x == 0) sums[blockIdx.x] = tot;
 }
 
+// tile_sums != nullptr: this block's offset is the sum of tile_sums[0, blockIdx.x), summed here
 template <typename T>
 __global__ __launch_bounds__(kScanThreads) void k_block_scan(const T* in, T* out, int64_t n,
-                                                             const T* __restrict__ offsets) {
+                                                             const T* __restrict__ offsets,
+                                                             const T* __restrict__ tile_sums = nullptr) {
+  __shared__ T shoff;
+  if (tile_sums) {
+    T s = 0;
+    for (int i = threadIdx.x; i < static_cast<int>(blockIdx.x); i += kScanThreads) s += tile_sums[i];
+    T tot;
+    block_excl_scan(s, &tot);
+    if (threadIdx.x == 0) shoff = tot;
+    __syncthreads();
+  }
   const int64_t base = static_cast<int64_t>(blockIdx.x) * kScanTile + threadIdx.x * kScanItems;
   T v[kScanItems];
   T s = 0;
@@ -78,6 +90,7 @@ __global__ __launch_bounds__(kScanThreads) void k_block_scan(const T* in, T* out
   T tot;
   T pre = block_excl_scan(s, &tot);
   if (offsets) pre += offsets[blockIdx.x];
+  if (tile_sums) pre += shoff;
 #pragma unroll
   for (int i = 0; i < kScanItems; ++i) {
     if (base + i < n) out[base + i] = pre;
@@ -103,6 +116,7 @@ int exclusive_scan(const T* in, T* out, int64_t n, void* ws, size_t ws_bytes, hi
     MSC_LAUNCH_CHECK();
     return kOk;
   }
+
   const int64_t nt = ceil_div(n, kScanTile);
   const size_t mine = ((nt * sizeof(T) + 255) / 256) * 256;
   if (mine > ws_bytes) {
@@ -112,6 +126,11 @@ int exclusive_scan(const T* in, T* out, int64_t n, void* ws, size_t ws_bytes, hi
   T* sums = static_cast<T*>(ws);
   k_block_reduce<T><<<static_cast<unsigned>(nt), kScanThreads, 0, st>>>(in, n, sums);
   MSC_LAUNCH_CHECK();
+  if (nt <= kScanLookback) {  // each block adds up the preceding tile sums itself
+    k_block_scan<T><<<static_cast<unsigned>(nt), kScanThreads, 0, st>>>(in, out, n, nullptr, sums);
+    MSC_LAUNCH_CHECK();
+    return kOk;
+  }
   MSC_TRY(exclusive_scan<T>(sums, sums, nt, static_cast<char*>(ws) + mine, ws_bytes - mine, st));
   k_block_scan<T><<<static_cast<unsigned>(nt), kScanThreads, 0, st>>>(in, out, n, sums);
   MSC_LAUNCH_CHECK();
@@ -126,10 +145,18 @@ template int exclusive_scan<int64_t>(const int64_t*, int64_t*, int64_t, void*, s
 // ------------------------------------------------------------------------------------------
 // radix sort
 // ------------------------------------------------------------------------------------------
+// One pass = histogram (per block, per digit) -> exclusive scan over (digit, block) ->
+// scatter.  A tile of kRsTile keys is split into kRsWaves contiguous wave segments; lane l
+// of wave w holds items seg(w) + 64 i + l, i < kRsItems, all loaded up front.  Within a
+// wave, keys of equal digit are ranked by ballot peer masks and a wave-private LDS counter
+// (written by one lane per peer group), so ranks follow input order: the sort is stable
+// and uses no atomics.  The scatter stages the tile in LDS in digit order so each digit's
+// run leaves the block as one contiguous, coalesced write.
 constexpr int kRsThreads = 256;
-constexpr int kRsRounds = 16;
-constexpr int kRsTile = kRsThreads * kRsRounds;
+constexpr int kRsItems = 8;
 constexpr int kRsWaves = kRsThreads / kWave;
+constexpr int kRsSeg = kRsItems * kWave;         // keys per wave segment
+constexpr int kRsTile = kRsThreads * kRsItems;   // keys per block
 
 // lanes of the wave holding the same 8-bit digit (valid lanes only)
 __device__ __forceinline__ uint64_t peer_mask(uint32_t digit, bool valid) {
@@ -148,16 +175,21 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_hist(const uint32_t* __restri
                                                         int32_t* __restrict__ counts) {
   __shared__ int32_t h[kRsWaves][256];
   const int tid = threadIdx.x, w = tid / kWave, l = lane_id();
-  for (int i = tid; i < kRsWaves * 256; i += kRsThreads) (&h[0][0])[i] = 0;
+#pragma unroll
+  for (int i = 0; i < kRsWaves; ++i) h[i][tid] = 0;
   __syncthreads();
-  const int64_t base = static_cast<int64_t>(blockIdx.x) * kRsTile;
-  for (int r = 0; r < kRsRounds; ++r) {
-    const int64_t pos = base + r * kRsThreads + tid;
-    const bool valid = pos < n;
-    const uint32_t digit = valid ? (keys[pos] >> shift) & 255u : 0u;
-    const uint64_t peers = peer_mask(digit, valid);
-    if (valid && (__ffsll(static_cast<long long>(peers)) - 1) == l)
-      h[w][digit] += __popcll(peers);
+  const int64_t seg = static_cast<int64_t>(blockIdx.x) * kRsTile + w * kRsSeg;
+  uint32_t dg[kRsItems];
+#pragma unroll
+  for (int i = 0; i < kRsItems; ++i) {
+    const int64_t pos = seg + i * kWave + l;
+    dg[i] = pos < n ? (keys[pos] >> shift) & 255u : 256u;
+  }
+#pragma unroll
+  for (int i = 0; i < kRsItems; ++i) {
+    const bool valid = dg[i] < 256u;
+    const uint64_t peers = peer_mask(dg[i], valid);
+    if (valid && (__ffsll(static_cast<long long>(peers)) - 1) == l) h[w][dg[i]] += __popcll(peers);
   }
   __syncthreads();
   int32_t s = 0;
@@ -169,39 +201,82 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_hist(const uint32_t* __restri
 __global__ __launch_bounds__(kRsThreads) void k_rs_scatter(
     const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin, uint32_t* __restrict__ kout,
     uint32_t* __restrict__ vout, int64_t n, int shift, int nblocks, const int32_t* __restrict__ offsets) {
-  __shared__ int32_t basep[256];
-  __shared__ int32_t wc[kRsWaves][256];
+  __shared__ int32_t wc[kRsWaves][256];   // per-wave digit counts, then per-wave digit offsets
+  __shared__ int32_t dstart[256];         // first tile slot of each digit
+  __shared__ int32_t gbase[256];          // global position of each digit's first key
+  __shared__ uint32_t sk[kRsTile];
+  __shared__ uint32_t sv[kRsTile];
   const int tid = threadIdx.x, w = tid / kWave, l = lane_id();
-  basep[tid] = offsets[static_cast<int64_t>(tid) * nblocks + blockIdx.x];
 #pragma unroll
   for (int i = 0; i < kRsWaves; ++i) wc[i][tid] = 0;
-  __syncthreads();
-  const int64_t base = static_cast<int64_t>(blockIdx.x) * kRsTile;
-  const uint64_t lt = (1ull << l) - 1ull;
-  for (int r = 0; r < kRsRounds; ++r) {
-    const int64_t pos = base + r * kRsThreads + tid;
-    const bool valid = pos < n;
-    const uint32_t key = valid ? kin[pos] : 0u;
-    const uint32_t digit = (key >> shift) & 255u;
-    const uint64_t peers = peer_mask(digit, valid);
-    const int rank = __popcll(peers & lt);
-    if (valid && (__ffsll(static_cast<long long>(peers)) - 1) == l) wc[w][digit] = __popcll(peers);
-    __syncthreads();
-    if (valid) {
-      int32_t off = basep[digit] + rank;
-      for (int i = 0; i < w; ++i) off += wc[i][digit];
-      kout[off] = key;
-      vout[off] = vin[pos];
-    }
-    __syncthreads();
-    int32_t add = 0;
+  gbase[tid] = offsets[static_cast<int64_t>(tid) * nblocks + blockIdx.x];
+  const int64_t tile0 = static_cast<int64_t>(blockIdx.x) * kRsTile;
+  const int64_t seg = tile0 + w * kRsSeg;
+  uint32_t key[kRsItems], val[kRsItems];
+  int rank[kRsItems];
 #pragma unroll
-    for (int i = 0; i < kRsWaves; ++i) {
-      add += wc[i][tid];
-      wc[i][tid] = 0;
+  for (int i = 0; i < kRsItems; ++i) {
+    const int64_t pos = seg + i * kWave + l;
+    const bool valid = pos < n;
+    key[i] = valid ? kin[pos] : 0u;
+    val[i] = valid ? vin[pos] : 0u;
+  }
+  __syncthreads();
+  const uint64_t lt = (1ull << l) - 1ull;
+  const int64_t nvalid = n - seg;  // keys of this wave segment that exist (may be <= 0)
+#pragma unroll
+  for (int i = 0; i < kRsItems; ++i) {
+    const bool valid = i * kWave + l < nvalid;
+    const uint32_t digit = (key[i] >> shift) & 255u;
+    const uint64_t peers = peer_mask(digit, valid);
+    const int before = valid ? wc[w][digit] : 0;   // all reads of this round precede the update
+    rank[i] = before + __popcll(peers & lt);
+    if (valid && (__ffsll(static_cast<long long>(peers)) - 1) == l) wc[w][digit] = before + __popcll(peers);
+  }
+  __syncthreads();
+  // thread tid owns digit tid: per-wave offsets within the digit and the digit's tile start
+  int32_t run = 0;
+#pragma unroll
+  for (int i = 0; i < kRsWaves; ++i) {
+    const int32_t c = wc[i][tid];
+    wc[i][tid] = run;
+    run += c;
+  }
+  // exclusive scan of the digit totals across the block (one digit per thread)
+  __shared__ int32_t wsum[kRsWaves];
+  int32_t inc = run;
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    const int32_t o = __shfl_up(inc, d, kWave);
+    if (l >= d) inc += o;
+  }
+  if (l == kWave - 1) wsum[w] = inc;
+  __syncthreads();
+  int32_t off = 0;
+#pragma unroll
+  for (int i = 0; i < kRsWaves; ++i)
+    if (i < w) off += wsum[i];
+  dstart[tid] = off + inc - run;
+  __syncthreads();
+  // stage in digit order
+#pragma unroll
+  for (int i = 0; i < kRsItems; ++i) {
+    if (i * kWave + l < nvalid) {
+      const uint32_t digit = (key[i] >> shift) & 255u;
+      const int slot = dstart[digit] + wc[w][digit] + rank[i];
+      sk[slot] = key[i];
+      sv[slot] = val[i];
     }
-    basep[tid] += add;
-    __syncthreads();
+  }
+  __syncthreads();
+  // coalesced write-out: slot p holds the (p - dstart[d])-th key of digit d in this tile
+  const int64_t m = n - tile0 < kRsTile ? n - tile0 : kRsTile;
+  for (int p = tid; p < m; p += kRsThreads) {
+    const uint32_t k = sk[p];
+    const uint32_t digit = (k >> shift) & 255u;
+    const int64_t dst = static_cast<int64_t>(gbase[digit]) + (p - dstart[digit]);
+    kout[dst] = k;
+    vout[dst] = sv[p];
   }
 }
 
@@ -253,17 +328,22 @@ int radix_sort_pairs(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t*
   return kOk;
 }
 
+// seg_start[j] = first p with min(keys[p], nseg) >= j, for j = 0..nseg (keys ascending)
 __global__ void k_segment_starts(const uint32_t* __restrict__ keys, int64_t m, int64_t nseg,
                                  int64_t* __restrict__ seg_start) {
-  const int64_t p = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
-  if (p > m) return;
-  const int64_t prev = (p == 0) ? -1 : std::min<int64_t>(keys[p - 1], nseg);
-  const int64_t cur = (p == m) ? nseg : std::min<int64_t>(keys[p], nseg);
-  for (int64_t j = prev + 1; j <= cur; ++j) seg_start[j] = p;
+  const int64_t j = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (j > nseg) return;
+  int64_t lo = 0, hi = m;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (static_cast<int64_t>(keys[mid]) < j) lo = mid + 1;
+    else hi = mid;
+  }
+  seg_start[j] = lo;
 }
 
 int segment_starts(const uint32_t* keys, int64_t m, int64_t nseg, int64_t* seg_start, hipStream_t st) {
-  const unsigned g = static_cast<unsigned>(ceil_div(m + 1, 256));
+  const unsigned g = static_cast<unsigned>(ceil_div(nseg + 1, 256));
   k_segment_starts<<<g, 256, 0, st>>>(keys, m, nseg, seg_start);
   MSC_LAUNCH_CHECK();
   return kOk;

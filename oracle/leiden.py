@@ -12,8 +12,8 @@ graph/csr_graph.py:24-63), vectorised in NumPy.  Per pass:
    Refinement (leiden.py:100-144) restricts neighbours to ``part[u] == part[v]`` and starts
    from singletons.
 3. ``wants = target != comm``; ``coin = u32 < floor(0.9 * 2^32)``; commit ``wants & coin``.
-4. Next active set = commit | N(commit) | (wants & ~coin), only when no vertex has degree
-   > 1024 (louvain.py:350-372).
+4. Next active set = commit | N(commit) | (wants & ~coin) (louvain.py:364-372), on every
+   graph (see the last deviation below).
 5. Convergence test every 4 passes (and at the last): stop when nothing wants to move.
 
 Levels (leiden.py:410-452): move from P; relabel densely (np.unique order); reject the level
@@ -32,7 +32,10 @@ the reference's MLX arithmetic):
 * the coin comes from the counter RNG in oracle/numerics.py keyed by (seed, phase, pass, v)
   instead of MLX's key/split stream;
 * vertices of degree > 1024 follow the same synchronous rule as all others (the reference
-  moves them sequentially on the host, louvain.py:151-181).
+  moves them sequentially on the host, louvain.py:151-181); since no vertex is moved outside
+  the synchronous pass, the frontier rule applies to every graph (the reference switches
+  pruning off when such vertices exist because its host moves bypass the frontier,
+  louvain.py:350-352).
 """
 
 from __future__ import annotations
@@ -41,7 +44,6 @@ import numpy as np
 
 from .numerics import coin_threshold, coin_u32, stream_key
 
-DEGREE_CAP = 1024
 MAX_LEVELS = 20
 TOL = 1e-9
 
@@ -150,7 +152,6 @@ def local_moving_sync(g, gamma, seed, init_comm, part=None, max_passes=200,
     comm = np.asarray(init_comm, np.int64).copy()
     key = stream_key(seed, 0 if part is None else 1)
     thr = coin_threshold(commit_prob)
-    prune_on = g.max_deg <= DEGREE_CAP
     active = np.ones(n, bool)
     vid = np.arange(n, dtype=np.uint64)
     passes = 0
@@ -162,12 +163,11 @@ def local_moving_sync(g, gamma, seed, init_comm, part=None, max_passes=200,
         coin = coin_u32(key, p, vid) < thr
         commit = wants & coin
         comm = np.where(commit, target, comm)
-        if prune_on:
-            nbr = np.zeros(n, bool)
-            if g.src.size:
-                hit = commit[g.indices]
-                nbr = np.bincount(g.src[hit], minlength=n) > 0
-            active = commit | nbr | (wants & ~coin)
+        nbr = np.zeros(n, bool)
+        if g.src.size:
+            hit = commit[g.indices]
+            nbr = np.bincount(g.src[hit], minlength=n) > 0
+        active = commit | nbr | (wants & ~coin)
         if (p % sync_every == sync_every - 1) or (p == max_passes - 1):
             if not wants.any():
                 break

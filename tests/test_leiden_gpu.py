@@ -52,7 +52,8 @@ def test_leiden_resolutions(c1_data, res):
 
 
 def test_leiden_heavy_vertices():
-    """Hubs with degree > 64 (and > 1024, which turns frontier pruning off)."""
+    """Hubs of degree 90 (two neighbours per lane on the one-wave path), 300 and 1500
+    (the sort path over heavy edges); the frontier rule applies with them present."""
     rng = np.random.default_rng(5)
     n = 3000
     rows, cols = [], []
