@@ -28,10 +28,13 @@ struct Geo {
   int gx, gy, gz, dim;
 };
 
+// per-block bounding box (grid-stride) into out + 8 * block: [min x3, max x3, any non-finite]
 __global__ void k_bbox(const float* __restrict__ X, int64_t n, int dim, float* __restrict__ out) {
   float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
   int bad = 0;
-  for (int64_t i = threadIdx.x; i < n; i += blockDim.x)
+  out += 8 * blockIdx.x;
+  for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<int64_t>(gridDim.x) * blockDim.x)
     for (int t = 0; t < dim; ++t) {
       const float v = X[i * dim + t];
       bad |= !isfinite(v);
@@ -71,6 +74,21 @@ __global__ void k_bbox(const float* __restrict__ X, int64_t n, int dim, float* _
       out[3 + t] = b;
     }
 }
+// combine nb per-block boxes (min / max / or are order independent)
+__global__ void k_bbox_final(const float* __restrict__ part, int nb, float* __restrict__ out) {
+  if (threadIdx.x != 0) return;
+  float r[7] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY, 0.0f};
+  for (int b = 0; b < nb; ++b) {
+    const float* q = part + 8 * b;
+    for (int t = 0; t < 3; ++t) {
+      r[t] = fminf(r[t], q[t]);
+      r[3 + t] = fmaxf(r[3 + t], q[3 + t]);
+    }
+    r[6] = fmaxf(r[6], q[6]);
+  }
+  for (int t = 0; t < 7; ++t) out[t] = r[t];
+}
+constexpr int kBBoxBlocks = 256;
 
 __device__ __forceinline__ int cell_coord(float v, float o, float cell, int g) {
   int c = static_cast<int>(floorf((v - o) / cell));
@@ -260,7 +278,7 @@ struct Bufs {
 
 static Bufs carve(Carve& cv, int64_t n) {
   Bufs b{};
-  b.bbox = cv.take<float>(8);
+  b.bbox = cv.take<float>(8 * (kBBoxBlocks + 1));
   b.key = cv.take<uint32_t>(n);
   b.val = cv.take<uint32_t>(n);
   b.skey = cv.take<uint32_t>(n);
@@ -308,7 +326,9 @@ extern "C" int msc_spatial_knn(const float* coords, int64_t n, int dim, int k, i
     set_error("spatial_knn: workspace too small (%zu < %zu)", workspace_bytes, cv.used);
     return kWorkspaceTooSmall;
   }
-  k_bbox<<<1, kThreads, 0, st>>>(coords, n, dim, b.bbox);
+  const int nbb = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(kBBoxBlocks, ceil_div(n, 4 * kThreads))));
+  k_bbox<<<nbb, kThreads, 0, st>>>(coords, n, dim, b.bbox + 8);
+  k_bbox_final<<<1, kWave, 0, st>>>(b.bbox + 8, nbb, b.bbox);
   MSC_LAUNCH_CHECK();
   float bb[8];
   MSC_HIP(hipMemcpyAsync(bb, b.bbox, 7 * sizeof(float), hipMemcpyDeviceToHost, st));

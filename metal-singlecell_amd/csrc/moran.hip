@@ -66,18 +66,29 @@ __global__ void k_row_normalize(const int64_t* __restrict__ indptr, const double
   const double inv = (transform && s != 0.0) ? 1.0 / s : 1.0;
   for (int64_t e = indptr[r]; e < indptr[r + 1]; ++e) w_out[e] = (transform ? (s != 0.0 ? w_in[e] * inv : 0.0) : w_in[e]);
 }
-__global__ void k_sum_f64(const double* __restrict__ v, int64_t m, double* __restrict__ out) {
-  // single block, fixed order: per-thread strided sums, then lanes/waves in order
+// Fixed-order float64 sum in two launches: block b sums its contiguous range (threads
+// strided, then the block's per-thread sums in thread order), then one block sums the block
+// results in block order.  The order depends only on m, so results are deterministic.
+constexpr int kSumBlocks = 512;
+__global__ void k_sum_f64(const double* __restrict__ v, int64_t m, int64_t per, double* __restrict__ out) {
+  const int64_t b0 = blockIdx.x * per, b1 = min(m, b0 + per);
   double s = 0.0;
-  for (int64_t i = threadIdx.x; i < m; i += blockDim.x) s += v[i];
+  for (int64_t i = b0 + threadIdx.x; i < b1; i += blockDim.x) s += v[i];
   __shared__ double sh[kThreads];
   sh[threadIdx.x] = s;
   __syncthreads();
   if (threadIdx.x == 0) {
     double t = 0.0;
     for (int i = 0; i < kThreads; ++i) t += sh[i];
-    *out = t;
+    out[blockIdx.x] = t;
   }
+}
+// out = sum of v[0, m); scratch holds kSumBlocks doubles
+static void sum_f64(const double* v, int64_t m, double* out, double* scratch, hipStream_t st) {
+  const int64_t nb = std::max<int64_t>(1, std::min<int64_t>(kSumBlocks, ceil_div(m, 4 * kThreads)));
+  const int64_t per = ceil_div(std::max<int64_t>(m, 1), nb);
+  k_sum_f64<<<static_cast<unsigned>(nb), kThreads, 0, st>>>(v, m, per, scratch);
+  k_sum_f64<<<1, kThreads, 0, st>>>(scratch, nb, nb, out);
 }
 
 // Y[r, :] = sum_e w_e (X[j_e, :] - mean), float64 accumulate, stored float32; block per row
@@ -122,49 +133,96 @@ __device__ __forceinline__ int64_t feistel_perm(int64_t r, int64_t n, const Feis
 }
 
 // ---- observed numerator and permuted numerators -----------------------------------------------
-// grid: (row chunks, perm batches, gene tiles); partial[chunk][p][g]
+// grid: (row chunks, perm batches, gene tiles); partial[chunk][p][g].  A thread owns VEC
+// consecutive genes (VEC = 4: 16-byte loads, needs G % 4 == 0 and 16-byte aligned rows); rows
+// of the chunk in order, kRowStep at a time: the block first writes the source rows pi_p(r)
+// of the step's rows and kPermBatch permutations to LDS (each computed once), then every
+// thread gathers X[pi_p(r), g] and accumulates x * Y[r, g] in float64, per (p, g) in row
+// order.
+constexpr int kRowStep = 64;
+__device__ __forceinline__ uint64_t perm_key(const Feistel& f, int p) {
+  return mix64(f.key + static_cast<uint64_t>(p + 1) * 0xD1B54A32D192ED03ull);
+}
+template <int VEC>
+__device__ __forceinline__ void load_vec(const float* __restrict__ p, float (&v)[VEC]) {
+  if constexpr (VEC == 4) {
+    const float4 q = *reinterpret_cast<const float4*>(p);
+    v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
+  } else {
+    v[0] = *p;
+  }
+}
+template <int VEC>
 __global__ __launch_bounds__(kThreads) void k_perm_partial(
     const float* __restrict__ X, const float* __restrict__ Y, int64_t n, int64_t G, const double* __restrict__ mean,
     int64_t per, int P, const int32_t* __restrict__ perms, Feistel fst, int observed, double* __restrict__ partial) {
+  __shared__ int32_t jtab[kPermBatch][kRowStep];
   const int64_t r0 = blockIdx.x * per, r1 = min(n, r0 + per);
   const int p0 = blockIdx.y * kPermBatch;
-  const int64_t g = static_cast<int64_t>(blockIdx.z) * kThreads + threadIdx.x;
-  const bool gok = g < G;
-  const double m = gok ? mean[g] : 0.0;
-  double acc[kPermBatch];
-  uint64_t pk[kPermBatch];
+  const int64_t g0 = (static_cast<int64_t>(blockIdx.z) * blockDim.x + threadIdx.x) * VEC;
+  const bool gok = g0 < G;
+  double m[VEC];
 #pragma unroll
-  for (int b = 0; b < kPermBatch; ++b) {
-    acc[b] = 0.0;
-    pk[b] = mix64(fst.key + static_cast<uint64_t>(p0 + b + 1) * 0xD1B54A32D192ED03ull);
-  }
-  for (int64_t r = r0; r < r1; ++r) {
-    const double y = gok ? static_cast<double>(Y[r * G + g]) : 0.0;
+  for (int v = 0; v < VEC; ++v) m[v] = gok ? mean[g0 + v] : 0.0;
+  double acc[kPermBatch][VEC];
+#pragma unroll
+  for (int b = 0; b < kPermBatch; ++b)
+#pragma unroll
+    for (int v = 0; v < VEC; ++v) acc[b][v] = 0.0;
+  for (int64_t rb = r0; rb < r1; rb += kRowStep) {
+    const int nr = static_cast<int>(min(static_cast<int64_t>(kRowStep), r1 - rb));
     if (observed) {
-      const double x = gok ? static_cast<double>(X[r * G + g]) - m : 0.0;
-      acc[0] += x * y;
+      if (!gok) continue;
+      for (int rr = 0; rr < nr; ++rr) {
+        const int64_t r = rb + rr;
+        float x[VEC], y[VEC];
+        load_vec<VEC>(X + r * G + g0, x);
+        load_vec<VEC>(Y + r * G + g0, y);
+#pragma unroll
+        for (int v = 0; v < VEC; ++v) acc[0][v] += (static_cast<double>(x[v]) - m[v]) * static_cast<double>(y[v]);
+      }
       continue;
     }
-#pragma unroll
-    for (int b = 0; b < kPermBatch; ++b) {
+    __syncthreads();  // the previous step's readers are done with jtab
+    for (int q = threadIdx.x; q < kPermBatch * kRowStep; q += blockDim.x) {
+      const int b = q / kRowStep, rr = q % kRowStep;
       const int p = p0 + b;
-      if (p < P) {
-        const int64_t j = perms ? static_cast<int64_t>(perms[static_cast<int64_t>(p) * n + r])
-                                : feistel_perm(r, n, fst, pk[b]);
-        const double x = gok ? static_cast<double>(X[j * G + g]) - m : 0.0;
-        acc[b] += x * y;
+      int32_t j = 0;
+      if (p < P && rr < nr) {
+        const int64_t r = rb + rr;
+        j = perms ? perms[static_cast<int64_t>(p) * n + r]
+                  : static_cast<int32_t>(feistel_perm(r, n, fst, perm_key(fst, p)));
+      }
+      jtab[b][rr] = j;
+    }
+    __syncthreads();
+    if (!gok) continue;
+    for (int rr = 0; rr < nr; ++rr) {
+      float y[VEC];
+      load_vec<VEC>(Y + (rb + rr) * G + g0, y);
+#pragma unroll
+      for (int b = 0; b < kPermBatch; ++b) {
+        if (p0 + b < P) {
+          float x[VEC];
+          load_vec<VEC>(X + static_cast<int64_t>(jtab[b][rr]) * G + g0, x);
+#pragma unroll
+          for (int v = 0; v < VEC; ++v) acc[b][v] += (static_cast<double>(x[v]) - m[v]) * static_cast<double>(y[v]);
+        }
       }
     }
   }
   if (!gok) return;
   if (observed) {
-    partial[blockIdx.x * G + g] = acc[0];
+#pragma unroll
+    for (int v = 0; v < VEC; ++v) partial[blockIdx.x * G + g0 + v] = acc[0][v];
     return;
   }
 #pragma unroll
   for (int b = 0; b < kPermBatch; ++b) {
     const int p = p0 + b;
-    if (p < P) partial[(static_cast<int64_t>(blockIdx.x) * P + p) * G + g] = acc[b];
+    if (p < P)
+#pragma unroll
+      for (int v = 0; v < VEC; ++v) partial[(static_cast<int64_t>(blockIdx.x) * P + p) * G + g0 + v] = acc[b][v];
   }
 }
 
@@ -185,6 +243,8 @@ struct Plan {
   int nchunks;
   int64_t per;
   int gtiles;
+  int vec;       // genes per thread in the permutation kernel (4 when G % 4 == 0)
+  int pthreads;  // its block size
 };
 
 static Plan make_plan(int64_t n, int64_t G, int64_t E, int P) {
@@ -193,7 +253,9 @@ static Plan make_plan(int64_t n, int64_t G, int64_t E, int P) {
   p.G = G;
   p.E = E;
   p.P = P;
-  p.gtiles = static_cast<int>(ceil_div(G, kThreads));
+  p.vec = (G % 4 == 0) ? 4 : 1;
+  p.pthreads = p.vec == 4 ? kThreads / 2 : kThreads;
+  p.gtiles = static_cast<int>(ceil_div(G, static_cast<int64_t>(p.pthreads) * p.vec));
   const int pb = std::max(1, static_cast<int>(ceil_div(std::max(P, 1), kPermBatch)));
   // about 2048 blocks for the permutation kernel, chunks of >= 256 rows
   int64_t chunks = ceil_div(2048, static_cast<int64_t>(pb) * p.gtiles);
@@ -205,11 +267,11 @@ static Plan make_plan(int64_t n, int64_t G, int64_t E, int P) {
 }
 
 struct Bufs {
-  double *mean, *denom, *w, *S0, *partial, *cpartial;
+  double *mean, *denom, *w, *S0, *partial, *cpartial, *sum_scratch;
   float* Y;
 };
 
-constexpr int kColBlocks = 128;
+constexpr int kColBlocks = 1024;
 
 static Bufs carve(Carve& cv, const Plan& p) {
   Bufs b{};
@@ -217,6 +279,7 @@ static Bufs carve(Carve& cv, const Plan& p) {
   b.denom = cv.take<double>(p.G);
   b.w = cv.take<double>(std::max<int64_t>(p.E, 1));
   b.S0 = cv.take<double>(1);
+  b.sum_scratch = cv.take<double>(kSumBlocks);
   b.cpartial = cv.take<double>(static_cast<size_t>(kColBlocks) * p.G);
   b.partial = cv.take<double>(static_cast<size_t>(p.nchunks) * std::max(p.P, 1) * p.G);
   b.Y = cv.take<float>(static_cast<size_t>(p.n) * p.G);
@@ -273,7 +336,7 @@ extern "C" int msc_moran(const float* X, int64_t n, int64_t n_genes, const int64
   MSC_LAUNCH_CHECK();
   k_row_normalize<<<static_cast<unsigned>(ceil_div(n, kThreads)), kThreads, 0, st>>>(w_indptr, w_data, n,
                                                                                        transformation, b.w);
-  k_sum_f64<<<1, kThreads, 0, st>>>(b.w, nnz, b.S0);
+  sum_f64(b.w, nnz, b.S0, b.sum_scratch, st);
   MSC_LAUNCH_CHECK();
   k_spmm<<<static_cast<unsigned>(n), kThreads, 0, st>>>(X, n, G, w_indptr, w_indices, b.w, b.mean, b.Y);
   MSC_LAUNCH_CHECK();
@@ -285,10 +348,19 @@ extern "C" int msc_moran(const float* X, int64_t n, int64_t n_genes, const int64
     fst.mask = (uint64_t(1) << fst.half) - 1;
     fst.key = stream_key(seed, 7);
   }
+  // 16-byte gathers need 16-byte aligned rows of X (Y comes from the workspace, 256-aligned)
+  const bool vec4 = p.vec == 4 && (reinterpret_cast<uintptr_t>(X) % 16 == 0);
+  auto perm_launch = [&](dim3 grid, int P_, const int32_t* pr, int observed, double* part) {
+    if (vec4) k_perm_partial<4><<<grid, p.pthreads, 0, st>>>(X, b.Y, n, G, b.mean, p.per, P_, pr, fst, observed, part);
+    else {
+      const dim3 g1(grid.x, grid.y, static_cast<unsigned>(ceil_div(G, kThreads)));
+      k_perm_partial<1><<<g1, kThreads, 0, st>>>(X, b.Y, n, G, b.mean, p.per, P_, pr, fst, observed, part);
+    }
+  };
   // observed statistic
   {
     const dim3 grid(static_cast<unsigned>(p.nchunks), 1, static_cast<unsigned>(p.gtiles));
-    k_perm_partial<<<grid, kThreads, 0, st>>>(X, b.Y, n, G, b.mean, p.per, 1, nullptr, fst, 1, b.partial);
+    perm_launch(grid, 1, nullptr, 1, b.partial);
     k_finish<<<static_cast<unsigned>(ceil_div(G, kThreads)), kThreads, 0, st>>>(
         b.partial, p.nchunks, 1, G, static_cast<double>(n), b.S0, b.denom, out_I);
     MSC_LAUNCH_CHECK();
@@ -296,7 +368,7 @@ extern "C" int msc_moran(const float* X, int64_t n, int64_t n_genes, const int64
   if (n_perms > 0) {
     const dim3 grid(static_cast<unsigned>(p.nchunks), static_cast<unsigned>(ceil_div(n_perms, kPermBatch)),
                     static_cast<unsigned>(p.gtiles));
-    k_perm_partial<<<grid, kThreads, 0, st>>>(X, b.Y, n, G, b.mean, p.per, n_perms, perms, fst, 0, b.partial);
+    perm_launch(grid, n_perms, perms, 0, b.partial);
     k_finish<<<static_cast<unsigned>(ceil_div(static_cast<int64_t>(n_perms) * G, kThreads)), kThreads, 0, st>>>(
         b.partial, p.nchunks, n_perms, G, static_cast<double>(n), b.S0, b.denom, out_sims);
     MSC_LAUNCH_CHECK();
@@ -398,6 +470,7 @@ extern "C" int msc_graph_moments_workspace_size(int64_t n, int64_t nnz, size_t* 
   cv.take<uint32_t>(nnz + 1); cv.take<uint32_t>(nnz + 1); cv.take<uint32_t>(nnz + 1); cv.take<uint32_t>(nnz + 1);
   cv.take<int64_t>(n + 2);
   cv.take<char>(msc::prims::radix_workspace(nnz + 1));
+  cv.take<double>(kSumBlocks);
   *bytes = cv.used + 256;
   return kOk;
 }
@@ -418,22 +491,23 @@ extern "C" int msc_graph_moments(const int64_t* indptr, const int32_t* indices, 
   int64_t* cstart = cv.take<int64_t>(n + 2);
   const size_t rsb = msc::prims::radix_workspace(nnz + 1);
   void* rws = cv.take<char>(rsb);
+  double* scratch = cv.take<double>(kSumBlocks);
   if (!cv.ok()) {
     set_error("graph_moments: workspace too small");
     return kWorkspaceTooSmall;
   }
   const unsigned g = static_cast<unsigned>(ceil_div(n, kThreads));
   double hs[3];
-  k_sum_f64<<<1, kThreads, 0, st>>>(w, nnz, red);
+  sum_f64(w, nnz, red, scratch, st);
   k_s1_terms<<<g, kThreads, 0, st>>>(indptr, indices, w, n, terms);
-  k_sum_f64<<<1, kThreads, 0, st>>>(terms, n, red + 1);
+  sum_f64(terms, n, red + 1, scratch, st);
   MSC_LAUNCH_CHECK();
   k_edge_dst<<<g, kThreads, 0, st>>>(indptr, indices, n, key, val);
   MSC_LAUNCH_CHECK();
   MSC_TRY(msc::prims::radix_sort_pairs(key, val, skey, sval, nnz, msc::prims::bits_for(n), rws, rsb, st));
   MSC_TRY(msc::prims::segment_starts(skey, nnz, n, cstart, st));
   k_s2_terms<<<g, kThreads, 0, st>>>(indptr, w, cstart, sval, n, terms);
-  k_sum_f64<<<1, kThreads, 0, st>>>(terms, n, red + 2);
+  sum_f64(terms, n, red + 2, scratch, st);
   MSC_LAUNCH_CHECK();
   MSC_HIP(hipMemcpyAsync(hs, red, 3 * sizeof(double), hipMemcpyDeviceToHost, st));
   MSC_HIP(hipStreamSynchronize(st));

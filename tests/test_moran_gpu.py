@@ -89,3 +89,19 @@ def test_moran_constant_gene_is_nan():
     X = np.c_[X, np.full(X.shape[0], 3.0, np.float32)]
     I, _, _ = _gpu(X, _knn_W(xy))
     assert np.isnan(I[-1]) and np.isfinite(I[:-1]).all()
+
+
+@pytest.mark.parametrize("n_genes", [8, 7])
+def test_moran_wide_genes_match_oracle(n_genes):
+    """G % 4 == 0 takes the four-genes-per-thread (16-byte) gather path, other G the scalar
+    one; both must meet the float64 oracle at 1e-6."""
+    from oracle import moran
+    xy, X3 = _lattice_signals(20, jitter=5.0, seed=2)
+    rng = np.random.default_rng(3)
+    X = np.c_[X3, rng.gamma(2.0, 1.0, (X3.shape[0], n_genes - 3))].astype(np.float32)
+    W = _knn_W(xy)
+    perms = moran.permutations(X.shape[0], 37, 5)
+    I, sims, _ = _gpu(X, W, 37, perms)
+    rI, rs = moran.moran_dense(X, moran.row_normalize(W), perms)
+    np.testing.assert_allclose(I, rI, atol=1e-6, rtol=0)
+    np.testing.assert_allclose(sims, rs, atol=1e-6, rtol=0)
