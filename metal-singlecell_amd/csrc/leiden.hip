@@ -14,10 +14,10 @@
 // of scheduling.
 //
 // Per-vertex best move: vertices of degree <= 256 take one wave (up to four neighbours per
-// lane); degree 257..1024 take one wave each from a list (up to sixteen per lane); both sort
-// the neighbours' communities inside the wave.  Heavier vertices (and every vertex above 256
-// when the graph has 2^22 or more vertices) go through a sort path over their edges keyed by
-// (vertex, community).
+// lane); degree 257..1024 and 1025..4096 take one wave each from a list (up to sixteen and
+// sixty-four per lane); all sort the neighbours' communities inside the wave.  Heavier
+// vertices (and the list ranges whose community ids would not fit the sort key: 2^22 and
+// 2^20 vertices and up) go through a sort path over their edges keyed by (vertex, community).
 #include <cmath>
 #include <vector>
 
@@ -293,8 +293,46 @@ constexpr int64_t kSortedMaxN = int64_t(1) << 24;  // NI <= 4
 constexpr int kMidItems = 16;
 constexpr int kMidDeg = kWave * kMidItems;           // 1024
 constexpr int64_t kMidMaxN = int64_t(1) << 22;      // NI = 16
+constexpr int kBigItems = 64;
+constexpr int kBigDeg = kWave * kBigItems;           // 4096
+constexpr int64_t kBigMaxN = int64_t(1) << 20;      // NI = 64
 constexpr int kMoveGroup = 8;  // consecutive vertices per wave in k_move_light / k_frontier_push
 constexpr int kMidGroup = 2;   // listed vertices per single-wave block in k_move_mid
+
+// Bitonic network over the 64 * NI slots e = i * 64 + l of a wave (keys in registers).  The
+// stage sequence is unrolled by template recursion so that every register index is a
+// compile-time constant for any NI.
+template <int NI, int K, int J>
+__device__ __forceinline__ void bitonic_stage(uint32_t (&key)[NI], int l) {
+  if constexpr (J >= kWave) {  // partner slot in another item of the same lane
+#pragma unroll
+    for (int i = 0; i < NI; ++i) {
+      const int ip = i ^ (J / kWave);
+      if (ip > i) {
+        const bool asc = ((i * kWave + l) & K) == 0;
+        const uint32_t a = key[i], b = key[ip];
+        const uint32_t lo = a < b ? a : b, hi = a < b ? b : a;
+        key[i] = asc ? lo : hi;
+        key[ip] = asc ? hi : lo;
+      }
+    }
+  } else {  // partner slot in lane l ^ J of the same item
+#pragma unroll
+    for (int i = 0; i < NI; ++i) {
+      const uint32_t p = static_cast<uint32_t>(__shfl_xor(static_cast<int>(key[i]), J, kWave));
+      const bool asc = ((i * kWave + l) & K) == 0;
+      const bool lower = (l & J) == 0;
+      const uint32_t lo = key[i] < p ? key[i] : p, hi = key[i] < p ? p : key[i];
+      key[i] = (lower == asc) ? lo : hi;
+    }
+  }
+  if constexpr (J > 1) bitonic_stage<NI, K, J / 2>(key, l);
+}
+template <int NI, int K>
+__device__ __forceinline__ void bitonic_sort_slots(uint32_t (&key)[NI], int l) {
+  bitonic_stage<NI, K, K / 2>(key, l);
+  if constexpr (K < NI * kWave) bitonic_sort_slots<NI, 2 * K>(key, l);
+}
 
 template <int NI>
 __device__ __forceinline__ void move_sorted(int64_t v, int64_t e0, int64_t deg, int32_t curr, int64_t kv,
@@ -302,10 +340,12 @@ __device__ __forceinline__ void move_sorted(int64_t v, int64_t e0, int64_t deg, 
                                             const int32_t* __restrict__ comm, const int32_t* __restrict__ part,
                                             const int64_t* __restrict__ sigtot, const GainParams& gp,
                                             int32_t* __restrict__ target, int64_t* __restrict__ lw,
-                                            int64_t* __restrict__ lsg, int64_t* __restrict__ lp) {
+                                            int64_t* __restrict__ lsg, int64_t* __restrict__ lp,
+                                            uint64_t* __restrict__ lhm) {
   constexpr int N = NI * kWave;
-  constexpr int IB = NI == 1 ? 6 : (NI == 2 ? 7 : (NI == 4 ? 8 : (NI == 8 ? 9 : 10)));  // slot bits
-  static_assert(NI == 1 || NI == 2 || NI == 4 || NI == 8 || NI == 16, "NI");
+  constexpr int IB = NI == 1 ? 6 : NI == 2 ? 7 : NI == 4 ? 8 : NI == 8 ? 9 : NI == 16 ? 10 : NI == 32 ? 11 : 12;
+  static_assert(NI == 1 || NI == 2 || NI == 4 || NI == 8 || NI == 16 || NI == 32 || NI == 64, "NI");
+  constexpr bool kHmLds = NI > 16;  // run-head masks in LDS (lhm, NI entries) instead of registers
   constexpr uint32_t kNone = 0xFFFFFFFFu;
   const int l = lane_id();
   const int32_t pv = part ? part[v] : 0;
@@ -331,40 +371,10 @@ __device__ __forceinline__ void move_sorted(int64_t v, int64_t e0, int64_t deg, 
   for (int i = 0; i < NI; ++i)
     lsg[i * kWave + l] = key[i] != kNone ? sigtot[key[i] >> IB] : 0;
 
-  // bitonic sort, ascending in slot order e = i * 64 + l
-#pragma unroll
-  for (int k = 2; k <= N; k <<= 1) {
-#pragma unroll
-    for (int j = k >> 1; j > 0; j >>= 1) {
-      if (j >= kWave) {
-#pragma unroll
-        for (int i = 0; i < NI; ++i) {
-          const int ip = i ^ (j / kWave);
-          if (ip > i) {
-            const bool asc = ((i * kWave + l) & k) == 0;
-            const uint32_t a = key[i], b = key[ip];
-            const uint32_t lo = a < b ? a : b, hi = a < b ? b : a;
-            key[i] = asc ? lo : hi;
-            key[ip] = asc ? hi : lo;
-          }
-        }
-      } else {
-#pragma unroll
-        for (int i = 0; i < NI; ++i) {
-          const uint32_t p = static_cast<uint32_t>(__shfl_xor(static_cast<int>(key[i]), j, kWave));
-          const bool asc = ((i * kWave + l) & k) == 0;
-          const bool lower = (l & j) == 0;
-          const uint32_t lo = key[i] < p ? key[i] : p, hi = key[i] < p ? p : key[i];
-          key[i] = (lower == asc) ? lo : hi;
-        }
-      }
-    }
-  }
+  bitonic_sort_slots<NI, 2>(key, l);  // ascending in slot order e = i * 64 + l
 
-  // per slot: community, weight prefix, run heads
-  int32_t c[NI];
-  int64_t sg[NI];
-  uint64_t hmask[NI];
+  // per slot: weight prefix and run heads (community = key >> IB)
+  uint64_t hmask[kHmLds ? 1 : NI];
   int64_t carry = 0;
   int32_t c_last = -1;  // community of slot (i * 64 - 1)
 #pragma unroll
@@ -372,9 +382,8 @@ __device__ __forceinline__ void move_sorted(int64_t v, int64_t e0, int64_t deg, 
     const int e = i * kWave + l;
     const bool valid = e < nvalid;
     const uint32_t src = key[i] & ((1u << IB) - 1u);
-    c[i] = valid ? static_cast<int32_t>(key[i] >> IB) : -1;
+    const int32_t ci = valid ? static_cast<int32_t>(key[i] >> IB) : -1;
     const int64_t w = valid ? lw[src] : 0;
-    sg[i] = valid ? lsg[src] : 0;
     // inclusive prefix of w over the slots
     int64_t x = w;
 #pragma unroll
@@ -385,10 +394,15 @@ __device__ __forceinline__ void move_sorted(int64_t v, int64_t e0, int64_t deg, 
     x += carry;
     carry = static_cast<int64_t>(shfl_u64(static_cast<uint64_t>(x), kWave - 1));
     lp[e] = x;
-    int32_t cprev = __shfl_up(c[i], 1, kWave);
+    int32_t cprev = __shfl_up(ci, 1, kWave);
     if (l == 0) cprev = c_last;
-    hmask[i] = __ballot(valid && (e == 0 || cprev != c[i]));
-    c_last = __shfl(c[i], kWave - 1, kWave);
+    const uint64_t hm = __ballot(valid && (e == 0 || cprev != ci));
+    if constexpr (kHmLds) {
+      if (l == 0) lhm[i] = hm;
+    } else {
+      hmask[kHmLds ? 0 : i] = hm;
+    }
+    c_last = __shfl(ci, kWave - 1, kWave);
   }
 
   double best_g = -1e300, stay = 0.0;
@@ -399,19 +413,27 @@ __device__ __forceinline__ void move_sorted(int64_t v, int64_t e0, int64_t deg, 
   for (int i = 0; i < NI; ++i) {
     const int e = i * kWave + l;
     const bool valid = e < nvalid;
-    const bool next_head = (l < kWave - 1) ? ((hmask[i] >> (l + 1)) & 1ull) != 0
-                                           : (i + 1 < NI ? (hmask[i + 1 < NI ? i + 1 : i] & 1ull) != 0 : false);
+    uint64_t hmi, hmn;  // run heads of this item and of the next one
+    if constexpr (kHmLds) {
+      hmi = lhm[i];
+      hmn = i + 1 < NI ? lhm[i + 1 < NI ? i + 1 : i] : 0ull;
+    } else {
+      hmi = hmask[kHmLds ? 0 : i];
+      hmn = i + 1 < NI ? hmask[kHmLds ? 0 : (i + 1 < NI ? i + 1 : i)] : 0ull;
+    }
+    const bool next_head = (l < kWave - 1) ? ((hmi >> (l + 1)) & 1ull) != 0 : (hmn & 1ull) != 0;
     const bool end = valid && (e + 1 == nvalid || next_head);
     const uint64_t upto = (l == kWave - 1) ? ~0ull : ((2ull << l) - 1ull);
-    const uint64_t hm = hmask[i] & upto;
+    const uint64_t hm = hmi & upto;
     const int64_t start = hm ? static_cast<int64_t>(i * kWave + 63 - __clzll(static_cast<long long>(hm))) : head_carry;
-    if (hmask[i]) head_carry = i * kWave + 63 - __clzll(static_cast<long long>(hmask[i]));
+    if (hmi) head_carry = i * kWave + 63 - __clzll(static_cast<long long>(hmi));
     if (end) {
+      const int32_t ci = static_cast<int32_t>(key[i] >> IB);
       const int64_t wc = lp[e] - (start > 0 ? lp[start - 1] : 0);
-      const int64_t sig = sg[i] - (c[i] == curr ? kv : 0);
+      const int64_t sig = lsg[key[i] & ((1u << IB) - 1u)] - (ci == curr ? kv : 0);
       const double g = gain_of(wc, sig, kv, gp);
-      if (c[i] == curr) { stay = g; has_stay = true; }
-      if (better(g, c[i], best_g, best_c)) { best_g = g; best_c = c[i]; }
+      if (ci == curr) { stay = g; has_stay = true; }
+      if (better(g, ci, best_g, best_c)) { best_g = g; best_c = ci; }
     }
   }
 #pragma unroll
@@ -438,10 +460,10 @@ __device__ __forceinline__ void move_vertex(int64_t v, const int64_t* __restrict
     int64_t* lw = lds[0];
     int64_t* lsg = lds[1];
     int64_t* lp = lds[2];
-    if (deg <= kWave) move_sorted<1>(v, e0, deg, curr, kv, indices, wq, comm, part, sigtot, gp, target, lw, lsg, lp);
+    if (deg <= kWave) move_sorted<1>(v, e0, deg, curr, kv, indices, wq, comm, part, sigtot, gp, target, lw, lsg, lp, nullptr);
     else if (deg <= 2 * kWave)
-      move_sorted<2>(v, e0, deg, curr, kv, indices, wq, comm, part, sigtot, gp, target, lw, lsg, lp);
-    else move_sorted<4>(v, e0, deg, curr, kv, indices, wq, comm, part, sigtot, gp, target, lw, lsg, lp);
+      move_sorted<2>(v, e0, deg, curr, kv, indices, wq, comm, part, sigtot, gp, target, lw, lsg, lp, nullptr);
+    else move_sorted<4>(v, e0, deg, curr, kv, indices, wq, comm, part, sigtot, gp, target, lw, lsg, lp, nullptr);
   } else {
     if (deg <= kWave) move_one<1>(v, e0, deg, curr, kv, indices, wq, comm, part, sigtot, gp, target);
     else if (deg <= 2 * kWave) move_one<2>(v, e0, deg, curr, kv, indices, wq, comm, part, sigtot, gp, target);
@@ -498,10 +520,33 @@ __global__ __launch_bounds__(kWave) void k_move_mid(
     const int64_t e0 = indptr[v], deg = indptr[v + 1] - e0;
     const int64_t kv = kdeg[v];
     if (deg <= 8 * kWave)
-      move_sorted<8>(v, e0, deg, curr, kv, indices, wq, comm, part, sigtot, gp, target, lds[0], lds[1], lds[2]);
+      move_sorted<8>(v, e0, deg, curr, kv, indices, wq, comm, part, sigtot, gp, target, lds[0], lds[1], lds[2], nullptr);
     else
-      move_sorted<16>(v, e0, deg, curr, kv, indices, wq, comm, part, sigtot, gp, target, lds[0], lds[1], lds[2]);
+      move_sorted<16>(v, e0, deg, curr, kv, indices, wq, comm, part, sigtot, gp, target, lds[0], lds[1], lds[2], nullptr);
   }
+}
+
+// degree 1025..4096 (graphs below 2^20 vertices): one single-wave block per listed vertex
+// (the wave's slot arrays take 96 KiB of LDS, so one such block per CU)
+__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_move_big(
+    const int32_t* __restrict__ verts, const int64_t* __restrict__ indptr, const int32_t* __restrict__ indices,
+    const int64_t* __restrict__ wq, const int64_t* __restrict__ kdeg, const int32_t* __restrict__ comm,
+    const int32_t* __restrict__ part, const int64_t* __restrict__ sigtot, const uint8_t* __restrict__ active,
+    GainParams gp, int32_t* __restrict__ target) {
+  __shared__ int64_t lds[3][kBigDeg];
+  __shared__ uint64_t lhm[kBigItems];
+  const int64_t v = verts[blockIdx.x];
+  const int32_t curr = comm[v];
+  if (!active[v]) {
+    if (lane_id() == 0) target[v] = curr;
+    return;
+  }
+  const int64_t e0 = indptr[v], deg = indptr[v + 1] - e0;
+  const int64_t kv = kdeg[v];
+  if (deg <= 32 * kWave)
+    move_sorted<32>(v, e0, deg, curr, kv, indices, wq, comm, part, sigtot, gp, target, lds[0], lds[1], lds[2], lhm);
+  else
+    move_sorted<64>(v, e0, deg, curr, kv, indices, wq, comm, part, sigtot, gp, target, lds[0], lds[1], lds[2], lhm);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -526,21 +571,23 @@ struct Heavy {
   int64_t* hstart;     // nh + 1: first sorted heavy edge per heavy vertex
 };
 
-// heavy: degree > heavy_min; mid: kLightDeg < degree <= heavy_min
-__global__ void k_heavy_flags(const int64_t* __restrict__ indptr, int64_t n, int64_t heavy_min,
+// mid: kLightDeg < degree <= mid_max; big: mid_max < degree <= big_max; heavy: above
+__global__ void k_heavy_flags(const int64_t* __restrict__ indptr, int64_t n, int64_t mid_max, int64_t big_max,
                               int32_t* __restrict__ flag, int64_t* __restrict__ deg_h,
-                              int32_t* __restrict__ mflag) {
+                              int32_t* __restrict__ mflag, int32_t* __restrict__ bflag) {
   const int64_t v = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
   if (v < n) {
     const int64_t d = indptr[v + 1] - indptr[v];
-    flag[v] = d > heavy_min ? 1 : 0;
-    deg_h[v] = d > heavy_min ? d : 0;
-    mflag[v] = (d > kLightDeg && d <= heavy_min) ? 1 : 0;
+    flag[v] = d > big_max ? 1 : 0;
+    deg_h[v] = d > big_max ? d : 0;
+    mflag[v] = (d > kLightDeg && d <= mid_max) ? 1 : 0;
+    bflag[v] = (d > mid_max && d <= big_max) ? 1 : 0;
   }
   if (v == n) {
     flag[n] = 0;
     deg_h[n] = 0;
     mflag[n] = 0;
+    bflag[n] = 0;
   }
 }
 __global__ void k_heavy_compact(const int32_t* __restrict__ flag, const int32_t* __restrict__ pos, int64_t n,
@@ -804,6 +851,9 @@ __global__ void k_scatter_pagg(const int32_t* __restrict__ R, const int32_t* __r
 // ------------------------------------------------------------------------------------------
 struct Ctx {
   hipStream_t st;
+  hipStream_t side = nullptr;       // big-vertex moves run here, beside k_move_light
+  hipStream_t side2 = nullptr;      // mid-vertex moves
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_join2 = nullptr;
   int64_t n0, E0;
   double gamma;
   uint64_t thr;
@@ -836,6 +886,8 @@ struct Ctx {
   int64_t *hdeg, *hdegoff;
   int32_t *mflag, *mpos, *mverts;  // mid-degree vertices (one-wave sorted path)
   int64_t nm = 0;
+  int32_t *bflag, *bpos, *bverts;  // big-degree vertices (one-wave sorted path, 96 KiB LDS)
+  int64_t nb = 0;
   msc_leiden_info* info;
 };
 
@@ -886,6 +938,7 @@ static void carve_ctx(Carve& cv, Ctx& c, int64_t n, int64_t E) {
   c.hflag = cv.take<int32_t>(n1); c.hpos = cv.take<int32_t>(n1);
   c.hdeg = cv.take<int64_t>(n1); c.hdegoff = cv.take<int64_t>(n1);
   c.mflag = cv.take<int32_t>(n1); c.mpos = cv.take<int32_t>(n1); c.mverts = cv.take<int32_t>(n1);
+  c.bflag = cv.take<int32_t>(n1); c.bpos = cv.take<int32_t>(n1); c.bverts = cv.take<int32_t>(n1);
 }
 
 template <typename T>
@@ -911,16 +964,27 @@ static int graph_setup(Ctx& c, Graph& g) {
   h.nh = 0;
   h.eh = 0;
   c.nm = 0;
+  c.nb = 0;
   if (g.max_deg > kLightDeg) {
-    const int64_t heavy_min = g.n < kMidMaxN ? kMidDeg : kLightDeg;
-    k_heavy_flags<<<grid_for(g.n + 1), kThreads, 0, st>>>(g.indptr, g.n, heavy_min, c.hflag, c.hdeg, c.mflag);
+    // community ids must fit the sorted paths' keys: 22 bits for NI = 16, 20 for NI = 64
+    const int64_t mid_max = g.n < kMidMaxN ? kMidDeg : kLightDeg;
+    const int64_t big_max = g.n < kBigMaxN ? kBigDeg : mid_max;
+    k_heavy_flags<<<grid_for(g.n + 1), kThreads, 0, st>>>(g.indptr, g.n, mid_max, big_max, c.hflag, c.hdeg, c.mflag,
+                                                          c.bflag);
     MSC_LAUNCH_CHECK();
     MSC_TRY(prims::exclusive_scan<int32_t>(c.mflag, c.mpos, g.n + 1, c.scan_ws, c.scan_ws_bytes, st));
-    int32_t nm32;
+    MSC_TRY(prims::exclusive_scan<int32_t>(c.bflag, c.bpos, g.n + 1, c.scan_ws, c.scan_ws_bytes, st));
+    int32_t nm32, nb32;
     MSC_TRY(read_scalar(c.mpos + g.n, &nm32, st));
+    MSC_TRY(read_scalar(c.bpos + g.n, &nb32, st));
     c.nm = nm32;
+    c.nb = nb32;
     if (c.nm > 0) {
       k_heavy_compact<<<grid_for(g.n), kThreads, 0, st>>>(c.mflag, c.mpos, g.n, c.mverts);
+      MSC_LAUNCH_CHECK();
+    }
+    if (c.nb > 0) {
+      k_heavy_compact<<<grid_for(g.n), kThreads, 0, st>>>(c.bflag, c.bpos, g.n, c.bverts);
       MSC_LAUNCH_CHECK();
     }
     MSC_TRY(prims::exclusive_scan<int32_t>(c.hflag, c.hpos, g.n + 1, c.scan_ws, c.scan_ws_bytes, st));
@@ -989,13 +1053,34 @@ static int local_moving(Ctx& c, const Graph& g, int64_t seed, const int32_t* par
   int p = 0;
   for (; p < kMaxPasses; ++p) {
     MSC_TRY(sigma_tot(c, g, c.comm));
+    // the three move kernels write target[] of disjoint vertex sets and only read the rest:
+    // the listed (mid / big) vertices run on the side stream concurrently with the light ones
+    const bool fork = c.nm > 0 || c.nb > 0;
+    if (fork) {
+      MSC_HIP(hipEventRecord(c.ev_fork, st));
+      if (c.nb > 0) MSC_HIP(hipStreamWaitEvent(c.side, c.ev_fork, 0));
+      if (c.nm > 0) MSC_HIP(hipStreamWaitEvent(c.side2, c.ev_fork, 0));
+    }
     k_move_light<<<grid_for(ceil_div(g.n, kMoveGroup) * kWave), kThreads, 0, st>>>(g.indptr, g.indices, g.wq, g.k, g.n, c.comm, part,
                                                             c.sigtot, c.active, gp, c.target);
     MSC_LAUNCH_CHECK();
+    if (c.nb > 0) {
+      k_move_big<<<static_cast<unsigned>(c.nb), kWave, 0, c.side>>>(c.bverts, g.indptr, g.indices, g.wq, g.k, c.comm,
+                                                                    part, c.sigtot, c.active, gp, c.target);
+      MSC_LAUNCH_CHECK();
+    }
     if (c.nm > 0) {
-      k_move_mid<<<static_cast<unsigned>(ceil_div(c.nm, kMidGroup)), kWave, 0, st>>>(
+      k_move_mid<<<static_cast<unsigned>(ceil_div(c.nm, kMidGroup)), kWave, 0, c.side2>>>(
           c.mverts, c.nm, g.indptr, g.indices, g.wq, g.k, c.comm, part, c.sigtot, c.active, gp, c.target);
       MSC_LAUNCH_CHECK();
+    }
+    if (c.nb > 0) {
+      MSC_HIP(hipEventRecord(c.ev_join, c.side));
+      MSC_HIP(hipStreamWaitEvent(st, c.ev_join, 0));
+    }
+    if (c.nm > 0) {
+      MSC_HIP(hipEventRecord(c.ev_join2, c.side2));
+      MSC_HIP(hipStreamWaitEvent(st, c.ev_join2, 0));
     }
     MSC_TRY(heavy_moves(c, g, c.comm, part, gp));
     k_commit<<<grid_for(g.n), kThreads, 0, st>>>(c.comm, c.target, g.n, pass_key(key, p), c.thr, c.commit, c.held,
@@ -1185,6 +1270,24 @@ extern "C" int msc_leiden(const int64_t* indptr, const int32_t* indices, const f
   MSC_REQUIRE(commit_prob > 0.0 && commit_prob <= 1.0, "leiden: commit_prob must be in (0, 1]");
   Ctx c{};
   c.st = static_cast<hipStream_t>(stream);
+  // side stream (default priority) and fork/join events, released on every return path
+  struct SideGuard {
+    Ctx& c;
+    ~SideGuard() {
+      for (hipStream_t s : {c.side, c.side2})
+        if (s) {
+          (void)hipStreamSynchronize(s);
+          (void)hipStreamDestroy(s);
+        }
+      for (hipEvent_t e : {c.ev_fork, c.ev_join, c.ev_join2})
+        if (e) (void)hipEventDestroy(e);
+    }
+  } side_guard{c};
+  MSC_HIP(hipStreamCreateWithFlags(&c.side, hipStreamNonBlocking));
+  MSC_HIP(hipStreamCreateWithFlags(&c.side2, hipStreamNonBlocking));
+  MSC_HIP(hipEventCreateWithFlags(&c.ev_fork, hipEventDisableTiming));
+  MSC_HIP(hipEventCreateWithFlags(&c.ev_join, hipEventDisableTiming));
+  MSC_HIP(hipEventCreateWithFlags(&c.ev_join2, hipEventDisableTiming));
   c.n0 = n;
   c.E0 = nnz;
   c.gamma = resolution;

@@ -52,17 +52,18 @@ def test_leiden_resolutions(c1_data, res):
 
 
 def test_leiden_heavy_vertices():
-    """Hubs of degree 90 (two neighbours per lane on the one-wave path), 300 and 1500
-    (the sort path over heavy edges); the frontier rule applies with them present."""
+    """Hubs of degree 90 (two neighbours per lane on the one-wave path), 300 (sixteen-per-lane
+    list path), 1500 (sixty-four-per-lane list path) and 5000 (the sort path over heavy
+    edges); the frontier rule applies with them present."""
     rng = np.random.default_rng(5)
-    n = 3000
+    n = 8000
     rows, cols = [], []
-    for hub, deg in [(0, 1500), (1, 300), (2, 90)]:
-        nb = rng.choice(np.arange(3, n), deg, replace=False)
+    for hub, deg in [(0, 5000), (1, 1500), (2, 300), (3, 90)]:
+        nb = rng.choice(np.arange(4, n), deg, replace=False)
         rows += [hub] * deg
         cols += nb.tolist()
     # planted blocks of 100
-    for b in range(30):
+    for b in range(n // 100):
         m = np.arange(b * 100, (b + 1) * 100)
         r = rng.choice(m, 800)
         c = rng.choice(m, 800)
@@ -73,6 +74,7 @@ def test_leiden_heavy_vertices():
     A = (A + A.T).tocsr()
     A.setdiag(0)
     A.eliminate_zeros()
+    assert np.diff(A.indptr).max() > 4096
     _check(A, random_state=1)
 
 
