@@ -163,6 +163,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=10.0)
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="first run one plain device copy of this many MiB (counter calibration)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
@@ -173,6 +175,15 @@ def main():
     from metalsinglecell import synthetic
 
     torch.cuda.set_device(0)
+    if a.copy_probe_mib > 0:
+        # a plain copy of known size: its FETCH_SIZE / WRITE_SIZE counters calibrate the
+        # counters read for the select kernel in the same profiled process
+        src = torch.ones(a.copy_probe_mib << 18, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copy probe: {a.copy_probe_mib} MiB read + {a.copy_probe_mib} MiB written")
+        del src, dst
     t = time.perf_counter()
     X, _ = synthetic.gaussian_mixture_zipf(n=a.n, seed=2)
     log(f"data {X.shape} in {time.perf_counter() - t:.1f}s")

@@ -314,13 +314,15 @@ __global__ __launch_bounds__(kThreads, 2) void select_kernel(
   }
 
   uint64_t lst[2][L];
-  float tau[2];
+  float tau[2];  // each list's own tail (the certificate threshold)
+  float tq[2];   // admission threshold: the smallest tail over the lanes (and lists) of the query
   int cnt[2] = {0, 0};
 #pragma unroll
   for (int nb = 0; nb < 2; ++nb) {
 #pragma unroll
     for (int i = 0; i < L; ++i) lst[nb][i] = kEmpty;
     tau[nb] = __uint_as_float(0x7F800000u);
+    tq[nb] = tau[nb];
   }
 
   const int64_t ntiles = (c_end > c_begin) ? ceil_div(c_end - c_begin, kTile) : 0;
@@ -412,13 +414,13 @@ __global__ __launch_bounds__(kThreads, 2) void select_kernel(
         const int li = WIDE ? (mb >> 1) : nb;  // list receiving this group
         bool h = false;
 #pragma unroll
-        for (int i = 0; i < 4; ++i) h |= acc[mb][nb][i] <= tau[li];
+        for (int i = 0; i < 4; ++i) h |= acc[mb][nb][i] <= tq[li];
         if (__builtin_amdgcn_ballot_w64(h) == 0ull) continue;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
           const float v = acc[mb][nb][i];
           const int64_t c = c0 + 16 * mb + 4 * g + i;
-          if (v <= tau[li] && (!tail_tile || c < c_end)) {
+          if (v <= tq[li] && (!tail_tile || c < c_end)) {
             hitbuf[wv][li][cnt[li]][l] = (static_cast<uint64_t>(float_ord(v)) << 32) | static_cast<uint64_t>(c);
             ++cnt[li];
           }
@@ -431,6 +433,17 @@ __global__ __launch_bounds__(kThreads, 2) void select_kernel(
           }
           cnt[li] = 0;
           tau[li] = tail_value(lst[li][L - 1]);
+          // A candidate above the smallest tail of its query's lists can never be needed:
+          // the certificate bounds every rejected candidate by that smallest tail.
+          float t = WIDE ? fminf(tau[0], tau[1]) : tau[li];
+          t = fminf(t, __shfl_xor(t, 16, kWave));
+          t = fminf(t, __shfl_xor(t, 32, kWave));
+          if (WIDE) {
+            tq[0] = t;
+            tq[1] = t;
+          } else {
+            tq[li] = t;
+          }
         }
       }
     }
@@ -483,6 +496,19 @@ __device__ __forceinline__ float exact_d2(const float* __restrict__ X, int d, in
   const float* a = X + i * d;
   const float* b = X + j * d;
   float acc = 0.0f;
+  if ((d & 1) == 0 && (reinterpret_cast<uintptr_t>(X) & 7) == 0) {  // 8-byte aligned rows: pairwise
+                                                                      // loads, same summation order
+    const float2* a2 = reinterpret_cast<const float2*>(a);
+    const float2* b2 = reinterpret_cast<const float2*>(b);
+    for (int t = 0; t < d / 2; ++t) {
+      const float2 x = a2[t], y = b2[t];
+      const float d0 = __fsub_rn(x.x, y.x);
+      acc = __fadd_rn(acc, __fmul_rn(d0, d0));
+      const float d1 = __fsub_rn(x.y, y.y);
+      acc = __fadd_rn(acc, __fmul_rn(d1, d1));
+    }
+    return acc;
+  }
   for (int t = 0; t < d; ++t) {
     const float diff = __fsub_rn(a[t], b[t]);
     acc = __fadd_rn(acc, __fmul_rn(diff, diff));

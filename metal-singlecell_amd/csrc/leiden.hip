@@ -504,6 +504,7 @@ __global__ __launch_bounds__(kWave) void k_move_mid(
     const int32_t* __restrict__ part, const int64_t* __restrict__ sigtot, const uint8_t* __restrict__ active,
     GainParams gp, int32_t* __restrict__ target) {
   __shared__ int64_t lds[3][kMidDeg];
+  __builtin_amdgcn_s_setprio(2);  // as in k_move_big
   const int l = lane_id();
   const int64_t i0 = static_cast<int64_t>(blockIdx.x) * kMidGroup;
   bool todo = false;
@@ -535,6 +536,9 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(1, 1))) v
     GainParams gp, int32_t* __restrict__ target) {
   __shared__ int64_t lds[3][kBigDeg];
   __shared__ uint64_t lhm[kBigItems];
+  // a few long single-wave sorts share SIMDs with the many short light-vertex waves:
+  // raise their issue priority so they are not the last to finish
+  __builtin_amdgcn_s_setprio(3);
   const int64_t v = verts[blockIdx.x];
   const int32_t curr = comm[v];
   if (!active[v]) {
@@ -1061,9 +1065,6 @@ static int local_moving(Ctx& c, const Graph& g, int64_t seed, const int32_t* par
       if (c.nb > 0) MSC_HIP(hipStreamWaitEvent(c.side, c.ev_fork, 0));
       if (c.nm > 0) MSC_HIP(hipStreamWaitEvent(c.side2, c.ev_fork, 0));
     }
-    k_move_light<<<grid_for(ceil_div(g.n, kMoveGroup) * kWave), kThreads, 0, st>>>(g.indptr, g.indices, g.wq, g.k, g.n, c.comm, part,
-                                                            c.sigtot, c.active, gp, c.target);
-    MSC_LAUNCH_CHECK();
     if (c.nb > 0) {
       k_move_big<<<static_cast<unsigned>(c.nb), kWave, 0, c.side>>>(c.bverts, g.indptr, g.indices, g.wq, g.k, c.comm,
                                                                     part, c.sigtot, c.active, gp, c.target);
@@ -1074,6 +1075,10 @@ static int local_moving(Ctx& c, const Graph& g, int64_t seed, const int32_t* par
           c.mverts, c.nm, g.indptr, g.indices, g.wq, g.k, c.comm, part, c.sigtot, c.active, gp, c.target);
       MSC_LAUNCH_CHECK();
     }
+    // the listed vertices go first so that their large-LDS blocks find free CUs
+    k_move_light<<<grid_for(ceil_div(g.n, kMoveGroup) * kWave), kThreads, 0, st>>>(g.indptr, g.indices, g.wq, g.k, g.n, c.comm, part,
+                                                            c.sigtot, c.active, gp, c.target);
+    MSC_LAUNCH_CHECK();
     if (c.nb > 0) {
       MSC_HIP(hipEventRecord(c.ev_join, c.side));
       MSC_HIP(hipStreamWaitEvent(st, c.ev_join, 0));
