@@ -69,11 +69,24 @@ int msc_knn_exact(const float* X, int64_t n, int d, int k, int precision, int32_
  * only its own sample).  X: total x d rows, samples contiguous; offsets (host): n_samples+1
  * row offsets.  out_idx / out_dist: total x k, column ids GLOBAL (offset of the sample
  * included), so the result is the block-diagonal kNN of the concatenation.  Every sample
- * needs >= k cells; d <= 125.  Replaces per-sample calls of neighbors._knn. */
+ * needs >= k cells; d <= 125.  splits (1..16, 0 = 1): each sample's candidates are scanned
+ * in that many ranges by separate blocks (a tile-shape knob; results do not depend on it).
+ * Replaces per-sample calls of neighbors._knn (src/metalsinglecell/neighbors.py:698-724),
+ * as bbknn's per-batch loop does (neighbors.py:491-559).
+ *   _prepare uploads the layout tables into the workspace (synchronises);
+ *   _launch enqueues the kernels only: no allocation, no host synchronisation, so it may be
+ *   captured in a HIP graph; it needs a prior _prepare with the same offsets, d, k,
+ *   precision, splits and workspace;
+ *   msc_knn_batched = _prepare + _launch (+ the fallback count in info). */
 int msc_knn_batched_workspace_size(const int64_t* offsets /*host*/, int n_samples, int d, int k,
-                                   int precision, size_t* bytes /*host*/);
+                                   int precision, int splits, size_t* bytes /*host*/);
+int msc_knn_batched_prepare(const int64_t* offsets /*host*/, int n_samples, int d, int k, int precision,
+                            int splits, void* workspace, size_t workspace_bytes, void* stream);
+int msc_knn_batched_launch(const float* X, const int64_t* offsets /*host*/, int n_samples, int d, int k,
+                           int precision, int splits, int32_t* out_idx, float* out_dist, void* workspace,
+                           size_t workspace_bytes, void* stream);
 int msc_knn_batched(const float* X, const int64_t* offsets /*host*/, int n_samples, int d, int k,
-                    int precision, int32_t* out_idx, float* out_dist, void* workspace,
+                    int precision, int splits, int32_t* out_idx, float* out_dist, void* workspace,
                     size_t workspace_bytes, void* stream, msc_knn_info* info /*host, may be NULL*/);
 
 /* obsp['distances'] as CSR from a self-inclusive kNN (neighbors.py:755-760): k-1 entries per
@@ -97,6 +110,20 @@ int msc_fuzzy_connectivities(const int32_t* knn_idx, const float* knn_dist, int6
                              int64_t capacity, float* out_sigma, float* out_rho,
                              int64_t* out_nnz /*host*/, void* workspace, size_t workspace_bytes,
                              void* stream);
+
+/* The same for a batch of samples whose kNN rows are concatenated (global column ids, as
+ * msc_knn_batched writes them): the output is the block-diagonal CSR whose block s equals
+ * msc_fuzzy_connectivities of sample s alone (the mean kNN distance used by the sigma floor
+ * is taken per sample).  group_offsets (DEVICE, n_groups+1 row offsets).  capacity must be
+ * >= 2*n*(k-1); nnz = out_indptr[n] stays on the device.  No allocation and no host
+ * synchronisation: may be captured in a HIP graph.  Replaces per-sample calls of the
+ * fuzzy_simplicial_set step (neighbors.py:742-753). */
+int msc_fuzzy_batched_workspace_size(int64_t n, int k, int n_groups, size_t* bytes /*host*/);
+int msc_fuzzy_connectivities_batched(const int32_t* knn_idx, const float* knn_dist, int64_t n, int k,
+                                     const int64_t* group_offsets, int n_groups, int64_t* out_indptr,
+                                     int32_t* out_indices, float* out_data, int64_t capacity,
+                                     float* out_sigma, float* out_rho, void* workspace,
+                                     size_t workspace_bytes, void* stream);
 
 /* ---------------------------------------------------------------------------------------
  * (d) Synchronous parallel Leiden (modularity, resolution gamma).

@@ -19,9 +19,26 @@ namespace fuzzy {
 
 constexpr int kThreads = 256;
 
-__global__ void k_dist_partial(const float* __restrict__ dist, int64_t m, int64_t per,
+// Mean of the kNN distances, per group of rows (batched samples) or over all rows (goff null):
+// group g (blockIdx.y) splits its m_g = rows_g * k distances into gridDim.x fixed ranges, so
+// a group's sum is formed exactly as for a call on that group alone.
+__device__ __forceinline__ void group_span(const int64_t* goff, int64_t m, int k, int g, int64_t* base, int64_t* mg) {
+  if (goff) {
+    *base = goff[g] * k;
+    *mg = (goff[g + 1] - goff[g]) * k;
+  } else {
+    *base = 0;
+    *mg = m;
+  }
+}
+__global__ void k_dist_partial(const float* __restrict__ dist, int64_t m, int k, const int64_t* __restrict__ goff,
                                double* __restrict__ partial) {
-  const int64_t b0 = blockIdx.x * per, b1 = min(m, b0 + per);
+  int64_t base, mg;
+  group_span(goff, m, k, blockIdx.y, &base, &mg);
+  dist += base;
+  partial += static_cast<int64_t>(blockIdx.y) * gridDim.x;
+  const int64_t per = (mg + gridDim.x - 1) / gridDim.x;
+  const int64_t b0 = blockIdx.x * per, b1 = min(mg, b0 + per);
   double s = 0.0;
   for (int64_t i = b0 + threadIdx.x; i < b1; i += blockDim.x) s += static_cast<double>(dist[i]);
   s = wave_sum_f64(s);
@@ -35,18 +52,22 @@ __global__ void k_dist_partial(const float* __restrict__ dist, int64_t m, int64_
   }
 }
 
-__global__ void k_dist_mean(const double* __restrict__ partial, int nb, int64_t m, double* __restrict__ mean) {
+__global__ void k_dist_mean(const double* __restrict__ partial, int nb, int64_t m, int k,
+                            const int64_t* __restrict__ goff, double* __restrict__ mean) {
   if (threadIdx.x == 0) {
+    int64_t base, mg;
+    group_span(goff, m, k, blockIdx.x, &base, &mg);
     double t = 0.0;
-    for (int b = 0; b < nb; ++b) t += partial[b];
-    *mean = t / static_cast<double>(m);
+    for (int b = 0; b < nb; ++b) t += partial[static_cast<int64_t>(blockIdx.x) * nb + b];
+    mean[blockIdx.x] = t / static_cast<double>(mg);
   }
 }
 
 // one thread per row; the row's distances staged in LDS (k <= 64)
 __global__ __launch_bounds__(64) void k_smooth_membership(
     const int32_t* __restrict__ idx, const float* __restrict__ dist, int64_t n, int k,
-    double target, const double* __restrict__ mean_all, float* __restrict__ sigma_out,
+    double target, const double* __restrict__ mean_all, const int64_t* __restrict__ goff, int n_groups,
+    float* __restrict__ sigma_out,
     float* __restrict__ rho_out, float* __restrict__ vals) {
   __shared__ float sd[64 * 65];
   const int64_t r0 = static_cast<int64_t>(blockIdx.x) * 64;
@@ -83,7 +104,17 @@ __global__ __launch_bounds__(64) void k_smooth_membership(
     }
   }
   float sigma = mid;
-  const double floor_v = (rho > 0.0f) ? 1e-3 * (rowsum / static_cast<double>(k)) : 1e-3 * (*mean_all);
+  int grp = 0;  // group (sample) of row i: last g with goff[g] <= i
+  if (goff) {
+    int lo = 0, hi = n_groups - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (goff[mid] <= i) lo = mid;
+      else hi = mid - 1;
+    }
+    grp = lo;
+  }
+  const double floor_v = (rho > 0.0f) ? 1e-3 * (rowsum / static_cast<double>(k)) : 1e-3 * mean_all[grp];
   if (static_cast<double>(sigma) < floor_v) sigma = static_cast<float>(floor_v);
   sigma_out[i] = sigma;
   rho_out[i] = rho;
@@ -129,7 +160,7 @@ __global__ __launch_bounds__(kThreads) void k_union(
     const int32_t* __restrict__ idx, const float* __restrict__ vals, int64_t n, int k,
     const int64_t* __restrict__ in_start, const uint32_t* __restrict__ in_eid,
     int64_t* __restrict__ row_nnz, const int64_t* __restrict__ indptr,
-    int32_t* __restrict__ out_idx, float* __restrict__ out_val) {
+    int32_t* __restrict__ out_idx, float* __restrict__ out_val, int64_t cap) {
   __shared__ int64_t fcol[kThreads / kWave][64];
   __shared__ int32_t fpre[kThreads / kWave][65];
   const int w = threadIdx.x / kWave, l = lane_id();
@@ -185,8 +216,10 @@ __global__ __launch_bounds__(kThreads) void k_union(
       v = __fsub_rn(__fadd_rn(a, b), __fmul_rn(a, b));
     }
     const int64_t pos = base + l + (lb - s0) - pre;
-    out_idx[pos] = static_cast<int32_t>(col);
-    out_val[pos] = v;
+    if (pos < cap) {
+      out_idx[pos] = static_cast<int32_t>(col);
+      out_val[pos] = v;
+    }
   }
   // in-only entries
   for (int64_t p = s0 + l; p < s1; p += kWave) {
@@ -202,8 +235,10 @@ __global__ __launch_bounds__(kThreads) void k_union(
     const bool dup = lo < nF && fcol[w][lo] == s;
     if (!dup) {
       const int64_t pos = base + lo + ((p - s0) - fpre[w][lo]);
-      out_idx[pos] = static_cast<int32_t>(s);
-      out_val[pos] = vals[e];
+      if (pos < cap) {
+        out_idx[pos] = static_cast<int32_t>(s);
+        out_val[pos] = vals[e];
+      }
     }
   }
 }
@@ -226,11 +261,11 @@ struct Bufs {
 
 constexpr int kMeanBlocks = 256;
 
-static Bufs carve(Carve& cv, int64_t n, int k) {
+static Bufs carve(Carve& cv, int64_t n, int k, int n_groups = 1) {
   Bufs b{};
   const int64_t m = n * k;
-  b.partial = cv.take<double>(kMeanBlocks);
-  b.mean = cv.take<double>(1);
+  b.partial = cv.take<double>(static_cast<size_t>(kMeanBlocks) * n_groups);
+  b.mean = cv.take<double>(n_groups);
   b.vals = cv.take<float>(m);
   b.key = cv.take<uint32_t>(m);
   b.eid = cv.take<uint32_t>(m);
@@ -266,6 +301,45 @@ extern "C" int msc_fuzzy_workspace_size(int64_t n, int k, size_t* bytes) {
   return kOk;
 }
 
+// all kernels of the connectivities pipeline; goff (device, n_groups + 1 row offsets) or null
+static int fuzzy_kernels(const int32_t* knn_idx, const float* knn_dist, int64_t n, int k, const int64_t* goff,
+                         int n_groups, int64_t* out_indptr, int32_t* out_indices, float* out_data, int64_t capacity,
+                         float* out_sigma, float* out_rho, const Bufs& b, hipStream_t st, int64_t* host_nnz) {
+  const int64_t m = n * k;
+  k_dist_partial<<<dim3(kMeanBlocks, n_groups), kThreads, 0, st>>>(knn_dist, m, k, goff, b.partial);
+  k_dist_mean<<<n_groups, 64, 0, st>>>(b.partial, kMeanBlocks, m, k, goff, b.mean);
+  MSC_LAUNCH_CHECK();
+  const double target = std::log2(static_cast<double>(k));
+  k_smooth_membership<<<static_cast<unsigned>(ceil_div(n, 64)), 64, 0, st>>>(
+      knn_idx, knn_dist, n, k, target, b.mean, goff, n_groups, out_sigma, out_rho, b.vals);
+  MSC_LAUNCH_CHECK();
+  k_edge_keys<<<static_cast<unsigned>(ceil_div(m, kThreads)), kThreads, 0, st>>>(knn_idx, b.vals, n, k,
+                                                                                 b.key, b.eid);
+  MSC_LAUNCH_CHECK();
+  MSC_TRY(prims::radix_sort_pairs(b.key, b.eid, b.skey, b.seid, m, prims::bits_for(n), b.sort_ws,
+                                  b.sort_ws_bytes, st));
+  MSC_TRY(prims::segment_starts(b.skey, m, n, b.in_start, st));
+  const unsigned wb = static_cast<unsigned>(ceil_div(n, kThreads / kWave));
+  k_union<false><<<wb, kThreads, 0, st>>>(knn_idx, b.vals, n, k, b.in_start, b.seid, b.row_nnz,
+                                          nullptr, nullptr, nullptr, 0);
+  MSC_LAUNCH_CHECK();
+  MSC_TRY(prims::exclusive_scan<int64_t>(b.row_nnz, out_indptr, n + 1, b.scan_ws, b.scan_ws_bytes, st));
+  if (host_nnz) {
+    int64_t nnz = 0;
+    MSC_HIP(hipMemcpyAsync(&nnz, out_indptr + n, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    MSC_HIP(hipStreamSynchronize(st));
+    if (nnz > capacity) {
+      set_error("fuzzy: output capacity %lld < nnz %lld", (long long)capacity, (long long)nnz);
+      return kInvalidArgument;
+    }
+    *host_nnz = nnz;
+  }
+  k_union<true><<<wb, kThreads, 0, st>>>(knn_idx, b.vals, n, k, b.in_start, b.seid, nullptr,
+                                         out_indptr, out_indices, out_data, capacity);
+  MSC_LAUNCH_CHECK();
+  return kOk;
+}
+
 extern "C" int msc_fuzzy_connectivities(const int32_t* knn_idx, const float* knn_dist, int64_t n,
                                         int k, int64_t* out_indptr, int32_t* out_indices,
                                         float* out_data, int64_t capacity, float* out_sigma,
@@ -282,36 +356,36 @@ extern "C" int msc_fuzzy_connectivities(const int32_t* knn_idx, const float* knn
     set_error("fuzzy: workspace too small (%zu < %zu)", workspace_bytes, cv.used);
     return kWorkspaceTooSmall;
   }
-  const int64_t m = n * k;
-  const int64_t per = ceil_div(m, kMeanBlocks);
-  k_dist_partial<<<kMeanBlocks, kThreads, 0, st>>>(knn_dist, m, per, b.partial);
-  k_dist_mean<<<1, 64, 0, st>>>(b.partial, kMeanBlocks, m, b.mean);
-  MSC_LAUNCH_CHECK();
-  const double target = std::log2(static_cast<double>(k));
-  k_smooth_membership<<<static_cast<unsigned>(ceil_div(n, 64)), 64, 0, st>>>(
-      knn_idx, knn_dist, n, k, target, b.mean, out_sigma, out_rho, b.vals);
-  MSC_LAUNCH_CHECK();
-  k_edge_keys<<<static_cast<unsigned>(ceil_div(m, kThreads)), kThreads, 0, st>>>(knn_idx, b.vals, n, k,
-                                                                                 b.key, b.eid);
-  MSC_LAUNCH_CHECK();
-  MSC_TRY(prims::radix_sort_pairs(b.key, b.eid, b.skey, b.seid, m, prims::bits_for(n), b.sort_ws,
-                                  b.sort_ws_bytes, st));
-  MSC_TRY(prims::segment_starts(b.skey, m, n, b.in_start, st));
-  const unsigned wb = static_cast<unsigned>(ceil_div(n, kThreads / kWave));
-  k_union<false><<<wb, kThreads, 0, st>>>(knn_idx, b.vals, n, k, b.in_start, b.seid, b.row_nnz,
-                                          nullptr, nullptr, nullptr);
-  MSC_LAUNCH_CHECK();
-  MSC_TRY(prims::exclusive_scan<int64_t>(b.row_nnz, out_indptr, n + 1, b.scan_ws, b.scan_ws_bytes, st));
-  int64_t nnz = 0;
-  MSC_HIP(hipMemcpyAsync(&nnz, out_indptr + n, sizeof(int64_t), hipMemcpyDeviceToHost, st));
-  MSC_HIP(hipStreamSynchronize(st));
-  if (nnz > capacity) {
-    set_error("fuzzy: output capacity %lld < nnz %lld", (long long)capacity, (long long)nnz);
-    return kInvalidArgument;
-  }
-  k_union<true><<<wb, kThreads, 0, st>>>(knn_idx, b.vals, n, k, b.in_start, b.seid, nullptr,
-                                         out_indptr, out_indices, out_data);
-  MSC_LAUNCH_CHECK();
-  *out_nnz = nnz;
+  return fuzzy_kernels(knn_idx, knn_dist, n, k, nullptr, 1, out_indptr, out_indices, out_data, capacity, out_sigma,
+                       out_rho, b, st, out_nnz);
+}
+
+extern "C" int msc_fuzzy_batched_workspace_size(int64_t n, int k, int n_groups, size_t* bytes) {
+  MSC_TRY(validate(n, k));
+  MSC_REQUIRE(bytes && n_groups >= 1, "fuzzy_batched: bad arguments");
+  Carve cv(nullptr, 0);
+  carve(cv, n, k, n_groups);
+  *bytes = cv.used + 256;
   return kOk;
+}
+
+extern "C" int msc_fuzzy_connectivities_batched(const int32_t* knn_idx, const float* knn_dist, int64_t n, int k,
+                                                const int64_t* group_offsets, int n_groups, int64_t* out_indptr,
+                                                int32_t* out_indices, float* out_data, int64_t capacity,
+                                                float* out_sigma, float* out_rho, void* workspace,
+                                                size_t workspace_bytes, void* stream) {
+  MSC_TRY(validate(n, k));
+  MSC_REQUIRE(knn_idx && knn_dist && group_offsets && n_groups >= 1 && out_indptr && out_indices && out_data &&
+                  out_sigma && out_rho && workspace,
+              "fuzzy_batched: null pointer argument");
+  MSC_REQUIRE(capacity >= 2 * n * (k - 1), "fuzzy_batched: capacity must be >= 2*n*(k-1)");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  Carve cv(workspace, workspace_bytes);
+  Bufs b = carve(cv, n, k, n_groups);
+  if (!cv.ok()) {
+    set_error("fuzzy_batched: workspace too small (%zu < %zu)", workspace_bytes, cv.used);
+    return kWorkspaceTooSmall;
+  }
+  return fuzzy_kernels(knn_idx, knn_dist, n, k, group_offsets, n_groups, out_indptr, out_indices, out_data, capacity,
+                       out_sigma, out_rho, b, st, nullptr);
 }

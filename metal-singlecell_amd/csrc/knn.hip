@@ -269,8 +269,11 @@ __global__ __launch_bounds__(kThreads, 2) void select_kernel(
   const int lm = l & 15;    // row within an MFMA block
   const int split = blockIdx.y;
   // candidate range: one split of [0, n), or the block's own range (batched samples)
-  const int64_t c_begin = blk_range ? blk_range[2 * blockIdx.x] : static_cast<int64_t>(split) * split_len;
-  const int64_t c_end = blk_range ? blk_range[2 * blockIdx.x + 1] : min(n, c_begin + split_len);
+  // candidate range: one split of [0, n), or this block's split of its own sample (batched
+  // samples: blk_range holds [begin, end) per (split, block))
+  const int64_t br = 2 * (static_cast<int64_t>(split) * gridDim.x + blockIdx.x);
+  const int64_t c_begin = blk_range ? blk_range[br] : static_cast<int64_t>(split) * split_len;
+  const int64_t c_end = blk_range ? blk_range[br + 1] : min(n, c_begin + split_len);
   // query slots: slot s is row s, or row qrows[s] when a row list is given
   const int64_t sbase = static_cast<int64_t>(blockIdx.x) * QPB + wv * (16 * NB);
   const char* Xb = static_cast<const char*>(Xp);
@@ -1025,30 +1028,38 @@ __global__ void k_unpad(const int32_t* __restrict__ idx_pad, const float* __rest
 }  // namespace msc
 
 namespace {
+// Padded layout: sample s occupies rows [P_s, P_s + n_s) of an n_tot x d copy, P_s % 128 == 0.
+// Each query block of a sample scans its own sample, split into S candidate ranges of whole
+// 64-row tiles (ranges past the sample's end are empty).
 struct BatchLayout {
   int64_t n_tot = 0, nblk = 0;
+  int S = 1;
   std::vector<int32_t> prow2row, prow_sample;
   std::vector<int64_t> row_range, blk_range;
 };
 
-int make_layout(const int64_t* offsets, int n_samples, int k, BatchLayout* L) {
+int make_layout(const int64_t* offsets, int n_samples, int k, int S, BatchLayout* L) {
+  using msc::knn::kQueriesPerBlock;
+  using msc::knn::kTile;
+  MSC_REQUIRE(S >= 1 && S <= 16, "knn_batched: splits must be in [1, 16]");
   int64_t P = 0;
   for (int s = 0; s < n_samples; ++s) {
     const int64_t ns = offsets[s + 1] - offsets[s];
     MSC_REQUIRE(ns >= k, "knn_batched: sample %d has %lld cells < k=%d", s, (long long)ns, k);
-    P += msc::ceil_div(ns, msc::knn::kQueriesPerBlock) * msc::knn::kQueriesPerBlock;
+    P += msc::ceil_div(ns, kQueriesPerBlock) * kQueriesPerBlock;
   }
   MSC_REQUIRE(P < (int64_t(1) << 31) - 256, "knn_batched: total padded rows too large");
   L->n_tot = P;
-  L->nblk = P / msc::knn::kQueriesPerBlock;
+  L->nblk = P / kQueriesPerBlock;
+  L->S = S;
   L->prow2row.assign(P, -1);
   L->prow_sample.assign(P, 0);
   L->row_range.assign(2 * P, 0);
-  L->blk_range.assign(2 * L->nblk, 0);
+  L->blk_range.assign(2 * L->nblk * S, 0);
   int64_t base = 0;
   for (int s = 0; s < n_samples; ++s) {
     const int64_t ns = offsets[s + 1] - offsets[s];
-    const int64_t padded = msc::ceil_div(ns, msc::knn::kQueriesPerBlock) * msc::knn::kQueriesPerBlock;
+    const int64_t padded = msc::ceil_div(ns, kQueriesPerBlock) * kQueriesPerBlock;
     for (int64_t i = 0; i < padded; ++i) {
       const int64_t r = base + i;
       L->prow_sample[r] = s;
@@ -1058,94 +1069,138 @@ int make_layout(const int64_t* offsets, int n_samples, int k, BatchLayout* L) {
         L->row_range[2 * r + 1] = base + ns;
       }
     }
-    for (int64_t b = base / msc::knn::kQueriesPerBlock; b < (base + padded) / msc::knn::kQueriesPerBlock; ++b) {
-      L->blk_range[2 * b] = base;
-      L->blk_range[2 * b + 1] = base + ns;
-    }
+    const int64_t slen = msc::ceil_div(msc::ceil_div(ns, S), kTile) * kTile;
+    for (int64_t b = base / kQueriesPerBlock; b < (base + padded) / kQueriesPerBlock; ++b)
+      for (int t = 0; t < S; ++t) {
+        const int64_t lo = std::min(base + ns, base + t * slen);
+        const int64_t hi = std::min(base + ns, lo + slen);
+        L->blk_range[2 * (t * L->nblk + b)] = lo;
+        L->blk_range[2 * (t * L->nblk + b) + 1] = hi;
+      }
     base += padded;
   }
   return msc::kOk;
 }
 
-size_t batched_bytes(int64_t n_in, int n_samples, const BatchLayout& L, int d, int k, int precision) {
+// device copies of the layout tables and the per-call buffers, carved identically by the
+// size query, prepare and launch
+struct BatchBufs {
+  msc::knn::Plan p;
+  msc::knn::Buffers b;
+  float* Xpad;
+  int32_t *prow2row, *prow_sample;
+  int64_t *row_range, *blk_range, *in_off;
+  int32_t* idx_pad;
+  float* dist_pad;
+};
+
+BatchBufs carve_batched(msc::Carve& cv, const BatchLayout& L, int n_samples, int d, int k, int precision) {
   using namespace msc::knn;
-  Plan p = make_plan(L.n_tot, d, k, precision);
-  msc::Carve cv(nullptr, 0);
-  carve(cv, p);
-  cv.take<float>(static_cast<size_t>(L.n_tot) * d);  // padded X
-  cv.take<int32_t>(L.n_tot);                          // prow2row
-  cv.take<int32_t>(L.n_tot);                          // prow_sample
-  cv.take<int64_t>(2 * L.n_tot);                      // row_range
-  cv.take<int64_t>(2 * L.nblk);                       // blk_range
-  cv.take<int64_t>(n_samples + 1);                    // input offsets
-  cv.take<int32_t>(static_cast<size_t>(L.n_tot) * k); // padded idx
-  cv.take<float>(static_cast<size_t>(L.n_tot) * k);   // padded dist
-  (void)n_in;
-  return cv.used + 256;
+  BatchBufs t{};
+  t.p = make_plan(L.n_tot, d, k, precision);
+  t.p.S = L.S;  // keys for S splits of every slot
+  t.b = carve(cv, t.p);
+  t.Xpad = cv.take<float>(static_cast<size_t>(L.n_tot) * d);
+  t.prow2row = cv.take<int32_t>(L.n_tot);
+  t.prow_sample = cv.take<int32_t>(L.n_tot);
+  t.row_range = cv.take<int64_t>(2 * L.n_tot);
+  t.blk_range = cv.take<int64_t>(2 * L.nblk * L.S);
+  t.in_off = cv.take<int64_t>(n_samples + 1);
+  t.idx_pad = cv.take<int32_t>(static_cast<size_t>(L.n_tot) * k);
+  t.dist_pad = cv.take<float>(static_cast<size_t>(L.n_tot) * k);
+  return t;
+}
+
+int batched_common(const int64_t* offsets, int n_samples, int d, int k, int precision, int splits, BatchLayout* L) {
+  MSC_REQUIRE(offsets && n_samples >= 1, "knn_batched: bad arguments");
+  MSC_REQUIRE(precision == MSC_PRECISION_FP32 || precision == MSC_PRECISION_BF16_RERANK, "knn_batched: precision");
+  MSC_REQUIRE(k >= 1 && k <= 64 && d >= 1 && d <= 125, "knn_batched: bad d/k");
+  return make_layout(offsets, n_samples, k, splits <= 0 ? 1 : splits, L);
 }
 }  // namespace
 
 extern "C" int msc_knn_batched_workspace_size(const int64_t* offsets, int n_samples, int d, int k, int precision,
-                                              size_t* bytes) {
-  MSC_REQUIRE(offsets && bytes && n_samples >= 1, "knn_batched: bad arguments");
-  MSC_REQUIRE(k >= 1 && k <= 64 && d >= 1, "knn_batched: bad d/k");
+                                              int splits, size_t* bytes) {
+  MSC_REQUIRE(bytes, "knn_batched: bytes is null");
   BatchLayout L;
-  MSC_TRY(make_layout(offsets, n_samples, k, &L));
-  *bytes = batched_bytes(offsets[n_samples], n_samples, L, d, k, precision);
+  MSC_TRY(batched_common(offsets, n_samples, d, k, precision, splits, &L));
+  Carve cv(nullptr, 0);
+  carve_batched(cv, L, n_samples, d, k, precision);
+  *bytes = cv.used + 256;
+  return kOk;
+}
+
+extern "C" int msc_knn_batched_prepare(const int64_t* offsets, int n_samples, int d, int k, int precision, int splits,
+                                       void* workspace, size_t workspace_bytes, void* stream) {
+  BatchLayout L;
+  MSC_TRY(batched_common(offsets, n_samples, d, k, precision, splits, &L));
+  MSC_REQUIRE(workspace, "knn_batched: null workspace");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  Carve cv(workspace, workspace_bytes);
+  BatchBufs t = carve_batched(cv, L, n_samples, d, k, precision);
+  if (!cv.ok()) {
+    set_error("knn_batched: workspace too small");
+    return kWorkspaceTooSmall;
+  }
+  MSC_HIP(hipMemcpyAsync(t.prow2row, L.prow2row.data(), L.n_tot * sizeof(int32_t), hipMemcpyHostToDevice, st));
+  MSC_HIP(hipMemcpyAsync(t.prow_sample, L.prow_sample.data(), L.n_tot * sizeof(int32_t), hipMemcpyHostToDevice, st));
+  MSC_HIP(hipMemcpyAsync(t.row_range, L.row_range.data(), 2 * L.n_tot * sizeof(int64_t), hipMemcpyHostToDevice, st));
+  MSC_HIP(hipMemcpyAsync(t.blk_range, L.blk_range.data(), 2 * L.nblk * L.S * sizeof(int64_t), hipMemcpyHostToDevice,
+                         st));
+  MSC_HIP(hipMemcpyAsync(t.in_off, offsets, (n_samples + 1) * sizeof(int64_t), hipMemcpyHostToDevice, st));
+  MSC_HIP(hipStreamSynchronize(st));  // the host tables are pageable and local to this call
+  return kOk;
+}
+
+extern "C" int msc_knn_batched_launch(const float* X, const int64_t* offsets, int n_samples, int d, int k,
+                                      int precision, int splits, int32_t* out_idx, float* out_dist, void* workspace,
+                                      size_t workspace_bytes, void* stream) {
+  BatchLayout L;
+  MSC_TRY(batched_common(offsets, n_samples, d, k, precision, splits, &L));
+  MSC_REQUIRE(X && out_idx && out_dist && workspace, "knn_batched: null pointer argument");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  Carve cv(workspace, workspace_bytes);
+  BatchBufs t = carve_batched(cv, L, n_samples, d, k, precision);
+  if (!cv.ok()) {
+    set_error("knn_batched: workspace too small");
+    return kWorkspaceTooSmall;
+  }
+  const Plan& p = t.p;
+  const Buffers& b = t.b;
+  const int64_t n = L.n_tot;
+  k_pad_copy<<<static_cast<unsigned>(ceil_div(n * d, kThreads)), kThreads, 0, st>>>(X, t.prow2row, n, d, t.Xpad);
+  MSC_LAUNCH_CHECK();
+  const bool bf16 = precision == MSC_PRECISION_BF16_RERANK;
+  Operands op = prepare(p, b, t.Xpad, n, bf16, st);
+  MSC_LAUNCH_CHECK();
+  MSC_TRY(launch_select(bf16, false, op.ks, op.X, n, n, d, n, L.S, nullptr, t.blk_range, b.keys, b.taus, st));
+  const unsigned wb = static_cast<unsigned>(ceil_div(n, kThreads / kWave));
+  finalize_kernel<<<wb, kThreads, 0, st>>>(t.Xpad, n, d, k, L.S, n, nullptr, b.keys, b.taus, op.qn2, op.qres,
+                                           op.gstat, op.bnd, t.row_range, t.idx_pad, t.dist_pad, b.flags);
+  MSC_LAUNCH_CHECK();
+  fallback_kernel<<<wb, kThreads, 0, st>>>(t.Xpad, n, d, k, b.flags, t.row_range, t.idx_pad, t.dist_pad);
+  k_unpad<<<static_cast<unsigned>(ceil_div(n * k, kThreads)), kThreads, 0, st>>>(
+      t.idx_pad, t.dist_pad, t.prow2row, t.row_range, t.in_off, t.prow_sample, n, k, out_idx, out_dist);
+  MSC_LAUNCH_CHECK();
   return kOk;
 }
 
 extern "C" int msc_knn_batched(const float* X, const int64_t* offsets, int n_samples, int d, int k, int precision,
-                               int32_t* out_idx, float* out_dist, void* workspace, size_t workspace_bytes,
+                               int splits, int32_t* out_idx, float* out_dist, void* workspace, size_t workspace_bytes,
                                void* stream, msc_knn_info* info) {
-  MSC_REQUIRE(X && offsets && out_idx && out_dist && workspace && n_samples >= 1, "knn_batched: bad arguments");
-  MSC_REQUIRE(precision == MSC_PRECISION_FP32 || precision == MSC_PRECISION_BF16_RERANK, "knn_batched: precision");
-  MSC_REQUIRE(k >= 1 && k <= 64 && d >= 1 && d <= 125, "knn_batched: bad d/k");
-  hipStream_t st = static_cast<hipStream_t>(stream);
-  BatchLayout L;
-  MSC_TRY(make_layout(offsets, n_samples, k, &L));
-  if (workspace_bytes < batched_bytes(offsets[n_samples], n_samples, L, d, k, precision)) {
-    set_error("knn_batched: workspace too small");
-    return kWorkspaceTooSmall;
-  }
-  Plan p = make_plan(L.n_tot, d, k, precision);
-  Carve cv(workspace, workspace_bytes);
-  Buffers b = carve(cv, p);
-  float* Xpad = cv.take<float>(static_cast<size_t>(L.n_tot) * d);
-  int32_t* prow2row = cv.take<int32_t>(L.n_tot);
-  int32_t* prow_sample = cv.take<int32_t>(L.n_tot);
-  int64_t* row_range = cv.take<int64_t>(2 * L.n_tot);
-  int64_t* blk_range = cv.take<int64_t>(2 * L.nblk);
-  int64_t* in_off = cv.take<int64_t>(n_samples + 1);
-  int32_t* idx_pad = cv.take<int32_t>(static_cast<size_t>(L.n_tot) * k);
-  float* dist_pad = cv.take<float>(static_cast<size_t>(L.n_tot) * k);
-  MSC_HIP(hipMemcpyAsync(prow2row, L.prow2row.data(), L.n_tot * sizeof(int32_t), hipMemcpyHostToDevice, st));
-  MSC_HIP(hipMemcpyAsync(prow_sample, L.prow_sample.data(), L.n_tot * sizeof(int32_t), hipMemcpyHostToDevice, st));
-  MSC_HIP(hipMemcpyAsync(row_range, L.row_range.data(), 2 * L.n_tot * sizeof(int64_t), hipMemcpyHostToDevice, st));
-  MSC_HIP(hipMemcpyAsync(blk_range, L.blk_range.data(), 2 * L.nblk * sizeof(int64_t), hipMemcpyHostToDevice, st));
-  MSC_HIP(hipMemcpyAsync(in_off, offsets, (n_samples + 1) * sizeof(int64_t), hipMemcpyHostToDevice, st));
-  MSC_HIP(hipStreamSynchronize(st));  // the host tables are pageable and local to this call
-  const int64_t n = L.n_tot;
-  k_pad_copy<<<static_cast<unsigned>(ceil_div(n * d, kThreads)), kThreads, 0, st>>>(X, prow2row, n, d, Xpad);
-  MSC_LAUNCH_CHECK();
-  const bool bf16 = precision == MSC_PRECISION_BF16_RERANK;
-  Operands op = prepare(p, b, Xpad, n, bf16, st);
-  MSC_LAUNCH_CHECK();
-  MSC_TRY(launch_select(bf16, false, op.ks, op.X, n, n, d, n, 1, nullptr, blk_range, b.keys, b.taus, st));
-  const unsigned wb = static_cast<unsigned>(ceil_div(n, kThreads / kWave));
-  finalize_kernel<<<wb, kThreads, 0, st>>>(Xpad, n, d, k, 1, n, nullptr, b.keys, b.taus, op.qn2, op.qres, op.gstat,
-                                           op.bnd, row_range, idx_pad, dist_pad, b.flags);
-  MSC_LAUNCH_CHECK();
-  fallback_kernel<<<wb, kThreads, 0, st>>>(Xpad, n, d, k, b.flags, row_range, idx_pad, dist_pad);
-  k_unpad<<<static_cast<unsigned>(ceil_div(n * k, kThreads)), kThreads, 0, st>>>(
-      idx_pad, dist_pad, prow2row, row_range, in_off, prow_sample, n, k, out_idx, out_dist);
-  MSC_LAUNCH_CHECK();
+  MSC_TRY(msc_knn_batched_prepare(offsets, n_samples, d, k, precision, splits, workspace, workspace_bytes, stream));
+  MSC_TRY(msc_knn_batched_launch(X, offsets, n_samples, d, k, precision, splits, out_idx, out_dist, workspace,
+                                 workspace_bytes, stream));
   if (info) {
+    BatchLayout L;
+    MSC_TRY(batched_common(offsets, n_samples, d, k, precision, splits, &L));
+    Carve cv(workspace, workspace_bytes);
+    BatchBufs t = carve_batched(cv, L, n_samples, d, k, precision);
     int64_t m = 0;
-    MSC_TRY(compact_flags(p, b, &m, st));
+    MSC_TRY(compact_flags(t.p, t.b, &m, static_cast<hipStream_t>(stream)));
     info->n_fallback = m;
     info->n_second_pass = 0;
-    info->n_splits = 1;
+    info->n_splits = L.S;
     info->list_len = 4 * kL;
   }
   return kOk;

@@ -56,13 +56,20 @@ SIGNATURES = {
     "msc_knn_workspace_size": (c_int, [c_i64, c_int, c_int, c_int, ctypes.POINTER(c_sz)]),
     "msc_knn_exact": (c_int, [c_vp, c_i64, c_int, c_int, c_int, c_vp, c_vp, c_vp, c_sz, c_vp,
                               ctypes.POINTER(KnnInfo)]),
-    "msc_knn_batched_workspace_size": (c_int, [c_vp, c_int, c_int, c_int, c_int, ctypes.POINTER(c_sz)]),
-    "msc_knn_batched": (c_int, [c_vp, c_vp, c_int, c_int, c_int, c_int, c_vp, c_vp, c_vp, c_sz, c_vp,
-                                ctypes.POINTER(KnnInfo)]),
+    "msc_knn_batched_workspace_size": (c_int, [c_vp, c_int, c_int, c_int, c_int, c_int,
+                                               ctypes.POINTER(c_sz)]),
+    "msc_knn_batched_prepare": (c_int, [c_vp, c_int, c_int, c_int, c_int, c_int, c_vp, c_sz, c_vp]),
+    "msc_knn_batched_launch": (c_int, [c_vp, c_vp, c_int, c_int, c_int, c_int, c_int, c_vp, c_vp, c_vp,
+                                       c_sz, c_vp]),
+    "msc_knn_batched": (c_int, [c_vp, c_vp, c_int, c_int, c_int, c_int, c_int, c_vp, c_vp, c_vp, c_sz,
+                                c_vp, ctypes.POINTER(KnnInfo)]),
     "msc_knn_distances_csr": (c_int, [c_vp, c_vp, c_i64, c_int, c_vp, c_vp, c_vp, c_vp]),
     "msc_fuzzy_workspace_size": (c_int, [c_i64, c_int, ctypes.POINTER(c_sz)]),
     "msc_fuzzy_connectivities": (c_int, [c_vp, c_vp, c_i64, c_int, c_vp, c_vp, c_vp, c_i64, c_vp,
                                          c_vp, ctypes.POINTER(c_i64), c_vp, c_sz, c_vp]),
+    "msc_fuzzy_batched_workspace_size": (c_int, [c_i64, c_int, c_int, ctypes.POINTER(c_sz)]),
+    "msc_fuzzy_connectivities_batched": (c_int, [c_vp, c_vp, c_i64, c_int, c_vp, c_int, c_vp, c_vp, c_vp,
+                                                 c_i64, c_vp, c_vp, c_vp, c_sz, c_vp]),
     "msc_spatial_knn_workspace_size": (c_int, [c_i64, c_int, c_int, ctypes.POINTER(c_sz)]),
     "msc_spatial_knn": (c_int, [c_vp, c_i64, c_int, c_int, c_vp, c_vp, c_vp, c_vp,
                                 ctypes.POINTER(SpatialInfo), c_vp, c_sz, c_vp]),
@@ -162,7 +169,16 @@ def knn_exact(X, k, precision="fp32", timing=False, return_info=False):
     return idx, dist
 
 
-def knn_batched(X, offsets, k, precision="bf16", return_info=False):
+def knn_batched_workspace_bytes(offsets, d, k, precision="bf16", splits=1):
+    lib = load_library()
+    off = np.ascontiguousarray(offsets, dtype=np.int64)
+    nbytes = c_sz(0)
+    check(lib.msc_knn_batched_workspace_size(off.ctypes.data, off.size - 1, int(d), int(k), PRECISION[precision],
+                                             int(splits), ctypes.byref(nbytes)))
+    return nbytes.value
+
+
+def knn_batched(X, offsets, k, precision="bf16", splits=1, return_info=False):
     """Block-diagonal exact kNN of concatenated samples: ``(idx int32 global, dist float32)``."""
     import torch
     lib = load_library()
@@ -171,15 +187,13 @@ def knn_batched(X, offsets, k, precision="bf16", return_info=False):
     S = off.size - 1
     d = Xd.shape[1]
     p = PRECISION[precision]
-    nbytes = c_sz(0)
-    check(lib.msc_knn_batched_workspace_size(off.ctypes.data, S, d, int(k), p, ctypes.byref(nbytes)))
-    ws = workspace(nbytes.value)
+    ws = workspace(knn_batched_workspace_bytes(off, d, k, precision, splits))
     n = Xd.shape[0]
     idx = torch.empty((n, k), dtype=torch.int32, device=Xd.device)
     dist = torch.empty((n, k), dtype=torch.float32, device=Xd.device)
     info = KnnInfo()
-    check(lib.msc_knn_batched(ptr(Xd), off.ctypes.data, S, d, int(k), p, ptr(idx), ptr(dist), ptr(ws),
-                              ws.numel(), stream_ptr(), ctypes.byref(info)))
+    check(lib.msc_knn_batched(ptr(Xd), off.ctypes.data, S, d, int(k), p, int(splits), ptr(idx), ptr(dist),
+                              ptr(ws), ws.numel(), stream_ptr(), ctypes.byref(info)))
     if return_info:
         return idx, dist, {f: getattr(info, f) for f, _ in KnnInfo._fields_}
     return idx, dist
@@ -223,6 +237,13 @@ def fuzzy_connectivities(idx, dist):
                                        ptr(ws), ws.numel(), stream_ptr()))
     m = nnz.value
     return (indptr, indices[:m], data[:m]), (sigma, rho)
+
+
+def fuzzy_batched_workspace_bytes(n, k, n_groups):
+    lib = load_library()
+    nbytes = c_sz(0)
+    check(lib.msc_fuzzy_batched_workspace_size(int(n), int(k), int(n_groups), ctypes.byref(nbytes)))
+    return nbytes.value
 
 
 # ----------------------------------------------------------------------------------------

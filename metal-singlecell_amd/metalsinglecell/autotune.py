@@ -1,0 +1,98 @@
+"""Tile-shape autotuner for the batched neighbour-graph launch, with an on-disk cache.
+
+The tunable is the number of candidate splits of the batched select launch
+(``msc_knn_batched_*(..., splits, ...)``): with one split every query block of a sample
+scans the whole sample, so the largest samples form a long tail; more splits give more,
+shorter blocks at the cost of merging more partial lists.  Results never depend on the
+choice (the exact re-rank and the certificate fix them), only the time does.
+
+Cache: a JSON object mapping keys ``"knn_batched|<precision>|d<d>|k<k>|max<2^a>|tot<2^b>"``
+(power-of-two buckets of the largest sample and of the total cell count) to
+``{"splits": s, "ms": {candidate: median ms}}``.  Location: ``$MSC_AUTOTUNE_CACHE`` or
+``~/.cache/metalsinglecell/autotune.json``.
+"""
+
+from __future__ import annotations
+
+import json
+import os
+import threading
+
+import numpy as np
+
+CANDIDATES = (1, 2, 4, 8)
+_lock = threading.Lock()
+
+
+def cache_path():
+    return os.environ.get("MSC_AUTOTUNE_CACHE",
+                          os.path.join(os.path.expanduser("~"), ".cache", "metalsinglecell", "autotune.json"))
+
+
+def _bucket(v):
+    return 1 << max(0, int(np.ceil(np.log2(max(int(v), 1)))))
+
+
+def key_for(offsets, d, k, precision):
+    sizes = np.diff(np.asarray(offsets, dtype=np.int64))
+    return (f"knn_batched|{precision}|d{int(d)}|k{int(k)}|max{_bucket(sizes.max())}"
+            f"|tot{_bucket(sizes.sum())}")
+
+
+def load():
+    try:
+        with open(cache_path()) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
+def store(key, entry):
+    with _lock:
+        table = load()
+        table[key] = entry
+        path = cache_path()
+        os.makedirs(os.path.dirname(path), exist_ok=True)
+        tmp = path + ".tmp"
+        with open(tmp, "w") as f:
+            json.dump(table, f, indent=1, sort_keys=True)
+        os.replace(tmp, path)
+
+
+def lookup(offsets, d, k, precision):
+    entry = load().get(key_for(offsets, d, k, precision))
+    return None if entry is None else int(entry["splits"])
+
+
+def tune_batched_splits(X, offsets, k, precision="bf16", candidates=CANDIDATES, reps=3):
+    """Time the batched select launch for each candidate split count on ``X`` (device rows of
+    the concatenated samples), cache and return the fastest."""
+    import torch
+    from . import _hip
+    lib = _hip.load_library()
+    off = np.ascontiguousarray(offsets, dtype=np.int64)
+    S = off.size - 1
+    n, d = X.shape
+    p = _hip.PRECISION[precision]
+    idx = torch.empty((n, k), dtype=torch.int32, device=X.device)
+    dist = torch.empty((n, k), dtype=torch.float32, device=X.device)
+    timings = {}
+    for s in candidates:
+        ws = _hip.workspace(_hip.knn_batched_workspace_bytes(off, d, k, precision, s))
+        _hip.check(lib.msc_knn_batched_prepare(off.ctypes.data, S, d, int(k), p, int(s), _hip.ptr(ws), ws.numel(),
+                                               _hip.stream_ptr()))
+        ms = []
+        for _ in range(reps + 1):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            _hip.check(lib.msc_knn_batched_launch(_hip.ptr(X), off.ctypes.data, S, d, int(k), p, int(s),
+                                                  _hip.ptr(idx), _hip.ptr(dist), _hip.ptr(ws), ws.numel(),
+                                                  _hip.stream_ptr()))
+            e1.record()
+            e1.synchronize()
+            ms.append(e0.elapsed_time(e1))
+        timings[s] = float(np.median(ms[1:]))
+        del ws
+    best = min(timings, key=timings.get)
+    store(key_for(off, d, k, precision), {"splits": int(best), "ms": {str(c): t for c, t in timings.items()}})
+    return best, timings

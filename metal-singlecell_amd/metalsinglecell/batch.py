@@ -1,0 +1,137 @@
+"""Batched form: neighbour graphs and Leiden clusterings of many independent samples.
+
+Samples of uneven cell counts are concatenated (rows of sample s at ``offsets[s]:offsets[s+1]``).
+
+* Graph stage, one HIP graph: the exact kNN of every sample in ONE select launch
+  (``msc_knn_batched_launch``: each query block scans only its own sample) followed by the
+  UMAP connectivities of every sample in one pass over the concatenation
+  (``msc_fuzzy_connectivities_batched``; block s of the block-diagonal result equals the
+  single-sample result).  Both enqueue kernels only, so the stage is captured once into a
+  HIP graph (``torch.cuda.CUDAGraph``) and later batches of the same shape replay it.
+* Clustering stage: Leiden per sample (its modularity is normalised by the sample's own
+  total weight, and its level/pass loops branch on device results, so it is not captured),
+  several samples at a time, each on its own stream (default priority).
+
+Results are identical to per-sample calls of the single-sample entry points, whatever the
+split count the autotuner picks, and identical on every graph replay.
+"""
+
+from __future__ import annotations
+
+from concurrent.futures import ThreadPoolExecutor
+
+import numpy as np
+
+from . import _hip, autotune
+
+
+class BatchGraph:
+    """Captured kNN + connectivities stage for a fixed set of sample sizes."""
+
+    def __init__(self, sizes, d, k=15, precision="bf16", splits=None, capture=True, tune_on=None):
+        import torch
+        lib = _hip.load_library()
+        self.sizes = np.asarray(sizes, dtype=np.int64)
+        if self.sizes.ndim != 1 or self.sizes.size == 0 or (self.sizes < k).any():
+            raise ValueError(f"every sample needs at least k={k} cells")
+        self.offsets = np.concatenate([[0], np.cumsum(self.sizes)]).astype(np.int64)
+        self.n = int(self.offsets[-1])
+        self.d, self.k, self.precision = int(d), int(k), precision
+        dev = _hip.device()
+        self.X = torch.empty((self.n, self.d), dtype=torch.float32, device=dev)   # static input
+        if splits is None:
+            splits = autotune.lookup(self.offsets, d, k, precision)
+        if splits is None and tune_on is not None:
+            self.X.copy_(_hip.to_device(tune_on, torch.float32))
+            splits, _ = autotune.tune_batched_splits(self.X, self.offsets, k, precision)
+        self.splits = int(splits or 1)
+        S = self.sizes.size
+        self.ws_knn = _hip.workspace(_hip.knn_batched_workspace_bytes(self.offsets, d, k, precision, self.splits))
+        self.ws_fz = _hip.workspace(_hip.fuzzy_batched_workspace_bytes(self.n, k, S))
+        self.goff = torch.from_numpy(self.offsets).to(dev)
+        self.idx = torch.empty((self.n, k), dtype=torch.int32, device=dev)
+        self.dist = torch.empty((self.n, k), dtype=torch.float32, device=dev)
+        cap = max(2 * self.n * (k - 1), 1)
+        self.indptr = torch.empty(self.n + 1, dtype=torch.int64, device=dev)
+        self.indices = torch.empty(cap, dtype=torch.int32, device=dev)
+        self.data = torch.empty(cap, dtype=torch.float32, device=dev)
+        self.sigma = torch.empty(self.n, dtype=torch.float32, device=dev)
+        self.rho = torch.empty(self.n, dtype=torch.float32, device=dev)
+        self._cap = cap
+        _hip.check(lib.msc_knn_batched_prepare(self.offsets.ctypes.data, S, self.d, self.k,
+                                               _hip.PRECISION[precision], self.splits, _hip.ptr(self.ws_knn),
+                                               self.ws_knn.numel(), _hip.stream_ptr()))
+        self.graph = None
+        if capture:
+            # capture on a side stream (torch.cuda.graph's own); the first eager run warms up
+            self._enqueue()
+            torch.cuda.synchronize()
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g):
+                self._enqueue()
+            self.graph = g
+
+    def _enqueue(self):
+        lib = _hip.load_library()
+        st = _hip.stream_ptr()
+        S = self.sizes.size
+        _hip.check(lib.msc_knn_batched_launch(_hip.ptr(self.X), self.offsets.ctypes.data, S, self.d, self.k,
+                                              _hip.PRECISION[self.precision], self.splits, _hip.ptr(self.idx),
+                                              _hip.ptr(self.dist), _hip.ptr(self.ws_knn), self.ws_knn.numel(), st))
+        _hip.check(lib.msc_fuzzy_connectivities_batched(
+            _hip.ptr(self.idx), _hip.ptr(self.dist), self.n, self.k, _hip.ptr(self.goff), S, _hip.ptr(self.indptr),
+            _hip.ptr(self.indices), _hip.ptr(self.data), self._cap, _hip.ptr(self.sigma), _hip.ptr(self.rho),
+            _hip.ptr(self.ws_fz), self.ws_fz.numel(), st))
+
+    def run(self, X):
+        """kNN and connectivities of the concatenated samples ``X`` (n x d)."""
+        import torch
+        src = _hip.to_device(X, torch.float32)
+        if tuple(src.shape) != (self.n, self.d):
+            raise ValueError(f"X must be {self.n} x {self.d}")
+        self.X.copy_(src)
+        if self.graph is not None:
+            self.graph.replay()
+        else:
+            self._enqueue()
+        return self.idx, self.dist, (self.indptr, self.indices, self.data)
+
+
+def leiden_samples(bg, resolution=1.0, random_state=0, n_iterations=2, workers=8):
+    """Leiden of every sample of a BatchGraph's last result: list of int32 label tensors."""
+    import torch
+    lo_hi = [(int(bg.offsets[s]), int(bg.offsets[s + 1])) for s in range(bg.sizes.size)]
+    ip_host = bg.indptr.cpu().numpy()      # one transfer: row starts of every sample
+    dev = bg.indptr.device
+
+    def one(s):
+        a, b = lo_hi[s]
+        e0, e1 = int(ip_host[a]), int(ip_host[b])
+        stream = torch.cuda.Stream(device=dev)
+        with torch.cuda.stream(stream):
+            ip = bg.indptr[a:b + 1] - e0
+            ind = bg.indices[e0:e1] - a
+            dat = bg.data[e0:e1]
+            lab, q, _ = _hip.leiden(ip, ind, dat, b - a, resolution=resolution, random_state=random_state,
+                                    n_iterations=n_iterations)
+            stream.synchronize()
+        return lab, q
+
+    with ThreadPoolExecutor(max_workers=max(1, int(workers))) as ex:
+        res = list(ex.map(one, range(bg.sizes.size)))
+    return [r[0] for r in res], [r[1] for r in res]
+
+
+def neighbors_leiden_batched(samples, k=15, precision="bf16", resolution=1.0, random_state=0, n_iterations=2,
+                             workers=8, splits=None, autotune_splits=True):
+    """kNN + connectivities + Leiden for a list of (n_s x d) float32 samples.
+
+    Returns ``(labels, modularities, graph)``: per-sample int32 label tensors and Q values,
+    and the BatchGraph holding the block-diagonal kNN and connectivities."""
+    sizes = [int(x.shape[0]) for x in samples]
+    X = np.concatenate([np.asarray(x, dtype=np.float32) for x in samples])
+    bg = BatchGraph(sizes, X.shape[1], k=k, precision=precision, splits=splits,
+                    tune_on=X if autotune_splits else None)
+    bg.run(X)
+    labels, qs = leiden_samples(bg, resolution, random_state, n_iterations, workers)
+    return labels, qs, bg

@@ -120,3 +120,17 @@ def test_api_errors_before_any_gpu_work():
     wide = AnnData(np.zeros((20, 60), np.float32))
     with pytest.raises(NotImplementedError, match="X_pca"):
         pp.neighbors(wide)
+
+
+def test_autotune_cache_roundtrip(tmp_path, monkeypatch):
+    """Cache keys bucket the largest sample and the total by powers of two; entries persist."""
+    from metalsinglecell import autotune
+    monkeypatch.setenv("MSC_AUTOTUNE_CACHE", str(tmp_path / "sub" / "t.json"))
+    off = np.array([0, 500, 1700, 60_000])
+    assert autotune.key_for(off, 30, 15, "bf16") == "knn_batched|bf16|d30|k15|max65536|tot65536"
+    assert autotune.lookup(off, 30, 15, "bf16") is None
+    autotune.store(autotune.key_for(off, 30, 15, "bf16"), {"splits": 4, "ms": {"1": 2.0, "4": 1.0}})
+    assert autotune.lookup(off, 30, 15, "bf16") == 4
+    assert autotune.lookup(off, 30, 15, "fp32") is None
+    off2 = np.array([0, 500, 1700, 58_000])          # same buckets -> same entry
+    assert autotune.lookup(off2, 30, 15, "bf16") == 4
