@@ -102,6 +102,46 @@ def cpu_baseline(X, k, threads):
             "stage_s": {"knn": t_knn, "connectivities": t_conn, "leiden": t_leiden}}
 
 
+def batched(reps=3, workers=8):
+    """256 samples, 500-60,000 cells log-uniform, d = 30, k = 15: the kNN + connectivities stage
+    as one HIP-graph replay (split count from the autotuner), then per-sample Leiden."""
+    import numpy as np
+    import torch
+    from metalsinglecell import _hip, synthetic
+    from metalsinglecell.batch import BatchGraph, leiden_samples
+    S = [x for x, _ in synthetic.batched_samples()]
+    X = np.concatenate(S)
+    sizes = [s.shape[0] for s in S]
+    t = time.perf_counter()
+    bg = BatchGraph(sizes, X.shape[1], k=15, precision="bf16", tune_on=X)
+    torch.cuda.synchronize()
+    setup_s = time.perf_counter() - t
+    g_ms, l_ms = [], []
+    for _ in range(reps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        bg.run(X)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        leiden_samples(bg, workers=workers)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        g_ms.append(1e3 * (t1 - t0))
+        l_ms.append(1e3 * (t2 - t1))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for x in S:                                   # the same stage as per-sample calls
+        i, d = _hip.knn_exact(x, 15, precision="bf16")
+        _hip.fuzzy_connectivities(i, d)
+    torch.cuda.synchronize()
+    per_sample_ms = 1e3 * (time.perf_counter() - t0)
+    total = float(np.median(g_ms) + np.median(l_ms))
+    return {"samples": len(S), "cells": int(X.shape[0]), "dims": int(X.shape[1]), "k": 15,
+            "splits": bg.splits, "setup_s": setup_s, "graph_stage_ms": float(np.median(g_ms)),
+            "graph_stage_per_sample_calls_ms": per_sample_ms, "leiden_ms": float(np.median(l_ms)),
+            "leiden_workers": workers, "cells_per_s": X.shape[0] / (total * 1e-3)}
+
+
 def serving(duration_s, rate=20.0, genes=500, n_perms=100, max_slots=6, seed=4):
     """Poisson stream of spatial jobs (grid kNN k=6 + Moran's I, P permutations) through the
     continuous-batching server; arrival -> completion latency percentiles."""
@@ -163,6 +203,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=10.0)
+    ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="first run one plain device copy of this many MiB (counter calibration)")
     a = ap.parse_args()
@@ -222,6 +263,12 @@ def main():
         cpu = cpu_baseline(X, a.k, threads)
         log(f"cpu baseline in {time.perf_counter() - t:.1f}s", cpu)
 
+    batch = None
+    if not a.no_batched:
+        t = time.perf_counter()
+        batch = batched()
+        log(f"batched in {time.perf_counter() - t:.1f}s", batch)
+
     serve = None
     if a.serve_seconds > 0:
         t = time.perf_counter()
@@ -239,7 +286,7 @@ def main():
                    "cells": n, "dims": d, "k": a.k, "parallelism": "single GPU"},
         "stages_s": {"knn": last["knn_s"], "connectivities": last["conn_s"], "leiden": last["leiden_s"]},
         "knn_info": last["knn"], "modularity": last["q"], "n_clusters": last["n_clusters"],
-        "fp32_form": fp32, "serving": serve, "roofline": roof, "cpu_baseline": cpu,
+        "fp32_form": fp32, "batched": batch, "serving": serve, "roofline": roof, "cpu_baseline": cpu,
     }
     print(json.dumps(out))
 

@@ -1060,6 +1060,13 @@ static int local_moving(Ctx& c, const Graph& g, int64_t seed, const int32_t* par
     // the three move kernels write target[] of disjoint vertex sets and only read the rest:
     // the listed (mid / big) vertices run on the side stream concurrently with the light ones
     const bool fork = c.nm > 0 || c.nb > 0;
+    if (fork && !c.side) {  // created on first need: most small graphs never list a vertex
+      MSC_HIP(hipStreamCreateWithFlags(&c.side, hipStreamNonBlocking));
+      MSC_HIP(hipStreamCreateWithFlags(&c.side2, hipStreamNonBlocking));
+      MSC_HIP(hipEventCreateWithFlags(&c.ev_fork, hipEventDisableTiming));
+      MSC_HIP(hipEventCreateWithFlags(&c.ev_join, hipEventDisableTiming));
+      MSC_HIP(hipEventCreateWithFlags(&c.ev_join2, hipEventDisableTiming));
+    }
     if (fork) {
       MSC_HIP(hipEventRecord(c.ev_fork, st));
       if (c.nb > 0) MSC_HIP(hipStreamWaitEvent(c.side, c.ev_fork, 0));
@@ -1275,7 +1282,8 @@ extern "C" int msc_leiden(const int64_t* indptr, const int32_t* indices, const f
   MSC_REQUIRE(commit_prob > 0.0 && commit_prob <= 1.0, "leiden: commit_prob must be in (0, 1]");
   Ctx c{};
   c.st = static_cast<hipStream_t>(stream);
-  // side stream (default priority) and fork/join events, released on every return path
+  // side streams (default priority) and fork/join events, created on first need and
+  // released on every return path
   struct SideGuard {
     Ctx& c;
     ~SideGuard() {
@@ -1288,11 +1296,6 @@ extern "C" int msc_leiden(const int64_t* indptr, const int32_t* indices, const f
         if (e) (void)hipEventDestroy(e);
     }
   } side_guard{c};
-  MSC_HIP(hipStreamCreateWithFlags(&c.side, hipStreamNonBlocking));
-  MSC_HIP(hipStreamCreateWithFlags(&c.side2, hipStreamNonBlocking));
-  MSC_HIP(hipEventCreateWithFlags(&c.ev_fork, hipEventDisableTiming));
-  MSC_HIP(hipEventCreateWithFlags(&c.ev_join, hipEventDisableTiming));
-  MSC_HIP(hipEventCreateWithFlags(&c.ev_join2, hipEventDisableTiming));
   c.n0 = n;
   c.E0 = nnz;
   c.gamma = resolution;
