@@ -338,20 +338,23 @@ __global__ __launch_bounds__(kThreads, 2) void select_kernel(
   constexpr int kNChunk = kTile * kRowChunks;
   constexpr int kPer = (kNChunk + kThreads - 1) / kThreads;
   static_assert(kPer <= 8, "tile prefetch needs more than 8 registers");
-  uint4 pf0{}, pf1{}, pf2{}, pf3{}, pf4{}, pf5{}, pf6{}, pf7{};
-#define MSC_PF(i) (i == 0 ? pf0 : i == 1 ? pf1 : i == 2 ? pf2 : i == 3 ? pf3 : i == 4 ? pf4 : i == 5 ? pf5 : i == 6 ? pf6 : pf7)
-#define MSC_LOAD1(i, T)                                                                         \
+  // Two register sets (A: odd tiles, B: even tiles) so that a tile's global load has two
+  // tiles of compute to complete before its LDS store.
+  uint4 pa0{}, pa1{}, pa2{}, pa3{}, pa4{}, pa5{}, pa6{}, pa7{};
+  uint4 pb0{}, pb1{}, pb2{}, pb3{}, pb4{}, pb5{}, pb6{}, pb7{};
+#define MSC_PF(S, i) (i == 0 ? S##0 : i == 1 ? S##1 : i == 2 ? S##2 : i == 3 ? S##3 : i == 4 ? S##4 : i == 5 ? S##5 : i == 6 ? S##6 : S##7)
+#define MSC_LOAD1(S, i, T)                                                                      \
   if constexpr (kPer > i) {                                                                     \
     const int c_ = threadIdx.x + i * kThreads;                                                  \
     if (kNChunk % kThreads == 0 || c_ < kNChunk)                                                \
-      MSC_PF(i) = *reinterpret_cast<const uint4*>(Xb + (c_begin + (T) * kTile) * C::ROWB + static_cast<int64_t>(c_) * 16); \
+      MSC_PF(S, i) = *reinterpret_cast<const uint4*>(Xb + (c_begin + (T) * kTile) * C::ROWB + static_cast<int64_t>(c_) * 16); \
   }
-#define MSC_STORE1(i, B)                                                                        \
+#define MSC_STORE1(S, i, B)                                                                     \
   if constexpr (kPer > i) {                                                                     \
     const int c_ = threadIdx.x + i * kThreads;                                                  \
     if (kNChunk % kThreads == 0 || c_ < kNChunk) {                                              \
       char* dst_ = &tile[(B)][(c_ / kRowChunks) * C::STRIDE + (c_ % kRowChunks) * 16];           \
-      const uint4 v_ = MSC_PF(i);                                                               \
+      const uint4 v_ = MSC_PF(S, i);                                                            \
       if constexpr (BF16) {                                                                     \
         *reinterpret_cast<uint4*>(dst_) = v_;                                                   \
       } else {                                                                                  \
@@ -360,20 +363,21 @@ __global__ __launch_bounds__(kThreads, 2) void select_kernel(
       }                                                                                         \
     }                                                                                           \
   }
-#define MSC_LOAD_TILE(T) { MSC_LOAD1(0, T) MSC_LOAD1(1, T) MSC_LOAD1(2, T) MSC_LOAD1(3, T) \
-                           MSC_LOAD1(4, T) MSC_LOAD1(5, T) MSC_LOAD1(6, T) MSC_LOAD1(7, T) }
-#define MSC_STORE_TILE(B) { MSC_STORE1(0, B) MSC_STORE1(1, B) MSC_STORE1(2, B) MSC_STORE1(3, B) \
-                            MSC_STORE1(4, B) MSC_STORE1(5, B) MSC_STORE1(6, B) MSC_STORE1(7, B) }
+#define MSC_LOAD_TILE(S, T) { MSC_LOAD1(S, 0, T) MSC_LOAD1(S, 1, T) MSC_LOAD1(S, 2, T) MSC_LOAD1(S, 3, T) \
+                              MSC_LOAD1(S, 4, T) MSC_LOAD1(S, 5, T) MSC_LOAD1(S, 6, T) MSC_LOAD1(S, 7, T) }
+#define MSC_STORE_TILE(S, B) { MSC_STORE1(S, 0, B) MSC_STORE1(S, 1, B) MSC_STORE1(S, 2, B) MSC_STORE1(S, 3, B) \
+                               MSC_STORE1(S, 4, B) MSC_STORE1(S, 5, B) MSC_STORE1(S, 6, B) MSC_STORE1(S, 7, B) }
 
   if (ntiles > 0) {
-    MSC_LOAD_TILE(0);
-    MSC_STORE_TILE(0);
+    MSC_LOAD_TILE(pb, 0);
+    MSC_STORE_TILE(pb, 0);
   }
   __syncthreads();
+  if (ntiles > 1) MSC_LOAD_TILE(pa, 1);
+  if (ntiles > 2) MSC_LOAD_TILE(pb, 2);
 
-  for (int64_t t = 0; t < ntiles; ++t) {
-    const int buf = static_cast<int>(t & 1);
-    if (t + 1 < ntiles) MSC_LOAD_TILE(t + 1);
+  // one candidate tile: MFMA distance block from LDS buffer buf, then the fused selection
+  auto tile_body = [&](const int buf, const int64_t t) {
 
     f32x4 acc[4][NB];
 #pragma unroll
@@ -415,16 +419,16 @@ __global__ __launch_bounds__(kThreads, 2) void select_kernel(
 #pragma unroll
       for (int mb = 0; mb < 4; ++mb) {
         const int li = WIDE ? (mb >> 1) : nb;  // list receiving this group
-        bool h = false;
-#pragma unroll
-        for (int i = 0; i < 4; ++i) h |= acc[mb][nb][i] <= tq[li];
+        const f32x4 a4 = acc[mb][nb];
+        const bool h = fminf(fminf(a4[0], a4[1]), fminf(a4[2], a4[3])) <= tq[li];
         if (__builtin_amdgcn_ballot_w64(h) == 0ull) continue;
+        const uint32_t cg = static_cast<uint32_t>(c0) + 16 * mb + 4 * g;  // candidate ids < 2^31
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-          const float v = acc[mb][nb][i];
-          const int64_t c = c0 + 16 * mb + 4 * g + i;
-          if (v <= tq[li] && (!tail_tile || c < c_end)) {
-            hitbuf[wv][li][cnt[li]][l] = (static_cast<uint64_t>(float_ord(v)) << 32) | static_cast<uint64_t>(c);
+          const float v = a4[i];
+          const uint32_t c = cg + i;
+          if (v <= tq[li] && (!tail_tile || static_cast<int64_t>(c) < c_end)) {
+            hitbuf[wv][li][cnt[li]][l] = (static_cast<uint64_t>(float_ord(v)) << 32) | c;
             ++cnt[li];
           }
         }
@@ -451,14 +455,25 @@ __global__ __launch_bounds__(kThreads, 2) void select_kernel(
       }
     }
 
-    // Compiler fence: keep the LDS store of the prefetched tile here, after this tile's
-    // compute, so the global-load latency of tile t+1 hides behind it (without the fence the
-    // store is hoisted next to its load and every tile waits on memory).
+  };
+
+  // Tiles in pairs: tile t reads buffer 0 while tile t+1 (set A, loaded two tiles earlier) is
+  // stored to buffer 1, whose last reader (tile t-1) finished before the previous barrier;
+  // then the same with the roles swapped.  One barrier per tile publishes the stored tile.
+  // The compiler fences keep each store after its tile's compute, so the load latency hides
+  // behind two tiles of work.
+  for (int64_t t = 0; t < ntiles; t += 2) {
+    tile_body(0, t);
     asm volatile("" ::: "memory");
-    // buf^1 was last read in iteration t-1, which every wave finished before the barrier
-    // that ended it; one barrier per tile publishes the new tile.
-    if (t + 1 < ntiles) MSC_STORE_TILE(buf ^ 1);
+    if (t + 1 < ntiles) MSC_STORE_TILE(pa, 1);
     __syncthreads();
+    if (t + 3 < ntiles) MSC_LOAD_TILE(pa, t + 3);
+    if (t + 1 >= ntiles) break;
+    tile_body(1, t + 1);
+    asm volatile("" ::: "memory");
+    if (t + 2 < ntiles) MSC_STORE_TILE(pb, 0);
+    __syncthreads();
+    if (t + 4 < ntiles) MSC_LOAD_TILE(pb, t + 4);
   }
 #undef MSC_LOAD_TILE
 #undef MSC_STORE_TILE
