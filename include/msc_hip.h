@@ -89,6 +89,30 @@ int msc_knn_batched(const float* X, const int64_t* offsets /*host*/, int n_sampl
                     int precision, int splits, int32_t* out_idx, float* out_dist, void* workspace,
                     size_t workspace_bytes, void* stream, msc_knn_info* info /*host, may be NULL*/);
 
+/* Batch-balanced kNN: for every cell, its min(k, n_t) exact nearest cells within each batch
+ * t, the per-batch lists concatenated in batch order and sorted by (float32 distance,
+ * position in the concatenation).  Replaces the per-batch loop of neighbors.bbknn
+ * (src/metalsinglecell/neighbors.py:491-541) as called by pp.bbknn (pp.py:809-833).
+ * X: n x d float32 (device); batch (host): n codes in [0, n_batches), every code present,
+ * n_batches <= 64.  Width K = sum_t min(k, n_t) <= 64 (returned in *out_width, host).
+ * out_idx / out_dist: n x K.  Within a batch the order is the kNN key (j != i, d2, j), so a
+ * cell's own batch lists the cell itself first at distance 0.  d <= 125 (BF16) / 127.
+ * Workspace grows as n_batches x n x 512 B (one candidate list per cell and batch). */
+int msc_bbknn_workspace_size(const int32_t* batch /*host*/, int64_t n, int n_batches, int d, int k,
+                             int precision, size_t* bytes /*host*/);
+int msc_bbknn(const float* X, int64_t n, int d, const int32_t* batch /*host*/, int n_batches, int k,
+              int precision, int32_t* out_idx, float* out_dist, int* out_width /*host*/,
+              void* workspace, size_t workspace_bytes, void* stream,
+              msc_knn_info* info /*host, may be NULL; n_fallback counts (cell, batch) lists*/);
+
+/* obsp['distances'] of bbknn (neighbors.py:554-557): the K combined neighbours per row
+ * without the row itself and without zero distances, columns ascending.  out_indptr n+1
+ * int64 (nnz = out_indptr[n], on the device); out_indices / out_data capacity n*K. */
+int msc_bbknn_distances_workspace_size(int64_t n, size_t* bytes /*host*/);
+int msc_bbknn_distances_csr(const int32_t* idx, const float* dist, int64_t n, int K,
+                            int64_t* out_indptr, int32_t* out_indices, float* out_data,
+                            void* workspace, size_t workspace_bytes, void* stream);
+
 /* obsp['distances'] as CSR from a self-inclusive kNN (neighbors.py:755-760): k-1 entries per
  * row (self column dropped), columns ascending, explicit zeros kept.
  * out_indptr n+1 int64, out_indices n*(k-1) int32, out_data n*(k-1) float32. */
@@ -125,6 +149,16 @@ int msc_fuzzy_connectivities_batched(const int32_t* knn_idx, const float* knn_di
                                      float* out_sigma, float* out_rho, void* workspace,
                                      size_t workspace_bytes, void* stream);
 
+/* Connectivity trimming of bbknn: neighbors._trim_graph (src/metalsinglecell/neighbors.py:
+ * 465-488) on a symmetric canonical CSR.  Row i's threshold is its trim-th largest value
+ * when it holds more than trim entries, else 0; entry (i, j) is kept when nonzero and >= the
+ * thresholds of rows i and j.  Output canonical CSR (capacity = input nnz); *out_nnz (host,
+ * may be NULL; when given the call synchronises).  trim >= 1. */
+int msc_trim_graph_workspace_size(int64_t n, size_t* bytes /*host*/);
+int msc_trim_graph(const int64_t* indptr, const int32_t* indices, const float* data, int64_t n,
+                   int64_t trim, int64_t* out_indptr, int32_t* out_indices, float* out_data,
+                   int64_t* out_nnz /*host*/, void* workspace, size_t workspace_bytes, void* stream);
+
 /* ---------------------------------------------------------------------------------------
  * (d) Synchronous parallel Leiden (modularity, resolution gamma).
  * Replaces cluster.leiden(backend="gpu") -> graph.leiden.leiden
@@ -159,6 +193,21 @@ int msc_leiden(const int64_t* indptr, const int32_t* indices, const float* weigh
                double commit_prob, const int32_t* init_labels, int32_t* out_labels,
                double* out_modularity /*host*/, msc_leiden_info* info /*host, may be NULL*/,
                void* workspace, size_t workspace_bytes, void* stream);
+
+/* Synchronous parallel multilevel Louvain (the same move kernels and contraction, no
+ * refinement).  Replaces tl.louvain(backend="gpu") -> graph.louvain.louvain
+ * (src/metalsinglecell/tl.py:76-91, graph/louvain.py:397-453) with variant="sync".
+ * Level l moves from singletons with seed + 100*l; the run stops, keeping the previous
+ * level's labels, when no vertex moved or Q rose by at most 1e-9; otherwise the graph is
+ * aggregated on the moved partition.  1 <= max_levels <= 20.  Input, output, fixed-point
+ * arithmetic, info records (one per level; n_refined = n_comms, refine_passes = 0) and the
+ * workspace size (= msc_leiden_workspace_size) as for msc_leiden.  Synchronises. */
+int msc_louvain_workspace_size(int64_t n, int64_t nnz, size_t* bytes /*host*/);
+int msc_louvain(const int64_t* indptr, const int32_t* indices, const float* weights, int64_t n,
+                int64_t nnz, double resolution, int64_t seed, int max_levels, double commit_prob,
+                int32_t* out_labels, double* out_modularity /*host*/,
+                msc_leiden_info* info /*host, may be NULL*/, void* workspace,
+                size_t workspace_bytes, void* stream);
 
 /* ---------------------------------------------------------------------------------------
  * (b) Exact spatial kNN on a uniform grid, self excluded (2-D or 3-D coordinates).

@@ -286,6 +286,88 @@ static int validate(int64_t n, int k) {
   return kOk;
 }
 
+// ---- trimming (bbknn): keep each row's strongest entries ----------------------------------
+// Row i's threshold is its trim-th largest value when it holds more than trim entries, else
+// 0; entry (i, j) survives when it is nonzero and >= both row thresholds (the row pass, the
+// transpose and the column pass of neighbors._trim_graph collapse to that test).
+// The trim-th largest is found by a bitwise search over the order-preserving uint32 image:
+// the largest T with #{v : ord(v) >= T} >= trim is exactly ord of the trim-th largest value.
+__global__ __launch_bounds__(kThreads) void k_trim_thresh(const int64_t* __restrict__ indptr,
+                                                          const float* __restrict__ data, int64_t n, int64_t trim,
+                                                          float* __restrict__ thr) {
+  const int l = lane_id();
+  const int64_t q = static_cast<int64_t>(blockIdx.x) * (kThreads / kWave) + threadIdx.x / kWave;
+  if (q >= n) return;
+  const int64_t b = indptr[q], e = indptr[q + 1];
+  if (e - b <= trim) {
+    if (l == 0) thr[q] = 0.0f;
+    return;
+  }
+  uint32_t T = 0;
+  for (int bit = 31; bit >= 0; --bit) {
+    const uint32_t cand = T | (1u << bit);
+    int64_t c = 0;
+    for (int64_t p = b + l; p < e; p += kWave) c += float_ord(data[p]) >= cand ? 1 : 0;
+    c = wave_sum_i64(c);
+    if (c >= trim) T = cand;
+  }
+  if (l == 0) thr[q] = ord_float(T);
+}
+
+__device__ __forceinline__ bool trim_keep(float v, float ti, float tj) { return v != 0.0f && v >= ti && v >= tj; }
+
+__global__ __launch_bounds__(kThreads) void k_trim_count(const int64_t* __restrict__ indptr,
+                                                         const int32_t* __restrict__ indices,
+                                                         const float* __restrict__ data, int64_t n,
+                                                         const float* __restrict__ thr, int64_t* __restrict__ cnt) {
+  const int l = lane_id();
+  const int64_t q = static_cast<int64_t>(blockIdx.x) * (kThreads / kWave) + threadIdx.x / kWave;
+  if (q > n) return;
+  if (q == n) {
+    if (l == 0) cnt[n] = 0;
+    return;
+  }
+  const int64_t b = indptr[q], e = indptr[q + 1];
+  const float ti = thr[q];
+  int64_t c = 0;
+  for (int64_t p = b + l; p < e; p += kWave) c += trim_keep(data[p], ti, thr[indices[p]]) ? 1 : 0;
+  c = wave_sum_i64(c);
+  if (l == 0) cnt[q] = c;
+}
+
+__global__ __launch_bounds__(kThreads) void k_trim_fill(const int64_t* __restrict__ indptr,
+                                                        const int32_t* __restrict__ indices,
+                                                        const float* __restrict__ data, int64_t n,
+                                                        const float* __restrict__ thr,
+                                                        const int64_t* __restrict__ out_indptr,
+                                                        int32_t* __restrict__ out_indices,
+                                                        float* __restrict__ out_data) {
+  const int l = lane_id();
+  const int64_t q = static_cast<int64_t>(blockIdx.x) * (kThreads / kWave) + threadIdx.x / kWave;
+  if (q >= n) return;
+  const int64_t b = indptr[q], e = indptr[q + 1];
+  const float ti = thr[q];
+  int64_t o = out_indptr[q];
+  for (int64_t base = b; base < e; base += kWave) {
+    const int64_t p = base + l;
+    bool keep = false;
+    int32_t j = 0;
+    float v = 0.0f;
+    if (p < e) {
+      j = indices[p];
+      v = data[p];
+      keep = trim_keep(v, ti, thr[j]);
+    }
+    const uint64_t m = __ballot(keep);
+    if (keep) {
+      const int64_t r = o + __popcll(m & ((1ull << l) - 1ull));
+      out_indices[r] = j;
+      out_data[r] = v;
+    }
+    o += __popcll(m);
+  }
+}
+
 }  // namespace fuzzy
 }  // namespace msc
 
@@ -388,4 +470,42 @@ extern "C" int msc_fuzzy_connectivities_batched(const int32_t* knn_idx, const fl
   }
   return fuzzy_kernels(knn_idx, knn_dist, n, k, group_offsets, n_groups, out_indptr, out_indices, out_data, capacity,
                        out_sigma, out_rho, b, st, nullptr);
+}
+
+extern "C" int msc_trim_graph_workspace_size(int64_t n, size_t* bytes) {
+  MSC_REQUIRE(bytes && n >= 1, "trim: bad arguments");
+  Carve cv(nullptr, 0);
+  cv.take<float>(n);
+  cv.take<char>(prims::scan_workspace<int64_t>(n + 1));
+  *bytes = cv.used + 256;
+  return kOk;
+}
+
+extern "C" int msc_trim_graph(const int64_t* indptr, const int32_t* indices, const float* data, int64_t n,
+                              int64_t trim, int64_t* out_indptr, int32_t* out_indices, float* out_data,
+                              int64_t* out_nnz, void* workspace, size_t workspace_bytes, void* stream) {
+  MSC_REQUIRE(indptr && indices && data && out_indptr && out_indices && out_data && workspace,
+              "trim: null pointer argument");
+  MSC_REQUIRE(n >= 1 && trim >= 1, "trim: need n >= 1 and trim >= 1");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  Carve cv(workspace, workspace_bytes);
+  float* thr = cv.take<float>(n);
+  const size_t sb = prims::scan_workspace<int64_t>(n + 1);
+  void* sws = cv.take<char>(sb);
+  if (!cv.ok()) {
+    set_error("trim: workspace too small (%zu < %zu)", workspace_bytes, cv.used);
+    return kWorkspaceTooSmall;
+  }
+  const unsigned wb = static_cast<unsigned>(ceil_div(n + 1, kThreads / kWave));
+  k_trim_thresh<<<wb, kThreads, 0, st>>>(indptr, data, n, trim, thr);
+  k_trim_count<<<wb, kThreads, 0, st>>>(indptr, indices, data, n, thr, out_indptr);
+  MSC_LAUNCH_CHECK();
+  MSC_TRY(prims::exclusive_scan<int64_t>(out_indptr, out_indptr, n + 1, sws, sb, st));
+  k_trim_fill<<<wb, kThreads, 0, st>>>(indptr, indices, data, n, thr, out_indptr, out_indices, out_data);
+  MSC_LAUNCH_CHECK();
+  if (out_nnz) {
+    MSC_HIP(hipMemcpyAsync(out_nnz, out_indptr + n, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    MSC_HIP(hipStreamSynchronize(st));
+  }
+  return kOk;
 }

@@ -551,6 +551,25 @@ struct Bound {
   double split_rel; // residual of the |c~|^2 split, relative to |c~|^2
 };
 
+// The certificate above: true when every candidate that the list (tail tau) did not keep is
+// provably farther than the list's k-th exact distance (key kth).
+__device__ __forceinline__ bool certified(float tau, uint64_t kth, int64_t q, int d, const float* __restrict__ nrm2,
+                                          const float* __restrict__ res, const float* __restrict__ gstat, Bound bnd) {
+  int64_t jk;
+  float d2k;
+  decode_exact(kth, q, &jk, &d2k);
+  const double nq2 = static_cast<double>(nrm2[q]);
+  const double nmax2 = static_cast<double>(gstat[0]);
+  const double nq = sqrt(nq2), nmax = sqrt(nmax2);
+  // sum |terms| <= |c~|^2 + 2 |q^| (|hi_c| + |lo_c|) <= nmax^2 + 2.02 |q^| nmax
+  const double E = bnd.gam * (2.02 * nq * nmax + nmax2) + bnd.split_rel * nmax2 + 4.0e-7 * (nq2 + nmax2);
+  const double lb2 = static_cast<double>(tau) + nq2 - E;
+  const double lb = (lb2 > 0.0 ? sqrt(lb2) : 0.0) -
+                    (static_cast<double>(res[q]) + static_cast<double>(gstat[1])) * (1.0 + 1e-6) - 1e-12 * nmax;
+  const double slack = 1.0 - 8.0 * (d + 2) * 5.9604644775390625e-08;
+  return lb > 0.0 && static_cast<double>(d2k) < lb * lb * slack;
+}
+
 __global__ __launch_bounds__(kThreads) void finalize_kernel(
     const float* __restrict__ X, int64_t nq, int d, int k, int S, int64_t n_slots,
     const int32_t* __restrict__ qrows, const uint64_t* __restrict__ keys, const float* __restrict__ taus,
@@ -584,23 +603,7 @@ __global__ __launch_bounds__(kThreads) void finalize_kernel(
   const uint64_t kth = shfl_u64(ek, k - 1);
   const bool self_first = shfl_u64(ek, 0) == 0ull;
   bool ok = self_first && kth != kEmpty;
-  if (ok && !isinf(tau)) {
-    int64_t jk;
-    float d2k;
-    decode_exact(kth, q, &jk, &d2k);
-    const double nq2 = static_cast<double>(nrm2[q]);
-    const double nmax2 = static_cast<double>(gstat[0]);
-    const double nq = sqrt(nq2), nmax = sqrt(nmax2);
-    // sum |terms| <= |c~|^2 + 2 |q^| (|hi_c| + |lo_c|) <= nmax^2 + 2.02 |q^| nmax
-    const double E = bnd.gam * (2.02 * nq * nmax + nmax2) + bnd.split_rel * nmax2 +
-                     4.0e-7 * (nq2 + nmax2);
-    const double lb2 = static_cast<double>(tau) + nq2 - E;
-    const double lb = (lb2 > 0.0 ? sqrt(lb2) : 0.0) -
-                      (static_cast<double>(res[q]) + static_cast<double>(gstat[1])) * (1.0 + 1e-6) -
-                      1e-12 * nmax;
-    const double slack = 1.0 - 8.0 * (d + 2) * 5.9604644775390625e-08;
-    ok = lb > 0.0 && static_cast<double>(d2k) < lb * lb * slack;
-  }
+  if (ok && !isinf(tau)) ok = certified(tau, kth, q, d, nrm2, res, gstat, bnd);
   if (ok) {
     if (l < k) {
       int64_t jj;
@@ -1039,6 +1042,137 @@ __global__ void k_unpad(const int32_t* __restrict__ idx_pad, const float* __rest
   out_idx[static_cast<int64_t>(dst) * k + t] = static_cast<int32_t>(idx_pad[e] - row_range[2 * r] + in_off[s]);
   out_dist[static_cast<int64_t>(dst) * k + t] = dist_pad[e];
 }
+
+// ---- batch-balanced kNN (one list per query and batch) ------------------------------------
+// Rows sorted by batch and padded per batch as in the batched layout; select split t of every
+// query block scans batch t.  Wave w = t * n_slots + q finalises query slot q against batch t:
+// exact re-rank, the certificate over batch t's range alone (the query itself is required
+// first only in its own batch), kk[t] results at columns coff[t].. of the K-wide padded row.
+__global__ __launch_bounds__(kThreads) void finalize_sep_kernel(
+    const float* __restrict__ X, int64_t n_slots, int d, int nb, const uint64_t* __restrict__ keys,
+    const float* __restrict__ taus, const float* __restrict__ nrm2, const float* __restrict__ res,
+    const float* __restrict__ gstat, Bound bnd, const int32_t* __restrict__ prow2row,
+    const int32_t* __restrict__ prow_batch, const int32_t* __restrict__ kk, const int32_t* __restrict__ coff, int K,
+    int32_t* __restrict__ idx_pad, float* __restrict__ dist_pad, int32_t* __restrict__ flags) {
+  const int l = lane_id();
+  const int64_t w = static_cast<int64_t>(blockIdx.x) * (kThreads / kWave) + threadIdx.x / kWave;
+  if (w >= static_cast<int64_t>(nb) * n_slots) return;
+  const int t = static_cast<int>(w / n_slots);
+  const int64_t q = w - static_cast<int64_t>(t) * n_slots;
+  if (prow2row[q] < 0) {  // padding row
+    if (l == 0) flags[w] = 0;
+    return;
+  }
+  const int k = kk[t];
+  const uint64_t cur = keys[w * kListStride + l];
+  const float tau = taus[w];
+  uint64_t ek = kEmpty;
+  if (cur != kEmpty) {
+    const int64_t j = static_cast<int64_t>(cur & 0xFFFFFFFFull);
+    ek = exact_key(q, j, exact_d2(X, d, q, j));
+  }
+  ek = wave_sort_u64(ek);
+  const uint64_t kth = shfl_u64(ek, k - 1);
+  const bool own = prow_batch[q] == t;
+  bool ok = kth != kEmpty && (!own || shfl_u64(ek, 0) == 0ull);
+  if (ok && !isinf(tau)) ok = certified(tau, kth, q, d, nrm2, res, gstat, bnd);
+  if (ok) {
+    if (l < k) {
+      int64_t jj;
+      float d2;
+      decode_exact(ek, q, &jj, &d2);
+      idx_pad[q * K + coff[t] + l] = static_cast<int32_t>(jj);
+      dist_pad[q * K + coff[t] + l] = sqrtf(d2);
+    }
+    if (l == 0) flags[w] = 0;
+  } else if (l == 0) {
+    flags[w] = 1;
+  }
+}
+
+// exact brute force of a flagged (batch t, slot q) over batch t's rows [brange[2t], brange[2t+1])
+__global__ __launch_bounds__(kThreads) void fallback_sep_kernel(
+    const float* __restrict__ X, int64_t n_slots, int d, int nb, const int32_t* __restrict__ flags,
+    const int64_t* __restrict__ brange, const int32_t* __restrict__ kk, const int32_t* __restrict__ coff, int K,
+    int32_t* __restrict__ idx_pad, float* __restrict__ dist_pad) {
+  const int l = lane_id();
+  const int64_t w = static_cast<int64_t>(blockIdx.x) * (kThreads / kWave) + threadIdx.x / kWave;
+  if (w >= static_cast<int64_t>(nb) * n_slots || flags[w] == 0) return;
+  const int t = static_cast<int>(w / n_slots);
+  const int64_t q = w - static_cast<int64_t>(t) * n_slots;
+  const int64_t cb = brange[2 * t], ce = brange[2 * t + 1];
+  uint64_t cur = kEmpty, tail = kEmpty;
+  for (int64_t base = cb; base < ce; base += kWave) {
+    const int64_t j = base + l;
+    const uint64_t ek = (j < ce) ? exact_key(q, j, exact_d2(X, d, q, j)) : kEmpty;
+    if (__any(ek < tail)) {
+      cur = wave_merge_min_u64(cur, wave_sort_u64(ek));
+      tail = shfl_u64(cur, kWave - 1);
+    }
+  }
+  if (l < kk[t]) {
+    int64_t jj;
+    float d2;
+    decode_exact(cur, q, &jj, &d2);
+    idx_pad[q * K + coff[t] + l] = static_cast<int32_t>(jj);
+    dist_pad[q * K + coff[t] + l] = sqrtf(d2);
+  }
+}
+
+// the K per-batch results of a row sorted by (float32 distance, column): the stable sort of
+// the batch-ordered concatenation; written at the input row with input column ids
+__global__ __launch_bounds__(kThreads) void k_bbknn_combine(
+    const int32_t* __restrict__ idx_pad, const float* __restrict__ dist_pad, int64_t n_slots, int K,
+    const int32_t* __restrict__ prow2row, int32_t* __restrict__ out_idx, float* __restrict__ out_dist) {
+  const int l = lane_id();
+  const int64_t q = static_cast<int64_t>(blockIdx.x) * (kThreads / kWave) + threadIdx.x / kWave;
+  if (q >= n_slots) return;
+  const int32_t row = prow2row[q];
+  if (row < 0) return;
+  uint64_t key = kEmpty;
+  if (l < K) key = (static_cast<uint64_t>(__float_as_uint(dist_pad[q * K + l])) << 32) | static_cast<uint32_t>(l);
+  key = wave_sort_u64(key);
+  if (l < K) {
+    const int64_t e = q * K + static_cast<int64_t>(key & 0xFFFFFFFFull);
+    out_idx[static_cast<int64_t>(row) * K + l] = prow2row[idx_pad[e]];
+    out_dist[static_cast<int64_t>(row) * K + l] = dist_pad[e];
+  }
+}
+
+// obsp['distances'] of the combined lists: the self column and zero distances dropped,
+// columns ascending (scipy's coo->csr, setdiag(0), eliminate_zeros)
+__global__ __launch_bounds__(kThreads) void k_bbknn_dcount(const int32_t* __restrict__ idx,
+                                                           const float* __restrict__ dist, int64_t n, int K,
+                                                           int64_t* __restrict__ cnt) {
+  const int l = lane_id();
+  const int64_t q = static_cast<int64_t>(blockIdx.x) * (kThreads / kWave) + threadIdx.x / kWave;
+  if (q > n) return;
+  if (q == n) {
+    if (l == 0) cnt[n] = 0;
+    return;
+  }
+  const bool keep = l < K && idx[q * K + l] != static_cast<int32_t>(q) && dist[q * K + l] != 0.0f;
+  const uint64_t m = __ballot(keep);
+  if (l == 0) cnt[q] = __popcll(m);
+}
+__global__ __launch_bounds__(kThreads) void k_bbknn_dfill(const int32_t* __restrict__ idx,
+                                                          const float* __restrict__ dist, int64_t n, int K,
+                                                          const int64_t* __restrict__ indptr,
+                                                          int32_t* __restrict__ indices, float* __restrict__ data) {
+  const int l = lane_id();
+  const int64_t q = static_cast<int64_t>(blockIdx.x) * (kThreads / kWave) + threadIdx.x / kWave;
+  if (q >= n) return;
+  uint64_t key = kEmpty;
+  if (l < K && idx[q * K + l] != static_cast<int32_t>(q) && dist[q * K + l] != 0.0f)
+    key = (static_cast<uint64_t>(static_cast<uint32_t>(idx[q * K + l])) << 32) | __float_as_uint(dist[q * K + l]);
+  key = wave_sort_u64(key);
+  const int64_t b = indptr[q], m = indptr[q + 1] - b;
+  if (l < m) {
+    indices[b + l] = static_cast<int32_t>(key >> 32);
+    data[b + l] = __uint_as_float(static_cast<uint32_t>(key));
+  }
+}
+
 }  // namespace knn
 }  // namespace msc
 
@@ -1218,5 +1352,193 @@ extern "C" int msc_knn_batched(const float* X, const int64_t* offsets, int n_sam
     info->n_splits = L.S;
     info->list_len = 4 * kL;
   }
+  return kOk;
+}
+
+// ------------------------------------------------------------------------------------------
+// batch-balanced kNN
+// ------------------------------------------------------------------------------------------
+namespace {
+// Batch t occupies padded rows [P_t, P_t + n_t) (its input rows in ascending order),
+// P_t % 128 == 0; every query block of every batch runs one select split per batch t, over
+// exactly batch t's rows.  Results: kk_t = min(k, n_t) columns per batch at offset coff_t of
+// a K = sum kk_t wide row.
+struct BbLayout {
+  int64_t n_tot = 0, nblk = 0;
+  int nb = 0, K = 0;
+  std::vector<int32_t> prow2row, prow_batch, kk, coff;
+  std::vector<int64_t> brange, blk_range;
+};
+
+int make_bb_layout(const int32_t* batch, int64_t n, int n_batches, int k, BbLayout* L) {
+  using msc::knn::kQueriesPerBlock;
+  MSC_REQUIRE(batch && n >= 1 && n < (int64_t(1) << 31) - 8192, "bbknn: bad n or null batch");
+  MSC_REQUIRE(n_batches >= 1 && n_batches <= 64, "bbknn: n_batches must be in [1, 64]");
+  MSC_REQUIRE(k >= 1 && k <= 64, "bbknn: neighbors_within_batch must be in [1, 64]");
+  std::vector<int64_t> cnt(n_batches, 0);
+  for (int64_t i = 0; i < n; ++i) {
+    const int32_t b = batch[i];
+    MSC_REQUIRE(b >= 0 && b < n_batches, "bbknn: batch code %d of row %lld outside [0, %d)", b, (long long)i,
+                n_batches);
+    ++cnt[b];
+  }
+  L->nb = n_batches;
+  L->kk.resize(n_batches);
+  L->coff.resize(n_batches);
+  L->brange.resize(2 * n_batches);
+  std::vector<int64_t> P(n_batches);
+  int64_t base = 0;
+  int K = 0;
+  for (int t = 0; t < n_batches; ++t) {
+    MSC_REQUIRE(cnt[t] >= 1, "bbknn: batch %d has no cells", t);
+    L->kk[t] = static_cast<int32_t>(std::min<int64_t>(k, cnt[t]));
+    L->coff[t] = K;
+    K += L->kk[t];
+    P[t] = base;
+    L->brange[2 * t] = base;
+    L->brange[2 * t + 1] = base + cnt[t];
+    base += msc::ceil_div(cnt[t], kQueriesPerBlock) * kQueriesPerBlock;
+  }
+  MSC_REQUIRE(K <= 64, "bbknn: %d combined neighbours (sum over batches of min(k, batch size)) exceed 64", K);
+  L->K = K;
+  L->n_tot = base;
+  L->nblk = base / kQueriesPerBlock;
+  L->prow2row.assign(base, -1);
+  L->prow_batch.assign(base, 0);
+  std::vector<int64_t> pos(P);
+  for (int64_t i = 0; i < n; ++i) L->prow2row[pos[batch[i]]++] = static_cast<int32_t>(i);
+  for (int t = 0; t < n_batches; ++t)
+    for (int64_t r = P[t]; r < P[t] + msc::ceil_div(cnt[t], kQueriesPerBlock) * kQueriesPerBlock; ++r)
+      L->prow_batch[r] = t;
+  L->blk_range.resize(2 * L->nblk * n_batches);
+  for (int t = 0; t < n_batches; ++t)
+    for (int64_t b = 0; b < L->nblk; ++b) {
+      L->blk_range[2 * (t * L->nblk + b)] = L->brange[2 * t];
+      L->blk_range[2 * (t * L->nblk + b) + 1] = L->brange[2 * t + 1];
+    }
+  return msc::kOk;
+}
+
+struct BbBufs {
+  msc::knn::Plan p;
+  msc::knn::Buffers b;
+  float* Xpad;
+  int32_t *prow2row, *prow_batch, *kk, *coff, *flags;
+  int64_t *brange, *blk_range, *nfall;
+  int32_t* idx_pad;
+  float* dist_pad;
+};
+
+BbBufs carve_bb(msc::Carve& cv, const BbLayout& L, int d, int k, int precision) {
+  using namespace msc::knn;
+  BbBufs t{};
+  t.p = make_plan(L.n_tot, d, k, precision);
+  t.p.S = L.nb;  // one key set per (batch, slot)
+  t.b = carve(cv, t.p);
+  t.Xpad = cv.take<float>(static_cast<size_t>(L.n_tot) * d);
+  t.prow2row = cv.take<int32_t>(L.n_tot);
+  t.prow_batch = cv.take<int32_t>(L.n_tot);
+  t.kk = cv.take<int32_t>(L.nb);
+  t.coff = cv.take<int32_t>(L.nb);
+  t.flags = cv.take<int32_t>(static_cast<size_t>(L.nb) * L.n_tot);
+  t.brange = cv.take<int64_t>(2 * L.nb);
+  t.blk_range = cv.take<int64_t>(2 * L.nblk * L.nb);
+  t.nfall = cv.take<int64_t>(1);
+  t.idx_pad = cv.take<int32_t>(static_cast<size_t>(L.n_tot) * L.K);
+  t.dist_pad = cv.take<float>(static_cast<size_t>(L.n_tot) * L.K);
+  return t;
+}
+
+int bb_common(const int32_t* batch, int64_t n, int n_batches, int d, int k, int precision, BbLayout* L) {
+  MSC_REQUIRE(precision == MSC_PRECISION_FP32 || precision == MSC_PRECISION_BF16_RERANK, "bbknn: precision");
+  MSC_REQUIRE(d >= 1 && (precision == MSC_PRECISION_FP32 ? d <= 127 : d <= 125), "bbknn: d=%d unsupported", d);
+  return make_bb_layout(batch, n, n_batches, k, L);
+}
+}  // namespace
+
+extern "C" int msc_bbknn_workspace_size(const int32_t* batch, int64_t n, int n_batches, int d, int k, int precision,
+                                        size_t* bytes) {
+  MSC_REQUIRE(bytes, "bbknn: bytes is null");
+  BbLayout L;
+  MSC_TRY(bb_common(batch, n, n_batches, d, k, precision, &L));
+  Carve cv(nullptr, 0);
+  carve_bb(cv, L, d, k, precision);
+  *bytes = cv.used + 256;
+  return kOk;
+}
+
+extern "C" int msc_bbknn(const float* X, int64_t n, int d, const int32_t* batch, int n_batches, int k,
+                         int precision, int32_t* out_idx, float* out_dist, int* out_width, void* workspace,
+                         size_t workspace_bytes, void* stream, msc_knn_info* info) {
+  BbLayout L;
+  MSC_TRY(bb_common(batch, n, n_batches, d, k, precision, &L));
+  MSC_REQUIRE(X && out_idx && out_dist && workspace, "bbknn: null pointer argument");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  Carve cv(workspace, workspace_bytes);
+  BbBufs t = carve_bb(cv, L, d, k, precision);
+  if (!cv.ok()) {
+    set_error("bbknn: workspace too small (%zu < %zu)", workspace_bytes, cv.used);
+    return kWorkspaceTooSmall;
+  }
+  const int64_t nt = L.n_tot;
+  const int nb = L.nb;
+  MSC_HIP(hipMemcpyAsync(t.prow2row, L.prow2row.data(), nt * sizeof(int32_t), hipMemcpyHostToDevice, st));
+  MSC_HIP(hipMemcpyAsync(t.prow_batch, L.prow_batch.data(), nt * sizeof(int32_t), hipMemcpyHostToDevice, st));
+  MSC_HIP(hipMemcpyAsync(t.kk, L.kk.data(), nb * sizeof(int32_t), hipMemcpyHostToDevice, st));
+  MSC_HIP(hipMemcpyAsync(t.coff, L.coff.data(), nb * sizeof(int32_t), hipMemcpyHostToDevice, st));
+  MSC_HIP(hipMemcpyAsync(t.brange, L.brange.data(), 2 * nb * sizeof(int64_t), hipMemcpyHostToDevice, st));
+  MSC_HIP(hipMemcpyAsync(t.blk_range, L.blk_range.data(), L.blk_range.size() * sizeof(int64_t),
+                         hipMemcpyHostToDevice, st));
+  MSC_HIP(hipStreamSynchronize(st));  // the host tables are pageable and local to this call
+  k_pad_copy<<<static_cast<unsigned>(ceil_div(nt * d, kThreads)), kThreads, 0, st>>>(X, t.prow2row, nt, d, t.Xpad);
+  MSC_LAUNCH_CHECK();
+  const bool bf16 = precision == MSC_PRECISION_BF16_RERANK;
+  Operands op = prepare(t.p, t.b, t.Xpad, nt, bf16, st);
+  MSC_LAUNCH_CHECK();
+  MSC_TRY(launch_select(bf16, false, op.ks, op.X, nt, nt, d, nt, nb, nullptr, t.blk_range, t.b.keys, t.b.taus, st));
+  const unsigned wb = static_cast<unsigned>(ceil_div(static_cast<int64_t>(nb) * nt, kThreads / kWave));
+  finalize_sep_kernel<<<wb, kThreads, 0, st>>>(t.Xpad, nt, d, nb, t.b.keys, t.b.taus, op.qn2, op.qres, op.gstat,
+                                               op.bnd, t.prow2row, t.prow_batch, t.kk, t.coff, L.K, t.idx_pad,
+                                               t.dist_pad, t.flags);
+  MSC_LAUNCH_CHECK();
+  fallback_sep_kernel<<<wb, kThreads, 0, st>>>(t.Xpad, nt, d, nb, t.flags, t.brange, t.kk, t.coff, L.K, t.idx_pad,
+                                               t.dist_pad);
+  MSC_LAUNCH_CHECK();
+  k_bbknn_combine<<<static_cast<unsigned>(ceil_div(nt, kThreads / kWave)), kThreads, 0, st>>>(
+      t.idx_pad, t.dist_pad, nt, L.K, t.prow2row, out_idx, out_dist);
+  MSC_LAUNCH_CHECK();
+  if (out_width) *out_width = L.K;
+  if (info) {
+    count_flags<<<1, kThreads, 0, st>>>(t.flags, static_cast<int64_t>(nb) * nt, t.nfall);
+    MSC_LAUNCH_CHECK();
+    int64_t m = 0;
+    MSC_HIP(hipMemcpyAsync(&m, t.nfall, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    MSC_HIP(hipStreamSynchronize(st));
+    info->n_fallback = m;
+    info->n_second_pass = 0;
+    info->n_splits = nb;
+    info->list_len = 4 * kL;
+  }
+  return kOk;
+}
+
+extern "C" int msc_bbknn_distances_workspace_size(int64_t n, size_t* bytes) {
+  MSC_REQUIRE(bytes && n >= 1, "bbknn_distances: bad arguments");
+  *bytes = prims::scan_workspace<int64_t>(n + 1) + 256;
+  return kOk;
+}
+
+extern "C" int msc_bbknn_distances_csr(const int32_t* idx, const float* dist, int64_t n, int K, int64_t* out_indptr,
+                                       int32_t* out_indices, float* out_data, void* workspace,
+                                       size_t workspace_bytes, void* stream) {
+  MSC_REQUIRE(idx && dist && out_indptr && out_indices && out_data && workspace, "bbknn_distances: null pointer");
+  MSC_REQUIRE(n >= 1 && K >= 1 && K <= 64, "bbknn_distances: need n >= 1 and 1 <= K <= 64");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const unsigned wb = static_cast<unsigned>(ceil_div(n + 1, kThreads / kWave));
+  k_bbknn_dcount<<<wb, kThreads, 0, st>>>(idx, dist, n, K, out_indptr);
+  MSC_LAUNCH_CHECK();
+  MSC_TRY(prims::exclusive_scan<int64_t>(out_indptr, out_indptr, n + 1, workspace, workspace_bytes, st));
+  k_bbknn_dfill<<<wb, kThreads, 0, st>>>(idx, dist, n, K, out_indptr, out_indices, out_data);
+  MSC_LAUNCH_CHECK();
   return kOk;
 }

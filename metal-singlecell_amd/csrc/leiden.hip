@@ -1242,6 +1242,49 @@ static int leiden_pass(Ctx& c, Graph& g0, int64_t seed, const int32_t* init) {
   return kOk;
 }
 
+// multilevel Louvain (graph/louvain.py:397-437): every level moves from singletons with seed
+// seed + 100 * level, relabels densely and stops, keeping the previous level's labels, when
+// no vertex moved or Q did not rise by more than kTol; otherwise it aggregates on the moved
+// partition itself (no refinement).  Result (dense) in c.final_lab.
+static int louvain_pass(Ctx& c, Graph& g0, int64_t seed, int max_levels) {
+  hipStream_t st = c.st;
+  const int64_t n0 = g0.n;
+  Graph g = g0;
+  int slot = 0;
+  k_iota_i32<<<grid_for(n0), kThreads, 0, st>>>(c.orig2node, n0);
+  MSC_LAUNCH_CHECK();
+  double q_prev = -1.0;
+  for (int level = 0; level < max_levels; ++level) {
+    k_iota_i32<<<grid_for(g.n), kThreads, 0, st>>>(c.comm, g.n);
+    MSC_LAUNCH_CHECK();
+    int mp = 0;
+    MSC_TRY(local_moving(c, g, seed + 100 * static_cast<int64_t>(level), nullptr, &mp));
+    int64_t ncomm;
+    MSC_TRY(dense_relabel(c, c.comm, g.n, c.P, &ncomm));
+    double q;
+    MSC_TRY(modularity(c, g, c.P, ncomm, &q));
+    if (c.info && c.info->n_levels < MSC_LEIDEN_MAX_RECORDS) {
+      msc_leiden_level* rec = &c.info->levels[c.info->n_levels++];
+      rec->n = g.n; rec->n_edges = g.E; rec->move_passes = mp; rec->refine_passes = 0;
+      rec->n_comms = ncomm; rec->n_refined = ncomm; rec->modularity = q;
+    }
+    if (ncomm == g.n || q <= q_prev + kTol) break;
+    k_gather_i32<<<grid_for(n0), kThreads, 0, st>>>(c.P, c.orig2node, c.tmp_lab, n0);
+    k_copy_i32<<<grid_for(n0), kThreads, 0, st>>>(c.tmp_lab, c.orig2node, n0);
+    MSC_LAUNCH_CHECK();
+    Graph ng;
+    MSC_TRY(contract(c, g, c.P, ncomm, slot, &ng));
+    slot ^= 1;
+    g = ng;
+    q_prev = q;
+  }
+  // orig2node composes dense surjective maps, so it is dense already; the relabel keeps the
+  // output contract independent of that argument
+  int64_t nf;
+  MSC_TRY(dense_relabel(c, c.orig2node, n0, c.final_lab, &nf));
+  return kOk;
+}
+
 static int validate(int64_t n, int64_t nnz) {
   MSC_REQUIRE(n >= 1 && n < (int64_t(1) << 31) - 1, "leiden: n out of range");
   MSC_REQUIRE(nnz >= 0 && nnz < (int64_t(1) << 31) - 1, "leiden: nnz out of range (< 2^31)");
@@ -1254,48 +1297,31 @@ static int ceil_log2(double x) {
   return m == 0.5 ? e - 1 : e;
 }
 
-}  // namespace leiden
-}  // namespace msc
+// side streams (default priority) and fork/join events are created on first need by
+// local_moving and released on every return path of an entry point
+struct SideGuard {
+  Ctx& c;
+  ~SideGuard() {
+    for (hipStream_t s : {c.side, c.side2})
+      if (s) {
+        (void)hipStreamSynchronize(s);
+        (void)hipStreamDestroy(s);
+      }
+    for (hipEvent_t e : {c.ev_fork, c.ev_join, c.ev_join2})
+      if (e) (void)hipEventDestroy(e);
+  }
+};
 
-using namespace msc;
-using namespace msc::leiden;
-
-extern "C" int msc_leiden_workspace_size(int64_t n, int64_t nnz, size_t* bytes) {
-  MSC_TRY(validate(n, nnz));
-  MSC_REQUIRE(bytes, "leiden: bytes is null");
-  Carve cv(nullptr, 0);
-  Ctx c{};
-  carve_ctx(cv, c, n, nnz);
-  cv.take<int64_t>(nnz + 1);  // level-0 quantised weights
-  cv.take<int64_t>(n + 1);    // level-0 degrees
-  *bytes = cv.used + 256;
-  return kOk;
-}
-
-extern "C" int msc_leiden(const int64_t* indptr, const int32_t* indices, const float* weights, int64_t n,
-                          int64_t nnz, double resolution, int64_t seed, int n_iterations, double commit_prob,
-                          const int32_t* init_labels, int32_t* out_labels, double* out_modularity,
-                          msc_leiden_info* info, void* workspace, size_t workspace_bytes, void* stream) {
+// set-up shared by msc_leiden and msc_louvain: arguments, workspace, the fixed-point
+// exponent and the quantised level-0 graph g0
+static int begin_run(Ctx& c, Graph& g0, const int64_t* indptr, const int32_t* indices, const float* weights,
+                     int64_t n, int64_t nnz, double resolution, double commit_prob, int32_t* out_labels,
+                     msc_leiden_info* info, void* workspace, size_t workspace_bytes, void* stream) {
   MSC_TRY(validate(n, nnz));
   MSC_REQUIRE(indptr && out_labels && workspace && (nnz == 0 || (indices && weights)),
               "leiden: null pointer argument");
   MSC_REQUIRE(commit_prob > 0.0 && commit_prob <= 1.0, "leiden: commit_prob must be in (0, 1]");
-  Ctx c{};
   c.st = static_cast<hipStream_t>(stream);
-  // side streams (default priority) and fork/join events, created on first need and
-  // released on every return path
-  struct SideGuard {
-    Ctx& c;
-    ~SideGuard() {
-      for (hipStream_t s : {c.side, c.side2})
-        if (s) {
-          (void)hipStreamSynchronize(s);
-          (void)hipStreamDestroy(s);
-        }
-      for (hipEvent_t e : {c.ev_fork, c.ev_join, c.ev_join2})
-        if (e) (void)hipEventDestroy(e);
-    }
-  } side_guard{c};
   c.n0 = n;
   c.E0 = nnz;
   c.gamma = resolution;
@@ -1327,7 +1353,6 @@ extern "C" int msc_leiden(const int64_t* indptr, const int32_t* indices, const f
     k_quantize<<<grid_for(nnz), kThreads, 0, st>>>(weights, nnz, c.s, wq0);
     MSC_LAUNCH_CHECK();
   }
-  Graph g0;
   g0.n = n;
   g0.E = nnz;
   g0.indptr = const_cast<int64_t*>(indptr);
@@ -1336,6 +1361,57 @@ extern "C" int msc_leiden(const int64_t* indptr, const int32_t* indices, const f
   g0.k = k0;
   MSC_TRY(graph_setup(c, g0));
   MSC_REQUIRE(g0.twom > 0, "leiden: graph has no positive edge weight");
+  return kOk;
+}
+
+// modularity of out_labels on g0, scale records, final synchronisation
+static int end_run(Ctx& c, const Graph& g0, const int32_t* out_labels, double* out_modularity,
+                   msc_leiden_info* info) {
+  if (out_modularity) {
+    int64_t C;
+    MSC_TRY(dense_relabel(c, out_labels, g0.n, c.tmp_lab, &C));
+    MSC_TRY(modularity(c, g0, c.tmp_lab, C, out_modularity));
+  }
+  if (info) {
+    info->scale_exponent = c.s;
+    info->twom_fixed = g0.twom;
+  }
+  MSC_HIP(hipStreamSynchronize(c.st));
+  return kOk;
+}
+
+}  // namespace leiden
+}  // namespace msc
+
+using namespace msc;
+using namespace msc::leiden;
+
+extern "C" int msc_leiden_workspace_size(int64_t n, int64_t nnz, size_t* bytes) {
+  MSC_TRY(validate(n, nnz));
+  MSC_REQUIRE(bytes, "leiden: bytes is null");
+  Carve cv(nullptr, 0);
+  Ctx c{};
+  carve_ctx(cv, c, n, nnz);
+  cv.take<int64_t>(nnz + 1);  // level-0 quantised weights
+  cv.take<int64_t>(n + 1);    // level-0 degrees
+  *bytes = cv.used + 256;
+  return kOk;
+}
+
+extern "C" int msc_louvain_workspace_size(int64_t n, int64_t nnz, size_t* bytes) {
+  return msc_leiden_workspace_size(n, nnz, bytes);
+}
+
+extern "C" int msc_leiden(const int64_t* indptr, const int32_t* indices, const float* weights, int64_t n,
+                          int64_t nnz, double resolution, int64_t seed, int n_iterations, double commit_prob,
+                          const int32_t* init_labels, int32_t* out_labels, double* out_modularity,
+                          msc_leiden_info* info, void* workspace, size_t workspace_bytes, void* stream) {
+  Ctx c{};
+  SideGuard side_guard{c};
+  Graph g0;
+  MSC_TRY(begin_run(c, g0, indptr, indices, weights, n, nnz, resolution, commit_prob, out_labels, info, workspace,
+                    workspace_bytes, stream));
+  hipStream_t st = c.st;
   const int iters = std::max(1, n_iterations);
   for (int it = 0; it < iters; ++it) {
     const int32_t* init = nullptr;
@@ -1352,15 +1428,21 @@ extern "C" int msc_leiden(const int64_t* indptr, const int32_t* indices, const f
     k_copy_i32<<<grid_for(n), kThreads, 0, st>>>(c.final_lab, out_labels, n);
     MSC_LAUNCH_CHECK();
   }
-  if (out_modularity) {
-    int64_t C;
-    MSC_TRY(dense_relabel(c, out_labels, n, c.tmp_lab, &C));
-    MSC_TRY(modularity(c, g0, c.tmp_lab, C, out_modularity));
-  }
-  if (info) {
-    info->scale_exponent = c.s;
-    info->twom_fixed = g0.twom;
-  }
-  MSC_HIP(hipStreamSynchronize(st));
-  return kOk;
+  return end_run(c, g0, out_labels, out_modularity, info);
+}
+
+extern "C" int msc_louvain(const int64_t* indptr, const int32_t* indices, const float* weights, int64_t n,
+                           int64_t nnz, double resolution, int64_t seed, int max_levels, double commit_prob,
+                           int32_t* out_labels, double* out_modularity, msc_leiden_info* info, void* workspace,
+                           size_t workspace_bytes, void* stream) {
+  MSC_REQUIRE(max_levels >= 1 && max_levels <= kMaxLevels, "louvain: max_levels must be in [1, %d]", kMaxLevels);
+  Ctx c{};
+  SideGuard side_guard{c};
+  Graph g0;
+  MSC_TRY(begin_run(c, g0, indptr, indices, weights, n, nnz, resolution, commit_prob, out_labels, info, workspace,
+                    workspace_bytes, stream));
+  MSC_TRY(louvain_pass(c, g0, seed, max_levels));
+  k_copy_i32<<<grid_for(n), kThreads, 0, c.st>>>(c.final_lab, out_labels, n);
+  MSC_LAUNCH_CHECK();
+  return end_run(c, g0, out_labels, out_modularity, info);
 }

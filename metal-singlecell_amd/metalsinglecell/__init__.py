@@ -3,7 +3,9 @@
 ``import metalsinglecell as msc`` gives the reference's API for this path:
 
 * ``msc.pp.neighbors``            exact kNN + UMAP connectivities
+* ``msc.pp.bbknn``                batch-balanced exact kNN + connectivities
 * ``msc.tl.leiden(backend="gpu")`` synchronous parallel Leiden
+* ``msc.tl.louvain(backend="gpu")`` synchronous parallel multilevel Louvain
 * ``msc.gr.spatial_neighbors_knn`` exact grid spatial kNN
 * ``msc.gr.spatial_autocorr``      Moran's I with a permutation null
 

@@ -64,6 +64,14 @@ SIGNATURES = {
     "msc_knn_batched": (c_int, [c_vp, c_vp, c_int, c_int, c_int, c_int, c_int, c_vp, c_vp, c_vp, c_sz,
                                 c_vp, ctypes.POINTER(KnnInfo)]),
     "msc_knn_distances_csr": (c_int, [c_vp, c_vp, c_i64, c_int, c_vp, c_vp, c_vp, c_vp]),
+    "msc_bbknn_workspace_size": (c_int, [c_vp, c_i64, c_int, c_int, c_int, c_int, ctypes.POINTER(c_sz)]),
+    "msc_bbknn": (c_int, [c_vp, c_i64, c_int, c_vp, c_int, c_int, c_int, c_vp, c_vp, ctypes.POINTER(c_int),
+                          c_vp, c_sz, c_vp, ctypes.POINTER(KnnInfo)]),
+    "msc_bbknn_distances_workspace_size": (c_int, [c_i64, ctypes.POINTER(c_sz)]),
+    "msc_bbknn_distances_csr": (c_int, [c_vp, c_vp, c_i64, c_int, c_vp, c_vp, c_vp, c_vp, c_sz, c_vp]),
+    "msc_trim_graph_workspace_size": (c_int, [c_i64, ctypes.POINTER(c_sz)]),
+    "msc_trim_graph": (c_int, [c_vp, c_vp, c_vp, c_i64, c_i64, c_vp, c_vp, c_vp, ctypes.POINTER(c_i64), c_vp,
+                               c_sz, c_vp]),
     "msc_fuzzy_workspace_size": (c_int, [c_i64, c_int, ctypes.POINTER(c_sz)]),
     "msc_fuzzy_connectivities": (c_int, [c_vp, c_vp, c_i64, c_int, c_vp, c_vp, c_vp, c_i64, c_vp,
                                          c_vp, ctypes.POINTER(c_i64), c_vp, c_sz, c_vp]),
@@ -82,6 +90,9 @@ SIGNATURES = {
     "msc_leiden_workspace_size": (c_int, [c_i64, c_i64, ctypes.POINTER(c_sz)]),
     "msc_leiden": (c_int, [c_vp, c_vp, c_vp, c_i64, c_i64, c_dbl, c_i64, c_int, c_dbl, c_vp, c_vp,
                            ctypes.POINTER(c_dbl), ctypes.POINTER(LeidenInfo), c_vp, c_sz, c_vp]),
+    "msc_louvain_workspace_size": (c_int, [c_i64, c_i64, ctypes.POINTER(c_sz)]),
+    "msc_louvain": (c_int, [c_vp, c_vp, c_vp, c_i64, c_i64, c_dbl, c_i64, c_int, c_dbl, c_vp,
+                            ctypes.POINTER(c_dbl), ctypes.POINTER(LeidenInfo), c_vp, c_sz, c_vp]),
 }
 
 
@@ -213,6 +224,74 @@ def knn_distances_csr(idx, dist):
     return indptr, indices[:m], data[:m]
 
 
+def bbknn(X, batch_codes, n_batches, k, precision="bf16", return_info=False):
+    """Batch-balanced kNN: ``(idx int32 (n,K), dist float32 (n,K))`` with K = sum_t min(k, n_t)."""
+    import torch
+    lib = load_library()
+    Xd = to_device(X, torch.float32)
+    if Xd.dim() != 2:
+        raise ValueError("X must be 2-D (cells x dims)")
+    n, d = Xd.shape
+    codes = np.ascontiguousarray(batch_codes, dtype=np.int32)
+    if codes.shape != (n,):
+        raise ValueError("batch codes must have one entry per cell")
+    p = PRECISION[precision]
+    nbytes = c_sz(0)
+    check(lib.msc_bbknn_workspace_size(codes.ctypes.data, n, int(n_batches), d, int(k), p, ctypes.byref(nbytes)))
+    ws = workspace(nbytes.value)
+    counts = np.bincount(codes, minlength=int(n_batches))
+    K = int(np.minimum(counts, int(k)).sum())
+    idx = torch.empty((n, K), dtype=torch.int32, device=Xd.device)
+    dist = torch.empty((n, K), dtype=torch.float32, device=Xd.device)
+    width = c_int(0)
+    info = KnnInfo()
+    check(lib.msc_bbknn(ptr(Xd), n, d, codes.ctypes.data, int(n_batches), int(k), p, ptr(idx), ptr(dist),
+                        ctypes.byref(width), ptr(ws), ws.numel(), stream_ptr(),
+                        ctypes.byref(info) if return_info else None))
+    assert width.value == K
+    if return_info:
+        return idx, dist, {f: getattr(info, f) for f, _ in KnnInfo._fields_}
+    return idx, dist
+
+
+def bbknn_distances_csr(idx, dist):
+    """Device CSR ``(indptr, indices, data)`` of bbknn's obsp['distances'] (self and zeros dropped)."""
+    import torch
+    lib = load_library()
+    n, K = idx.shape
+    nbytes = c_sz(0)
+    check(lib.msc_bbknn_distances_workspace_size(n, ctypes.byref(nbytes)))
+    ws = workspace(nbytes.value)
+    indptr = torch.empty(n + 1, dtype=torch.int64, device=idx.device)
+    indices = torch.empty(max(n * K, 1), dtype=torch.int32, device=idx.device)
+    data = torch.empty(max(n * K, 1), dtype=torch.float32, device=idx.device)
+    check(lib.msc_bbknn_distances_csr(ptr(idx), ptr(dist), n, K, ptr(indptr), ptr(indices), ptr(data), ptr(ws),
+                                      ws.numel(), stream_ptr()))
+    m = int(indptr[n].item())
+    return indptr, indices[:m], data[:m]
+
+
+def trim_graph(indptr, indices, data, n, trim):
+    """Device CSR of ``neighbors._trim_graph`` (keep entries >= both rows' trim-th largest)."""
+    import torch
+    lib = load_library()
+    indptr = to_device(indptr, torch.int64)
+    indices = to_device(indices, torch.int32)
+    data = to_device(data, torch.float32)
+    nbytes = c_sz(0)
+    check(lib.msc_trim_graph_workspace_size(n, ctypes.byref(nbytes)))
+    ws = workspace(nbytes.value)
+    cap = max(int(indices.numel()), 1)
+    o_ip = torch.empty(n + 1, dtype=torch.int64, device=indptr.device)
+    o_ind = torch.empty(cap, dtype=torch.int32, device=indptr.device)
+    o_dat = torch.empty(cap, dtype=torch.float32, device=indptr.device)
+    nnz = c_i64(0)
+    check(lib.msc_trim_graph(ptr(indptr), ptr(indices), ptr(data), n, int(trim), ptr(o_ip), ptr(o_ind), ptr(o_dat),
+                             ctypes.byref(nnz), ptr(ws), ws.numel(), stream_ptr()))
+    m = nnz.value
+    return o_ip, o_ind[:m], o_dat[:m]
+
+
 # ----------------------------------------------------------------------------------------
 # (c) fuzzy simplicial set connectivities
 # ----------------------------------------------------------------------------------------
@@ -269,6 +348,29 @@ def leiden(indptr, indices, data, n, resolution=1.0, random_state=0, n_iteration
                          int(random_state), int(n_iterations), float(commit_prob),
                          None if init is None else ptr(init), ptr(labels), ctypes.byref(q),
                          ctypes.byref(info), ptr(ws), ws.numel(), stream_ptr()))
+    levels = [{f: getattr(info.levels[i], f) for f, _ in LeidenLevel._fields_}
+              for i in range(info.n_levels)]
+    return labels, q.value, {"scale_exponent": info.scale_exponent,
+                             "twom_fixed": info.twom_fixed, "levels": levels}
+
+
+def louvain(indptr, indices, data, n, resolution=1.0, random_state=0, max_levels=20, commit_prob=0.9):
+    """Multilevel Louvain on a device CSR: ``(labels int32 device (n,), modularity, info dict)``."""
+    import torch
+    lib = load_library()
+    indptr = to_device(indptr, torch.int64)
+    indices = to_device(indices, torch.int32)
+    data = to_device(data, torch.float32)
+    nnz = int(indices.numel())
+    nbytes = c_sz(0)
+    check(lib.msc_louvain_workspace_size(n, nnz, ctypes.byref(nbytes)))
+    ws = workspace(nbytes.value)
+    labels = torch.empty(n, dtype=torch.int32, device=indptr.device)
+    q = c_dbl(0.0)
+    info = LeidenInfo()
+    check(lib.msc_louvain(ptr(indptr), ptr(indices), ptr(data), n, nnz, float(resolution),
+                          int(random_state), int(max_levels), float(commit_prob), ptr(labels),
+                          ctypes.byref(q), ctypes.byref(info), ptr(ws), ws.numel(), stream_ptr()))
     levels = [{f: getattr(info.levels[i], f) for f, _ in LeidenLevel._fields_}
               for i in range(info.n_levels)]
     return labels, q.value, {"scale_exponent": info.scale_exponent,

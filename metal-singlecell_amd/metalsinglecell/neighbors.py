@@ -10,6 +10,9 @@ Mirrors the reference's ``metalsinglecell.neighbors`` seams:
   (neighbors.py:727-760), connectivities from the fuzzy simplicial set (csrc/fuzzy.hip).
 * ``_knn_grid(coords, k)`` -> ``(indices, dists)`` self-inclusive exact spatial kNN
   (neighbors.py:333-417) on the device grid (csrc/grid.hip).
+* ``bbknn(X_pca, batch, ...)`` -> ``(distances, connectivities, n_neighbors)``
+  (neighbors.py:491-559 with ``_trim_graph`` :465-488): per-batch exact kNN in one select
+  launch (one candidate split per batch), connectivities and trimming on the device.
 
 All computation is on the GPU; without one these functions raise.
 """
@@ -94,3 +97,42 @@ def _knn_grid(coords, k, min_n=None):
     idx = np.c_[np.arange(n), idx.cpu().numpy()].astype(np.int64)
     dist = np.c_[np.zeros(n, np.float32), dist.cpu().numpy()]
     return idx, dist
+
+
+def _batch_codes(batch):
+    """Codes 0..B-1 of the present categories in ``np.unique`` order (neighbors.py:510-511)."""
+    cats, codes = np.unique(np.asarray(batch), return_inverse=True)
+    return codes.astype(np.int32).ravel(), int(cats.size)
+
+
+def bbknn_device(X, batch, neighbors_within_batch=3, trim=None, precision=DEFAULT_PRECISION):
+    """Batch-balanced kNN graph as device tensors: dict with ``idx``/``dist`` (n x K),
+    ``distances`` and ``connectivities`` CSR tuples (connectivities trimmed), and ``K``."""
+    codes, nb = _batch_codes(batch)
+    idx, dist = _hip.bbknn(X, codes, nb, int(neighbors_within_batch), precision=precision)
+    K = int(idx.shape[1])
+    dcsr = _hip.bbknn_distances_csr(idx, dist)
+    conn, _ = _hip.fuzzy_connectivities(idx, dist)
+    t = 10 * K if trim is None else int(trim)
+    if t > 0:
+        conn = _hip.trim_graph(*conn, idx.shape[0], t)
+    return {"idx": idx, "dist": dist, "distances": dcsr, "connectivities": conn, "K": K}
+
+
+def bbknn(X_pca, batch, neighbors_within_batch: int = 3, n_pcs: int | None = 50, trim: int | None = None,
+          random_state: int = 0, precision=DEFAULT_PRECISION):
+    """Batch-balanced kNN (neighbors.py:491-559): ``(distances, connectivities, n_neighbors)``.
+
+    For every cell the ``min(neighbors_within_batch, n_b)`` exact nearest cells of each batch
+    b (the cell itself first in its own batch), merged and sorted by distance (ties keep the
+    batch order), then one fuzzy simplicial set over the K combined neighbours, trimmed to
+    each cell's ``10 K`` (or ``trim``) strongest connectivities.  ``random_state`` does not
+    affect the exact results."""
+    X = np.asarray(X_pca, dtype=np.float32)
+    if n_pcs is not None:
+        X = X[:, :n_pcs]
+    n = X.shape[0]
+    if np.asarray(batch).shape[0] != n:
+        raise ValueError("batch must have one entry per cell")
+    out = bbknn_device(np.ascontiguousarray(X), batch, neighbors_within_batch, trim, precision)
+    return _csr_to_scipy(*out["distances"], n), _csr_to_scipy(*out["connectivities"], n), out["K"]

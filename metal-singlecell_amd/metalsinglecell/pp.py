@@ -1,4 +1,5 @@
-"""``pp`` namespace — the neighbour-graph drop-in of ``scanpy.pp.neighbors``.
+"""``pp`` namespace — the neighbour-graph drop-ins of ``scanpy.pp.neighbors`` and
+``sc.external.pp.bbknn``.
 
 Mirrors the reference's ``metalsinglecell.pp.neighbors`` (src/metalsinglecell/pp.py:753-788)
 and its representation choice (:722-745): same signature (``n_pcs`` positional after
@@ -68,4 +69,29 @@ def neighbors(adata, n_neighbors: int = 15, n_pcs: int | None = None, *, use_rep
         params["n_pcs"] = n_pcs
     adata.uns["neighbors" if key_added is None else key_added] = {
         "connectivities_key": ck, "distances_key": dk, "params": params}
+    return adata if copy else None
+
+
+def bbknn(adata, batch_key: str = "batch", *, use_rep: str = "X_pca", neighbors_within_batch: int = 3,
+          n_pcs: int | None = 50, trim: int | None = None, random_state: int = 0, copy: bool = False,
+          precision: str = "bf16"):
+    """Batch-balanced kNN (``sc.external.pp.bbknn``, reference pp.py:809-833); writes
+    ``obsp['distances']``, ``obsp['connectivities']`` and ``uns['neighbors']`` whose
+    ``n_neighbors`` is the combined width K = sum over batches of min(k, batch size)."""
+    from .neighbors import bbknn as _bbknn
+    adata = adata.copy() if copy else adata
+    if use_rep not in adata.obsm:
+        raise ValueError(f"Did not find {use_rep} in .obsm.keys(). You need to compute it first.")
+    if batch_key not in adata.obs:
+        raise KeyError(f"batch_key {batch_key!r} not in adata.obs")
+    dist, conn, K = _bbknn(np.asarray(adata.obsm[use_rep], dtype=np.float32), np.asarray(adata.obs[batch_key]),
+                           neighbors_within_batch=neighbors_within_batch, n_pcs=n_pcs, trim=trim,
+                           random_state=random_state, precision=precision)
+    adata.obsp["distances"] = dist
+    adata.obsp["connectivities"] = conn
+    adata.uns["neighbors"] = {
+        "connectivities_key": "connectivities", "distances_key": "distances",
+        "params": {"n_neighbors": K, "method": "umap", "metric": "euclidean",
+                   "random_state": random_state, "use_rep": use_rep, "n_pcs": n_pcs,
+                   "bbknn": {"trim": 10 * K if trim is None else trim, "computation": "hip"}}}
     return adata if copy else None

@@ -1,10 +1,12 @@
-"""``tl`` namespace — the Leiden drop-in of ``scanpy.tl.leiden``.
+"""``tl`` namespace — the Leiden and Louvain drop-ins of ``scanpy.tl.leiden`` / ``tl.louvain``.
 
-Mirrors the reference's ``metalsinglecell.tl.leiden`` (src/metalsinglecell/tl.py:55-73) with
+Mirrors the reference's ``metalsinglecell.tl.leiden`` (src/metalsinglecell/tl.py:55-73) and
+``tl.louvain`` (:76-105) with
 ``_conn`` (:14-21) and ``_categorical`` (:35-52): the connectivity graph is resolved through
 ``uns[neighbors_key]``, labels are written to ``obs[key_added]`` as a Categorical of string
 labels with natural-sorted categories, and ``uns[key_added]['params']`` records resolution,
-random_state and n_iterations.  ``backend="gpu"`` runs csrc/leiden.hip on the MI355X.
+random_state and n_iterations (Louvain: resolution and random_state).  ``backend="gpu"`` runs
+csrc/leiden.hip on the MI355X.
 """
 
 from __future__ import annotations
@@ -46,4 +48,17 @@ def leiden(adata, resolution: float = 1.0, key_added: str = "leiden", random_sta
     adata.obs[key_added] = _categorical(lab)
     adata.uns[key_added] = {"params": {"resolution": resolution, "random_state": random_state,
                                        "n_iterations": n_iterations}}
+    return adata if copy else None
+
+
+def louvain(adata, resolution: float = 1.0, key_added: str = "louvain", random_state: int = 0,
+            backend: str = "igraph", variant: str = "sync", commit_prob: float = 0.9,
+            neighbors_key: str = "neighbors", copy: bool = False):
+    """Louvain clustering; writes ``obs[key_added]`` (categorical) and ``uns[key_added]``."""
+    from .cluster import louvain as _louvain
+    adata = adata.copy() if copy else adata
+    lab = _louvain(_conn(adata, neighbors_key), resolution=resolution, random_state=random_state,
+                   backend=backend, variant=variant, commit_prob=commit_prob)
+    adata.obs[key_added] = _categorical(lab)
+    adata.uns[key_added] = {"params": {"resolution": resolution, "random_state": random_state}}
     return adata if copy else None

@@ -22,6 +22,9 @@ is the identity; aggregate on the refined partition (intra edges become self-loo
 summed) with P lifted; at most 20 levels.  ``n_iterations`` repeats the pass with
 ``seed + 17 * it`` from the previous labels (leiden.py:374-377).
 
+``louvain`` restates the multilevel Louvain of the same module (louvain.py:397-453) on the
+same move pass and contraction: levels move from singletons, no refinement.
+
 Deliberate definitions shared bit-for-bit with the HIP kernels (documented deviations from
 the reference's MLX arithmetic):
 * edge weights are quantised once to 64-bit fixed point, ``wq = rint(w * 2^s)`` with
@@ -261,3 +264,29 @@ def modularity_of_labels(conn, labels, resolution=1.0):
     """Modularity of a labelling on the quantised graph (float64)."""
     g = from_scipy(conn)
     return modularity(g, dense_labels(np.asarray(labels)), resolution)
+
+
+def louvain(conn, resolution=1.0, random_state=0, max_levels=MAX_LEVELS, commit_prob=0.9, stats=None):
+    """Multilevel Louvain, ``graph.louvain.louvain`` with ``variant="sync"`` (louvain.py:397-437).
+
+    Level l: synchronous local moving from singletons with seed ``random_state + 100 l``;
+    dense relabel; stop, keeping the previous level's labels, when every vertex stayed alone
+    or Q <= Q_prev + 1e-9; otherwise map the original vertices through the moved partition
+    and aggregate on it (``_contract_dense``, louvain.py:440-453).  Returns dense labels."""
+    g = from_scipy(conn)
+    orig2super = np.arange(g.n, dtype=np.int64)
+    q_prev = -1.0
+    for level in range(max_levels):
+        comm, mp = local_moving_sync(g, resolution, random_state + 100 * level, np.arange(g.n),
+                                     commit_prob=commit_prob)
+        cd = dense_labels(comm)
+        C = int(cd.max()) + 1
+        q = modularity(g, cd, resolution)
+        if stats is not None:
+            stats.append(dict(level=level, n=g.n, E=g.indices.size, move_passes=mp, n_comms=C, q=q))
+        if C == g.n or q <= q_prev + TOL:
+            break
+        orig2super = cd[orig2super]
+        g = contract(g, cd, C)
+        q_prev = q
+    return dense_labels(orig2super)

@@ -40,6 +40,10 @@ def main():
     q = leiden.modularity_of_labels(P, lab)
     np.savez_compressed(os.path.join(HERE, "leiden_mixture600.npz"), labels=lab.astype(np.int8),
                         q=np.float64(q))
+    lab = leiden.louvain(P, resolution=1.0, random_state=0)
+    q = leiden.modularity_of_labels(P, lab)
+    np.savez_compressed(os.path.join(HERE, "louvain_mixture600.npz"), labels=lab.astype(np.int8),
+                        q=np.float64(q))
     xy, r, c = lattice(30, jitter=1.0, seed=0)
     si, sd = grid.spatial_knn(xy.astype(np.float32), 6)
     np.savez_compressed(os.path.join(HERE, "spatial_hex30_k6.npz"), xy=xy,

@@ -31,6 +31,9 @@ def test_oracle_reproduces_fixtures():
     assert np.array_equal(P.data, f["data"]) and np.array_equal(sigma, f["sigma"])
     lab = leiden.leiden(P, n_iterations=2)
     assert np.array_equal(lab, _load("leiden_mixture600.npz")["labels"])
+    lv = _load("louvain_mixture600.npz")
+    lab = leiden.louvain(P)
+    assert np.array_equal(lab, lv["labels"]) and leiden.modularity_of_labels(P, lab) == float(lv["q"])
     s = _load("spatial_hex30_k6.npz")
     si, sd = grid.spatial_knn(s["xy"].astype(np.float32), 6)
     assert np.array_equal(si, s["idx"]) and np.array_equal(sd, s["dist"])
@@ -56,6 +59,9 @@ def test_hip_reproduces_fixtures():
     assert np.array_equal(ind.cpu().numpy(), f["indices"]) and np.array_equal(dat.cpu().numpy(), f["data"])
     lab, q, _ = _hip.leiden(ip, ind, dat, 600, n_iterations=2)
     L = _load("leiden_mixture600.npz")
+    assert np.array_equal(lab.cpu().numpy(), L["labels"]) and q == float(L["q"])
+    lab, q, _ = _hip.louvain(ip, ind, dat, 600)
+    L = _load("louvain_mixture600.npz")
     assert np.array_equal(lab.cpu().numpy(), L["labels"]) and q == float(L["q"])
     s = _load("spatial_hex30_k6.npz")
     si, sd = _hip.spatial_knn(s["xy"].astype(np.float32), 6)

@@ -105,6 +105,12 @@ def test_api_errors_before_any_gpu_work():
     a.obsp["connectivities"] = sp.identity(20, format="csr")
     with pytest.raises(ValueError, match="backend"):
         tl.leiden(a, backend="banana")
+    with pytest.raises(ValueError, match="backend"):
+        tl.louvain(a, backend="banana")
+    with pytest.raises(NotImplementedError, match="variant"):
+        tl.louvain(a, backend="gpu", variant="colored")
+    with pytest.raises(ValueError, match="commit_prob"):
+        tl.louvain(a, backend="gpu", commit_prob=0.0)
     with pytest.raises(ValueError, match="mode"):
         gr.spatial_autocorr(a, mode="Moran")
     with pytest.raises(KeyError, match="spatial"):
