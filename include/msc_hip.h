@@ -228,6 +228,25 @@ int msc_spatial_knn(const float* coords, int64_t n, int dim, int k, int32_t* out
                     msc_spatial_info* info /*host, may be NULL*/, void* workspace,
                     size_t workspace_bytes, void* stream);
 
+/* Fixed-radius spatial neighbours, self excluded, as a canonical CSR (columns ascending).
+ * Replaces neighbors._radius_grid and its grid_radius kernel (src/metalsinglecell/
+ * neighbors.py:255-330, 199-252) as used by gr.spatial_neighbors_radius (gr.py:228-252).
+ * Pair (i, j) is kept when r_min^2 <= d2 <= r_max^2 with d2 the float32 squared distance of
+ * msc_spatial_knn and the bounds rounded to float32 from float64 squares.  Two calls:
+ * _count writes out_indptr (n+1 int64) and *out_nnz (host; synchronises); _fill (same
+ * coords, radii and indptr) writes out_indices (nnz int32), optionally out_dist (nnz
+ * float32, sqrt(d2)) and, with coords64 (n x dim float64), out_len64 (nnz float64 edge
+ * lengths).  Workspace: msc_spatial_radius_workspace_size(n, dim, nnz) — nnz = 0 suffices
+ * for _count, the fill needs the pair count. */
+int msc_spatial_radius_workspace_size(int64_t n, int dim, int64_t nnz, size_t* bytes /*host*/);
+int msc_spatial_radius_count(const float* coords, int64_t n, int dim, double r_min, double r_max,
+                             int64_t* out_indptr, int64_t* out_nnz /*host*/, void* workspace,
+                             size_t workspace_bytes, void* stream);
+int msc_spatial_radius_fill(const float* coords, int64_t n, int dim, double r_min, double r_max,
+                            const int64_t* indptr, int64_t nnz, int32_t* out_indices, float* out_dist,
+                            const double* coords64, double* out_len64, void* workspace,
+                            size_t workspace_bytes, void* stream);
+
 /* ---------------------------------------------------------------------------------------
  * (e) Moran's I with a permutation null (float64 reductions).
  * Replaces the Moran branch of spatial.spatial_autocorr (src/metalsinglecell/spatial.py:

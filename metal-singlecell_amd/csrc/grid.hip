@@ -207,6 +207,100 @@ __global__ void k_edge_len(const double* __restrict__ X, int64_t n, int dim, int
   out[e] = sqrt(s);
 }
 
+// ---- fixed-radius neighbours ------------------------------------------------------------
+// Pair (i, j), j != i, is kept when r2lo <= d2 <= r2hi with d2 the separately rounded float32
+// sum of the kNN path.  One thread per query, queries in cell order; the cells within `ring`
+// of the query's cell hold every kept pair (the cell side includes the rounding margin).
+// COUNT writes counts[i]; otherwise columns go to out_col from indptr[i] on, in walk order.
+template <bool COUNT>
+__global__ __launch_bounds__(kThreads) void k_radius(const float* __restrict__ X, int64_t n, Geo geo, int ring,
+                                                     float r2lo, float r2hi, const uint32_t* __restrict__ order,
+                                                     const int64_t* __restrict__ cell_start,
+                                                     int64_t* __restrict__ rows, uint32_t* __restrict__ out_col) {
+  const int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (t >= n) {
+    if (COUNT && t == n) rows[n] = 0;  // the scan's last input
+    return;
+  }
+  const int dim = geo.dim;
+  const int64_t i = order[t];
+  const float qx = X[i * dim], qy = X[i * dim + 1], qz = dim == 3 ? X[i * dim + 2] : 0.0f;
+  const int cx = cell_coord(qx, geo.ox, geo.cell, geo.gx);
+  const int cy = cell_coord(qy, geo.oy, geo.cell, geo.gy);
+  const int cz = dim == 3 ? cell_coord(qz, geo.oz, geo.cell, geo.gz) : 0;
+  int64_t w = COUNT ? 0 : rows[i];
+  const int64_t wend = COUNT ? 0 : rows[i + 1];  // the fill never writes past its row
+  int64_t cnt = 0;
+  const int rz = dim == 3 ? ring : 0;
+  for (int z = max(cz - rz, 0); z <= min(cz + rz, geo.gz - 1); ++z)
+    for (int y = max(cy - ring, 0); y <= min(cy + ring, geo.gy - 1); ++y) {
+      const int64_t row = (static_cast<int64_t>(z) * geo.gy + y) * geo.gx;
+      // the cells x0..x1 of one grid row are contiguous in the cell list
+      const int64_t p0 = cell_start[row + max(cx - ring, 0)];
+      const int64_t p1 = cell_start[row + min(cx + ring, geo.gx - 1) + 1];
+      for (int64_t p = p0; p < p1; ++p) {
+        const int64_t j = order[p];
+        if (j == i) continue;
+        const float ddx = __fsub_rn(X[j * dim], qx);
+        const float ddy = __fsub_rn(X[j * dim + 1], qy);
+        float d2 = __fadd_rn(__fmul_rn(ddx, ddx), __fmul_rn(ddy, ddy));
+        if (dim == 3) {
+          const float ddz = __fsub_rn(X[j * dim + 2], qz);
+          d2 = __fadd_rn(d2, __fmul_rn(ddz, ddz));
+        }
+        if (d2 < r2lo || d2 > r2hi) continue;
+        if (COUNT) ++cnt;
+        else if (w < wend) out_col[w++] = static_cast<uint32_t>(j);
+      }
+    }
+  if (COUNT) rows[i] = cnt;
+}
+
+// row id of every CSR position (one wave per row)
+__global__ __launch_bounds__(kThreads) void k_row_of(const int64_t* __restrict__ indptr, int64_t n,
+                                                     uint32_t* __restrict__ row) {
+  const int64_t q = static_cast<int64_t>(blockIdx.x) * (kThreads / kWave) + threadIdx.x / kWave;
+  if (q >= n) return;
+  for (int64_t p = indptr[q] + lane_id(); p < indptr[q + 1]; p += kWave) row[p] = static_cast<uint32_t>(q);
+}
+__global__ void k_iota_u32(uint32_t* __restrict__ v, int64_t m) {
+  const int64_t e = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (e < m) v[e] = static_cast<uint32_t>(e);
+}
+__global__ void k_gather_u32(const uint32_t* __restrict__ src, const uint32_t* __restrict__ idx, int64_t m,
+                             uint32_t* __restrict__ dst) {
+  const int64_t e = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (e < m) dst[e] = src[idx[e]];
+}
+// final columns (sorted within rows) with the float32 distance and float64 length of each edge
+__global__ void k_radius_emit(const float* __restrict__ X, const double* __restrict__ X64, int dim, int64_t m,
+                              const uint32_t* __restrict__ row, const uint32_t* __restrict__ perm,
+                              const uint32_t* __restrict__ col, int32_t* __restrict__ out_col,
+                              float* __restrict__ out_dist, double* __restrict__ out_len64) {
+  const int64_t e = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (e >= m) return;
+  const int64_t i = row[e], j = col[perm[e]];
+  out_col[e] = static_cast<int32_t>(j);
+  if (out_dist) {
+    const float ddx = __fsub_rn(X[j * dim], X[i * dim]);
+    const float ddy = __fsub_rn(X[j * dim + 1], X[i * dim + 1]);
+    float d2 = __fadd_rn(__fmul_rn(ddx, ddx), __fmul_rn(ddy, ddy));
+    if (dim == 3) {
+      const float ddz = __fsub_rn(X[j * dim + 2], X[i * dim + 2]);
+      d2 = __fadd_rn(d2, __fmul_rn(ddz, ddz));
+    }
+    out_dist[e] = sqrtf(d2);
+  }
+  if (out_len64) {
+    double s = 0.0;
+    for (int t = 0; t < dim; ++t) {
+      const double d = X64[i * dim + t] - X64[j * dim + t];
+      s += d * d;
+    }
+    out_len64[e] = sqrt(s);
+  }
+}
+
 __global__ void k_max_i32(const int32_t* __restrict__ v, int64_t n, int32_t* __restrict__ out) {
   int32_t m = 0;
   for (int64_t i = threadIdx.x; i < n; i += blockDim.x) m = max(m, v[i]);
@@ -266,6 +360,55 @@ static int choose_geometry(const float* mn, const float* mx, int64_t n, int dim,
   return kOk;
 }
 
+// radius geometry: cell side >= the reach of a query (r_hi, widened by the float32 rounding
+// of the kept test and of the cell assignment at these coordinate magnitudes), coarsened while
+// the table exceeds 4n + 16 cells; ring = cells to scan on each side.
+static int radius_geometry(const float* mn, const float* mx, int64_t n, int dim, double r_hi, Geo* geo, int* ring) {
+  double extent[3] = {0, 0, 0}, mag = 0.0;
+  bool nz[3] = {false, false, false};
+  int eff = 0;
+  for (int t = 0; t < dim; ++t) {
+    extent[t] = static_cast<double>(mx[t]) - static_cast<double>(mn[t]);
+    nz[t] = extent[t] > 0;
+    eff += nz[t];
+    mag = std::max(mag, std::max(std::fabs(static_cast<double>(mn[t])), std::fabs(static_cast<double>(mx[t]))));
+  }
+  geo->dim = dim;
+  geo->ox = mn[0];
+  geo->oy = mn[1];
+  geo->oz = dim == 3 ? mn[2] : 0.0f;
+  *ring = 1;
+  if (eff == 0) {
+    geo->cell = 1.0f;
+    geo->gx = geo->gy = geo->gz = 1;
+    return kOk;
+  }
+  const double reach = r_hi * (1.0 + 1e-5) + 4.0 * 4.8e-7 * (mag + 1e-30);
+  double cell = std::max(reach, 1e-30);
+  int64_t g[3] = {1, 1, 1};
+  for (int it = 0; it < 256; ++it) {
+    int64_t ncells = 1;
+    bool big = false;
+    for (int t = 0; t < dim; ++t) {
+      const double gt = nz[t] ? std::ceil(extent[t] / cell) + 1.0 : 1.0;
+      big |= gt > 1e9;
+      g[t] = big ? 1 : static_cast<int64_t>(gt);
+      ncells *= g[t];
+    }
+    if (!big && ncells <= max_cells(n)) break;
+    cell *= big ? 16.0 : std::pow(static_cast<double>(ncells) / (4.0 * n + 16.0), 1.0 / eff) * 1.0001;
+  }
+  MSC_REQUIRE(g[0] * g[1] * g[2] <= max_cells(n), "spatial_radius: no cell geometry fits the coordinate box");
+  float cf = static_cast<float>(cell);
+  if (static_cast<double>(cf) < cell) cf = std::nextafter(cf, INFINITY);  // keep the side >= the reach
+  geo->cell = cf;
+  geo->gx = static_cast<int>(g[0]);
+  geo->gy = static_cast<int>(g[1]);
+  geo->gz = dim == 3 ? static_cast<int>(g[2]) : 1;
+  *ring = std::max(1, static_cast<int>(std::ceil(reach / static_cast<double>(geo->cell))));
+  return kOk;
+}
+
 struct Bufs {
   float* bbox;
   uint32_t *key, *val, *skey, *order;
@@ -289,6 +432,31 @@ static Bufs carve(Carve& cv, int64_t n) {
   b.sort_ws_bytes = prims::radix_workspace(n);
   b.sort_ws = cv.take<char>(b.sort_ws_bytes);
   return b;
+}
+
+// device bounding box -> host bb[0..2] min, bb[3..5] max; rejects non-finite coordinates
+static int bounding_box(const float* coords, int64_t n, int dim, const Bufs& b, hipStream_t st, float* bb,
+                        const char* who) {
+  const int nbb = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(kBBoxBlocks, ceil_div(n, 4 * kThreads))));
+  k_bbox<<<nbb, kThreads, 0, st>>>(coords, n, dim, b.bbox + 8);
+  k_bbox_final<<<1, kWave, 0, st>>>(b.bbox + 8, nbb, b.bbox);
+  MSC_LAUNCH_CHECK();
+  MSC_HIP(hipMemcpyAsync(bb, b.bbox, 7 * sizeof(float), hipMemcpyDeviceToHost, st));
+  MSC_HIP(hipStreamSynchronize(st));
+  MSC_REQUIRE(bb[6] == 0.0f, "%s: coordinates must be finite", who);
+  for (int t = 0; t < dim; ++t)
+    MSC_REQUIRE(std::isfinite(bb[t]) && std::isfinite(bb[3 + t]), "%s: coordinates must be finite", who);
+  return kOk;
+}
+
+// cell list: points sorted by cell id (stable: index order within a cell), cell starts
+static int sort_cells(const float* coords, int64_t n, const Geo& geo, const Bufs& b, hipStream_t st) {
+  const int64_t ncells = static_cast<int64_t>(geo.gx) * geo.gy * geo.gz;
+  k_cell_keys<<<static_cast<unsigned>(ceil_div(n, kThreads)), kThreads, 0, st>>>(coords, n, geo, b.key, b.val);
+  MSC_LAUNCH_CHECK();
+  MSC_TRY(prims::radix_sort_pairs(b.key, b.val, b.skey, b.order, n, prims::bits_for(std::max<int64_t>(ncells - 1, 1)),
+                                  b.sort_ws, b.sort_ws_bytes, st));
+  return prims::segment_starts(b.skey, n, ncells, b.cell_start, st);
 }
 
 static int validate(int64_t n, int dim, int k) {
@@ -326,25 +494,12 @@ extern "C" int msc_spatial_knn(const float* coords, int64_t n, int dim, int k, i
     set_error("spatial_knn: workspace too small (%zu < %zu)", workspace_bytes, cv.used);
     return kWorkspaceTooSmall;
   }
-  const int nbb = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(kBBoxBlocks, ceil_div(n, 4 * kThreads))));
-  k_bbox<<<nbb, kThreads, 0, st>>>(coords, n, dim, b.bbox + 8);
-  k_bbox_final<<<1, kWave, 0, st>>>(b.bbox + 8, nbb, b.bbox);
-  MSC_LAUNCH_CHECK();
   float bb[8];
-  MSC_HIP(hipMemcpyAsync(bb, b.bbox, 7 * sizeof(float), hipMemcpyDeviceToHost, st));
-  MSC_HIP(hipStreamSynchronize(st));
-  MSC_REQUIRE(bb[6] == 0.0f, "spatial_knn: coordinates must be finite");
-  for (int t = 0; t < dim; ++t)
-    MSC_REQUIRE(std::isfinite(bb[t]) && std::isfinite(bb[3 + t]), "spatial_knn: coordinates must be finite");
+  MSC_TRY(bounding_box(coords, n, dim, b, st, bb, "spatial_knn"));
   Geo geo;
   MSC_TRY(choose_geometry(bb, bb + 3, n, dim, k + 5, &geo));
-  const int64_t ncells = static_cast<int64_t>(geo.gx) * geo.gy * geo.gz;
+  MSC_TRY(sort_cells(coords, n, geo, b, st));
   const unsigned g = static_cast<unsigned>(ceil_div(n, kThreads));
-  k_cell_keys<<<g, kThreads, 0, st>>>(coords, n, geo, b.key, b.val);
-  MSC_LAUNCH_CHECK();
-  MSC_TRY(prims::radix_sort_pairs(b.key, b.val, b.skey, b.order, n, prims::bits_for(std::max<int64_t>(ncells - 1, 1)),
-                                  b.sort_ws, b.sort_ws_bytes, st));
-  MSC_TRY(prims::segment_starts(b.skey, n, ncells, b.cell_start, st));
   int32_t* shells = info ? b.shells : nullptr;
   if (k <= 8)
     k_grid_query<8><<<g, kThreads, 0, st>>>(coords, n, k, geo, b.order, b.cell_start, out_idx, out_dist, shells);
@@ -370,5 +525,126 @@ extern "C" int msc_spatial_knn(const float* coords, int64_t n, int dim, int k, i
     info->grid_z = geo.gz;
     info->cell = geo.cell;
   }
+  return kOk;
+}
+
+// ---- fixed-radius neighbours (count, then fill) -------------------------------------------
+namespace {
+int radius_args(int64_t n, int dim, double r_min, double r_max) {
+  MSC_REQUIRE(dim == 2 || dim == 3, "spatial_radius: dim must be 2 or 3");
+  MSC_REQUIRE(n >= 1 && n < (int64_t(1) << 31) / 4, "spatial_radius: need 1 <= n < 2^29");
+  MSC_REQUIRE(std::isfinite(r_min) && std::isfinite(r_max) && r_min >= 0.0 && r_max >= r_min,
+              "spatial_radius: need 0 <= r_min <= r_max, finite");
+  return kOk;
+}
+
+// radix scratch for the (row, column) sort of the filled CSR
+struct RadiusSort {
+  uint32_t *col, *row, *k1, *v0, *v1, *k2, *v2;
+  void* ws;
+  size_t ws_bytes;
+};
+RadiusSort carve_sort(Carve& cv, int64_t nnz) {
+  RadiusSort r{};
+  const size_t m = static_cast<size_t>(std::max<int64_t>(nnz, 1));
+  r.col = cv.take<uint32_t>(m);
+  r.row = cv.take<uint32_t>(m);
+  r.k1 = cv.take<uint32_t>(m);
+  r.v0 = cv.take<uint32_t>(m);
+  r.v1 = cv.take<uint32_t>(m);
+  r.k2 = cv.take<uint32_t>(m);
+  r.v2 = cv.take<uint32_t>(m);
+  r.ws_bytes = prims::radix_workspace(static_cast<int64_t>(m));
+  r.ws = cv.take<char>(r.ws_bytes);
+  return r;
+}
+
+// the grid of a radius query: bounding box, geometry, cell list
+int radius_grid(const float* coords, int64_t n, int dim, double r_max, const Bufs& b, hipStream_t st, Geo* geo,
+                int* ring) {
+  float bb[8];
+  MSC_TRY(bounding_box(coords, n, dim, b, st, bb, "spatial_radius"));
+  MSC_TRY(radius_geometry(bb, bb + 3, n, dim, r_max, geo, ring));
+  return sort_cells(coords, n, *geo, b, st);
+}
+}  // namespace
+
+extern "C" int msc_spatial_radius_workspace_size(int64_t n, int dim, int64_t nnz, size_t* bytes) {
+  MSC_TRY(radius_args(n, dim, 0.0, 0.0));
+  MSC_REQUIRE(bytes && nnz >= 0 && nnz < (int64_t(1) << 31), "spatial_radius: bad nnz or null bytes");
+  Carve cv(nullptr, 0);
+  carve(cv, n);
+  const size_t count_bytes = cv.used + prims::scan_workspace<int64_t>(n + 1) + 256;  // the count call's layout
+  carve_sort(cv, nnz);
+  *bytes = std::max(cv.used + 256, count_bytes);
+  return kOk;
+}
+
+extern "C" int msc_spatial_radius_count(const float* coords, int64_t n, int dim, double r_min, double r_max,
+                                        int64_t* out_indptr, int64_t* out_nnz, void* workspace,
+                                        size_t workspace_bytes, void* stream) {
+  MSC_TRY(radius_args(n, dim, r_min, r_max));
+  MSC_REQUIRE(coords && out_indptr && out_nnz && workspace, "spatial_radius: null pointer argument");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  Carve cv(workspace, workspace_bytes);
+  Bufs b = carve(cv, n);
+  const size_t scan_bytes = prims::scan_workspace<int64_t>(n + 1);
+  void* scan_ws = cv.take<char>(scan_bytes);
+  if (!cv.ok()) {
+    set_error("spatial_radius: workspace too small (%zu < %zu)", workspace_bytes, cv.used);
+    return kWorkspaceTooSmall;
+  }
+  Geo geo;
+  int ring;
+  MSC_TRY(radius_grid(coords, n, dim, r_max, b, st, &geo, &ring));
+  const float r2lo = static_cast<float>(r_min * r_min), r2hi = static_cast<float>(r_max * r_max);
+  k_radius<true><<<static_cast<unsigned>(ceil_div(n + 1, kThreads)), kThreads, 0, st>>>(
+      coords, n, geo, ring, r2lo, r2hi, b.order, b.cell_start, out_indptr, nullptr);
+  MSC_LAUNCH_CHECK();
+  MSC_TRY(prims::exclusive_scan<int64_t>(out_indptr, out_indptr, n + 1, scan_ws, scan_bytes, st));
+  MSC_HIP(hipMemcpyAsync(out_nnz, out_indptr + n, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  MSC_HIP(hipStreamSynchronize(st));
+  MSC_REQUIRE(*out_nnz < (int64_t(1) << 31), "spatial_radius: %lld pairs exceed 2^31 - 1", (long long)*out_nnz);
+  return kOk;
+}
+
+extern "C" int msc_spatial_radius_fill(const float* coords, int64_t n, int dim, double r_min, double r_max,
+                                       const int64_t* indptr, int64_t nnz, int32_t* out_indices, float* out_dist,
+                                       const double* coords64, double* out_len64, void* workspace,
+                                       size_t workspace_bytes, void* stream) {
+  MSC_TRY(radius_args(n, dim, r_min, r_max));
+  MSC_REQUIRE(coords && indptr && workspace && (nnz == 0 || out_indices), "spatial_radius: null pointer argument");
+  MSC_REQUIRE((coords64 == nullptr) == (out_len64 == nullptr), "spatial_radius: coords64 and out_len64 go together");
+  MSC_REQUIRE(nnz >= 0 && nnz < (int64_t(1) << 31), "spatial_radius: bad nnz");
+  if (nnz == 0) return kOk;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  Carve cv(workspace, workspace_bytes);
+  Bufs b = carve(cv, n);
+  RadiusSort r = carve_sort(cv, nnz);
+  if (!cv.ok()) {
+    set_error("spatial_radius: workspace too small for %lld pairs (%zu < %zu)", (long long)nnz, workspace_bytes,
+              cv.used);
+    return kWorkspaceTooSmall;
+  }
+  Geo geo;
+  int ring;
+  MSC_TRY(radius_grid(coords, n, dim, r_max, b, st, &geo, &ring));
+  const float r2lo = static_cast<float>(r_min * r_min), r2hi = static_cast<float>(r_max * r_max);
+  k_radius<false><<<static_cast<unsigned>(ceil_div(n, kThreads)), kThreads, 0, st>>>(
+      coords, n, geo, ring, r2lo, r2hi, b.order, b.cell_start, const_cast<int64_t*>(indptr), r.col);
+  MSC_LAUNCH_CHECK();
+  // (row, column) order: stable sort by column, then stable sort by row
+  const unsigned ge = static_cast<unsigned>(ceil_div(nnz, kThreads));
+  k_row_of<<<static_cast<unsigned>(ceil_div(n, kThreads / kWave)), kThreads, 0, st>>>(indptr, n, r.row);
+  k_iota_u32<<<ge, kThreads, 0, st>>>(r.v0, nnz);
+  MSC_LAUNCH_CHECK();
+  const int bits = prims::bits_for(std::max<int64_t>(n - 1, 1));
+  MSC_TRY(prims::radix_sort_pairs(r.col, r.v0, r.k1, r.v1, nnz, bits, r.ws, r.ws_bytes, st));
+  k_gather_u32<<<ge, kThreads, 0, st>>>(r.row, r.v1, nnz, r.k1);
+  MSC_LAUNCH_CHECK();
+  MSC_TRY(prims::radix_sort_pairs(r.k1, r.v1, r.k2, r.v2, nnz, bits, r.ws, r.ws_bytes, st));
+  k_radius_emit<<<ge, kThreads, 0, st>>>(coords, coords64, dim, nnz, r.k2, r.v2, r.col, out_indices, out_dist,
+                                         out_len64);
+  MSC_LAUNCH_CHECK();
   return kOk;
 }

@@ -81,6 +81,11 @@ SIGNATURES = {
     "msc_spatial_knn_workspace_size": (c_int, [c_i64, c_int, c_int, ctypes.POINTER(c_sz)]),
     "msc_spatial_knn": (c_int, [c_vp, c_i64, c_int, c_int, c_vp, c_vp, c_vp, c_vp,
                                 ctypes.POINTER(SpatialInfo), c_vp, c_sz, c_vp]),
+    "msc_spatial_radius_workspace_size": (c_int, [c_i64, c_int, c_i64, ctypes.POINTER(c_sz)]),
+    "msc_spatial_radius_count": (c_int, [c_vp, c_i64, c_int, c_dbl, c_dbl, c_vp, ctypes.POINTER(c_i64), c_vp, c_sz,
+                                         c_vp]),
+    "msc_spatial_radius_fill": (c_int, [c_vp, c_i64, c_int, c_dbl, c_dbl, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp,
+                                        c_sz, c_vp]),
     "msc_moran_workspace_size": (c_int, [c_i64, c_i64, c_i64, c_int, ctypes.POINTER(c_sz)]),
     "msc_moran": (c_int, [c_vp, c_i64, c_i64, c_vp, c_vp, c_vp, c_i64, c_int, c_int, c_vp, c_i64,
                           c_vp, c_vp, ctypes.POINTER(c_dbl), c_vp, c_sz, c_vp]),
@@ -407,6 +412,35 @@ def spatial_knn(coords, k, coords64=None, return_info=False):
 # ----------------------------------------------------------------------------------------
 # (e) Moran's I
 # ----------------------------------------------------------------------------------------
+def spatial_radius(coords, r_max, r_min=0.0, coords64=None):
+    """Fixed-radius neighbours (self excluded), canonical CSR on the device:
+    ``(indptr int64, indices int32, dist float32[, len64 float64])``."""
+    import torch
+    lib = load_library()
+    X = to_device(coords, torch.float32)
+    n, dim = X.shape
+    nbytes = c_sz(0)
+    check(lib.msc_spatial_radius_workspace_size(n, dim, 0, ctypes.byref(nbytes)))
+    ws = workspace(nbytes.value)
+    indptr = torch.empty(n + 1, dtype=torch.int64, device=X.device)
+    nnz = c_i64(0)
+    check(lib.msc_spatial_radius_count(ptr(X), n, dim, float(r_min), float(r_max), ptr(indptr), ctypes.byref(nnz),
+                                       ptr(ws), ws.numel(), stream_ptr()))
+    m = nnz.value
+    check(lib.msc_spatial_radius_workspace_size(n, dim, m, ctypes.byref(nbytes)))
+    if nbytes.value > ws.numel():
+        ws = workspace(nbytes.value)
+    indices = torch.empty(max(m, 1), dtype=torch.int32, device=X.device)
+    dist = torch.empty(max(m, 1), dtype=torch.float32, device=X.device)
+    X64 = None if coords64 is None else to_device(coords64, torch.float64)
+    L64 = None if coords64 is None else torch.empty(max(m, 1), dtype=torch.float64, device=X.device)
+    check(lib.msc_spatial_radius_fill(ptr(X), n, dim, float(r_min), float(r_max), ptr(indptr), m, ptr(indices),
+                                      ptr(dist), None if X64 is None else ptr(X64), None if L64 is None else ptr(L64),
+                                      ptr(ws), ws.numel(), stream_ptr()))
+    out = (indptr, indices[:m], dist[:m])
+    return out if L64 is None else out + (L64[:m],)
+
+
 def moran(X, w_indptr, w_indices, w_data, n_perms=0, perms=None, seed=0, transformation=True):
     """Moran's I on the device: ``(I float64 (G,), sims float64 (P, G) or None, S0)``."""
     import torch

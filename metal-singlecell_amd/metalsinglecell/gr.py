@@ -1,5 +1,5 @@
-"""``gr`` namespace — the spatial drop-ins of ``squidpy.gr``: the kNN spatial graph and
-Moran's I.
+"""``gr`` namespace — the spatial drop-ins of ``squidpy.gr``: the kNN and fixed-radius spatial
+graphs and Moran's I.
 
 Mirrors the reference's ``metalsinglecell.gr`` (src/metalsinglecell/gr.py):
 
@@ -9,6 +9,9 @@ Mirrors the reference's ``metalsinglecell.gr`` (src/metalsinglecell/gr.py):
   coordinates; per-library blocks (:107-147); percentile / set_diag / transform tail
   (:37-104); slots ``obsp[f"{key_added}_connectivities"]``, ``obsp[f"{key_added}_distances"]``
   and ``uns[f"{key_added}_neighbors"]`` (:157-168).  The kNN runs in csrc/grid.hip.
+* ``spatial_neighbors_radius`` (:228-252): every pair within ``radius`` (or a ``(min, max)``
+  interval), self excluded, binary connectivities and float64 lengths, the same tail and
+  slots as the kNN graph; pairs from csrc/grid.hip's count / fill passes.
 * ``spatial_autocorr`` (:418-503), Moran mode: statistic and permutation null in
   csrc/moran.hip, the frame ``uns['moranI']`` with squidpy's columns, sorted by I descending.
 """
@@ -158,6 +161,38 @@ def spatial_neighbors_knn(adata, *, n_neighs: int = 6, spatial_key: str = "spati
     adj, dst = _assemble(adata, spatial_key, library_key, build, transform, set_diag, percentile)
     return _write_graph(adata, adj, dst, {"coord_type": "generic", "n_neighbors": n_neighs},
                         key_added, transform, copy)
+
+
+def radius_graph(coords, exact, r_min, r_max):
+    """``(connectivities, distances)`` CSR of the fixed-radius graph of one section."""
+    import scipy.sparse as sp
+    from . import _hip
+    n = coords.shape[0]
+    ip, ind, _, len64 = _hip.spatial_radius(coords, r_max, r_min, coords64=exact)
+    indptr = ip.cpu().numpy()
+    cols = ind.cpu().numpy()
+    dist = len64.cpu().numpy()
+    return (sp.csr_matrix((np.ones(cols.size, np.float32), cols, indptr), shape=(n, n)),
+            sp.csr_matrix((dist, cols.copy(), indptr.copy()), shape=(n, n)))
+
+
+def spatial_neighbors_radius(adata, *, radius, spatial_key: str = "spatial", library_key: str | None = None,
+                             percentile: float | None = None, transform: str | None = None,
+                             set_diag: bool = False, key_added: str = "spatial", copy: bool = False):
+    """Fixed-radius spatial graph (``sq.gr.spatial_neighbors_radius``) on the MI355X grid.
+
+    ``radius`` is a maximum distance or a ``(min, max)`` interval; every pair within it is
+    connected (symmetric graph, degrees follow local density)."""
+    if transform not in (None, "none", "spectral", "cosine"):
+        raise NotImplementedError(f"transform={transform!r} is not implemented "
+                                  f"(expected None, 'spectral' or 'cosine')")
+    lo, hi = (0.0, float(radius)) if np.isscalar(radius) else (float(min(radius)), float(max(radius)))
+
+    def build(coords, exact):
+        return radius_graph(coords, exact, lo, hi)
+
+    adj, dst = _assemble(adata, spatial_key, library_key, build, transform, set_diag, percentile)
+    return _write_graph(adata, adj, dst, {"coord_type": "generic", "radius": radius}, key_added, transform, copy)
 
 
 def _autocorr_values(adata, attr, genes, layer, use_raw):

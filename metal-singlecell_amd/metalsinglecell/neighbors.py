@@ -10,6 +10,8 @@ Mirrors the reference's ``metalsinglecell.neighbors`` seams:
   (neighbors.py:727-760), connectivities from the fuzzy simplicial set (csrc/fuzzy.hip).
 * ``_knn_grid(coords, k)`` -> ``(indices, dists)`` self-inclusive exact spatial kNN
   (neighbors.py:333-417) on the device grid (csrc/grid.hip).
+* ``_radius_grid(coords, radius, min_radius)`` -> CSR of all pairs within the radius
+  (neighbors.py:255-330), count and fill passes on the device grid (csrc/grid.hip).
 * ``bbknn(X_pca, batch, ...)`` -> ``(distances, connectivities, n_neighbors)``
   (neighbors.py:491-559 with ``_trim_graph`` :465-488): per-batch exact kNN in one select
   launch (one candidate split per batch), connectivities and trimming on the device.
@@ -97,6 +99,20 @@ def _knn_grid(coords, k, min_n=None):
     idx = np.c_[np.arange(n), idx.cpu().numpy()].astype(np.int64)
     dist = np.c_[np.zeros(n, np.float32), dist.cpu().numpy()]
     return idx, dist
+
+
+def _radius_grid(coords, radius, min_radius=0.0):
+    """All neighbours within ``radius`` (and at least ``min_radius``), self excluded:
+    ``(indptr int64, indices int32, distances float32)`` of an (n, n) CSR with columns
+    ascending (neighbors.py:255-330 returns the same pairs in cell-walk order)."""
+    X = np.ascontiguousarray(coords, dtype=np.float32)
+    n = X.shape[0]
+    if n == 0:
+        return np.zeros(1, np.int64), np.zeros(0, np.int32), np.zeros(0, np.float32)
+    if X.ndim != 2 or X.shape[1] not in (2, 3):
+        raise ValueError(f"radius search expects 2-D or 3-D coordinates, got {X.shape[-1]}-D")
+    ip, ind, dist = _hip.spatial_radius(X, float(radius), float(min_radius))
+    return ip.cpu().numpy(), ind.cpu().numpy(), dist.cpu().numpy()
 
 
 def _batch_codes(batch):

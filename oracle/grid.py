@@ -1,4 +1,4 @@
-"""Exact spatial kNN oracle (2-D / 3-D) — TEST INFRASTRUCTURE ONLY.
+"""Exact spatial kNN and fixed-radius oracle (2-D / 3-D) — TEST INFRASTRUCTURE ONLY.
 
 Restates what ``gr.spatial_neighbors_knn`` builds (src/metalsinglecell/gr.py:174-225 on
 ``neighbors._knn_grid``, neighbors.py:333-417): a directed graph with ``n_neighs`` edges per
@@ -81,4 +81,50 @@ def knn_graph(coords, n_neighs=6, exact=None):
     indptr = np.arange(n + 1, dtype=np.int64) * n_neighs
     A = sp.csr_matrix((np.ones(dist.size, np.float32), cols, indptr), shape=(n, n))
     D = sp.csr_matrix((dist, cols, indptr), shape=(n, n))
+    return A, D
+
+
+def radius_neighbors(coords, r_max, r_min=0.0, block=512):
+    """All pairs with ``r_min^2 <= d2 <= r_max^2``, self excluded (neighbors.py:255-330 and its
+    kernel :199-252): ``(indptr int64, indices int64, dist float32)``, columns ascending.
+
+    ``d2`` is the separately rounded float32 sum of this module; the bounds are float64
+    squares rounded to float32, as the reference passes them to its kernel (:306-307)."""
+    X = np.ascontiguousarray(coords, dtype=np.float32)
+    n, d = X.shape
+    r2hi = np.float32(float(r_max) ** 2)
+    r2lo = np.float32(float(r_min) ** 2)
+    counts = np.zeros(n, np.int64)
+    cols, dists = [], []
+    for s in range(0, n, block):
+        Q = X[s:s + block]
+        acc = None
+        for t in range(d):
+            diff = (X[None, :, t] - Q[:, t, None]).astype(np.float32)
+            sq = (diff * diff).astype(np.float32)
+            acc = sq if acc is None else (acc + sq).astype(np.float32)
+        keep = (acc >= r2lo) & (acc <= r2hi)
+        keep[np.arange(Q.shape[0]), np.arange(s, s + Q.shape[0])] = False
+        r, c = np.nonzero(keep)
+        counts[s:s + Q.shape[0]] = np.bincount(r, minlength=Q.shape[0])
+        cols.append(c)
+        dists.append(np.sqrt(acc[r, c]).astype(np.float32))
+    indptr = np.zeros(n + 1, np.int64)
+    np.cumsum(counts, out=indptr[1:])
+    return indptr, np.concatenate(cols).astype(np.int64), np.concatenate(dists)
+
+
+def radius_graph(coords, r_max, r_min=0.0, exact=None):
+    """``(connectivities, distances)`` CSR pair as written by ``spatial_neighbors_radius``
+    (gr.py:228-252) before the percentile / diagonal / transform tail."""
+    import scipy.sparse as sp
+    X = np.asarray(coords)
+    n = X.shape[0]
+    indptr, cols, _ = radius_neighbors(X, r_max, r_min)
+    ex = np.asarray(X if exact is None else exact, dtype=np.float64)
+    rows = np.repeat(np.arange(n), np.diff(indptr))
+    diff = ex[rows] - ex[cols]
+    dist = np.sqrt((diff * diff).sum(1))
+    A = sp.csr_matrix((np.ones(dist.size, np.float32), cols, indptr), shape=(n, n))
+    D = sp.csr_matrix((dist, cols, indptr.copy()), shape=(n, n))
     return A, D
