@@ -264,6 +264,19 @@ int msc_moran(const float* X, int64_t n, int64_t n_genes, const int64_t* w_indpt
               int n_perms, const int32_t* perms, int64_t seed, double* out_I, double* out_sims,
               double* out_S0 /*host*/, void* workspace, size_t workspace_bytes, void* stream);
 
+/* Geary's C with the same permutation null: the Geary branch of spatial.spatial_autocorr
+ * (src/metalsinglecell/spatial.py:388-393, 396-406) as called by gr.spatial_autocorr
+ * (gr.py:455-503, mode="geary").  C_g = (n-1)/(2 S0) sum_e w_e (x_src - x_dst)^2 / denom_g,
+ * the observed numerator summed directly over the edges in float64; each permuted numerator
+ * (sources pi[src]) by its exact per-row expansion in float64.  Arguments as msc_moran;
+ * out_C: n_genes float64. */
+int msc_geary_workspace_size(int64_t n, int64_t n_genes, int64_t nnz, int n_perms,
+                             size_t* bytes /*host*/);
+int msc_geary(const float* X, int64_t n, int64_t n_genes, const int64_t* w_indptr,
+              const int32_t* w_indices, const double* w_data, int64_t nnz, int transformation,
+              int n_perms, const int32_t* perms, int64_t seed, double* out_C, double* out_sims,
+              double* out_S0 /*host*/, void* workspace, size_t workspace_bytes, void* stream);
+
 /* Dense n x n_cols float32 row-major copy of a CSR matrix (duplicates: last write wins; a
  * canonical CSR has none).  Moran densifies the selected genes (gr.py:452). */
 int msc_csr_to_dense(const int64_t* indptr, const int32_t* indices, const float* data, int64_t n,

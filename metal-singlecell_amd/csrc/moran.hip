@@ -516,3 +516,260 @@ extern "C" int msc_graph_moments(const int64_t* indptr, const int32_t* indices, 
   out_s[2] = hs[2];
   return kOk;
 }
+
+// ------------------------------------------------------------------------------------------
+// Geary's C (spatial.py:388-393): C_g = (n - 1) / (2 S0) * sum_e w_e (x_src - x_dst)^2 / denom_g.
+// Observed numerator summed directly over the edges in float64.  Permuted numerators (the
+// reference scatters from pi[src], :401-405) are expanded per source row r,
+//   N_pi = sum_r [ a_pi(r)^2 R_r - 2 a_pi(r) Y_r ] + Z,   a = x - mean, R_r = row sum of w,
+//   Y = W a (float64), Z = sum_r sum_e w_e a_dst^2 (independent of pi),
+// with Z taken from the observed numerator and the identity's expansion, all in float64.
+// ------------------------------------------------------------------------------------------
+namespace msc {
+namespace moran {
+
+// Y64[r, g] = sum_e w_e a_dst, D[r, g] = sum_e w_e (x_r - x_dst)^2, R[r] = sum_e w_e; block per row
+__global__ __launch_bounds__(kThreads) void k_spmm_geary(const float* __restrict__ X, int64_t G,
+                                                         const int64_t* __restrict__ indptr,
+                                                         const int32_t* __restrict__ indices,
+                                                         const double* __restrict__ w, const double* __restrict__ mean,
+                                                         double* __restrict__ Y64, double* __restrict__ D,
+                                                         double* __restrict__ R) {
+  const int64_t r = blockIdx.x;
+  const int64_t e0 = indptr[r], e1 = indptr[r + 1];
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int64_t e = e0; e < e1; ++e) s += w[e];
+    R[r] = s;
+  }
+  for (int64_t g = threadIdx.x; g < G; g += kThreads) {
+    const double m = mean[g];
+    const double xr = static_cast<double>(X[r * G + g]);
+    double y = 0.0, dd = 0.0;
+    for (int64_t e = e0; e < e1; ++e) {
+      const double xj = static_cast<double>(X[static_cast<int64_t>(indices[e]) * G + g]);
+      y += w[e] * (xj - m);
+      const double df = xr - xj;
+      dd += w[e] * (df * df);
+    }
+    Y64[r * G + g] = y;
+    D[r * G + g] = dd;
+  }
+}
+
+// column sums of a float64 n x G matrix over row chunks (fixed order): partial[b][g]
+__global__ void k_col_partial_f64(const double* __restrict__ M, int64_t n, int64_t G, int64_t per,
+                                  double* __restrict__ partial) {
+  const int64_t r0 = blockIdx.x * per, r1 = min(n, r0 + per);
+  for (int64_t g = threadIdx.x + static_cast<int64_t>(blockIdx.y) * blockDim.x; g < G;
+       g += static_cast<int64_t>(gridDim.y) * blockDim.x) {
+    double s = 0.0;
+    for (int64_t r = r0; r < r1; ++r) s += M[r * G + g];
+    partial[blockIdx.x * G + g] = s;
+  }
+}
+
+// grid (row chunks, perm batches, gene tiles): partA/partB[chunk][p][g] of sum a_pi(r)^2 R_r and
+// sum a_pi(r) Y_r; `observed` runs the identity as permutation 0
+template <int VEC>
+__global__ __launch_bounds__(kThreads) void k_geary_partial(
+    const float* __restrict__ X, const double* __restrict__ Y64, const double* __restrict__ R, int64_t n, int64_t G,
+    const double* __restrict__ mean, int64_t per, int P, const int32_t* __restrict__ perms, Feistel fst, int observed,
+    double* __restrict__ partA, double* __restrict__ partB) {
+  __shared__ int32_t jtab[kPermBatch][kRowStep];
+  const int64_t r0 = blockIdx.x * per, r1 = min(n, r0 + per);
+  const int p0 = blockIdx.y * kPermBatch;
+  const int64_t g0 = (static_cast<int64_t>(blockIdx.z) * blockDim.x + threadIdx.x) * VEC;
+  const bool gok = g0 < G;
+  double m[VEC];
+#pragma unroll
+  for (int v = 0; v < VEC; ++v) m[v] = gok ? mean[g0 + v] : 0.0;
+  double accA[kPermBatch][VEC], accB[kPermBatch][VEC];
+#pragma unroll
+  for (int b = 0; b < kPermBatch; ++b)
+#pragma unroll
+    for (int v = 0; v < VEC; ++v) accA[b][v] = accB[b][v] = 0.0;
+  for (int64_t rb = r0; rb < r1; rb += kRowStep) {
+    const int nr = static_cast<int>(min(static_cast<int64_t>(kRowStep), r1 - rb));
+    __syncthreads();  // the previous step's readers are done with jtab
+    for (int q = threadIdx.x; q < kPermBatch * kRowStep; q += blockDim.x) {
+      const int b = q / kRowStep, rr = q % kRowStep;
+      const int p = p0 + b;
+      int32_t j = 0;
+      if (p < P && rr < nr) {
+        const int64_t r = rb + rr;
+        j = observed ? static_cast<int32_t>(r)
+                     : (perms ? perms[static_cast<int64_t>(p) * n + r]
+                              : static_cast<int32_t>(feistel_perm(r, n, fst, perm_key(fst, p))));
+      }
+      jtab[b][rr] = j;
+    }
+    __syncthreads();
+    if (!gok) continue;
+    for (int rr = 0; rr < nr; ++rr) {
+      const int64_t r = rb + rr;
+      const double Rr = R[r];
+      double y[VEC];
+#pragma unroll
+      for (int v = 0; v < VEC; ++v) y[v] = Y64[r * G + g0 + v];
+#pragma unroll
+      for (int b = 0; b < kPermBatch; ++b) {
+        if (p0 + b < P) {
+          float x[VEC];
+          load_vec<VEC>(X + static_cast<int64_t>(jtab[b][rr]) * G + g0, x);
+#pragma unroll
+          for (int v = 0; v < VEC; ++v) {
+            const double a = static_cast<double>(x[v]) - m[v];
+            accA[b][v] += (a * a) * Rr;
+            accB[b][v] += a * y[v];
+          }
+        }
+      }
+    }
+  }
+  if (!gok) return;
+#pragma unroll
+  for (int b = 0; b < kPermBatch; ++b) {
+    const int p = p0 + b;
+    if (p < P)
+#pragma unroll
+      for (int v = 0; v < VEC; ++v) {
+        const int64_t o = (static_cast<int64_t>(blockIdx.x) * P + p) * G + g0 + v;
+        partA[o] = accA[b][v];
+        partB[o] = accB[b][v];
+      }
+  }
+}
+
+// observed: C_g from the direct numerator; Zc_g = Nobs - A_id + 2 B_id kept for the null
+__global__ void k_geary_observed(const double* __restrict__ partA, const double* __restrict__ partB, int nchunks,
+                                 int64_t G, double n, const double* __restrict__ S0, const double* __restrict__ denom,
+                                 const double* __restrict__ Nobs, double* __restrict__ Zc, double* __restrict__ out) {
+  const int64_t g = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (g >= G) return;
+  double A = 0.0, B = 0.0;
+  for (int c = 0; c < nchunks; ++c) {
+    A += partA[static_cast<int64_t>(c) * G + g];
+    B += partB[static_cast<int64_t>(c) * G + g];
+  }
+  Zc[g] = Nobs[g] - A + 2.0 * B;
+  out[g] = ((n - 1.0) / (2.0 * *S0)) * Nobs[g] / denom[g];
+}
+__global__ void k_geary_finish(const double* __restrict__ partA, const double* __restrict__ partB, int nchunks, int P,
+                               int64_t G, double n, const double* __restrict__ S0, const double* __restrict__ denom,
+                               const double* __restrict__ Zc, double* __restrict__ out) {
+  const int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (t >= static_cast<int64_t>(P) * G) return;
+  const int64_t p = t / G, g = t % G;
+  double A = 0.0, B = 0.0;
+  for (int c = 0; c < nchunks; ++c) {
+    A += partA[(static_cast<int64_t>(c) * P + p) * G + g];
+    B += partB[(static_cast<int64_t>(c) * P + p) * G + g];
+  }
+  out[t] = ((n - 1.0) / (2.0 * *S0)) * ((A - 2.0 * B) + Zc[g]) / denom[g];
+}
+
+struct GearyBufs {
+  Bufs b;  // mean, denom, w, S0, sums, column partials
+  double *Y64, *D, *R, *Nobs, *Zc, *partA, *partB;
+};
+static GearyBufs carve_geary(Carve& cv, const Plan& p) {
+  GearyBufs t{};
+  Plan q = p;
+  q.P = 0;      // the Moran partials / float32 Y are not used
+  t.b = carve(cv, q);
+  t.Y64 = cv.take<double>(static_cast<size_t>(p.n) * p.G);
+  t.D = cv.take<double>(static_cast<size_t>(p.n) * p.G);
+  t.R = cv.take<double>(p.n);
+  t.Nobs = cv.take<double>(p.G);
+  t.Zc = cv.take<double>(p.G);
+  const size_t pp = static_cast<size_t>(p.nchunks) * std::max(p.P, 1) * p.G;
+  t.partA = cv.take<double>(pp);
+  t.partB = cv.take<double>(pp);
+  return t;
+}
+
+}  // namespace moran
+}  // namespace msc
+
+extern "C" int msc_geary_workspace_size(int64_t n, int64_t n_genes, int64_t nnz, int n_perms, size_t* bytes) {
+  MSC_TRY(validate(n, n_genes, nnz, n_perms));
+  MSC_REQUIRE(bytes, "geary: bytes is null");
+  Plan p = make_plan(n, n_genes, nnz, n_perms);
+  Carve cv(nullptr, 0);
+  carve_geary(cv, p);
+  *bytes = cv.used + 256;
+  return kOk;
+}
+
+extern "C" int msc_geary(const float* X, int64_t n, int64_t n_genes, const int64_t* w_indptr,
+                         const int32_t* w_indices, const double* w_data, int64_t nnz, int transformation,
+                         int n_perms, const int32_t* perms, int64_t seed, double* out_C, double* out_sims,
+                         double* out_S0, void* workspace, size_t workspace_bytes, void* stream) {
+  MSC_TRY(validate(n, n_genes, nnz, n_perms));
+  MSC_REQUIRE(X && w_indptr && out_C && workspace && (nnz == 0 || (w_indices && w_data)),
+              "geary: null pointer argument");
+  MSC_REQUIRE(n_perms == 0 || out_sims, "geary: out_sims required when n_perms > 0");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const int64_t G = n_genes;
+  Plan p = make_plan(n, G, nnz, n_perms);
+  Carve cv(workspace, workspace_bytes);
+  GearyBufs t = carve_geary(cv, p);
+  if (!cv.ok()) {
+    set_error("geary: workspace too small (%zu < %zu)", workspace_bytes, cv.used);
+    return kWorkspaceTooSmall;
+  }
+  Bufs& b = t.b;
+  const int64_t per = ceil_div(n, kColBlocks);
+  const dim3 cg(kColBlocks, static_cast<unsigned>(std::min<int64_t>(ceil_div(G, kThreads), 64)));
+  const unsigned gb = static_cast<unsigned>(ceil_div(G, kThreads));
+  k_col_partial<<<cg, kThreads, 0, st>>>(X, n, G, per, nullptr, b.cpartial);
+  k_col_final<<<gb, kThreads, 0, st>>>(b.cpartial, kColBlocks, G, 1.0 / static_cast<double>(n), b.mean);
+  k_col_partial<<<cg, kThreads, 0, st>>>(X, n, G, per, b.mean, b.cpartial);
+  k_col_final<<<gb, kThreads, 0, st>>>(b.cpartial, kColBlocks, G, 1.0, b.denom);
+  MSC_LAUNCH_CHECK();
+  k_row_normalize<<<static_cast<unsigned>(ceil_div(n, kThreads)), kThreads, 0, st>>>(w_indptr, w_data, n,
+                                                                                       transformation, b.w);
+  sum_f64(b.w, nnz, b.S0, b.sum_scratch, st);
+  MSC_LAUNCH_CHECK();
+  k_spmm_geary<<<static_cast<unsigned>(n), kThreads, 0, st>>>(X, G, w_indptr, w_indices, b.w, b.mean, t.Y64, t.D, t.R);
+  k_col_partial_f64<<<cg, kThreads, 0, st>>>(t.D, n, G, per, b.cpartial);
+  k_col_final<<<gb, kThreads, 0, st>>>(b.cpartial, kColBlocks, G, 1.0, t.Nobs);
+  MSC_LAUNCH_CHECK();
+  Feistel fst{};
+  {
+    int bits = 2;
+    while ((int64_t(1) << bits) < n) bits += 2;
+    fst.half = bits / 2;
+    fst.mask = (uint64_t(1) << fst.half) - 1;
+    fst.key = stream_key(seed, 7);
+  }
+  const bool vec4 = p.vec == 4 && (reinterpret_cast<uintptr_t>(X) % 16 == 0);
+  auto launch = [&](dim3 grid, int P_, const int32_t* pr, int observed) {
+    if (vec4)
+      k_geary_partial<4><<<grid, p.pthreads, 0, st>>>(X, t.Y64, t.R, n, G, b.mean, p.per, P_, pr, fst, observed,
+                                                      t.partA, t.partB);
+    else {
+      const dim3 g1(grid.x, grid.y, static_cast<unsigned>(ceil_div(G, kThreads)));
+      k_geary_partial<1><<<g1, kThreads, 0, st>>>(X, t.Y64, t.R, n, G, b.mean, p.per, P_, pr, fst, observed, t.partA,
+                                                  t.partB);
+    }
+  };
+  launch(dim3(static_cast<unsigned>(p.nchunks), 1, static_cast<unsigned>(p.gtiles)), 1, nullptr, 1);
+  k_geary_observed<<<gb, kThreads, 0, st>>>(t.partA, t.partB, p.nchunks, G, static_cast<double>(n), b.S0, b.denom,
+                                            t.Nobs, t.Zc, out_C);
+  MSC_LAUNCH_CHECK();
+  if (n_perms > 0) {
+    launch(dim3(static_cast<unsigned>(p.nchunks), static_cast<unsigned>(ceil_div(n_perms, kPermBatch)),
+                static_cast<unsigned>(p.gtiles)),
+           n_perms, perms, 0);
+    k_geary_finish<<<static_cast<unsigned>(ceil_div(static_cast<int64_t>(n_perms) * G, kThreads)), kThreads, 0, st>>>(
+        t.partA, t.partB, p.nchunks, n_perms, G, static_cast<double>(n), b.S0, b.denom, t.Zc, out_sims);
+    MSC_LAUNCH_CHECK();
+  }
+  if (out_S0) {
+    MSC_HIP(hipMemcpyAsync(out_S0, b.S0, sizeof(double), hipMemcpyDeviceToHost, st));
+    MSC_HIP(hipStreamSynchronize(st));
+  }
+  return kOk;
+}

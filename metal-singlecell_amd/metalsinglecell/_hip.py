@@ -89,6 +89,9 @@ SIGNATURES = {
     "msc_moran_workspace_size": (c_int, [c_i64, c_i64, c_i64, c_int, ctypes.POINTER(c_sz)]),
     "msc_moran": (c_int, [c_vp, c_i64, c_i64, c_vp, c_vp, c_vp, c_i64, c_int, c_int, c_vp, c_i64,
                           c_vp, c_vp, ctypes.POINTER(c_dbl), c_vp, c_sz, c_vp]),
+    "msc_geary_workspace_size": (c_int, [c_i64, c_i64, c_i64, c_int, ctypes.POINTER(c_sz)]),
+    "msc_geary": (c_int, [c_vp, c_i64, c_i64, c_vp, c_vp, c_vp, c_i64, c_int, c_int, c_vp, c_i64,
+                          c_vp, c_vp, ctypes.POINTER(c_dbl), c_vp, c_sz, c_vp]),
     "msc_csr_to_dense": (c_int, [c_vp, c_vp, c_vp, c_i64, c_i64, c_vp, c_vp]),
     "msc_graph_moments_workspace_size": (c_int, [c_i64, c_i64, ctypes.POINTER(c_sz)]),
     "msc_graph_moments": (c_int, [c_vp, c_vp, c_vp, c_i64, c_i64, c_vp, c_vp, c_sz, c_vp]),
@@ -441,8 +444,7 @@ def spatial_radius(coords, r_max, r_min=0.0, coords64=None):
     return out if L64 is None else out + (L64[:m],)
 
 
-def moran(X, w_indptr, w_indices, w_data, n_perms=0, perms=None, seed=0, transformation=True):
-    """Moran's I on the device: ``(I float64 (G,), sims float64 (P, G) or None, S0)``."""
+def _autocorr(kind, X, w_indptr, w_indices, w_data, n_perms, perms, seed, transformation):
     import torch
     lib = load_library()
     Xd = to_device(X, torch.float32)
@@ -458,15 +460,25 @@ def moran(X, w_indptr, w_indices, w_data, n_perms=0, perms=None, seed=0, transfo
         if tuple(pr.shape) != (P, n):
             raise ValueError(f"perms must have shape ({P}, {n})")
     nbytes = c_sz(0)
-    check(lib.msc_moran_workspace_size(n, G, nnz, P, ctypes.byref(nbytes)))
+    check(getattr(lib, f"msc_{kind}_workspace_size")(n, G, nnz, P, ctypes.byref(nbytes)))
     ws = workspace(nbytes.value)
-    I = torch.empty(G, dtype=torch.float64, device=Xd.device)
+    stat = torch.empty(G, dtype=torch.float64, device=Xd.device)
     sims = torch.empty((max(P, 1), G), dtype=torch.float64, device=Xd.device)
     S0 = c_dbl(0.0)
-    check(lib.msc_moran(ptr(Xd), n, G, ptr(ip), ptr(ind), ptr(dat), nnz, int(bool(transformation)), P,
-                        None if pr is None else ptr(pr), int(seed), ptr(I), ptr(sims),
-                        ctypes.byref(S0), ptr(ws), ws.numel(), stream_ptr()))
-    return I, (sims[:P] if P else None), S0.value
+    check(getattr(lib, f"msc_{kind}")(ptr(Xd), n, G, ptr(ip), ptr(ind), ptr(dat), nnz, int(bool(transformation)), P,
+                                      None if pr is None else ptr(pr), int(seed), ptr(stat), ptr(sims),
+                                      ctypes.byref(S0), ptr(ws), ws.numel(), stream_ptr()))
+    return stat, (sims[:P] if P else None), S0.value
+
+
+def moran(X, w_indptr, w_indices, w_data, n_perms=0, perms=None, seed=0, transformation=True):
+    """Moran's I on the device: ``(I float64 (G,), sims float64 (P, G) or None, S0)``."""
+    return _autocorr("moran", X, w_indptr, w_indices, w_data, n_perms, perms, seed, transformation)
+
+
+def geary(X, w_indptr, w_indices, w_data, n_perms=0, perms=None, seed=0, transformation=True):
+    """Geary's C on the device: ``(C float64 (G,), sims float64 (P, G) or None, S0)``."""
+    return _autocorr("geary", X, w_indptr, w_indices, w_data, n_perms, perms, seed, transformation)
 
 
 def csr_to_dense(indptr, indices, data, n_rows, n_cols):

@@ -12,8 +12,9 @@ Mirrors the reference's ``metalsinglecell.gr`` (src/metalsinglecell/gr.py):
 * ``spatial_neighbors_radius`` (:228-252): every pair within ``radius`` (or a ``(min, max)``
   interval), self excluded, binary connectivities and float64 lengths, the same tail and
   slots as the kNN graph; pairs from csrc/grid.hip's count / fill passes.
-* ``spatial_autocorr`` (:418-503), Moran mode: statistic and permutation null in
-  csrc/moran.hip, the frame ``uns['moranI']`` with squidpy's columns, sorted by I descending.
+* ``spatial_autocorr`` (:418-503), Moran and Geary modes: statistic and permutation null in
+  csrc/moran.hip, the frame ``uns['moranI']`` / ``uns['gearyC']`` with squidpy's columns,
+  sorted by I descending / C ascending.
 """
 
 from __future__ import annotations
@@ -228,11 +229,13 @@ def _autocorr_values(adata, attr, genes, layer, use_raw):
     return np.asarray(Xg.todense() if sp.issparse(Xg) else Xg, dtype=np.float32), genes
 
 
-def moran_frame(score, sims, g, index, two_tailed=False, corr_method="fdr_bh"):
-    """The ``uns['moranI']`` frame from I, the permutation null and the weight matrix."""
+def moran_frame(score, sims, g, index, two_tailed=False, corr_method="fdr_bh", mode="moran"):
+    """The ``uns['moranI']`` / ``uns['gearyC']`` frame from the statistic, the permutation null
+    and the weight matrix (gr.py:484-498)."""
     import pandas as pd
-    p_norm, var_norm = _sp.analytic_pval(score, g, mode="moran", two_tailed=two_tailed)
-    cols = {"I": score, "pval_norm": p_norm, "var_norm": np.full(score.shape, var_norm)}
+    stat = "I" if mode == "moran" else "C"
+    p_norm, var_norm = _sp.analytic_pval(score, g, mode=mode, two_tailed=two_tailed)
+    cols = {stat: score, "pval_norm": p_norm, "var_norm": np.full(score.shape, var_norm)}
     if sims is not None:
         p_sim, p_z_sim, var_sim = _sp.permutation_pvals(score, sims)
         cols.update(pval_z_sim=p_z_sim, pval_sim=p_sim, var_sim=var_sim)
@@ -240,7 +243,7 @@ def moran_frame(score, sims, g, index, two_tailed=False, corr_method="fdr_bh"):
     if corr_method is not None:
         for pv in [c for c in df.columns if "pval" in c]:
             df[f"{pv}_{corr_method}"] = _sp.multiple_testing(df[pv].to_numpy(), corr_method)
-    df.sort_values(by="I", ascending=False, inplace=True)
+    df.sort_values(by=stat, ascending=(mode == "geary"), inplace=True)
     return df
 
 
@@ -249,22 +252,21 @@ def spatial_autocorr(adata, mode: str = "moran", genes=None, n_perms: int | None
                      transformation: bool = True, two_tailed: bool = False,
                      corr_method: str | None = "fdr_bh", attr: str = "X", use_raw: bool = False,
                      seed: int = 0, copy: bool = False, permutations: str = "numpy"):
-    """Moran's I per gene (``sq.gr.spatial_autocorr``); writes ``uns['moranI']``.
+    """Moran's I / Geary's C per gene (``sq.gr.spatial_autocorr``); writes ``uns['moranI']`` /
+    ``uns['gearyC']``.
 
     ``permutations="numpy"`` (default) draws the null from ``default_rng(seed)`` as the
     reference does; ``"device"`` generates it on the GPU (no host permutation stream).
     """
     if mode not in ("moran", "geary"):
         raise ValueError(f"mode must be 'moran' or 'geary', got {mode!r}")
-    if mode == "geary":
-        raise NotImplementedError("mode='geary' is not implemented on the MI355X (Moran only)")
     Xg, index = _autocorr_values(adata, attr, genes, layer, use_raw)
     g = adata.obsp[connectivity_key]
     g = _sp.row_normalize(g) if transformation else g.astype(np.float64)
-    out = _sp.spatial_autocorr(Xg, g, mode="moran", n_perms=n_perms, random_state=seed,
+    out = _sp.spatial_autocorr(Xg, g, mode=mode, n_perms=n_perms, random_state=seed,
                                permutations=permutations)
-    df = moran_frame(out["moran"], out["sims"], g, index, two_tailed, corr_method)
+    df = moran_frame(out[mode], out["sims"], g, index, two_tailed, corr_method, mode=mode)
     if copy:
         return df
-    adata.uns["moranI"] = df
+    adata.uns["moranI" if mode == "moran" else "gearyC"] = df
     return None

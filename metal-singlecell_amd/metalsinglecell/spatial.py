@@ -1,8 +1,10 @@
-"""Spatial autocorrelation (squidpy ``gr.spatial_autocorr`` compute), Moran's I on the MI355X.
+"""Spatial autocorrelation (squidpy ``gr.spatial_autocorr`` compute), Moran's I and Geary's C
+on the MI355X.
 
 Mirrors the reference's ``metalsinglecell.spatial`` (src/metalsinglecell/spatial.py):
 ``spatial_autocorr(X, connectivity, mode, n_perms, random_state) -> {mode: I, "sims": ...}``
-(:357-406) with the statistic and the permutation null computed by csrc/moran.hip, and the
+(:357-406) with the statistic and the permutation null computed by csrc/moran.hip (Geary:
+:388-393), and the
 small host-side statistics over the (genes,) / (perms, genes) results:
 ``row_normalize`` (:268-279), ``graph_moments`` and ``analytic_pval`` (:282-313, Cliff & Ord
 1981 / pysal), ``permutation_pvals`` (:316-331) and ``multiple_testing`` (:334-354).
@@ -45,15 +47,21 @@ def _normal_tail(z):
 
 
 def analytic_pval(score, g, mode="moran", two_tailed=False):
-    """Normality-assumption p-value and variance of Moran's I (randomisation-free form)."""
-    if mode != "moran":
-        raise NotImplementedError(f"mode={mode!r}: only Moran's I is implemented on the MI355X")
+    """Normality-assumption p-value and variance (Cliff & Ord 1981, as pysal/esda): Moran's
+    I around E = -1/(n-1), Geary's C around E = 1 with its own variance."""
+    if mode not in ("moran", "geary"):
+        raise ValueError(f"mode must be 'moran' or 'geary', got {mode!r}")
     s0, s1, s2 = graph_moments(g)
     n = g.shape[0]
-    e_i = -1.0 / (n - 1)
-    var = (n * n * s1 - n * s2 + 3.0 * s0 * s0) / ((n * n - 1.0) * s0 * s0) - e_i * e_i
+    s02 = s0 * s0
+    if mode == "geary":
+        var = ((2.0 * s1 + s2) * (n - 1) - 4.0 * s02) / (2.0 * (n + 1) * s02)
+        expected = 1.0
+    else:
+        expected = -1.0 / (n - 1)
+        var = (n * n * s1 - n * s2 + 3.0 * s02) / ((n * n - 1.0) * s02) - expected * expected
     with np.errstate(divide="ignore", invalid="ignore"):
-        p = _normal_tail((np.asarray(score, np.float64) - e_i) / np.sqrt(var))
+        p = _normal_tail((np.asarray(score, np.float64) - expected) / np.sqrt(var))
     return (2.0 * p if two_tailed else p), var
 
 
@@ -97,7 +105,7 @@ def permutations_numpy(n, n_perms, random_state):
 
 def spatial_autocorr(X, connectivity, mode="moran", n_perms=None, random_state=0,
                      permutations="numpy", transformation=False):
-    """Per-gene Moran's I and (optionally) its ``(n_perms, genes)`` permutation null.
+    """Per-gene Moran's I or Geary's C and (optionally) its ``(n_perms, genes)`` permutation null.
 
     ``connectivity`` is used as given (the caller row-normalises, as in the reference),
     unless ``transformation=True``, which normalises on the device.  ``permutations`` picks
@@ -105,8 +113,8 @@ def spatial_autocorr(X, connectivity, mode="moran", n_perms=None, random_state=0
     on the host; ``"device"`` uses a keyed Feistel bijection generated in the kernel.
     """
     import scipy.sparse as sp
-    if mode != "moran":
-        raise NotImplementedError(f"mode={mode!r}: only Moran's I is implemented on the MI355X")
+    if mode not in ("moran", "geary"):
+        raise ValueError(f"mode must be 'moran' or 'geary', got {mode!r}")
     W = sp.csr_matrix(connectivity)
     n = W.shape[0]
     P = int(n_perms or 0)
@@ -115,6 +123,7 @@ def spatial_autocorr(X, connectivity, mode="moran", n_perms=None, random_state=0
         perms = permutations_numpy(n, P, random_state)
     elif P and permutations != "device":
         raise ValueError("permutations must be 'numpy' or 'device'")
-    I, sims, _ = _hip.moran(X, W.indptr, W.indices, W.data.astype(np.float64), n_perms=P,
-                            perms=perms, seed=random_state, transformation=transformation)
-    return {mode: I.cpu().numpy(), "sims": None if sims is None else sims.cpu().numpy()}
+    fn = _hip.moran if mode == "moran" else _hip.geary
+    stat, sims, _ = fn(X, W.indptr, W.indices, W.data.astype(np.float64), n_perms=P, perms=perms,
+                       seed=random_state, transformation=transformation)
+    return {mode: stat.cpu().numpy(), "sims": None if sims is None else sims.cpu().numpy()}

@@ -1,4 +1,4 @@
-"""Moran's I oracle (float64, dense and sparse forms) — TEST INFRASTRUCTURE ONLY.
+"""Moran's I and Geary's C oracle (float64, dense and sparse forms) — TEST INFRASTRUCTURE ONLY.
 
 Restates src/metalsinglecell/spatial.py:357-406 (statistic and permutation null),
 :268-279 (row normalisation), :282-313 (pysal weight moments and the normality p-value,
@@ -9,7 +9,8 @@ Statistic: ``Xc = X - mean``, ``denom_g = sum_i Xc_ig^2``, ``Y = W Xc``,
 ``I_g = (n / S0) * sum_i Xc_ig Y_ig / denom_g``.  Null: for permutation pi drawn from
 ``np.random.default_rng(seed).permutation(n)`` in sequence, the reference scatters edges from
 ``pi[src]`` (spatial.py:401-405), which equals ``sim_g = (n/S0) sum_r Xc[pi(r), g] Y[r, g] / denom_g``.
-Everything here is float64.
+Geary's C (:388-393) is the direct edge sum ``(n-1)/(2 S0) sum_e w_e (Xc_src - Xc_dst)^2 /
+denom`` with the same permuted sources.  Everything here is float64.
 """
 
 from __future__ import annotations
@@ -68,6 +69,37 @@ def moran_scatter(X, W, perms=None):
     I = stat(src)
     sims = None if perms is None else np.stack([stat(p[src]) for p in perms])
     return I, sims
+
+
+def geary_scatter(X, W, perms=None):
+    """Geary's C in the reference's edge form (spatial.py:388-393, null :396-406), float64:
+    ``C_g = (n-1)/(2 S0) sum_e w_e (Xc[src_e] - Xc[dst_e])^2 / denom_g``; permutation pi
+    moves the sources to ``pi[src]``."""
+    import scipy.sparse as sp
+    coo = sp.coo_matrix(W)
+    src, dst, w = coo.row, coo.col, coo.data.astype(np.float64)
+    X = np.asarray(X, np.float64)
+    n = X.shape[0]
+    Xc = X - X.mean(axis=0)
+    denom = (Xc * Xc).sum(axis=0)
+    S0 = float(w.sum())
+
+    def stat(s):
+        de = Xc[s] - Xc[dst]
+        with np.errstate(divide="ignore", invalid="ignore"):
+            return (n - 1) / (2.0 * S0) * (w[:, None] * de * de).sum(axis=0) / denom
+
+    C = stat(src)
+    sims = None if perms is None else np.stack([stat(p[src]) for p in perms])
+    return C, sims
+
+
+def geary_variance(g):
+    """Normality variance of Geary's C (Cliff & Ord 1981, as pysal/esda; spatial.py:304-306)."""
+    s0, s1, s2 = graph_moments(g)
+    n = g.shape[0]
+    s02 = s0 * s0
+    return ((2 * s1 + s2) * (n - 1) - 4 * s02) / (2 * (n + 1) * s02)
 
 
 def graph_moments(g):
