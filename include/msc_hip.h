@@ -277,6 +277,19 @@ int msc_geary(const float* X, int64_t n, int64_t n_genes, const int64_t* w_indpt
               int n_perms, const int32_t* perms, int64_t seed, double* out_C, double* out_sims,
               double* out_S0 /*host*/, void* workspace, size_t workspace_bytes, void* stream);
 
+/* Cluster co-occurrence pair histogram: the fused kernel of spatial.co_occurrence
+ * (src/metalsinglecell/spatial.py:123-171) as called by gr.co_occurrence (gr.py:514-530).
+ * out_hist[c_i][c_j][r] (device int64, n_codes^2 x n_bins) counts the ordered pairs (i, j),
+ * i != j, whose squared distance d2 has r = the first bin with thr2[r] >= d2; pairs beyond
+ * thr2[n_bins-1] are not counted.  d2 is the float32 squared distance of msc_spatial_knn.
+ * coords: n x dim float32 (dim 2 or 3); codes (host): n values in [0, n_codes); thr2 (host):
+ * n_bins ascending float32.  n < 2^24; the LDS histogram needs n_codes x n_bins <= ~15000. */
+int msc_co_occurrence_workspace_size(const int32_t* codes /*host*/, int64_t n, int dim, int n_codes,
+                                     int n_bins, size_t* bytes /*host*/);
+int msc_co_occurrence(const float* coords, int64_t n, int dim, const int32_t* codes /*host*/,
+                      int n_codes, const float* thr2 /*host*/, int n_bins, int64_t* out_hist,
+                      void* workspace, size_t workspace_bytes, void* stream);
+
 /* Dense n x n_cols float32 row-major copy of a CSR matrix (duplicates: last write wins; a
  * canonical CSR has none).  Moran densifies the selected genes (gr.py:452). */
 int msc_csr_to_dense(const int64_t* indptr, const int32_t* indices, const float* data, int64_t n,

@@ -92,6 +92,8 @@ SIGNATURES = {
     "msc_geary_workspace_size": (c_int, [c_i64, c_i64, c_i64, c_int, ctypes.POINTER(c_sz)]),
     "msc_geary": (c_int, [c_vp, c_i64, c_i64, c_vp, c_vp, c_vp, c_i64, c_int, c_int, c_vp, c_i64,
                           c_vp, c_vp, ctypes.POINTER(c_dbl), c_vp, c_sz, c_vp]),
+    "msc_co_occurrence_workspace_size": (c_int, [c_vp, c_i64, c_int, c_int, c_int, ctypes.POINTER(c_sz)]),
+    "msc_co_occurrence": (c_int, [c_vp, c_i64, c_int, c_vp, c_int, c_vp, c_int, c_vp, c_vp, c_sz, c_vp]),
     "msc_csr_to_dense": (c_int, [c_vp, c_vp, c_vp, c_i64, c_i64, c_vp, c_vp]),
     "msc_graph_moments_workspace_size": (c_int, [c_i64, c_i64, ctypes.POINTER(c_sz)]),
     "msc_graph_moments": (c_int, [c_vp, c_vp, c_vp, c_i64, c_i64, c_vp, c_vp, c_sz, c_vp]),
@@ -479,6 +481,26 @@ def moran(X, w_indptr, w_indices, w_data, n_perms=0, perms=None, seed=0, transfo
 def geary(X, w_indptr, w_indices, w_data, n_perms=0, perms=None, seed=0, transformation=True):
     """Geary's C on the device: ``(C float64 (G,), sims float64 (P, G) or None, S0)``."""
     return _autocorr("geary", X, w_indptr, w_indices, w_data, n_perms, perms, seed, transformation)
+
+
+def co_occurrence_hist(coords, codes, n_codes, thr2):
+    """Per-bin ordered-pair counts ``(n_codes, n_codes, len(thr2))`` int64 on the device."""
+    import torch
+    lib = load_library()
+    X = to_device(coords, torch.float32)
+    n, dim = X.shape
+    c = np.ascontiguousarray(codes, dtype=np.int32)
+    t = np.ascontiguousarray(thr2, dtype=np.float32)
+    if c.shape != (n,):
+        raise ValueError("codes must have one entry per row")
+    L = int(t.size)
+    nbytes = c_sz(0)
+    check(lib.msc_co_occurrence_workspace_size(c.ctypes.data, n, dim, int(n_codes), L, ctypes.byref(nbytes)))
+    ws = workspace(nbytes.value)
+    out = torch.empty((int(n_codes), int(n_codes), L), dtype=torch.int64, device=X.device)
+    check(lib.msc_co_occurrence(ptr(X), n, dim, c.ctypes.data, int(n_codes), t.ctypes.data, L, ptr(out), ptr(ws),
+                                ws.numel(), stream_ptr()))
+    return out
 
 
 def csr_to_dense(indptr, indices, data, n_rows, n_cols):

@@ -12,6 +12,8 @@ Mirrors the reference's ``metalsinglecell.gr`` (src/metalsinglecell/gr.py):
 * ``spatial_neighbors_radius`` (:228-252): every pair within ``radius`` (or a ``(min, max)``
   interval), self excluded, binary connectivities and float64 lengths, the same tail and
   slots as the kNN graph; pairs from csrc/grid.hip's count / fill passes.
+* ``co_occurrence`` (:514-530): the all-pairs cluster co-occurrence histogram in
+  csrc/cooccur.hip, squidpy's thresholds and conditional normalisation on the host.
 * ``spatial_autocorr`` (:418-503), Moran and Geary modes: statistic and permutation null in
   csrc/moran.hip, the frame ``uns['moranI']`` / ``uns['gearyC']`` with squidpy's columns,
   sorted by I descending / C ascending.
@@ -194,6 +196,29 @@ def spatial_neighbors_radius(adata, *, radius, spatial_key: str = "spatial", lib
 
     adj, dst = _assemble(adata, spatial_key, library_key, build, transform, set_diag, percentile)
     return _write_graph(adata, adj, dst, {"coord_type": "generic", "radius": radius}, key_added, transform, copy)
+
+
+def co_occurrence(adata, cluster_key, interval=50, spatial_key: str = "spatial", copy: bool = False):
+    """Cluster co-occurrence vs distance (``sq.gr.co_occurrence``, reference gr.py:514-530);
+    writes ``uns[f'{cluster_key}_co_occurrence']`` = ``{"occ", "interval"}``.  ``interval`` is
+    the number of thresholds or an explicit ascending array of them (``occ`` has one entry
+    fewer along its last axis); ``copy=True`` returns ``(occ, interval)``."""
+    import pandas as pd
+    if cluster_key not in adata.obs:
+        raise KeyError(f"no adata.obs[{cluster_key!r}]")
+    if not isinstance(adata.obs[cluster_key].dtype, pd.CategoricalDtype):
+        raise TypeError(f"expected adata.obs[{cluster_key!r}] to be categorical, got {adata.obs[cluster_key].dtype}")
+    if spatial_key not in adata.obsm:
+        raise KeyError(f"no adata.obsm[{spatial_key!r}] with spatial coordinates")
+    kw = ({"interval": np.asarray(interval, dtype=np.float64)} if np.ndim(interval)
+          else {"n_intervals": int(interval)})
+    res = _sp.co_occurrence(np.asarray(adata.obsm[spatial_key], dtype=np.float32),
+                            adata.obs[cluster_key].to_numpy(), **kw)
+    out = {"occ": res["occ"], "interval": res["interval"]}
+    if copy:
+        return out["occ"], out["interval"]
+    adata.uns[f"{cluster_key}_co_occurrence"] = out
+    return None
 
 
 def _autocorr_values(adata, attr, genes, layer, use_raw):

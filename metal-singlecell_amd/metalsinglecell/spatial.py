@@ -127,3 +127,54 @@ def spatial_autocorr(X, connectivity, mode="moran", n_perms=None, random_state=0
     stat, sims, _ = fn(X, W.indptr, W.indices, W.data.astype(np.float64), n_perms=P, perms=perms,
                        seed=random_state, transformation=transformation)
     return {mode: stat.cpu().numpy(), "sims": None if sims is None else sims.cpu().numpy()}
+
+
+def cooccur_thresholds(spatial):
+    """squidpy's ``_find_min_max`` heuristic (spatial.py:174-187 of the reference): anchored on
+    the two points with the smallest coordinate sums and half the span to the largest."""
+    coord_sum = spatial.sum(axis=1)
+    lo_a, lo_b = np.argpartition(coord_sum, 2)[:2]
+    hi = int(np.argmax(coord_sum))
+    thres_max = float(np.linalg.norm(spatial[lo_a] - spatial[hi])) / 2.0
+    thres_min = float(np.linalg.norm(spatial[lo_a] - spatial[lo_b]))
+    return thres_min, thres_max
+
+
+def cooccur_ratios(hist):
+    """Cumulative counts and squidpy's conditional normalisation (reference spatial.py:240-258):
+    ``occ[:, c, r] = P(type | c, within thr_r) / P(type)``."""
+    counts = np.cumsum(np.asarray(hist, dtype=np.float64), axis=2)
+    K, _, L = counts.shape
+    row_sums = counts.sum(axis=0)
+    totals = row_sums.sum(axis=0)
+    occ = np.zeros((K, K, L))
+    for r in range(L):
+        probs = row_sums[:, r] / np.maximum(totals[r], 1e-12)
+        for c in range(K):
+            if row_sums[c, r] != 0.0:
+                occ[:, c, r] = (counts[c, :, r] / row_sums[c, r]) / np.maximum(probs, 1e-12)
+    return occ
+
+
+def co_occurrence(coords, labels, n_intervals=50, interval=None, max_dist=None):
+    """Cumulative cluster co-occurrence ratio (squidpy ``gr.co_occurrence``, reference
+    spatial.py:201-258): ``{"occ": (K, K, L-1), "interval": thresholds, "categories": cats}``.
+
+    The all-pairs per-bin histogram runs in csrc/cooccur.hip; thresholds, the cumulative sum
+    and the normalisation over the small (K, K, L) array run here."""
+    X = np.ascontiguousarray(coords, dtype=np.float32)
+    labels = np.asarray(labels)
+    cats = np.unique(labels)
+    code = np.searchsorted(cats, labels).astype(np.int32)
+    if interval is not None:
+        thr = np.sort(np.asarray(interval, dtype=np.float64))
+    else:
+        lo, hi = cooccur_thresholds(np.asarray(X, dtype=np.float64))
+        if max_dist is not None:
+            hi = float(max_dist)
+        thr = np.linspace(lo, hi, num=n_intervals)
+    if len(thr) <= 1:
+        raise ValueError(f"expected interval to be of length >= 2, found {len(thr)}")
+    thr2 = (thr[1:] ** 2).astype(np.float32)
+    hist = _hip.co_occurrence_hist(X, code, len(cats), thr2).cpu().numpy()
+    return {"occ": cooccur_ratios(hist), "interval": thr, "categories": cats}
