@@ -193,6 +193,96 @@ def serving(duration_s, rate=20.0, genes=500, n_perms=100, max_slots=6, seed=4):
     return st
 
 
+def _ev_ms(fn, reps=3):
+    """Median HIP-event time (ms) of fn() on torch's current stream, after one warm-up call;
+    returns (ms, last result)."""
+    import numpy as np
+    import torch
+    out = fn()
+    ts = []
+    for _ in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        out = fn()
+        e1.record()
+        e1.synchronize()
+        ts.append(e0.elapsed_time(e1))
+    return float(np.median(ts)), out
+
+
+def extensions(Xd, k):
+    """The callers either side of the path, each at a production size:
+    Louvain on the 1M-cell graph; batch-balanced kNN; fixed-radius spatial graph; Geary's C
+    beside Moran's I; the cluster co-occurrence histogram."""
+    import numpy as np
+    import torch
+    from metalsinglecell import _hip, synthetic
+    out = {}
+    n, d = Xd.shape
+    # Louvain on the same connectivity graph as the Leiden stage
+    idx, dist = _hip.knn_exact(Xd, k, precision="bf16")
+    (ip, ind, dat), _ = _hip.fuzzy_connectivities(idx, dist)
+    del idx, dist
+    ms, (lab, q, info) = _ev_ms(lambda: _hip.louvain(ip, ind, dat, n), reps=2)
+    out["louvain"] = {"cells": n, "edges": int(ind.numel()), "ms": ms, "modularity": q,
+                      "levels": len(info["levels"]), "n_clusters": int(lab.max().item()) + 1,
+                      "cells_per_s": n / (ms * 1e-3)}
+    log("louvain", out["louvain"])
+    del ip, ind, dat, lab
+    # batch-balanced kNN: 500k cells x 50 dims in 8 batches, 3 neighbours per batch
+    nb_n, nb_b = min(n, 500_000), 8
+    Xb = Xd[:nb_n].contiguous()
+    codes = np.random.default_rng(7).integers(0, nb_b, nb_n)
+    ms, (bi, _) = _ev_ms(lambda: _hip.bbknn(Xb, codes, nb_b, 3, precision="bf16"), reps=2)
+    _, _, binfo = _hip.bbknn(Xb, codes, nb_b, 3, precision="bf16", return_info=True)   # untimed diagnostics
+    fl = 2.0 * nb_n * nb_n * d
+    out["bbknn"] = {"cells": nb_n, "batches": nb_b, "k_per_batch": 3, "width": int(bi.shape[1]), "ms": ms,
+                    "cells_per_s": nb_n / (ms * 1e-3), "select_flops": fl,
+                    "select_tflops_incl_finalize": fl / (ms * 1e-3) / 1e12,
+                    "fallback_lists": binfo["n_fallback"]}
+    log("bbknn", out["bbknn"])
+    del Xb, bi
+    # fixed-radius spatial graph: 1M cells of tissue, radius for ~15 neighbours
+    rng = np.random.default_rng(11)
+    xy = synthetic.tissue_coordinates(1_000_000, rng=rng).astype(np.float32)
+    xyd = torch.from_numpy(xy).cuda()
+    area = float(np.ptp(xy[:, 0]) * np.ptp(xy[:, 1]))
+    r = float(np.sqrt(15.0 * area / (np.pi * xy.shape[0])))
+    ms, (rip, rind, _) = _ev_ms(lambda: _hip.spatial_radius(xyd, r), reps=3)
+    pairs = int(rind.numel())
+    out["radius"] = {"cells": xy.shape[0], "radius": r, "pairs": pairs, "ms": ms,
+                     "pairs_per_s": pairs / (ms * 1e-3), "cells_per_s": xy.shape[0] / (ms * 1e-3)}
+    log("radius", out["radius"])
+    del rip, rind
+    # Geary's C beside Moran's I: 50k spots x 500 genes, kNN(6) graph, 100 device permutations
+    ns, G, P = 50_000, 500, 100
+    sxy = xyd[:ns].contiguous()
+    si, _ = _hip.spatial_knn(sxy, 6)
+    sip = torch.arange(0, 6 * ns + 1, 6, dtype=torch.int64, device="cuda")
+    sw = torch.ones(6 * ns, dtype=torch.float64, device="cuda")
+    Xg = torch.from_numpy(synthetic.nb_counts(ns, G, rng=rng).toarray().astype(np.float32)).cuda()
+    ms_g, _ = _ev_ms(lambda: _hip.geary(Xg, sip, si.reshape(-1), sw, n_perms=P, seed=1), reps=2)
+    ms_m, _ = _ev_ms(lambda: _hip.moran(Xg, sip, si.reshape(-1), sw, n_perms=P, seed=1), reps=2)
+    gathered = float(P + 1) * ns * G * 4.0
+    out["geary"] = {"cells": ns, "genes": G, "n_perms": P, "ms": ms_g, "moran_ms": ms_m,
+                    "gathered_GBps": gathered / (ms_g * 1e-3) / 1e9}
+    log("geary", out["geary"])
+    del Xg, si, sip, sw
+    # cluster co-occurrence: 50k cells, 10 clusters, 50 thresholds (all ordered pairs)
+    from metalsinglecell import spatial as _sp
+    cn = 50_000
+    cxy = xy[:cn]
+    labels = rng.integers(0, 10, cn)
+    lo, hi = _sp.cooccur_thresholds(cxy.astype(np.float64))
+    thr2 = (np.linspace(lo, hi, 50)[1:] ** 2).astype(np.float32)
+    cxyd = torch.from_numpy(np.ascontiguousarray(cxy)).cuda()
+    ms, _ = _ev_ms(lambda: _hip.co_occurrence_hist(cxyd, labels, 10, thr2), reps=2)
+    out["co_occurrence"] = {"cells": cn, "clusters": 10, "thresholds": 50, "ms": ms,
+                            "pairs_per_s": float(cn) * (cn - 1) / (ms * 1e-3)}
+    log("co_occurrence", out["co_occurrence"])
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -204,6 +294,7 @@ def main():
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=10.0)
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-extensions", action="store_true")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="first run one plain device copy of this many MiB (counter calibration)")
     a = ap.parse_args()
@@ -269,6 +360,12 @@ def main():
         batch = batched()
         log(f"batched in {time.perf_counter() - t:.1f}s", batch)
 
+    ext = None
+    if not a.no_extensions:
+        t = time.perf_counter()
+        ext = extensions(Xd, a.k)
+        log(f"extensions in {time.perf_counter() - t:.1f}s")
+
     serve = None
     if a.serve_seconds > 0:
         t = time.perf_counter()
@@ -286,7 +383,8 @@ def main():
                    "cells": n, "dims": d, "k": a.k, "parallelism": "single GPU"},
         "stages_s": {"knn": last["knn_s"], "connectivities": last["conn_s"], "leiden": last["leiden_s"]},
         "knn_info": last["knn"], "modularity": last["q"], "n_clusters": last["n_clusters"],
-        "fp32_form": fp32, "batched": batch, "serving": serve, "roofline": roof, "cpu_baseline": cpu,
+        "fp32_form": fp32, "batched": batch, "serving": serve, "extensions": ext, "roofline": roof,
+        "cpu_baseline": cpu,
     }
     print(json.dumps(out))
 

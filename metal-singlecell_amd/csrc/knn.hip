@@ -684,6 +684,8 @@ __global__ __launch_bounds__(kThreads) void distances_csr_kernel(
 // host orchestration
 // ------------------------------------------------------------------------------------------
 constexpr int kL = 16;              // per-lane list length: 4 lanes x 16 = 64 admitted candidates
+constexpr int kLShort = 8;          // the short-list form (batch-balanced kNN with k <= kShortK)
+constexpr int kShortK = 4;
 
 // compact flagged rows into a list (deterministic order: ascending row)
 __global__ void k_flag_rows(const int32_t* __restrict__ flags, const int32_t* __restrict__ pos, int64_t n,
@@ -779,23 +781,28 @@ static Buffers carve(Carve& cv, const Plan& p) {
   return b;
 }
 
-template <bool BF16, int KS, bool WIDE>
+template <bool BF16, int KS, bool WIDE, int L>
 static void launch_select_t(const void* Xp, int64_t n, int64_t n_slots, int d, int64_t split_len, int S,
                             const int32_t* qrows, const int64_t* blk_range, uint64_t* keys, float* taus,
                             hipStream_t st) {
   const int64_t qpb = WIDE ? kQueriesPerBlock / 2 : kQueriesPerBlock;
   dim3 grid(static_cast<unsigned>(n_slots / qpb), static_cast<unsigned>(S));
-  select_kernel<BF16, KS, kL, WIDE><<<grid, kThreads, 0, st>>>(Xp, n, n_slots, d, split_len, qrows, blk_range, keys,
-                                                               taus);
+  select_kernel<BF16, KS, L, WIDE><<<grid, kThreads, 0, st>>>(Xp, n, n_slots, d, split_len, qrows, blk_range, keys,
+                                                              taus);
 }
 
-// n_slots must be a multiple of kQueriesPerBlock; the wide form writes 2 * S key sets
+// n_slots must be a multiple of kQueriesPerBlock; the wide form writes 2 * S key sets.
+// short_list keeps kLShort keys per lane (4 kLShort per query) instead of kL: for small k,
+// where many short candidate ranges would otherwise spend their time filling long lists.
 static int launch_select(bool bf16, bool wide, int ks, const void* Xp, int64_t n, int64_t n_slots, int d,
                          int64_t split_len, int S, const int32_t* qrows, const int64_t* blk_range, uint64_t* keys,
-                         float* taus, hipStream_t st) {
-#define MSC_SEL(B, K)                                                                                         \
-  (wide ? launch_select_t<B, K, true>(Xp, n, n_slots, d, split_len, S, qrows, blk_range, keys, taus, st)     \
-        : launch_select_t<B, K, false>(Xp, n, n_slots, d, split_len, S, qrows, blk_range, keys, taus, st))
+                         float* taus, hipStream_t st, bool short_list = false) {
+#define MSC_SEL(B, K)                                                                                             \
+  (wide ? launch_select_t<B, K, true, kL>(Xp, n, n_slots, d, split_len, S, qrows, blk_range, keys, taus, st)     \
+        : (short_list                                                                                             \
+               ? launch_select_t<B, K, false, kLShort>(Xp, n, n_slots, d, split_len, S, qrows, blk_range, keys,   \
+                                                       taus, st)                                                  \
+               : launch_select_t<B, K, false, kL>(Xp, n, n_slots, d, split_len, S, qrows, blk_range, keys, taus, st)))
   if (bf16) {
     switch (ks) {
       case 1: MSC_SEL(true, 1); break;
@@ -1495,7 +1502,11 @@ extern "C" int msc_bbknn(const float* X, int64_t n, int d, const int32_t* batch,
   const bool bf16 = precision == MSC_PRECISION_BF16_RERANK;
   Operands op = prepare(t.p, t.b, t.Xpad, nt, bf16, st);
   MSC_LAUNCH_CHECK();
-  MSC_TRY(launch_select(bf16, false, op.ks, op.X, nt, nt, d, nt, nb, nullptr, t.blk_range, t.b.keys, t.b.taus, st));
+  // the certificate compares the k-th exact distance with the smallest lane tail, which is at
+  // least the (per-lane list length)-th approximate distance: 8 per lane keeps that margin
+  // wide for k <= 4 at half the list upkeep of the 16-per-lane lists
+  MSC_TRY(launch_select(bf16, false, op.ks, op.X, nt, nt, d, nt, nb, nullptr, t.blk_range, t.b.keys, t.b.taus, st,
+                        k <= kShortK));
   const unsigned wb = static_cast<unsigned>(ceil_div(static_cast<int64_t>(nb) * nt, kThreads / kWave));
   finalize_sep_kernel<<<wb, kThreads, 0, st>>>(t.Xpad, nt, d, nb, t.b.keys, t.b.taus, op.qn2, op.qres, op.gstat,
                                                op.bnd, t.prow2row, t.prow_batch, t.kk, t.coff, L.K, t.idx_pad,
@@ -1517,7 +1528,7 @@ extern "C" int msc_bbknn(const float* X, int64_t n, int d, const int32_t* batch,
     info->n_fallback = m;
     info->n_second_pass = 0;
     info->n_splits = nb;
-    info->list_len = 4 * kL;
+    info->list_len = 4 * (k <= kShortK ? kLShort : kL);
   }
   return kOk;
 }
