@@ -283,6 +283,89 @@ def extensions(Xd, k):
     return out
 
 
+def extensions_cpu(X, threads, ext):
+    """CPU timings beside each extension, on bounded samples of the same workloads (the
+    oracle restatements, kind "port"; the radius graph uses scipy's k-d tree, kind "library")."""
+    import numpy as np
+    from scipy.spatial import cKDTree
+    from metalsinglecell import synthetic
+    from oracle import cooccur, fuzzy, knn, leiden, moran
+    rng = np.random.default_rng(0)
+    n = X.shape[0]
+    out = {}
+    # Louvain: a 20k-cell subsample graph, extrapolated linearly to n cells
+    m = min(n, 20_000)
+    sub = np.ascontiguousarray(X[rng.choice(n, m, replace=False)])
+    si, _, sd = knn.exact_knn_c(sub, 30, threads=threads)
+    P, _, _ = fuzzy.fuzzy_simplicial_set(si, sd)
+    t = time.perf_counter()
+    leiden.louvain(P)
+    out["louvain"] = {"ms": (time.perf_counter() - t) * 1e3 * n / m, "kind": "port", "cores": 1,
+                      "sample": f"NumPy Louvain on a {m}-cell subsample graph (k=30), x{n / m:.0f}"}
+    # bbknn: 96 query rows per batch against that batch (C oracle), extrapolated to n x n_b pairs
+    nb_n = ext["bbknn"]["cells"]
+    codes = np.random.default_rng(7).integers(0, 8, nb_n)
+    Xb = X[:nb_n]
+    tot = 0.0
+    for b in range(8):
+        Xs = np.ascontiguousarray(Xb[codes == b])
+        rows = rng.choice(Xs.shape[0], 96, replace=False)
+        t = time.perf_counter()
+        knn.exact_knn_c(Xs, 3, rows=rows, threads=threads)
+        tot += (time.perf_counter() - t) * nb_n / 96
+    out["bbknn"] = {"ms": tot * 1e3, "kind": "port", "cores": threads,
+                    "sample": f"96 queries x each of 8 batches (C oracle, {threads} threads), scaled to {nb_n} queries"}
+    # radius: scipy k-d tree over the same 1M points
+    xy = synthetic.tissue_coordinates(1_000_000, rng=np.random.default_rng(11)).astype(np.float32)
+    t = time.perf_counter()
+    tree = cKDTree(xy.astype(np.float64))
+    tree.query_ball_point(xy.astype(np.float64), ext["radius"]["radius"], workers=threads)
+    out["radius"] = {"ms": (time.perf_counter() - t) * 1e3, "kind": "library", "cores": threads,
+                     "sample": "scipy cKDTree build + query_ball_point, all 1M points"}
+    # Geary: the float64 edge-form oracle, observed + 2 permutations, scaled to 100 permutations
+    import scipy.sparse as sp
+    ns, G = ext["geary"]["cells"], ext["geary"]["genes"]
+    sxy = xy[:ns].astype(np.float64)
+    _, nn = cKDTree(sxy).query(sxy, 7, workers=threads)
+    W = sp.csr_matrix((np.ones(ns * 6), nn[:, 1:].ravel(), np.arange(0, 6 * ns + 1, 6)), shape=(ns, ns))
+    Xg = synthetic.nb_counts(ns, G, rng=np.random.default_rng(12)).toarray()
+    perms = moran.permutations(ns, 2, 1)
+    t = time.perf_counter()
+    moran.geary_scatter(Xg, moran.row_normalize(W), perms)
+    el = time.perf_counter() - t
+    out["geary"] = {"ms": el * 1e3 * (1 + ext["geary"]["n_perms"]) / 3.0, "kind": "port", "cores": 1,
+                    "sample": f"NumPy float64 Geary, observed + 2 permutations at {ns} x {G}, scaled to "
+                              f"{ext['geary']['n_perms']} permutations"}
+    # co-occurrence: 1000 query rows against all cells, scaled to every row
+    cn = ext["co_occurrence"]["cells"]
+    cxy = xy[:cn]
+    lab = rng.integers(0, 10, cn)
+    thr = cooccur.thresholds(cxy.astype(np.float64), 50)
+    q = rng.choice(cn, 1000, replace=False)
+    t = time.perf_counter()
+    _qhist(cxy, q, lab, (thr[1:] ** 2).astype(np.float32))
+    out["co_occurrence"] = {"ms": (time.perf_counter() - t) * 1e3 * cn / 1000, "kind": "port", "cores": 1,
+                            "sample": f"NumPy pair histogram of 1000 query rows x {cn} cells, x{cn / 1000:.0f}"}
+    return out
+
+
+def _qhist(xy, q, lab, thr2):
+    """The oracle's per-pair binning for a subset of query rows (timing only)."""
+    import numpy as np
+    Q = xy[q]
+    acc = None
+    for t in range(xy.shape[1]):
+        diff = (xy[None, :, t] - Q[:, t, None]).astype(np.float32)
+        sq = (diff * diff).astype(np.float32)
+        acc = sq if acc is None else (acc + sq).astype(np.float32)
+    r = np.searchsorted(thr2, acc, side="left")
+    keep = r < thr2.size
+    hist = np.zeros((10, 10, thr2.size), np.int64)
+    qi, j = np.nonzero(keep)
+    np.add.at(hist, (lab[q][qi], lab[j], r[qi, j]), 1)
+    return hist
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -365,6 +448,13 @@ def main():
         t = time.perf_counter()
         ext = extensions(Xd, a.k)
         log(f"extensions in {time.perf_counter() - t:.1f}s")
+        if not a.no_cpu:
+            t = time.perf_counter()
+            ecpu = extensions_cpu(X, len(os.sched_getaffinity(0)), ext)
+            for key, v in ecpu.items():
+                ext[key]["cpu_baseline"] = v
+                ext[key]["gpu_vs_cpu"] = v["ms"] / ext[key]["ms"]
+            log(f"extension cpu baselines in {time.perf_counter() - t:.1f}s", ecpu)
 
     serve = None
     if a.serve_seconds > 0:

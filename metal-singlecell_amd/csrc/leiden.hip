@@ -14,10 +14,9 @@
 // of scheduling.
 //
 // Per-vertex best move: vertices of degree <= 256 take one wave (up to four neighbours per
-// lane); degree 257..1024 and 1025..4096 take one wave each from a list (up to sixteen and
-// sixty-four per lane); all sort the neighbours' communities inside the wave.  Heavier
-// vertices (and the list ranges whose community ids would not fit the sort key: 2^22 and
-// 2^20 vertices and up) go through a sort path over their edges keyed by (vertex, community).
+// lane, sorted inside the wave); degree 257..1024 and 1025..4096 take one workgroup each from
+// a list (256 / 1024 threads, aggregated in an LDS table).  Heavier vertices go through a
+// sort path over their edges keyed by (vertex, community).
 #include <cmath>
 #include <vector>
 
@@ -290,14 +289,9 @@ __device__ __forceinline__ void move_one(int64_t v, int64_t e0, int64_t deg, int
 // once at the end of its run and takes the wave argmax (ties to the smaller community).
 // Requires community ids < 2^(32 - IB) so that a key fits 32 bits.
 constexpr int64_t kSortedMaxN = int64_t(1) << 24;  // NI <= 4
-constexpr int kMidItems = 16;
-constexpr int kMidDeg = kWave * kMidItems;           // 1024
-constexpr int64_t kMidMaxN = int64_t(1) << 22;      // NI = 16
-constexpr int kBigItems = 64;
-constexpr int kBigDeg = kWave * kBigItems;           // 4096
-constexpr int64_t kBigMaxN = int64_t(1) << 20;      // NI = 64
+constexpr int kMidDeg = 1024;                        // k_move_hash<256, 4, 11>
+constexpr int kBigDeg = 4096;                        // k_move_hash<1024, 4, 13>
 constexpr int kMoveGroup = 8;  // consecutive vertices per wave in k_move_light / k_frontier_push
-constexpr int kMidGroup = 2;   // listed vertices per single-wave block in k_move_mid
 
 // Bitonic network over the 64 * NI slots e = i * 64 + l of a wave (keys in registers).  The
 // stage sequence is unrolled by template recursion so that every register index is a
@@ -340,12 +334,9 @@ __device__ __forceinline__ void move_sorted(int64_t v, int64_t e0, int64_t deg, 
                                             const int32_t* __restrict__ comm, const int32_t* __restrict__ part,
                                             const int64_t* __restrict__ sigtot, const GainParams& gp,
                                             int32_t* __restrict__ target, int64_t* __restrict__ lw,
-                                            int64_t* __restrict__ lsg, int64_t* __restrict__ lp,
-                                            uint64_t* __restrict__ lhm) {
-  constexpr int N = NI * kWave;
-  constexpr int IB = NI == 1 ? 6 : NI == 2 ? 7 : NI == 4 ? 8 : NI == 8 ? 9 : NI == 16 ? 10 : NI == 32 ? 11 : 12;
-  static_assert(NI == 1 || NI == 2 || NI == 4 || NI == 8 || NI == 16 || NI == 32 || NI == 64, "NI");
-  constexpr bool kHmLds = NI > 16;  // run-head masks in LDS (lhm, NI entries) instead of registers
+                                            int64_t* __restrict__ lsg, int64_t* __restrict__ lp) {
+  constexpr int IB = NI == 1 ? 6 : NI == 2 ? 7 : 8;
+  static_assert(NI == 1 || NI == 2 || NI == 4, "NI");
   constexpr uint32_t kNone = 0xFFFFFFFFu;
   const int l = lane_id();
   const int32_t pv = part ? part[v] : 0;
@@ -374,7 +365,7 @@ __device__ __forceinline__ void move_sorted(int64_t v, int64_t e0, int64_t deg, 
   bitonic_sort_slots<NI, 2>(key, l);  // ascending in slot order e = i * 64 + l
 
   // per slot: weight prefix and run heads (community = key >> IB)
-  uint64_t hmask[kHmLds ? 1 : NI];
+  uint64_t hmask[NI];
   int64_t carry = 0;
   int32_t c_last = -1;  // community of slot (i * 64 - 1)
 #pragma unroll
@@ -396,12 +387,7 @@ __device__ __forceinline__ void move_sorted(int64_t v, int64_t e0, int64_t deg, 
     lp[e] = x;
     int32_t cprev = __shfl_up(ci, 1, kWave);
     if (l == 0) cprev = c_last;
-    const uint64_t hm = __ballot(valid && (e == 0 || cprev != ci));
-    if constexpr (kHmLds) {
-      if (l == 0) lhm[i] = hm;
-    } else {
-      hmask[kHmLds ? 0 : i] = hm;
-    }
+    hmask[i] = __ballot(valid && (e == 0 || cprev != ci));
     c_last = __shfl(ci, kWave - 1, kWave);
   }
 
@@ -413,14 +399,8 @@ __device__ __forceinline__ void move_sorted(int64_t v, int64_t e0, int64_t deg, 
   for (int i = 0; i < NI; ++i) {
     const int e = i * kWave + l;
     const bool valid = e < nvalid;
-    uint64_t hmi, hmn;  // run heads of this item and of the next one
-    if constexpr (kHmLds) {
-      hmi = lhm[i];
-      hmn = i + 1 < NI ? lhm[i + 1 < NI ? i + 1 : i] : 0ull;
-    } else {
-      hmi = hmask[kHmLds ? 0 : i];
-      hmn = i + 1 < NI ? hmask[kHmLds ? 0 : (i + 1 < NI ? i + 1 : i)] : 0ull;
-    }
+    const uint64_t hmi = hmask[i];                                   // run heads of this item
+    const uint64_t hmn = i + 1 < NI ? hmask[i + 1 < NI ? i + 1 : i] : 0ull;  // and of the next one
     const bool next_head = (l < kWave - 1) ? ((hmi >> (l + 1)) & 1ull) != 0 : (hmn & 1ull) != 0;
     const bool end = valid && (e + 1 == nvalid || next_head);
     const uint64_t upto = (l == kWave - 1) ? ~0ull : ((2ull << l) - 1ull);
@@ -460,10 +440,10 @@ __device__ __forceinline__ void move_vertex(int64_t v, const int64_t* __restrict
     int64_t* lw = lds[0];
     int64_t* lsg = lds[1];
     int64_t* lp = lds[2];
-    if (deg <= kWave) move_sorted<1>(v, e0, deg, curr, kv, indices, wq, comm, part, sigtot, gp, target, lw, lsg, lp, nullptr);
+    if (deg <= kWave) move_sorted<1>(v, e0, deg, curr, kv, indices, wq, comm, part, sigtot, gp, target, lw, lsg, lp);
     else if (deg <= 2 * kWave)
-      move_sorted<2>(v, e0, deg, curr, kv, indices, wq, comm, part, sigtot, gp, target, lw, lsg, lp, nullptr);
-    else move_sorted<4>(v, e0, deg, curr, kv, indices, wq, comm, part, sigtot, gp, target, lw, lsg, lp, nullptr);
+      move_sorted<2>(v, e0, deg, curr, kv, indices, wq, comm, part, sigtot, gp, target, lw, lsg, lp);
+    else move_sorted<4>(v, e0, deg, curr, kv, indices, wq, comm, part, sigtot, gp, target, lw, lsg, lp);
   } else {
     if (deg <= kWave) move_one<1>(v, e0, deg, curr, kv, indices, wq, comm, part, sigtot, gp, target);
     else if (deg <= 2 * kWave) move_one<2>(v, e0, deg, curr, kv, indices, wq, comm, part, sigtot, gp, target);
@@ -496,61 +476,101 @@ __global__ __launch_bounds__(kThreads) void k_move_light(
   }
 }
 
-// degree 257..1024 (graphs below 2^22 vertices): one single-wave block per kMidGroup listed
-// vertices, active ones moved one after another
-__global__ __launch_bounds__(kWave) void k_move_mid(
-    const int32_t* __restrict__ verts, int64_t nm, const int64_t* __restrict__ indptr, const int32_t* __restrict__ indices,
-    const int64_t* __restrict__ wq, const int64_t* __restrict__ kdeg, const int32_t* __restrict__ comm,
-    const int32_t* __restrict__ part, const int64_t* __restrict__ sigtot, const uint8_t* __restrict__ active,
-    GainParams gp, int32_t* __restrict__ target) {
-  __shared__ int64_t lds[3][kMidDeg];
-  __builtin_amdgcn_s_setprio(2);  // as in k_move_big
-  const int l = lane_id();
-  const int64_t i0 = static_cast<int64_t>(blockIdx.x) * kMidGroup;
-  bool todo = false;
-  if (l < kMidGroup && i0 + l < nm) {
-    const int64_t vv = verts[i0 + l];
-    if (!active[vv]) target[vv] = comm[vv];
-    else todo = true;
-  }
-  uint64_t pending = __ballot(todo);
-  while (pending) {
-    const int64_t v = verts[i0 + __ffsll(static_cast<long long>(pending)) - 1];
-    pending &= pending - 1;
-    const int32_t curr = comm[v];
-    const int64_t e0 = indptr[v], deg = indptr[v + 1] - e0;
-    const int64_t kv = kdeg[v];
-    if (deg <= 8 * kWave)
-      move_sorted<8>(v, e0, deg, curr, kv, indices, wq, comm, part, sigtot, gp, target, lds[0], lds[1], lds[2], nullptr);
-    else
-      move_sorted<16>(v, e0, deg, curr, kv, indices, wq, comm, part, sigtot, gp, target, lds[0], lds[1], lds[2], nullptr);
-  }
-}
-
-// degree 1025..4096 (graphs below 2^20 vertices): one single-wave block per listed vertex
-// (the wave's slot arrays take 96 KiB of LDS, so one such block per CU)
-__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_move_big(
+// Degree 257..1024 and 1025..4096: one workgroup of
+// THREADS threads per listed vertex, ITEMS neighbours per thread.  The same decision as
+// move_sorted, by aggregation instead of sorting: every eligible neighbour adds its int64
+// weight into an LDS open-addressing table keyed by community (2^TBITS slots, twice the
+// degree cap, so a probe always ends), then every occupied slot is scored and the block
+// takes the argmax (ties to the smaller community).  The table's contents — the set of
+// (community, exact integer weight sum) — do not depend on the order of the LDS atomic
+// updates, so the decision is deterministic and equals the sorted paths'.
+template <int THREADS, int ITEMS, int TBITS>
+__global__ __launch_bounds__(THREADS) void k_move_hash(
     const int32_t* __restrict__ verts, const int64_t* __restrict__ indptr, const int32_t* __restrict__ indices,
     const int64_t* __restrict__ wq, const int64_t* __restrict__ kdeg, const int32_t* __restrict__ comm,
     const int32_t* __restrict__ part, const int64_t* __restrict__ sigtot, const uint8_t* __restrict__ active,
     GainParams gp, int32_t* __restrict__ target) {
-  __shared__ int64_t lds[3][kBigDeg];
-  __shared__ uint64_t lhm[kBigItems];
-  // a few long single-wave sorts share SIMDs with the many short light-vertex waves:
-  // raise their issue priority so they are not the last to finish
-  __builtin_amdgcn_s_setprio(3);
+  constexpr int T = 1 << TBITS;
+  constexpr int NW = THREADS / kWave;
+  static_assert(T >= 2 * THREADS * ITEMS, "table must hold twice the neighbours");
+  __shared__ int32_t tkey[T];
+  __shared__ unsigned long long tval[T];
+  __shared__ double wbest_g[NW];
+  __shared__ int32_t wbest_c[NW];
+  __shared__ double s_stay;
+  const int t = threadIdx.x, l = lane_id(), wv = t / kWave;
   const int64_t v = verts[blockIdx.x];
   const int32_t curr = comm[v];
-  if (!active[v]) {
-    if (lane_id() == 0) target[v] = curr;
+  if (!active[v]) {  // uniform over the block
+    if (t == 0) target[v] = curr;
     return;
   }
+  for (int i = t; i < T; i += THREADS) {
+    tkey[i] = -1;
+    tval[i] = 0ull;
+  }
+  if (t == 0) s_stay = 0.0;
   const int64_t e0 = indptr[v], deg = indptr[v + 1] - e0;
   const int64_t kv = kdeg[v];
-  if (deg <= 32 * kWave)
-    move_sorted<32>(v, e0, deg, curr, kv, indices, wq, comm, part, sigtot, gp, target, lds[0], lds[1], lds[2], lhm);
-  else
-    move_sorted<64>(v, e0, deg, curr, kv, indices, wq, comm, part, sigtot, gp, target, lds[0], lds[1], lds[2], lhm);
+  const int32_t pv = part ? part[v] : 0;
+  __syncthreads();
+#pragma unroll
+  for (int s = 0; s < ITEMS; ++s) {
+    const int e = s * THREADS + t;
+    if (e >= deg) continue;
+    const int32_t u = indices[e0 + e];
+    if (u == static_cast<int32_t>(v) || (part && part[u] != pv)) continue;
+    const int32_t c = comm[u];
+    const unsigned long long w = static_cast<unsigned long long>(wq[e0 + e]);
+    uint32_t h = (static_cast<uint32_t>(c) * 2654435761u) >> (32 - TBITS);
+    for (;;) {
+      const int32_t prev = atomicCAS(&tkey[h], -1, c);
+      if (prev == -1 || prev == c) {
+        atomicAdd(&tval[h], w);
+        break;
+      }
+      h = (h + 1) & (T - 1);
+    }
+  }
+  __syncthreads();
+  double best_g = -1e300;
+  int32_t best_c = 0x7FFFFFFF;
+  for (int i = t; i < T; i += THREADS) {
+    const int32_t c = tkey[i];
+    if (c < 0) continue;
+    const int64_t wc = static_cast<int64_t>(tval[i]);
+    const int64_t sig = sigtot[c] - (c == curr ? kv : 0);
+    const double g = gain_of(wc, sig, kv, gp);
+    if (c == curr) s_stay = g;  // one slot holds the current community
+    if (better(g, c, best_g, best_c)) {
+      best_g = g;
+      best_c = c;
+    }
+  }
+#pragma unroll
+  for (int m = 1; m < kWave; m <<= 1) {
+    const double og = __shfl_xor(best_g, m, kWave);
+    const int32_t oc = __shfl_xor(best_c, m, kWave);
+    if (better(og, oc, best_g, best_c)) {
+      best_g = og;
+      best_c = oc;
+    }
+  }
+  if (l == 0) {
+    wbest_g[wv] = best_g;
+    wbest_c[wv] = best_c;
+  }
+  __syncthreads();
+  if (t == 0) {
+    double bg = wbest_g[0];
+    int32_t bc = wbest_c[0];
+    for (int w = 1; w < NW; ++w)
+      if (better(wbest_g[w], wbest_c[w], bg, bc)) {
+        bg = wbest_g[w];
+        bc = wbest_c[w];
+      }
+    target[v] = (bg > s_stay + kTol) ? bc : curr;
+  }
 }
 
 // ------------------------------------------------------------------------------------------
@@ -970,10 +990,8 @@ static int graph_setup(Ctx& c, Graph& g) {
   c.nm = 0;
   c.nb = 0;
   if (g.max_deg > kLightDeg) {
-    // community ids must fit the sorted paths' keys: 22 bits for NI = 16, 20 for NI = 64
-    const int64_t mid_max = g.n < kMidMaxN ? kMidDeg : kLightDeg;
-    const int64_t big_max = g.n < kBigMaxN ? kBigDeg : mid_max;
-    k_heavy_flags<<<grid_for(g.n + 1), kThreads, 0, st>>>(g.indptr, g.n, mid_max, big_max, c.hflag, c.hdeg, c.mflag,
+    // the listed tiers aggregate by community in LDS tables: any community id fits
+    k_heavy_flags<<<grid_for(g.n + 1), kThreads, 0, st>>>(g.indptr, g.n, kMidDeg, kBigDeg, c.hflag, c.hdeg, c.mflag,
                                                           c.bflag);
     MSC_LAUNCH_CHECK();
     MSC_TRY(prims::exclusive_scan<int32_t>(c.mflag, c.mpos, g.n + 1, c.scan_ws, c.scan_ws_bytes, st));
@@ -1073,13 +1091,13 @@ static int local_moving(Ctx& c, const Graph& g, int64_t seed, const int32_t* par
       if (c.nm > 0) MSC_HIP(hipStreamWaitEvent(c.side2, c.ev_fork, 0));
     }
     if (c.nb > 0) {
-      k_move_big<<<static_cast<unsigned>(c.nb), kWave, 0, c.side>>>(c.bverts, g.indptr, g.indices, g.wq, g.k, c.comm,
-                                                                    part, c.sigtot, c.active, gp, c.target);
+      k_move_hash<1024, 4, 13><<<static_cast<unsigned>(c.nb), 1024, 0, c.side>>>(
+          c.bverts, g.indptr, g.indices, g.wq, g.k, c.comm, part, c.sigtot, c.active, gp, c.target);
       MSC_LAUNCH_CHECK();
     }
     if (c.nm > 0) {
-      k_move_mid<<<static_cast<unsigned>(ceil_div(c.nm, kMidGroup)), kWave, 0, c.side2>>>(
-          c.mverts, c.nm, g.indptr, g.indices, g.wq, g.k, c.comm, part, c.sigtot, c.active, gp, c.target);
+      k_move_hash<256, 4, 11><<<static_cast<unsigned>(c.nm), 256, 0, c.side2>>>(
+          c.mverts, g.indptr, g.indices, g.wq, g.k, c.comm, part, c.sigtot, c.active, gp, c.target);
       MSC_LAUNCH_CHECK();
     }
     // the listed vertices go first so that their large-LDS blocks find free CUs
