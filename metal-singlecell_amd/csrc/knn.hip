@@ -644,20 +644,6 @@ __global__ __launch_bounds__(kThreads) void fallback_kernel(
   }
 }
 
-__global__ void count_flags(const int32_t* __restrict__ flags, int64_t n, int64_t* __restrict__ out) {
-  int64_t s = 0;
-  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) s += flags[i];
-  s = wave_sum_i64(s);
-  __shared__ int64_t sh[kThreads / kWave];
-  if (lane_id() == 0) sh[threadIdx.x / kWave] = s;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    int64_t t = 0;
-    for (int w = 0; w < static_cast<int>(blockDim.x) / kWave; ++w) t += sh[w];
-    *out = t;
-  }
-}
-
 // distances CSR: row i holds the k-1 non-self neighbours sorted by column (canonical CSR,
 // explicit zeros kept, as scipy's coo->csr in neighbors.py:755-760)
 __global__ __launch_bounds__(kThreads) void distances_csr_kernel(
@@ -1063,7 +1049,11 @@ __global__ __launch_bounds__(kThreads) void finalize_sep_kernel(
     int32_t* __restrict__ idx_pad, float* __restrict__ dist_pad, int32_t* __restrict__ flags) {
   const int l = lane_id();
   const int64_t w = static_cast<int64_t>(blockIdx.x) * (kThreads / kWave) + threadIdx.x / kWave;
-  if (w >= static_cast<int64_t>(nb) * n_slots) return;
+  const int64_t M = static_cast<int64_t>(nb) * n_slots;
+  if (w >= M) {
+    if (w == M && l == 0) flags[M] = 0;  // the flag scan's last input
+    return;
+  }
   const int t = static_cast<int>(w / n_slots);
   const int64_t q = w - static_cast<int64_t>(t) * n_slots;
   if (prow2row[q] < 0) {  // padding row
@@ -1097,32 +1087,38 @@ __global__ __launch_bounds__(kThreads) void finalize_sep_kernel(
   }
 }
 
-// exact brute force of a flagged (batch t, slot q) over batch t's rows [brange[2t], brange[2t+1])
+// exact brute force of each flagged (batch t, slot q) — the compacted list lists[0, *count)
+// of w = t * n_slots + q — over batch t's rows [brange[2t], brange[2t+1]); a fixed grid of
+// waves strides over the list
+constexpr int kFallbackBlocks = 1024;
 __global__ __launch_bounds__(kThreads) void fallback_sep_kernel(
-    const float* __restrict__ X, int64_t n_slots, int d, int nb, const int32_t* __restrict__ flags,
-    const int64_t* __restrict__ brange, const int32_t* __restrict__ kk, const int32_t* __restrict__ coff, int K,
-    int32_t* __restrict__ idx_pad, float* __restrict__ dist_pad) {
+    const float* __restrict__ X, int64_t n_slots, int d, const int32_t* __restrict__ lists,
+    const int32_t* __restrict__ count, const int64_t* __restrict__ brange, const int32_t* __restrict__ kk,
+    const int32_t* __restrict__ coff, int K, int32_t* __restrict__ idx_pad, float* __restrict__ dist_pad) {
   const int l = lane_id();
-  const int64_t w = static_cast<int64_t>(blockIdx.x) * (kThreads / kWave) + threadIdx.x / kWave;
-  if (w >= static_cast<int64_t>(nb) * n_slots || flags[w] == 0) return;
-  const int t = static_cast<int>(w / n_slots);
-  const int64_t q = w - static_cast<int64_t>(t) * n_slots;
-  const int64_t cb = brange[2 * t], ce = brange[2 * t + 1];
-  uint64_t cur = kEmpty, tail = kEmpty;
-  for (int64_t base = cb; base < ce; base += kWave) {
-    const int64_t j = base + l;
-    const uint64_t ek = (j < ce) ? exact_key(q, j, exact_d2(X, d, q, j)) : kEmpty;
-    if (__any(ek < tail)) {
-      cur = wave_merge_min_u64(cur, wave_sort_u64(ek));
-      tail = shfl_u64(cur, kWave - 1);
+  const int64_t m = *count;
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * (kThreads / kWave);
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * (kThreads / kWave) + threadIdx.x / kWave; i < m; i += stride) {
+    const int64_t w = lists[i];
+    const int t = static_cast<int>(w / n_slots);
+    const int64_t q = w - static_cast<int64_t>(t) * n_slots;
+    const int64_t cb = brange[2 * t], ce = brange[2 * t + 1];
+    uint64_t cur = kEmpty, tail = kEmpty;
+    for (int64_t base = cb; base < ce; base += kWave) {
+      const int64_t j = base + l;
+      const uint64_t ek = (j < ce) ? exact_key(q, j, exact_d2(X, d, q, j)) : kEmpty;
+      if (__any(ek < tail)) {
+        cur = wave_merge_min_u64(cur, wave_sort_u64(ek));
+        tail = shfl_u64(cur, kWave - 1);
+      }
     }
-  }
-  if (l < kk[t]) {
-    int64_t jj;
-    float d2;
-    decode_exact(cur, q, &jj, &d2);
-    idx_pad[q * K + coff[t] + l] = static_cast<int32_t>(jj);
-    dist_pad[q * K + coff[t] + l] = sqrtf(d2);
+    if (l < kk[t]) {
+      int64_t jj;
+      float d2;
+      decode_exact(cur, q, &jj, &d2);
+      idx_pad[q * K + coff[t] + l] = static_cast<int32_t>(jj);
+      dist_pad[q * K + coff[t] + l] = sqrtf(d2);
+    }
   }
 }
 
@@ -1408,6 +1404,7 @@ int make_bb_layout(const int32_t* batch, int64_t n, int n_batches, int k, BbLayo
   }
   MSC_REQUIRE(K <= 64, "bbknn: %d combined neighbours (sum over batches of min(k, batch size)) exceed 64", K);
   L->K = K;
+  MSC_REQUIRE(base * n_batches < (int64_t(1) << 31) - 1, "bbknn: n_batches x padded cells must stay below 2^31");
   L->n_tot = base;
   L->nblk = base / kQueriesPerBlock;
   L->prow2row.assign(base, -1);
@@ -1430,8 +1427,10 @@ struct BbBufs {
   msc::knn::Plan p;
   msc::knn::Buffers b;
   float* Xpad;
-  int32_t *prow2row, *prow_batch, *kk, *coff, *flags;
-  int64_t *brange, *blk_range, *nfall;
+  int32_t *prow2row, *prow_batch, *kk, *coff, *flags, *fpos, *flist;
+  int64_t *brange, *blk_range;
+  void* fscan;
+  size_t fscan_bytes;
   int32_t* idx_pad;
   float* dist_pad;
 };
@@ -1447,10 +1446,14 @@ BbBufs carve_bb(msc::Carve& cv, const BbLayout& L, int d, int k, int precision) 
   t.prow_batch = cv.take<int32_t>(L.n_tot);
   t.kk = cv.take<int32_t>(L.nb);
   t.coff = cv.take<int32_t>(L.nb);
-  t.flags = cv.take<int32_t>(static_cast<size_t>(L.nb) * L.n_tot);
+  const int64_t M = static_cast<int64_t>(L.nb) * L.n_tot;
+  t.flags = cv.take<int32_t>(M + 1);
+  t.fpos = cv.take<int32_t>(M + 1);
+  t.flist = cv.take<int32_t>(M);
+  t.fscan_bytes = msc::prims::scan_workspace<int32_t>(M + 1);
+  t.fscan = cv.take<char>(t.fscan_bytes);
   t.brange = cv.take<int64_t>(2 * L.nb);
   t.blk_range = cv.take<int64_t>(2 * L.nblk * L.nb);
-  t.nfall = cv.take<int64_t>(1);
   t.idx_pad = cv.take<int32_t>(static_cast<size_t>(L.n_tot) * L.K);
   t.dist_pad = cv.take<float>(static_cast<size_t>(L.n_tot) * L.K);
   return t;
@@ -1507,23 +1510,25 @@ extern "C" int msc_bbknn(const float* X, int64_t n, int d, const int32_t* batch,
   // wide for k <= 4 at half the list upkeep of the 16-per-lane lists
   MSC_TRY(launch_select(bf16, false, op.ks, op.X, nt, nt, d, nt, nb, nullptr, t.blk_range, t.b.keys, t.b.taus, st,
                         k <= kShortK));
-  const unsigned wb = static_cast<unsigned>(ceil_div(static_cast<int64_t>(nb) * nt, kThreads / kWave));
+  const int64_t M = static_cast<int64_t>(nb) * nt;
+  const unsigned wb = static_cast<unsigned>(ceil_div(M + 1, kThreads / kWave));
   finalize_sep_kernel<<<wb, kThreads, 0, st>>>(t.Xpad, nt, d, nb, t.b.keys, t.b.taus, op.qn2, op.qres, op.gstat,
                                                op.bnd, t.prow2row, t.prow_batch, t.kk, t.coff, L.K, t.idx_pad,
                                                t.dist_pad, t.flags);
   MSC_LAUNCH_CHECK();
-  fallback_sep_kernel<<<wb, kThreads, 0, st>>>(t.Xpad, nt, d, nb, t.flags, t.brange, t.kk, t.coff, L.K, t.idx_pad,
-                                               t.dist_pad);
+  // compact the uncertified (batch, slot) lists; the count stays on the device
+  MSC_TRY(prims::exclusive_scan<int32_t>(t.flags, t.fpos, M + 1, t.fscan, t.fscan_bytes, st));
+  k_flag_rows<<<static_cast<unsigned>(ceil_div(M, kThreads)), kThreads, 0, st>>>(t.flags, t.fpos, M, t.flist);
+  fallback_sep_kernel<<<kFallbackBlocks, kThreads, 0, st>>>(t.Xpad, nt, d, t.flist, t.fpos + M, t.brange, t.kk,
+                                                            t.coff, L.K, t.idx_pad, t.dist_pad);
   MSC_LAUNCH_CHECK();
   k_bbknn_combine<<<static_cast<unsigned>(ceil_div(nt, kThreads / kWave)), kThreads, 0, st>>>(
       t.idx_pad, t.dist_pad, nt, L.K, t.prow2row, out_idx, out_dist);
   MSC_LAUNCH_CHECK();
   if (out_width) *out_width = L.K;
   if (info) {
-    count_flags<<<1, kThreads, 0, st>>>(t.flags, static_cast<int64_t>(nb) * nt, t.nfall);
-    MSC_LAUNCH_CHECK();
-    int64_t m = 0;
-    MSC_HIP(hipMemcpyAsync(&m, t.nfall, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    int32_t m = 0;
+    MSC_HIP(hipMemcpyAsync(&m, t.fpos + M, sizeof(int32_t), hipMemcpyDeviceToHost, st));
     MSC_HIP(hipStreamSynchronize(st));
     info->n_fallback = m;
     info->n_second_pass = 0;
