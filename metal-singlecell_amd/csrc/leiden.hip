@@ -15,8 +15,9 @@
 //
 // Per-vertex best move: vertices of degree <= 256 take one wave (up to four neighbours per
 // lane, sorted inside the wave); degree 257..1024 and 1025..4096 take one workgroup each from
-// a list (256 / 1024 threads, aggregated in an LDS table).  Heavier vertices go through a
-// sort path over their edges keyed by (vertex, community).
+// a list (256 / 1024 threads, sorted across the workgroup).  Heavier vertices (and the list
+// ranges whose community ids would not fit the sort key: 2^22 and 2^20 vertices and up) go
+// through a sort path over their edges keyed by (vertex, community).
 #include <cmath>
 #include <vector>
 
@@ -289,8 +290,10 @@ __device__ __forceinline__ void move_one(int64_t v, int64_t e0, int64_t deg, int
 // once at the end of its run and takes the wave argmax (ties to the smaller community).
 // Requires community ids < 2^(32 - IB) so that a key fits 32 bits.
 constexpr int64_t kSortedMaxN = int64_t(1) << 24;  // NI <= 4
-constexpr int kMidDeg = 1024;                        // k_move_hash<256, 4, 11>
-constexpr int kBigDeg = 4096;                        // k_move_hash<1024, 4, 13>
+constexpr int kMidDeg = 1024;                        // k_move_block<256>: 4 slots per thread
+constexpr int64_t kMidMaxN = int64_t(1) << 22;      // community ids below 2^(32 - 10)
+constexpr int kBigDeg = 4096;                        // k_move_block<1024>
+constexpr int64_t kBigMaxN = int64_t(1) << 20;      // community ids below 2^(32 - 12)
 constexpr int kMoveGroup = 8;  // consecutive vertices per wave in k_move_light / k_frontier_push
 
 // Bitonic network over the 64 * NI slots e = i * 64 + l of a wave (keys in registers).  The
@@ -476,25 +479,75 @@ __global__ __launch_bounds__(kThreads) void k_move_light(
   }
 }
 
-// Degree 257..1024 and 1025..4096: one workgroup of
-// THREADS threads per listed vertex, ITEMS neighbours per thread.  The same decision as
-// move_sorted, by aggregation instead of sorting: every eligible neighbour adds its int64
-// weight into an LDS open-addressing table keyed by community (2^TBITS slots, twice the
-// degree cap, so a probe always ends), then every occupied slot is scored and the block
-// takes the argmax (ties to the smaller community).  The table's contents — the set of
-// (community, exact integer weight sum) — do not depend on the order of the LDS atomic
-// updates, so the decision is deterministic and equals the sorted paths'.
-template <int THREADS, int ITEMS, int TBITS>
-__global__ __launch_bounds__(THREADS) void k_move_hash(
+// Degree 257..1024 and 1025..4096: one workgroup of THREADS threads per listed vertex, 4
+// neighbour slots per thread (N = 4 THREADS).  The same decision as move_sorted, atomics-free:
+// keys (community << IB | slot) sorted by a bitonic network whose stages run where the
+// partner lives — inside the thread (stride < 4), across the wave by shuffles (stride < 256)
+// or through LDS with barriers (stride >= 256: 3 of the 55 stages at N = 1024, 10 of 78 at
+// 4096) — then the weights of each run of equal community summed by a block int64 prefix
+// scan in sorted order, each run's head found by a block max-scan, each community scored
+// once at the end of its run and the block argmax taken (ties to the smaller community).
+// Weights and sigma_tot are re-read by slot from global memory, so the LDS holds only the
+// keys and the prefix (12 N bytes) and the blocks share CUs with the light kernel.
+template <int THREADS>
+__device__ __forceinline__ int64_t block_incl_sum_i64(int64_t x, int64_t* __restrict__ wtot) {
+  const int l = lane_id(), wv = threadIdx.x / kWave;
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    const int64_t o = static_cast<int64_t>(shfl_up_u64(static_cast<uint64_t>(x), d));
+    if (l >= d) x += o;
+  }
+  if (l == kWave - 1) wtot[wv] = x;
+  __syncthreads();
+  int64_t off = 0;
+  for (int w = 0; w < wv; ++w) off += wtot[w];
+  __syncthreads();  // wtot is free for the next scan
+  return x + off;
+}
+template <int THREADS>
+__device__ __forceinline__ int32_t block_excl_max_i32(int32_t x, int32_t* __restrict__ wtot) {
+  const int l = lane_id(), wv = threadIdx.x / kWave;
+  int32_t inc = x;
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    const int32_t o = __shfl_up(inc, d, kWave);
+    if (l >= d) inc = max(inc, o);
+  }
+  if (l == kWave - 1) wtot[wv] = inc;
+  __syncthreads();
+  int32_t off = -1;
+  for (int w = 0; w < wv; ++w) off = max(off, wtot[w]);
+  __syncthreads();
+  int32_t ex = __shfl_up(inc, 1, kWave);
+  if (l == 0) ex = -1;
+  return max(ex, off);
+}
+
+// one compare-exchange of position e (value mine) with its partner p (value other) in the
+// stage of block size k: ascending where (e & k) == 0
+__device__ __forceinline__ uint32_t bitonic_pick(uint32_t mine, uint32_t other, int e, int p, int k) {
+  const uint32_t lo = mine < other ? mine : other, hi = mine < other ? other : mine;
+  return ((e < p) == ((e & k) == 0)) ? lo : hi;
+}
+
+template <int THREADS>
+__global__ __launch_bounds__(THREADS) void k_move_block(
     const int32_t* __restrict__ verts, const int64_t* __restrict__ indptr, const int32_t* __restrict__ indices,
     const int64_t* __restrict__ wq, const int64_t* __restrict__ kdeg, const int32_t* __restrict__ comm,
     const int32_t* __restrict__ part, const int64_t* __restrict__ sigtot, const uint8_t* __restrict__ active,
     GainParams gp, int32_t* __restrict__ target) {
-  constexpr int T = 1 << TBITS;
+  constexpr int ITEMS = 4;
+  constexpr int N = THREADS * ITEMS;
+  constexpr int IB = THREADS == 256 ? 10 : THREADS == 1024 ? 12 : -1;
+  static_assert(IB > 0, "THREADS must be 256 or 1024");
   constexpr int NW = THREADS / kWave;
-  static_assert(T >= 2 * THREADS * ITEMS, "table must hold twice the neighbours");
-  __shared__ int32_t tkey[T];
-  __shared__ unsigned long long tval[T];
+  constexpr int kWaveSpan = ITEMS * kWave;  // positions held by one wave
+  constexpr uint32_t kNone = 0xFFFFFFFFu;
+  constexpr uint32_t kSlot = (1u << IB) - 1u;
+  __shared__ uint32_t skey[N];
+  __shared__ int64_t lp[N];    // inclusive weight prefix, by sorted position
+  __shared__ int64_t wsum[NW];
+  __shared__ int32_t wmax[NW];
   __shared__ double wbest_g[NW];
   __shared__ int32_t wbest_c[NW];
   __shared__ double s_stay;
@@ -505,46 +558,94 @@ __global__ __launch_bounds__(THREADS) void k_move_hash(
     if (t == 0) target[v] = curr;
     return;
   }
-  for (int i = t; i < T; i += THREADS) {
-    tkey[i] = -1;
-    tval[i] = 0ull;
-  }
-  if (t == 0) s_stay = 0.0;
   const int64_t e0 = indptr[v], deg = indptr[v + 1] - e0;
   const int64_t kv = kdeg[v];
   const int32_t pv = part ? part[v] : 0;
-  __syncthreads();
+  const int eb = ITEMS * t;  // this thread's first position
+  uint32_t key[ITEMS];
 #pragma unroll
   for (int s = 0; s < ITEMS; ++s) {
-    const int e = s * THREADS + t;
-    if (e >= deg) continue;
-    const int32_t u = indices[e0 + e];
-    if (u == static_cast<int32_t>(v) || (part && part[u] != pv)) continue;
-    const int32_t c = comm[u];
-    const unsigned long long w = static_cast<unsigned long long>(wq[e0 + e]);
-    uint32_t h = (static_cast<uint32_t>(c) * 2654435761u) >> (32 - TBITS);
-    for (;;) {
-      const int32_t prev = atomicCAS(&tkey[h], -1, c);
-      if (prev == -1 || prev == c) {
-        atomicAdd(&tval[h], w);
-        break;
-      }
-      h = (h + 1) & (T - 1);
+    const int e = eb + s;
+    key[s] = kNone;
+    if (e < deg) {
+      const int32_t u = indices[e0 + e];
+      if (u != static_cast<int32_t>(v) && (!part || part[u] == pv))
+        key[s] = (static_cast<uint32_t>(comm[u]) << IB) | static_cast<uint32_t>(e);
     }
   }
+  if (t == 0) s_stay = 0.0;
+  for (int k = 2; k <= N; k <<= 1)
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      if (j >= kWaveSpan) {  // partner in another wave
+        __syncthreads();     // earlier readers of skey are done
+#pragma unroll
+        for (int s = 0; s < ITEMS; ++s) skey[eb + s] = key[s];
+        __syncthreads();
+#pragma unroll
+        for (int s = 0; s < ITEMS; ++s) key[s] = bitonic_pick(key[s], skey[(eb + s) ^ j], eb + s, (eb + s) ^ j, k);
+      } else if (j >= ITEMS) {  // partner in lane l ^ (j / ITEMS) of this wave
+#pragma unroll
+        for (int s = 0; s < ITEMS; ++s)
+          key[s] = bitonic_pick(key[s], __shfl_xor(key[s], j / ITEMS, kWave), eb + s, (eb + s) ^ j, k);
+      } else if (j == 2) {
+        const uint32_t a0 = key[0], a1 = key[1], a2 = key[2], a3 = key[3];
+        key[0] = bitonic_pick(a0, a2, eb, eb + 2, k);
+        key[2] = bitonic_pick(a2, a0, eb + 2, eb, k);
+        key[1] = bitonic_pick(a1, a3, eb + 1, eb + 3, k);
+        key[3] = bitonic_pick(a3, a1, eb + 3, eb + 1, k);
+      } else {  // j == 1
+        const uint32_t a0 = key[0], a1 = key[1], a2 = key[2], a3 = key[3];
+        key[0] = bitonic_pick(a0, a1, eb, eb + 1, k);
+        key[1] = bitonic_pick(a1, a0, eb + 1, eb, k);
+        key[2] = bitonic_pick(a2, a3, eb + 2, eb + 3, k);
+        key[3] = bitonic_pick(a3, a2, eb + 3, eb + 2, k);
+      }
+    }
+  __syncthreads();  // the last LDS stage's readers are done
+#pragma unroll
+  for (int s = 0; s < ITEMS; ++s) skey[eb + s] = key[s];
   __syncthreads();
-  double best_g = -1e300;
+  // thread t owns the sorted positions eb .. eb + 3
+  int64_t pre[ITEMS];
+  int32_t head[ITEMS];  // last run head at or before the position within this thread, or -1
+  int64_t run = 0;
+  int32_t hrun = -1;
+  const uint32_t before = t > 0 ? skey[eb - 1] : kNone;
+  const uint32_t after = eb + ITEMS < N ? skey[eb + ITEMS] : kNone;
+#pragma unroll
+  for (int s = 0; s < ITEMS; ++s) {
+    const bool valid = key[s] != kNone;
+    run += valid ? wq[e0 + (key[s] & kSlot)] : 0;
+    pre[s] = run;
+    const uint32_t prev = s > 0 ? key[s > 0 ? s - 1 : 0] : before;
+    if (valid && (eb + s == 0 || prev == kNone || (prev >> IB) != (key[s] >> IB))) hrun = eb + s;
+    head[s] = hrun;
+  }
+  const int64_t base = block_incl_sum_i64<THREADS>(run, wsum) - run;
+  const int32_t hprev = block_excl_max_i32<THREADS>(hrun, wmax);  // last head of earlier threads
+#pragma unroll
+  for (int s = 0; s < ITEMS; ++s) lp[eb + s] = base + pre[s];
+  __syncthreads();
+  double best_g = -1e300, stay = 0.0;
   int32_t best_c = 0x7FFFFFFF;
-  for (int i = t; i < T; i += THREADS) {
-    const int32_t c = tkey[i];
-    if (c < 0) continue;
-    const int64_t wc = static_cast<int64_t>(tval[i]);
-    const int64_t sig = sigtot[c] - (c == curr ? kv : 0);
+  bool has_stay = false;
+#pragma unroll
+  for (int s = 0; s < ITEMS; ++s) {
+    if (key[s] == kNone) continue;
+    const uint32_t next = s + 1 < ITEMS ? key[s + 1 < ITEMS ? s + 1 : s] : after;
+    const int32_t ci = static_cast<int32_t>(key[s] >> IB);
+    if (next != kNone && static_cast<int32_t>(next >> IB) == ci) continue;  // not the end of its run
+    const int32_t h = head[s] >= 0 ? head[s] : hprev;
+    const int64_t wc = lp[eb + s] - (h > 0 ? lp[h - 1] : 0);
+    const int64_t sig = sigtot[ci] - (ci == curr ? kv : 0);
     const double g = gain_of(wc, sig, kv, gp);
-    if (c == curr) s_stay = g;  // one slot holds the current community
-    if (better(g, c, best_g, best_c)) {
+    if (ci == curr) {
+      stay = g;
+      has_stay = true;
+    }
+    if (better(g, ci, best_g, best_c)) {
       best_g = g;
-      best_c = c;
+      best_c = ci;
     }
   }
 #pragma unroll
@@ -560,6 +661,7 @@ __global__ __launch_bounds__(THREADS) void k_move_hash(
     wbest_g[wv] = best_g;
     wbest_c[wv] = best_c;
   }
+  if (has_stay) s_stay = stay;  // one run holds the current community
   __syncthreads();
   if (t == 0) {
     double bg = wbest_g[0];
@@ -990,8 +1092,10 @@ static int graph_setup(Ctx& c, Graph& g) {
   c.nm = 0;
   c.nb = 0;
   if (g.max_deg > kLightDeg) {
-    // the listed tiers aggregate by community in LDS tables: any community id fits
-    k_heavy_flags<<<grid_for(g.n + 1), kThreads, 0, st>>>(g.indptr, g.n, kMidDeg, kBigDeg, c.hflag, c.hdeg, c.mflag,
+    // community ids must fit the sort keys: below 2^22 for the mid tier, 2^20 for the big one
+    const int64_t mid_max = g.n < kMidMaxN ? kMidDeg : kLightDeg;
+    const int64_t big_max = g.n < kBigMaxN ? kBigDeg : mid_max;
+    k_heavy_flags<<<grid_for(g.n + 1), kThreads, 0, st>>>(g.indptr, g.n, mid_max, big_max, c.hflag, c.hdeg, c.mflag,
                                                           c.bflag);
     MSC_LAUNCH_CHECK();
     MSC_TRY(prims::exclusive_scan<int32_t>(c.mflag, c.mpos, g.n + 1, c.scan_ws, c.scan_ws_bytes, st));
@@ -1091,12 +1195,12 @@ static int local_moving(Ctx& c, const Graph& g, int64_t seed, const int32_t* par
       if (c.nm > 0) MSC_HIP(hipStreamWaitEvent(c.side2, c.ev_fork, 0));
     }
     if (c.nb > 0) {
-      k_move_hash<1024, 4, 13><<<static_cast<unsigned>(c.nb), 1024, 0, c.side>>>(
+      k_move_block<1024><<<static_cast<unsigned>(c.nb), 1024, 0, c.side>>>(
           c.bverts, g.indptr, g.indices, g.wq, g.k, c.comm, part, c.sigtot, c.active, gp, c.target);
       MSC_LAUNCH_CHECK();
     }
     if (c.nm > 0) {
-      k_move_hash<256, 4, 11><<<static_cast<unsigned>(c.nm), 256, 0, c.side2>>>(
+      k_move_block<256><<<static_cast<unsigned>(c.nm), 256, 0, c.side2>>>(
           c.mverts, g.indptr, g.indices, g.wq, g.k, c.comm, part, c.sigtot, c.active, gp, c.target);
       MSC_LAUNCH_CHECK();
     }
