@@ -85,3 +85,18 @@ def test_leiden_init_labels_and_single_iteration(c1_data):
     lab1, _, _ = _gpu_leiden(P, n_iterations=1)
     ref1 = leiden.leiden(P, n_iterations=1)
     assert np.array_equal(lab1, ref1)
+
+
+def test_large_graph_70k():
+    """70,000 cells, 12 clusters, k = 15 (1M-edge graph): Leiden and Louvain labels and Q equal
+    the oracle's."""
+    from metalsinglecell import _hip, synthetic
+    from oracle import leiden
+    X, _ = synthetic.gaussian_mixture_isotropic(n=70_000, d=20, n_clusters=12, seed=3)
+    P = _knn_graph(X, 15)
+    assert P.shape[0] >= 1 << 16
+    _check(P)
+    lab, q, _ = _hip.louvain(P.indptr, P.indices, P.data, P.shape[0])
+    ref = leiden.louvain(P)
+    assert np.array_equal(lab.cpu().numpy().astype(np.int64), ref)
+    assert q == leiden.modularity_of_labels(P, ref)
