@@ -144,3 +144,19 @@ def test_knn_lane_partition_rescued_by_wide_pass(precision):
     assert info["n_second_pass"] >= 21 * n_cl[0]
     assert info["n_fallback"] == 0
     assert np.array_equal(idx, ri) and np.array_equal(dist, rd)
+
+
+@pytest.mark.parametrize("d", [1, 6, 7, 14, 22, 30, 46, 50, 61, 77, 93, 109, 125])
+def test_knn_bf16_every_k_step_shape(d):
+    """BF16 operand rows of 2d + 3 values padded to 32-element MFMA steps (d = 1..125 covers
+    every step count 1..8, several at a step's last full d) and FP32 rows of d + 1
+    values in 8-element steps; both precisions equal the exact oracle."""
+    from metalsinglecell import _hip
+    rng = np.random.default_rng(d)
+    centers = rng.normal(0, 4.0, (12, d))
+    X = (centers[rng.integers(0, 12, 1500)] + rng.normal(0, 1.0, (1500, d))).astype(np.float32)
+    ri, rd = _oracle(X, 10)
+    for precision in ("bf16", "fp32"):
+        idx, dist = _hip.knn_exact(X, 10, precision=precision)
+        assert np.array_equal(idx.cpu().numpy().astype(np.int64), ri), (d, precision)
+        assert np.array_equal(dist.cpu().numpy(), rd), (d, precision)
