@@ -178,6 +178,9 @@ typedef struct {
   int32_t move_passes, refine_passes;
   int64_t n_comms, n_refined;
   double modularity;
+  /* passes of each phase whose sigma_tot came from the previous pass's committed vertices
+   * alone (the rest rebuild it over all vertices); both forms give the same exact sums */
+  int32_t move_delta_passes, refine_delta_passes;
 } msc_leiden_level;
 
 typedef struct {

@@ -836,29 +836,157 @@ __global__ __launch_bounds__(kThreads) void k_heavy_argmax(
 // ------------------------------------------------------------------------------------------
 // commit, frontier, relabel, modularity, contraction
 // ------------------------------------------------------------------------------------------
-__global__ void k_commit(int32_t* __restrict__ comm, const int32_t* __restrict__ target, int64_t n, uint64_t pkey,
-                         uint64_t thr, uint8_t* __restrict__ commit, uint8_t* __restrict__ held,
-                         int32_t* __restrict__ wants_flag) {
-  const int64_t v = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
-  if (v >= n) return;
-  const int32_t t = target[v], c = comm[v];
-  const bool wants = t != c;
-  const bool coin = static_cast<uint64_t>(coin_u32(pkey, static_cast<uint64_t>(v))) < thr;
-  const bool cm = wants && coin;
-  if (cm) comm[v] = t;
-  commit[v] = cm;
-  held[v] = wants && !coin;
-  if (wants) *wants_flag = 1;  // every writer stores the same value
+// Commit the coin-approved moves (louvain.py:355-362) and start the next active set
+// (louvain.py:364-372): commit | held = (wants & coin) | (wants & ~coin) = wants, so the
+// vertex's own part is its wants bit; k_frontier_push adds N(commit) afterwards.  Each
+// block also lists its committed vertices (id, previous community) in vertex order at
+// [blockIdx.x * kCommitThreads, + mv_cnt[blockIdx.x]) for k_sigtot_delta.
+constexpr int kCommitThreads = 1024;
+__global__ __launch_bounds__(kCommitThreads) void k_commit(
+    int32_t* __restrict__ comm, const int32_t* __restrict__ target, int64_t n, uint64_t pkey, uint64_t thr,
+    uint8_t* __restrict__ commit, uint8_t* __restrict__ active, int32_t* __restrict__ wants_flag,
+    int32_t* __restrict__ mv_v, int32_t* __restrict__ mv_old, int32_t* __restrict__ mv_cnt) {
+  __shared__ int wcnt[kCommitThreads / kWave];
+  const int64_t v = static_cast<int64_t>(blockIdx.x) * kCommitThreads + threadIdx.x;
+  bool cm = false;
+  int32_t c = 0;
+  if (v < n) {
+    const int32_t t = target[v];
+    c = comm[v];
+    const bool wants = t != c;
+    const bool coin = static_cast<uint64_t>(coin_u32(pkey, static_cast<uint64_t>(v))) < thr;
+    cm = wants && coin;
+    if (cm) comm[v] = t;
+    commit[v] = cm;
+    active[v] = wants;
+    if (wants) *wants_flag = 1;  // every writer stores the same value
+  }
+  const int w = threadIdx.x / kWave, l = lane_id();
+  const uint64_t b = __ballot(cm);
+  if (l == 0) wcnt[w] = __popcll(b);
+  __syncthreads();
+  int base = 0, total = 0;
+#pragma unroll
+  for (int i = 0; i < kCommitThreads / kWave; ++i) {
+    const int x = wcnt[i];
+    base += (i < w) ? x : 0;
+    total += x;
+  }
+  if (cm) {
+    const int64_t slot = static_cast<int64_t>(blockIdx.x) * kCommitThreads + base +
+                         __popcll(b & ((uint64_t(1) << l) - 1));
+    mv_v[slot] = static_cast<int32_t>(v);
+    mv_old[slot] = c;
+  }
+  if (threadIdx.x == 0) mv_cnt[blockIdx.x] = total;
+}
+
+// Incremental sigma_tot after a commit, atomics-free: each committed vertex v contributes
+// (previous community, -k_v) and (new community, +k_v).  The entries are taken kDeltaChunk
+// vertices at a time in vertex order, bitonic-sorted by community in LDS, and the first
+// entry of each equal-community run adds the run's exact int64 sum to sigtot.  Chunks run
+// one after another inside this single workgroup (barrier between them), so a community
+// is written by one thread at a time, and the integer result equals a full recomputation.
+// The moved total goes to *moved_out; apply = 0 stops there.
+constexpr int kDeltaThreads = 1024;
+constexpr int kDeltaChunk = 1024;          // committed vertices per sort (2048 entries)
+constexpr int kDeltaMaxBlocks = 8192;      // commit blocks: graphs up to 8,388,608 vertices
+constexpr int64_t kDeltaSwitch = 8192;     // moved count at a sync below which passes go incremental
+__global__ __launch_bounds__(kDeltaThreads) void k_sigtot_delta(
+    const int32_t* __restrict__ mv_cnt, int nblk, const int32_t* __restrict__ mv_v,
+    const int32_t* __restrict__ mv_old, const int32_t* __restrict__ comm, const int64_t* __restrict__ kdeg,
+    int64_t* __restrict__ sigtot, int apply, int32_t* __restrict__ moved_out) {
+  __shared__ int32_t pre[kDeltaMaxBlocks + 1];
+  __shared__ int32_t key[2 * kDeltaChunk];
+  __shared__ int64_t val[2 * kDeltaChunk];
+  __shared__ int32_t wsum[kDeltaThreads / kWave];
+  const int t = threadIdx.x, w = t / kWave, l = lane_id();
+  // exclusive prefix of the per-block counts; thread t owns a contiguous run of blocks
+  const int per = (nblk + kDeltaThreads - 1) / kDeltaThreads;
+  const int b0 = min(t * per, nblk), b1 = min(b0 + per, nblk);
+  int s = 0;
+  for (int b = b0; b < b1; ++b) s += mv_cnt[b];
+  int x = s;
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    const int y = __shfl_up(x, d, kWave);
+    if (l >= d) x += y;
+  }
+  if (l == kWave - 1) wsum[w] = x;
+  __syncthreads();
+  int run = x - s, total = 0;
+#pragma unroll
+  for (int i = 0; i < kDeltaThreads / kWave; ++i) {
+    run += (i < w) ? wsum[i] : 0;
+    total += wsum[i];
+  }
+  for (int b = b0; b < b1; ++b) {
+    pre[b] = run;
+    run += mv_cnt[b];
+  }
+  if (t == 0) {
+    pre[nblk] = total;
+    *moved_out = total;
+  }
+  __syncthreads();
+  if (!apply) return;
+  for (int s0 = 0; s0 < total; s0 += kDeltaChunk) {
+    const int cnt = min(kDeltaChunk, total - s0);
+    int n2 = 2;
+    while (n2 < 2 * cnt) n2 <<= 1;
+    if (t < n2 / 2) {
+      int32_t k0 = INT32_MAX, k1 = INT32_MAX;  // sentinels sort last (ids are < n <= INT32_MAX)
+      int64_t v0 = 0, v1 = 0;
+      if (t < cnt) {
+        const int e = s0 + t;
+        int lo = 0, hi = nblk - 1;  // last block with pre[b] <= e: it holds e
+        while (lo < hi) {
+          const int mid = (lo + hi + 1) >> 1;
+          if (pre[mid] <= e) lo = mid;
+          else hi = mid - 1;
+        }
+        const int64_t slot = static_cast<int64_t>(lo) * kCommitThreads + (e - pre[lo]);
+        const int32_t v = mv_v[slot];
+        const int64_t kv = kdeg[v];
+        k0 = mv_old[slot];
+        v0 = -kv;
+        k1 = comm[v];
+        v1 = kv;
+      }
+      key[2 * t] = k0; val[2 * t] = v0;
+      key[2 * t + 1] = k1; val[2 * t + 1] = v1;
+    }
+    __syncthreads();
+    for (int size = 2; size <= n2; size <<= 1) {
+      for (int stride = size >> 1; stride > 0; stride >>= 1) {
+        if (t < n2 / 2) {
+          const int i = 2 * t - (t & (stride - 1));
+          const int j = i + stride;
+          const int32_t ki = key[i], kj = key[j];
+          if ((ki > kj) == ((i & size) == 0)) {
+            key[i] = kj; key[j] = ki;
+            const int64_t vi = val[i];
+            val[i] = val[j]; val[j] = vi;
+          }
+        }
+        __syncthreads();
+      }
+    }
+    for (int i = t; i < n2; i += kDeltaThreads) {
+      const int32_t ki = key[i];
+      if (ki != INT32_MAX && (i == 0 || key[i - 1] != ki)) {
+        int64_t sum = 0;
+        for (int j = i; j < n2 && key[j] == ki; ++j) sum += val[j];
+        sigtot[ki] += sum;
+      }
+    }
+    __syncthreads();
+  }
 }
 
 // next active set = commit | held | N(commit) (louvain.py:364-372); the graph stores both
 // directions, so marking the neighbours of each committed vertex equals testing every
 // vertex's neighbours.  Concurrent marks all store 1.
-__global__ void k_frontier_self(int64_t n, const uint8_t* __restrict__ commit, const uint8_t* __restrict__ held,
-                                uint8_t* __restrict__ active) {
-  const int64_t v = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
-  if (v < n) active[v] = (commit[v] || held[v]) ? 1 : 0;
-}
 // one wave per kMoveGroup consecutive vertices; only committed ones touch their edges
 __global__ __launch_bounds__(kThreads) void k_frontier_push(const int64_t* __restrict__ indptr,
                                                             const int32_t* __restrict__ indices, int64_t n,
@@ -987,10 +1115,11 @@ struct Ctx {
   double inv_s;
   // per-vertex (n0)
   int32_t *comm, *target, *P, *R, *Pagg, *orig2node, *final_lab, *prev_lab, *dense, *used, *map, *tmp_lab;
-  uint8_t *active, *commit, *held;
+  uint8_t *active, *commit;
   int64_t *sigtot, *iw, *tot, *inw, *kbuf[2];
   double* terms;
-  int32_t* wants_flags;     // kMaxPasses
+  int32_t* pass_info;       // 2 * kMaxPasses: per pass (any vertex wanted to move, moved count)
+  int32_t *mv_v, *mv_old, *mv_cnt;  // committed vertices per commit block (k_commit)
   int64_t* red;             // small reduction outputs
   float* fred;
   // graph slots
@@ -1023,11 +1152,13 @@ static void carve_ctx(Carve& cv, Ctx& c, int64_t n, int64_t E) {
   c.R = cv.take<int32_t>(n1); c.Pagg = cv.take<int32_t>(n1); c.orig2node = cv.take<int32_t>(n1);
   c.final_lab = cv.take<int32_t>(n1); c.prev_lab = cv.take<int32_t>(n1); c.dense = cv.take<int32_t>(n1);
   c.used = cv.take<int32_t>(n1); c.map = cv.take<int32_t>(n1); c.tmp_lab = cv.take<int32_t>(n1);
-  c.active = cv.take<uint8_t>(n1); c.commit = cv.take<uint8_t>(n1); c.held = cv.take<uint8_t>(n1);
+  c.active = cv.take<uint8_t>(n1); c.commit = cv.take<uint8_t>(n1);
   c.sigtot = cv.take<int64_t>(n1); c.iw = cv.take<int64_t>(n1); c.tot = cv.take<int64_t>(n1);
   c.inw = cv.take<int64_t>(n1); c.kbuf[0] = cv.take<int64_t>(n1); c.kbuf[1] = cv.take<int64_t>(n1);
   c.terms = cv.take<double>(n1);
-  c.wants_flags = cv.take<int32_t>(kMaxPasses);
+  c.pass_info = cv.take<int32_t>(2 * kMaxPasses);
+  c.mv_v = cv.take<int32_t>(n1); c.mv_old = cv.take<int32_t>(n1);
+  c.mv_cnt = cv.take<int32_t>(ceil_div(n1, kCommitThreads));
   c.red = cv.take<int64_t>(8);
   c.fred = cv.take<float>(kMaxBlocks + 8);
   for (int i = 0; i < 2; ++i) {
@@ -1166,7 +1297,8 @@ static int heavy_moves(Ctx& c, const Graph& g, const int32_t* comm, const int32_
 }
 
 // synchronous local moving (part == nullptr) or refinement; comm updated in place
-static int local_moving(Ctx& c, const Graph& g, int64_t seed, const int32_t* part, int* passes_out) {
+static int local_moving(Ctx& c, const Graph& g, int64_t seed, const int32_t* part, int* passes_out,
+                        int* delta_passes_out) {
   hipStream_t st = c.st;
   GainParams gp;
   gp.inv_s = c.inv_s;
@@ -1174,11 +1306,19 @@ static int local_moving(Ctx& c, const Graph& g, int64_t seed, const int32_t* par
   gp.gamma = c.gamma;
   const uint64_t key = stream_key(seed, part ? 1 : 0);
   k_fill_u8<<<grid_for(g.n), kThreads, 0, st>>>(c.active, g.n, 1);
-  k_fill_i32<<<grid_for(kMaxPasses), kThreads, 0, st>>>(c.wants_flags, kMaxPasses, 0);
+  k_fill_i32<<<grid_for(2 * kMaxPasses), kThreads, 0, st>>>(c.pass_info, 2 * kMaxPasses, 0);
   MSC_LAUNCH_CHECK();
+  // sigma_tot is rebuilt from scratch (sort + segmented sums) at a pass start unless the
+  // previous pass updated it from its committed vertices (k_sigtot_delta).  The host picks
+  // the incremental form at each sync from the last moved count: the commit counts decay
+  // over the passes, and the late passes move few vertices.
+  const int nblk = static_cast<int>(ceil_div(g.n, kCommitThreads));
+  const bool can_inc = nblk <= kDeltaMaxBlocks;
+  bool inc = false, sig_valid = false;
+  int n_inc = 0;
   int p = 0;
   for (; p < kMaxPasses; ++p) {
-    MSC_TRY(sigma_tot(c, g, c.comm));
+    if (!sig_valid) MSC_TRY(sigma_tot(c, g, c.comm));
     // the three move kernels write target[] of disjoint vertex sets and only read the rest:
     // the listed (mid / big) vertices run on the side stream concurrently with the light ones
     const bool fork = c.nm > 0 || c.nb > 0;
@@ -1217,23 +1357,35 @@ static int local_moving(Ctx& c, const Graph& g, int64_t seed, const int32_t* par
       MSC_HIP(hipStreamWaitEvent(st, c.ev_join2, 0));
     }
     MSC_TRY(heavy_moves(c, g, c.comm, part, gp));
-    k_commit<<<grid_for(g.n), kThreads, 0, st>>>(c.comm, c.target, g.n, pass_key(key, p), c.thr, c.commit, c.held,
-                                                 c.wants_flags + p);
+    k_commit<<<static_cast<unsigned>(nblk), kCommitThreads, 0, st>>>(
+        c.comm, c.target, g.n, pass_key(key, p), c.thr, c.commit, c.active, c.pass_info + 2 * p, c.mv_v, c.mv_old,
+        c.mv_cnt);
     MSC_LAUNCH_CHECK();
-    k_frontier_self<<<grid_for(g.n), kThreads, 0, st>>>(g.n, c.commit, c.held, c.active);
+    const bool sync = (p % kSyncEvery == kSyncEvery - 1) || p == kMaxPasses - 1;
+    // incremental passes update sigma_tot; a full-mode sync pass only counts the moves
+    if (inc || (sync && can_inc)) {
+      k_sigtot_delta<<<1, kDeltaThreads, 0, st>>>(c.mv_cnt, nblk, c.mv_v, c.mv_old, c.comm, g.k, c.sigtot,
+                                                  inc ? 1 : 0, c.pass_info + 2 * p + 1);
+      MSC_LAUNCH_CHECK();
+    }
+    sig_valid = inc;
+    n_inc += inc ? 1 : 0;
     k_frontier_push<<<grid_for(ceil_div(g.n, kMoveGroup) * kWave), kThreads, 0, st>>>(g.indptr, g.indices, g.n,
                                                                                      c.commit, c.active);
     MSC_LAUNCH_CHECK();
-    if ((p % kSyncEvery == kSyncEvery - 1) || p == kMaxPasses - 1) {
-      int32_t any;
-      MSC_TRY(read_scalar(c.wants_flags + p, &any, st));
-      if (!any) {
+    if (sync) {
+      int32_t info[2];  // (any vertex wanted to move, moved count)
+      MSC_HIP(hipMemcpyAsync(info, c.pass_info + 2 * p, sizeof(info), hipMemcpyDeviceToHost, st));
+      MSC_HIP(hipStreamSynchronize(st));
+      if (!info[0]) {
         ++p;
         break;
       }
+      inc = can_inc && info[1] <= kDeltaSwitch;
     }
   }
   *passes_out = std::min(p, kMaxPasses);
+  if (delta_passes_out) *delta_passes_out = n_inc;
   return kOk;
 }
 
@@ -1320,8 +1472,8 @@ static int leiden_pass(Ctx& c, Graph& g0, int64_t seed, const int32_t* init) {
     k_gather_i32<<<grid_for(n0), kThreads, 0, st>>>(c.P, c.orig2node, c.prev_lab, n0);
     k_copy_i32<<<grid_for(g.n), kThreads, 0, st>>>(c.P, c.comm, g.n);
     MSC_LAUNCH_CHECK();
-    int mp = 0, rp = 0;
-    MSC_TRY(local_moving(c, g, seed + level, nullptr, &mp));
+    int mp = 0, rp = 0, md = 0, rd = 0;
+    MSC_TRY(local_moving(c, g, seed + level, nullptr, &mp, &md));
     int64_t ncomm;
     MSC_TRY(dense_relabel(c, c.comm, g.n, c.P, &ncomm));
     double q;
@@ -1331,6 +1483,7 @@ static int leiden_pass(Ctx& c, Graph& g0, int64_t seed, const int32_t* init) {
       rec = &c.info->levels[c.info->n_levels++];
       rec->n = g.n; rec->n_edges = g.E; rec->move_passes = mp; rec->refine_passes = 0;
       rec->n_comms = ncomm; rec->n_refined = 0; rec->modularity = q;
+      rec->move_delta_passes = md; rec->refine_delta_passes = 0;
     }
     if (q <= q_prev + kTol) {
       k_copy_i32<<<grid_for(n0), kThreads, 0, st>>>(c.prev_lab, c.final_lab, n0);
@@ -1343,10 +1496,10 @@ static int leiden_pass(Ctx& c, Graph& g0, int64_t seed, const int32_t* init) {
     if (ncomm == g.n) break;
     k_iota_i32<<<grid_for(g.n), kThreads, 0, st>>>(c.comm, g.n);
     MSC_LAUNCH_CHECK();
-    MSC_TRY(local_moving(c, g, seed + level, c.P, &rp));
+    MSC_TRY(local_moving(c, g, seed + level, c.P, &rp, &rd));
     int64_t nR;
     MSC_TRY(dense_relabel(c, c.comm, g.n, c.R, &nR));
-    if (rec) { rec->refine_passes = rp; rec->n_refined = nR; }
+    if (rec) { rec->refine_passes = rp; rec->n_refined = nR; rec->refine_delta_passes = rd; }
     if (nR == g.n) break;
     k_scatter_pagg<<<grid_for(g.n), kThreads, 0, st>>>(c.R, c.P, g.n, c.Pagg);
     k_gather_i32<<<grid_for(n0), kThreads, 0, st>>>(c.R, c.orig2node, c.tmp_lab, n0);
@@ -1379,8 +1532,8 @@ static int louvain_pass(Ctx& c, Graph& g0, int64_t seed, int max_levels) {
   for (int level = 0; level < max_levels; ++level) {
     k_iota_i32<<<grid_for(g.n), kThreads, 0, st>>>(c.comm, g.n);
     MSC_LAUNCH_CHECK();
-    int mp = 0;
-    MSC_TRY(local_moving(c, g, seed + 100 * static_cast<int64_t>(level), nullptr, &mp));
+    int mp = 0, md = 0;
+    MSC_TRY(local_moving(c, g, seed + 100 * static_cast<int64_t>(level), nullptr, &mp, &md));
     int64_t ncomm;
     MSC_TRY(dense_relabel(c, c.comm, g.n, c.P, &ncomm));
     double q;
@@ -1389,6 +1542,7 @@ static int louvain_pass(Ctx& c, Graph& g0, int64_t seed, int max_levels) {
       msc_leiden_level* rec = &c.info->levels[c.info->n_levels++];
       rec->n = g.n; rec->n_edges = g.E; rec->move_passes = mp; rec->refine_passes = 0;
       rec->n_comms = ncomm; rec->n_refined = ncomm; rec->modularity = q;
+      rec->move_delta_passes = md; rec->refine_delta_passes = 0;
     }
     if (ncomm == g.n || q <= q_prev + kTol) break;
     k_gather_i32<<<grid_for(n0), kThreads, 0, st>>>(c.P, c.orig2node, c.tmp_lab, n0);

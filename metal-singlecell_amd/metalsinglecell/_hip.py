@@ -29,7 +29,8 @@ c_i64, c_int, c_vp, c_sz, c_dbl = ctypes.c_int64, ctypes.c_int, ctypes.c_void_p,
 class LeidenLevel(ctypes.Structure):
     _fields_ = [("n", c_i64), ("n_edges", c_i64), ("move_passes", ctypes.c_int32),
                 ("refine_passes", ctypes.c_int32), ("n_comms", c_i64), ("n_refined", c_i64),
-                ("modularity", c_dbl)]
+                ("modularity", c_dbl), ("move_delta_passes", ctypes.c_int32),
+                ("refine_delta_passes", ctypes.c_int32)]
 
 
 class LeidenInfo(ctypes.Structure):

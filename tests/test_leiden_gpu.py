@@ -26,7 +26,7 @@ def _check(P, **kw):
     rq = leiden.modularity_of_labels(P, ref, kw.get("resolution", 1.0))
     assert np.array_equal(lab, ref), (info["levels"], stats)
     assert q == rq
-    return lab, q
+    return lab, q, info
 
 
 def _knn_graph(X, k):
@@ -41,7 +41,7 @@ def test_leiden_mixture_3000(c1_data):
     from sklearn.metrics import adjusted_rand_score
     X, lab_true = c1_data
     P = _knn_graph(X, 15)
-    lab, q = _check(P)
+    lab, q, _ = _check(P)
     assert adjusted_rand_score(lab_true, lab) > 0.95
 
 
@@ -95,8 +95,13 @@ def test_large_graph_70k():
     X, _ = synthetic.gaussian_mixture_isotropic(n=70_000, d=20, n_clusters=12, seed=3)
     P = _knn_graph(X, 15)
     assert P.shape[0] >= 1 << 16
-    _check(P)
-    lab, q, _ = _hip.louvain(P.indptr, P.indices, P.data, P.shape[0])
+    _, _, info = _check(P)
+    # the late passes of both level-0 phases took sigma_tot from the committed vertices only
+    top = info["levels"][0]
+    assert 0 < top["move_delta_passes"] < top["move_passes"]
+    assert 0 < top["refine_delta_passes"] < top["refine_passes"]
+    lab, q, linfo = _hip.louvain(P.indptr, P.indices, P.data, P.shape[0])
     ref = leiden.louvain(P)
     assert np.array_equal(lab.cpu().numpy().astype(np.int64), ref)
     assert q == leiden.modularity_of_labels(P, ref)
+    assert linfo["levels"][0]["move_delta_passes"] > 0
