@@ -47,6 +47,14 @@ struct Graph {
   int64_t max_deg = 0;
 };
 
+// Per-vertex record for the light move kernel: a neighbour's community, refinement part
+// and that community's sigma_tot in one 16-byte gather (instead of three dependent ones).
+struct alignas(16) CommSig {
+  int32_t comm;
+  int32_t part;
+  int64_t sig;
+};
+
 struct GainParams {
   double inv_s;   // 2^-s
   double inv;     // 1 / (twom * 2^-s)
@@ -334,10 +342,10 @@ __device__ __forceinline__ void bitonic_sort_slots(uint32_t (&key)[NI], int l) {
 template <int NI>
 __device__ __forceinline__ void move_sorted(int64_t v, int64_t e0, int64_t deg, int32_t curr, int64_t kv,
                                             const int32_t* __restrict__ indices, const int64_t* __restrict__ wq,
-                                            const int32_t* __restrict__ comm, const int32_t* __restrict__ part,
-                                            const int64_t* __restrict__ sigtot, const GainParams& gp,
-                                            int32_t* __restrict__ target, int64_t* __restrict__ lw,
-                                            int64_t* __restrict__ lsg, int64_t* __restrict__ lp) {
+                                            const CommSig* __restrict__ cs, const int32_t* __restrict__ part,
+                                            const GainParams& gp, int32_t* __restrict__ target,
+                                            int64_t* __restrict__ lw, int64_t* __restrict__ lsg,
+                                            int64_t* __restrict__ lp) {
   constexpr int IB = NI == 1 ? 6 : NI == 2 ? 7 : 8;
   static_assert(NI == 1 || NI == 2 || NI == 4, "NI");
   constexpr uint32_t kNone = 0xFFFFFFFFu;
@@ -349,21 +357,21 @@ __device__ __forceinline__ void move_sorted(int64_t v, int64_t e0, int64_t deg, 
   for (int i = 0; i < NI; ++i) {
     const int64_t j = l + kWave * i;
     key[i] = kNone;
-    int64_t w = 0;
+    int64_t w = 0, sg = 0;
     if (j < deg) {
       const int32_t u = indices[e0 + j];
-      if (u != static_cast<int32_t>(v) && (!part || part[u] == pv)) {
-        const int32_t c = comm[u];
+      const int4 r = reinterpret_cast<const int4*>(cs)[u];  // (comm, part, sigtot lo, hi)
+      if (u != static_cast<int32_t>(v) && (!part || r.y == pv)) {
         w = wq[e0 + j];
-        key[i] = (static_cast<uint32_t>(c) << IB) | static_cast<uint32_t>(i * kWave + l);
+        sg = static_cast<int64_t>((static_cast<uint64_t>(static_cast<uint32_t>(r.w)) << 32) |
+                                  static_cast<uint32_t>(r.z));
+        key[i] = (static_cast<uint32_t>(r.x) << IB) | static_cast<uint32_t>(i * kWave + l);
       }
     }
     lw[i * kWave + l] = w;
+    lsg[i * kWave + l] = sg;
     nvalid += __popcll(__ballot(key[i] != kNone));
   }
-#pragma unroll
-  for (int i = 0; i < NI; ++i)
-    lsg[i * kWave + l] = key[i] != kNone ? sigtot[key[i] >> IB] : 0;
 
   bitonic_sort_slots<NI, 2>(key, l);  // ascending in slot order e = i * 64 + l
 
@@ -434,8 +442,9 @@ __device__ __forceinline__ void move_vertex(int64_t v, const int64_t* __restrict
                                             const int32_t* __restrict__ indices, const int64_t* __restrict__ wq,
                                             const int64_t* __restrict__ kdeg, int64_t n,
                                             const int32_t* __restrict__ comm, const int32_t* __restrict__ part,
-                                            const int64_t* __restrict__ sigtot, const GainParams& gp,
-                                            int32_t* __restrict__ target, int64_t (*lds)[kLightDeg]) {
+                                            const int64_t* __restrict__ sigtot, const CommSig* __restrict__ cs,
+                                            const GainParams& gp, int32_t* __restrict__ target,
+                                            int64_t (*lds)[kLightDeg]) {
   const int64_t e0 = indptr[v], deg = indptr[v + 1] - e0;
   const int32_t curr = comm[v];
   const int64_t kv = kdeg[v];
@@ -443,10 +452,9 @@ __device__ __forceinline__ void move_vertex(int64_t v, const int64_t* __restrict
     int64_t* lw = lds[0];
     int64_t* lsg = lds[1];
     int64_t* lp = lds[2];
-    if (deg <= kWave) move_sorted<1>(v, e0, deg, curr, kv, indices, wq, comm, part, sigtot, gp, target, lw, lsg, lp);
-    else if (deg <= 2 * kWave)
-      move_sorted<2>(v, e0, deg, curr, kv, indices, wq, comm, part, sigtot, gp, target, lw, lsg, lp);
-    else move_sorted<4>(v, e0, deg, curr, kv, indices, wq, comm, part, sigtot, gp, target, lw, lsg, lp);
+    if (deg <= kWave) move_sorted<1>(v, e0, deg, curr, kv, indices, wq, cs, part, gp, target, lw, lsg, lp);
+    else if (deg <= 2 * kWave) move_sorted<2>(v, e0, deg, curr, kv, indices, wq, cs, part, gp, target, lw, lsg, lp);
+    else move_sorted<4>(v, e0, deg, curr, kv, indices, wq, cs, part, gp, target, lw, lsg, lp);
   } else {
     if (deg <= kWave) move_one<1>(v, e0, deg, curr, kv, indices, wq, comm, part, sigtot, gp, target);
     else if (deg <= 2 * kWave) move_one<2>(v, e0, deg, curr, kv, indices, wq, comm, part, sigtot, gp, target);
@@ -457,8 +465,8 @@ __device__ __forceinline__ void move_vertex(int64_t v, const int64_t* __restrict
 __global__ __launch_bounds__(kThreads) void k_move_light(
     const int64_t* __restrict__ indptr, const int32_t* __restrict__ indices, const int64_t* __restrict__ wq,
     const int64_t* __restrict__ kdeg, int64_t n, const int32_t* __restrict__ comm,
-    const int32_t* __restrict__ part, const int64_t* __restrict__ sigtot, const uint8_t* __restrict__ active,
-    GainParams gp, int32_t* __restrict__ target) {
+    const int32_t* __restrict__ part, const int64_t* __restrict__ sigtot, const CommSig* __restrict__ cs,
+    const uint8_t* __restrict__ active, GainParams gp, int32_t* __restrict__ target) {
   __shared__ int64_t lds[kWavesPerBlock][3][kLightDeg];
   // each wave owns kMoveGroup consecutive vertices: inactive ones keep their community
   // (one coalesced store), active light ones are moved one after another
@@ -475,7 +483,7 @@ __global__ __launch_bounds__(kThreads) void k_move_light(
   while (pending) {
     const int64_t v = v0 + __ffsll(static_cast<long long>(pending)) - 1;
     pending &= pending - 1;
-    move_vertex(v, indptr, indices, wq, kdeg, n, comm, part, sigtot, gp, target, lds[threadIdx.x / kWave]);
+    move_vertex(v, indptr, indices, wq, kdeg, n, comm, part, sigtot, cs, gp, target, lds[threadIdx.x / kWave]);
   }
 }
 
@@ -984,16 +992,38 @@ __global__ __launch_bounds__(kDeltaThreads) void k_sigtot_delta(
   }
 }
 
+__device__ __forceinline__ void refresh_one(int64_t v, const int32_t* __restrict__ comm,
+                                            const int32_t* __restrict__ part, const int64_t* __restrict__ sigtot,
+                                            CommSig* __restrict__ cs) {
+  const int32_t c = comm[v];
+  const int64_t s = sigtot[c];
+  reinterpret_cast<int4*>(cs)[v] = make_int4(c, part ? part[v] : 0, static_cast<int32_t>(s),
+                                             static_cast<int32_t>(static_cast<uint64_t>(s) >> 32));
+}
+// cs[v] = (comm[v], part[v], sigtot[comm[v]]) after sigma_tot has been rebuilt
+__global__ void k_refresh_cs(int64_t n, const int32_t* __restrict__ comm, const int32_t* __restrict__ part,
+                             const int64_t* __restrict__ sigtot, CommSig* __restrict__ cs) {
+  const int64_t v = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (v < n) refresh_one(v, comm, part, sigtot, cs);
+}
+
 // next active set = commit | held | N(commit) (louvain.py:364-372); the graph stores both
 // directions, so marking the neighbours of each committed vertex equals testing every
 // vertex's neighbours.  Concurrent marks all store 1.
-// one wave per kMoveGroup consecutive vertices; only committed ones touch their edges
+// one wave per kMoveGroup consecutive vertices; only committed ones touch their edges.
+// refresh != 0 (sigma_tot already updated for this commit): also rewrite cs of the wave's
+// vertices, as k_refresh_cs does.
 __global__ __launch_bounds__(kThreads) void k_frontier_push(const int64_t* __restrict__ indptr,
                                                             const int32_t* __restrict__ indices, int64_t n,
                                                             const uint8_t* __restrict__ commit,
-                                                            uint8_t* __restrict__ active) {
+                                                            uint8_t* __restrict__ active, int refresh,
+                                                            const int32_t* __restrict__ comm,
+                                                            const int32_t* __restrict__ part,
+                                                            const int64_t* __restrict__ sigtot,
+                                                            CommSig* __restrict__ cs) {
   const int l = lane_id();
   const int64_t v0 = (static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + threadIdx.x / kWave) * kMoveGroup;
+  if (refresh && l < kMoveGroup && v0 + l < n) refresh_one(v0 + l, comm, part, sigtot, cs);
   uint64_t pending = __ballot(l < kMoveGroup && v0 + l < n && commit[v0 + l]);
   while (pending) {
     const int64_t v = v0 + __ffsll(static_cast<long long>(pending)) - 1;
@@ -1120,6 +1150,7 @@ struct Ctx {
   double* terms;
   int32_t* pass_info;       // 2 * kMaxPasses: per pass (any vertex wanted to move, moved count)
   int32_t *mv_v, *mv_old, *mv_cnt;  // committed vertices per commit block (k_commit)
+  CommSig* cs;              // (comm, part, sigtot[comm]) per vertex for k_move_light
   int64_t* red;             // small reduction outputs
   float* fred;
   // graph slots
@@ -1159,6 +1190,7 @@ static void carve_ctx(Carve& cv, Ctx& c, int64_t n, int64_t E) {
   c.pass_info = cv.take<int32_t>(2 * kMaxPasses);
   c.mv_v = cv.take<int32_t>(n1); c.mv_old = cv.take<int32_t>(n1);
   c.mv_cnt = cv.take<int32_t>(ceil_div(n1, kCommitThreads));
+  c.cs = cv.take<CommSig>(n1);
   c.red = cv.take<int64_t>(8);
   c.fred = cv.take<float>(kMaxBlocks + 8);
   for (int i = 0; i < 2; ++i) {
@@ -1318,7 +1350,11 @@ static int local_moving(Ctx& c, const Graph& g, int64_t seed, const int32_t* par
   int n_inc = 0;
   int p = 0;
   for (; p < kMaxPasses; ++p) {
-    if (!sig_valid) MSC_TRY(sigma_tot(c, g, c.comm));
+    if (!sig_valid) {
+      MSC_TRY(sigma_tot(c, g, c.comm));
+      k_refresh_cs<<<grid_for(g.n), kThreads, 0, st>>>(g.n, c.comm, part, c.sigtot, c.cs);
+      MSC_LAUNCH_CHECK();
+    }
     // the three move kernels write target[] of disjoint vertex sets and only read the rest:
     // the listed (mid / big) vertices run on the side stream concurrently with the light ones
     const bool fork = c.nm > 0 || c.nb > 0;
@@ -1345,8 +1381,8 @@ static int local_moving(Ctx& c, const Graph& g, int64_t seed, const int32_t* par
       MSC_LAUNCH_CHECK();
     }
     // the listed vertices go first so that their large-LDS blocks find free CUs
-    k_move_light<<<grid_for(ceil_div(g.n, kMoveGroup) * kWave), kThreads, 0, st>>>(g.indptr, g.indices, g.wq, g.k, g.n, c.comm, part,
-                                                            c.sigtot, c.active, gp, c.target);
+    k_move_light<<<grid_for(ceil_div(g.n, kMoveGroup) * kWave), kThreads, 0, st>>>(
+        g.indptr, g.indices, g.wq, g.k, g.n, c.comm, part, c.sigtot, c.cs, c.active, gp, c.target);
     MSC_LAUNCH_CHECK();
     if (c.nb > 0) {
       MSC_HIP(hipEventRecord(c.ev_join, c.side));
@@ -1370,8 +1406,9 @@ static int local_moving(Ctx& c, const Graph& g, int64_t seed, const int32_t* par
     }
     sig_valid = inc;
     n_inc += inc ? 1 : 0;
-    k_frontier_push<<<grid_for(ceil_div(g.n, kMoveGroup) * kWave), kThreads, 0, st>>>(g.indptr, g.indices, g.n,
-                                                                                     c.commit, c.active);
+    // after an incremental update the push also rewrites cs for the next pass
+    k_frontier_push<<<grid_for(ceil_div(g.n, kMoveGroup) * kWave), kThreads, 0, st>>>(
+        g.indptr, g.indices, g.n, c.commit, c.active, inc ? 1 : 0, c.comm, part, c.sigtot, c.cs);
     MSC_LAUNCH_CHECK();
     if (sync) {
       int32_t info[2];  // (any vertex wanted to move, moved count)
