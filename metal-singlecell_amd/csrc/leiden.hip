@@ -339,40 +339,30 @@ __device__ __forceinline__ void bitonic_sort_slots(uint32_t (&key)[NI], int l) {
   if constexpr (K < NI * kWave) bitonic_sort_slots<NI, 2 * K>(key, l);
 }
 
+constexpr uint32_t kNoKey = 0xFFFFFFFFu;
+
+__device__ __forceinline__ int64_t readlane_i64(int64_t x, int lane) {
+  const uint64_t u = static_cast<uint64_t>(x);
+  const uint32_t lo = __builtin_amdgcn_readlane(static_cast<int>(u), lane);
+  const uint32_t hi = __builtin_amdgcn_readlane(static_cast<int>(u >> 32), lane);
+  return static_cast<int64_t>((static_cast<uint64_t>(hi) << 32) | lo);
+}
+
+__device__ __forceinline__ int64_t sig_of(const int4& r) {
+  return static_cast<int64_t>((static_cast<uint64_t>(static_cast<uint32_t>(r.w)) << 32) |
+                              static_cast<uint32_t>(r.z));
+}
+
+// The decision for vertex v from its eligible neighbour slots e = i * 64 + lane: key[i] =
+// (community << IB | e) or kNoKey, their weights in lw[e] and sigma_tot in lsg[e].
 template <int NI>
-__device__ __forceinline__ void move_sorted(int64_t v, int64_t e0, int64_t deg, int32_t curr, int64_t kv,
-                                            const int32_t* __restrict__ indices, const int64_t* __restrict__ wq,
-                                            const CommSig* __restrict__ cs, const int32_t* __restrict__ part,
-                                            const GainParams& gp, int32_t* __restrict__ target,
-                                            int64_t* __restrict__ lw, int64_t* __restrict__ lsg,
-                                            int64_t* __restrict__ lp) {
+__device__ __forceinline__ void move_core(int64_t v, uint32_t (&key)[NI], int nvalid, int32_t curr, int64_t kv,
+                                          const GainParams& gp, int32_t* __restrict__ target,
+                                          const int64_t* __restrict__ lw, const int64_t* __restrict__ lsg,
+                                          int64_t* __restrict__ lp) {
   constexpr int IB = NI == 1 ? 6 : NI == 2 ? 7 : 8;
   static_assert(NI == 1 || NI == 2 || NI == 4, "NI");
-  constexpr uint32_t kNone = 0xFFFFFFFFu;
   const int l = lane_id();
-  const int32_t pv = part ? part[v] : 0;
-  uint32_t key[NI];
-  int nvalid = 0;
-#pragma unroll
-  for (int i = 0; i < NI; ++i) {
-    const int64_t j = l + kWave * i;
-    key[i] = kNone;
-    int64_t w = 0, sg = 0;
-    if (j < deg) {
-      const int32_t u = indices[e0 + j];
-      const int4 r = reinterpret_cast<const int4*>(cs)[u];  // (comm, part, sigtot lo, hi)
-      if (u != static_cast<int32_t>(v) && (!part || r.y == pv)) {
-        w = wq[e0 + j];
-        sg = static_cast<int64_t>((static_cast<uint64_t>(static_cast<uint32_t>(r.w)) << 32) |
-                                  static_cast<uint32_t>(r.z));
-        key[i] = (static_cast<uint32_t>(r.x) << IB) | static_cast<uint32_t>(i * kWave + l);
-      }
-    }
-    lw[i * kWave + l] = w;
-    lsg[i * kWave + l] = sg;
-    nvalid += __popcll(__ballot(key[i] != kNone));
-  }
-
   bitonic_sort_slots<NI, 2>(key, l);  // ascending in slot order e = i * 64 + l
 
   // per slot: weight prefix and run heads (community = key >> IB)
@@ -438,6 +428,50 @@ __device__ __forceinline__ void move_sorted(int64_t v, int64_t e0, int64_t deg, 
   if (l == 0) target[v] = (best_g > stay + kTol) ? best_c : curr;
 }
 
+// one neighbour slot: (community, weight, sigma_tot) of neighbour j of v when eligible
+template <int IB>
+__device__ __forceinline__ void gather_slot(int64_t v, int64_t e0, int64_t deg, int64_t j, int slot, int32_t pv,
+                                            const int32_t* __restrict__ indices, const int64_t* __restrict__ wq,
+                                            const CommSig* __restrict__ cs, const int32_t* __restrict__ part,
+                                            uint32_t& key, int64_t& w, int64_t& sg) {
+  key = kNoKey;
+  w = 0;
+  sg = 0;
+  if (j < deg) {
+    const int32_t u = indices[e0 + j];
+    const int64_t wj = wq[e0 + j];
+    const int4 r = reinterpret_cast<const int4*>(cs)[u];  // (comm, part, sigtot lo, hi)
+    if (u != static_cast<int32_t>(v) && (!part || r.y == pv)) {
+      w = wj;
+      sg = sig_of(r);
+      key = (static_cast<uint32_t>(r.x) << IB) | static_cast<uint32_t>(slot);
+    }
+  }
+}
+
+template <int NI>
+__device__ __forceinline__ void move_sorted(int64_t v, int64_t e0, int64_t deg, int32_t curr, int64_t kv,
+                                            const int32_t* __restrict__ indices, const int64_t* __restrict__ wq,
+                                            const CommSig* __restrict__ cs, const int32_t* __restrict__ part,
+                                            const GainParams& gp, int32_t* __restrict__ target,
+                                            int64_t* __restrict__ lw, int64_t* __restrict__ lsg,
+                                            int64_t* __restrict__ lp) {
+  constexpr int IB = NI == 1 ? 6 : NI == 2 ? 7 : 8;
+  const int l = lane_id();
+  const int32_t pv = part ? part[v] : 0;
+  uint32_t key[NI];
+  int nvalid = 0;
+#pragma unroll
+  for (int i = 0; i < NI; ++i) {
+    int64_t w, sg;
+    gather_slot<IB>(v, e0, deg, l + kWave * i, i * kWave + l, pv, indices, wq, cs, part, key[i], w, sg);
+    lw[i * kWave + l] = w;
+    lsg[i * kWave + l] = sg;
+    nvalid += __popcll(__ballot(key[i] != kNoKey));
+  }
+  move_core<NI>(v, key, nvalid, curr, kv, gp, target, lw, lsg, lp);
+}
+
 __device__ __forceinline__ void move_vertex(int64_t v, const int64_t* __restrict__ indptr,
                                             const int32_t* __restrict__ indices, const int64_t* __restrict__ wq,
                                             const int64_t* __restrict__ kdeg, int64_t n,
@@ -473,13 +507,64 @@ __global__ __launch_bounds__(kThreads) void k_move_light(
   const int l = lane_id();
   const int64_t v0 = (static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + threadIdx.x / kWave) * kMoveGroup;
   bool todo = false;
+  int64_t e0v = 0, degv = 0, kvv = 0;
+  int32_t currv = 0, pvv = 0;
   if (l < kMoveGroup && v0 + l < n) {
     const int64_t vv = v0 + l;
-    const bool light = indptr[vv + 1] - indptr[vv] <= kLightDeg;  // heavy path writes target[v]
-    if (light && !active[vv]) target[vv] = comm[vv];
-    todo = light && active[vv];
+    e0v = indptr[vv];
+    degv = indptr[vv + 1] - e0v;
+    currv = comm[vv];
+    const bool light = degv <= kLightDeg;  // heavy path writes target[v]
+    const bool act = active[vv];
+    if (light && !act) target[vv] = currv;
+    todo = light && act;
+    kvv = kdeg[vv];
+    pvv = part ? part[vv] : 0;
   }
   uint64_t pending = __ballot(todo);
+  int64_t(*wl)[kLightDeg] = lds[threadIdx.x / kWave];
+  if (n < kSortedMaxN) {
+    // Vertices of degree <= 64 (one slot per lane): the gathers of all of them are issued
+    // before the first decision (neighbour ids and weights, then every (comm, part, sigma)
+    // record), so a wave keeps up to kMoveGroup vertices' loads in flight at once.
+    const uint64_t small = __ballot(todo && degv <= kWave);
+    if (small) {
+      int32_t u[kMoveGroup];
+      int64_t wr[kMoveGroup];
+      int4 r[kMoveGroup];
+#pragma unroll
+      for (int i = 0; i < kMoveGroup; ++i) {
+        const int64_t e0 = readlane_i64(e0v, i);
+        const int deg = ((small >> i) & 1) ? static_cast<int>(readlane_i64(degv, i)) : 0;
+        u[i] = -1;
+        wr[i] = 0;
+        if (l < deg) {
+          u[i] = indices[e0 + l];
+          wr[i] = wq[e0 + l];
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < kMoveGroup; ++i) {
+        r[i] = make_int4(0, 0, 0, 0);
+        if (u[i] >= 0) r[i] = reinterpret_cast<const int4*>(cs)[u[i]];
+      }
+#pragma unroll
+      for (int i = 0; i < kMoveGroup; ++i) {
+        if ((small >> i) & 1) {
+          const int64_t v = v0 + i;
+          const int32_t pv = __builtin_amdgcn_readlane(pvv, i);
+          const bool ok = u[i] >= 0 && u[i] != static_cast<int32_t>(v) && (!part || r[i].y == pv);
+          uint32_t key[1] = {ok ? ((static_cast<uint32_t>(r[i].x) << 6) | static_cast<uint32_t>(l)) : kNoKey};
+          wl[0][l] = ok ? wr[i] : 0;
+          wl[1][l] = ok ? sig_of(r[i]) : 0;
+          const int nvalid = __popcll(__ballot(ok));
+          move_core<1>(v, key, nvalid, __builtin_amdgcn_readlane(currv, i), readlane_i64(kvv, i), gp, target,
+                       wl[0], wl[1], wl[2]);
+        }
+      }
+      pending &= ~small;
+    }
+  }
   while (pending) {
     const int64_t v = v0 + __ffsll(static_cast<long long>(pending)) - 1;
     pending &= pending - 1;
