@@ -231,12 +231,16 @@ template <bool BF16, int KS>
 struct SelCfg {
   static constexpr int K = BF16 ? 32 * KS : 8 * KS;         // elements per row
   static constexpr int ROWB = BF16 ? 2 * K : 4 * K;          // bytes per row
-  // LDS row stride (bytes): BF16 reads 16 B per lane -> stride = 4 (mod 32) dwords;
-  // FP32 reads 8 B per lane -> stride = 2 (mod 32) dwords.  Both conflict-free.
-  static constexpr int want = BF16 ? 4 : 2;
+  // LDS row stride = 8 (mod 16) dwords.  gfx950 serves 512 B of a wave's LDS read per cycle
+  // over 128 banks: 32 lanes of a 16-byte read (16 rows x 2 column groups, BF16) or 64 lanes
+  // of an 8-byte read (16 rows x 4 groups, FP32), i.e. 8 consecutive dwords of each of 16
+  // rows.  Rows 8 * odd dwords apart start on 16 distinct multiples of 8 (mod 128), so the
+  // 128 dwords fall in 128 distinct banks.  (A stride of 4 (mod 32) dwords costs 4 conflict
+  // cycles per 16-byte read on gfx950.)
   static constexpr int rdw = ROWB / 4;
-  static constexpr int SDW = rdw + ((want - (rdw % 32) + 32) % 32);
+  static constexpr int SDW = rdw + ((8 - (rdw % 16) + 16) % 16);
   static constexpr int STRIDE = 4 * SDW;
+  static_assert(STRIDE % 32 == 0 && SDW % 16 == 8, "LDS row stride");
   static constexpr int CHUNK = BF16 ? 16 : 8;                // bytes per global->LDS move
   static constexpr int NCHUNK = kTile * ROWB / CHUNK;
   static constexpr int PER_THREAD = (NCHUNK + kThreads - 1) / kThreads;
@@ -331,9 +335,8 @@ __global__ __launch_bounds__(kThreads, 2) void select_kernel(
   const int64_t ntiles = (c_end > c_begin) ? ceil_div(c_end - c_begin, kTile) : 0;
 
   // Prefetch registers for one tile: up to eight named 16-byte registers chosen at compile
-  // time (an array here gets demoted to LDS or scratch by the compiler).  Global reads are
-  // 16-byte chunks for both precisions; FP32 rows are stored to LDS as two 8-byte halves
-  // because their padded LDS stride is only 8-byte aligned.
+  // time (an array here gets demoted to LDS or scratch by the compiler).  Global reads and
+  // LDS stores are 16-byte chunks for both precisions (the row stride is a multiple of 32 B).
   constexpr int kRowChunks = C::ROWB / 16;
   constexpr int kNChunk = kTile * kRowChunks;
   constexpr int kPer = (kNChunk + kThreads - 1) / kThreads;
@@ -354,13 +357,7 @@ __global__ __launch_bounds__(kThreads, 2) void select_kernel(
     const int c_ = threadIdx.x + i * kThreads;                                                  \
     if (kNChunk % kThreads == 0 || c_ < kNChunk) {                                              \
       char* dst_ = &tile[(B)][(c_ / kRowChunks) * C::STRIDE + (c_ % kRowChunks) * 16];           \
-      const uint4 v_ = MSC_PF(S, i);                                                            \
-      if constexpr (BF16) {                                                                     \
-        *reinterpret_cast<uint4*>(dst_) = v_;                                                   \
-      } else {                                                                                  \
-        reinterpret_cast<uint2*>(dst_)[0] = make_uint2(v_.x, v_.y);                             \
-        reinterpret_cast<uint2*>(dst_)[1] = make_uint2(v_.z, v_.w);                             \
-      }                                                                                         \
+      *reinterpret_cast<uint4*>(dst_) = MSC_PF(S, i);                                           \
     }                                                                                           \
   }
 #define MSC_LOAD_TILE(S, T) { MSC_LOAD1(S, 0, T) MSC_LOAD1(S, 1, T) MSC_LOAD1(S, 2, T) MSC_LOAD1(S, 3, T) \
