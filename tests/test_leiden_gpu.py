@@ -105,3 +105,37 @@ def test_large_graph_70k():
     assert np.array_equal(lab.cpu().numpy().astype(np.int64), ref)
     assert q == leiden.modularity_of_labels(P, ref)
     assert linfo["levels"][0]["move_delta_passes"] > 0
+
+
+def _planted_partition(n, group, k_in, k_out, seed):
+    """Groups of `group` vertices, k_in random in-group and k_out random out-group edges per
+    vertex (symmetrised, uniform weights), vertices shuffled as real cell orders are."""
+    rng = np.random.default_rng(seed)
+    g = np.arange(n) // group
+    rows = np.repeat(np.arange(n), k_in + k_out)
+    cols = np.empty((n, k_in + k_out), np.int64)
+    cols[:, :k_in] = (g * group)[:, None] + rng.integers(0, group, (n, k_in))
+    cols[:, k_in:] = rng.integers(0, n, (n, k_out))
+    w = rng.uniform(0.2, 1.0, rows.size).astype(np.float32)
+    A = sp.coo_matrix((w, (rows, cols.ravel())), shape=(n, n)).tocsr()
+    A = (A + A.T).tocsr()
+    A.setdiag(0)
+    A.eliminate_zeros()
+    perm = rng.permutation(n)
+    A = A[perm][:, perm].tocsr()
+    A.sort_indices()
+    return A.astype(np.float32)
+
+
+def test_large_graph_planted_300k():
+    """300,000 shuffled vertices in 3,000 planted groups (4M edges): Leiden and Louvain labels and
+    Q equal the oracle's, with both sigma_tot forms (rebuild and incremental) in use."""
+    from metalsinglecell import _hip
+    from oracle import leiden
+    P = _planted_partition(300_000, 100, 6, 1, 11)
+    _, _, info = _check(P)
+    assert info["levels"][0]["move_delta_passes"] > 0 and info["levels"][0]["refine_delta_passes"] > 0
+    lab, q, _ = _hip.louvain(P.indptr, P.indices, P.data, P.shape[0])
+    ref = leiden.louvain(P)
+    assert np.array_equal(lab.cpu().numpy().astype(np.int64), ref)
+    assert q == leiden.modularity_of_labels(P, ref)
