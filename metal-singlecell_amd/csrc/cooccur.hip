@@ -16,6 +16,7 @@
 // lane t takes candidate (t + s) mod 256 at step s so that the lanes of a wave hit different
 // candidates (and mostly different bins) at once.  Tile histograms are summed per code in
 // tile order, as 64-bit integers.
+#include <cmath>
 #include <vector>
 
 #include "common.h"
@@ -44,7 +45,7 @@ __global__ void k_gather_rows(const float* __restrict__ X, const int32_t* __rest
 __global__ __launch_bounds__(kThreads) void k_pair_hist(const float4* __restrict__ Xs, int64_t n, int dim,
                                                         const int64_t* __restrict__ tiles,
                                                         const float* __restrict__ thr2, int L, int K, int pow2L,
-                                                        uint32_t* __restrict__ partial) {
+                                                        float ra, float rinv, uint32_t* __restrict__ partial) {
   extern __shared__ __align__(16) unsigned char smem[];
   float4* cand = reinterpret_cast<float4*>(smem);                          // kThreads
   float* thr = reinterpret_cast<float*>(smem + kThreads * sizeof(float4));  // L
@@ -78,11 +79,21 @@ __global__ __launch_bounds__(kThreads) void k_pair_hist(const float4* __restrict
         d2 = __fadd_rn(d2, __fmul_rn(dz, dz));
       }
       if (!(d2 <= tmax)) continue;
-      // number of thresholds below d2 = the first r with thr[r] >= d2 (branchless search)
-      int pos = 0;
-      for (int step = pow2L; step > 0; step >>= 1) {
-        const int np = pos + step;
-        if (np <= L && thr[np - 1] < d2) pos = np;
+      // number of thresholds below d2 = the first r with thr[r] >= d2 (thr[L - 1] >= d2 here)
+      int pos;
+      if (rinv > 0.0f) {
+        // thresholds evenly spaced in distance: estimate ceil((sqrt(d2) - ra) * rinv), then walk
+        // to the exact bin, so the result is the search's for any estimate
+        const float est = ceilf((__builtin_sqrtf(d2) - ra) * rinv);
+        pos = est <= 0.0f ? 0 : est >= static_cast<float>(L - 1) ? L - 1 : static_cast<int>(est);
+        while (pos > 0 && thr[pos - 1] >= d2) --pos;
+        while (thr[pos] < d2) ++pos;
+      } else {
+        pos = 0;  // branchless binary search
+        for (int step = pow2L; step > 0; step >>= 1) {
+          const int np = pos + step;
+          if (np <= L && thr[np - 1] < d2) pos = np;
+        }
       }
       atomicAdd(&hist[cj * L + pos], 1u);
     }
@@ -218,10 +229,24 @@ extern "C" int msc_co_occurrence(const float* coords, int64_t n, int dim, const 
   MSC_LAUNCH_CHECK();
   int pow2L = 1;
   while (pow2L < L) pow2L <<= 1;
+  // Thresholds evenly spaced in distance (the default linspace intervals) let the kernel start
+  // each bin lookup from an arithmetic estimate; any other spacing uses the binary search.
+  float ra = 0.0f, rinv = 0.0f;
+  if (L >= 2 && thr2[0] >= 0.0f) {
+    const double a = std::sqrt(static_cast<double>(thr2[0]));
+    const double h = (std::sqrt(static_cast<double>(thr2[L - 1])) - a) / (L - 1);
+    bool even = h > 0.0;
+    for (int r = 0; r < L && even; ++r)
+      even = std::fabs(std::sqrt(static_cast<double>(thr2[r])) - (a + r * h)) <= 0.25 * h;
+    if (even) {
+      ra = static_cast<float>(a);  // t_r = a + r h: #{r : t_r < x} = ceil((x - a) / h)
+      rinv = static_cast<float>(1.0 / h);
+    }
+  }
   const size_t shm = kThreads * sizeof(float4) + L * sizeof(float) + static_cast<size_t>(K) * L * sizeof(uint32_t);
   MSC_REQUIRE(shm <= 65536, "co_occurrence: %d codes x %d bins exceed the 64 KB workgroup histogram", K, L);
   k_pair_hist<<<static_cast<unsigned>(Lo.ntiles), kThreads, shm, st>>>(b.Xs, n, dim, b.tiles, b.thr2, L, K, pow2L,
-                                                                       b.partial);
+                                                                       ra, rinv, b.partial);
   MSC_LAUNCH_CHECK();
   const int64_t m = static_cast<int64_t>(K) * K * L;
   k_sum_tiles<<<static_cast<unsigned>(ceil_div(m, kThreads)), kThreads, 0, st>>>(b.partial, b.tc, K, L, out_hist);

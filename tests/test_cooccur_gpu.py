@@ -52,6 +52,10 @@ def test_hist_lattice_ties_and_duplicates():
     thr2 = np.array([0.0, 100.0, 200.0, 400.0, 2500.0], np.float32)
     h = _check(xy, codes, 6, thr2)
     assert h[:, :, 0].sum() == 20
+    # evenly spaced distances (the arithmetic bin estimate) landing exactly on lattice distances,
+    # with and without a zero first threshold
+    for thr in (np.linspace(0.0, 50.0, 6)[1:], np.linspace(0.0, 40.0, 5), np.linspace(5.0, 95.0, 19)):
+        _check(xy, codes, 6, (thr ** 2).astype(np.float32))
 
 
 def test_gr_co_occurrence_contract():
