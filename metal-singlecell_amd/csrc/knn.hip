@@ -531,6 +531,42 @@ __device__ __forceinline__ float exact_d2(const float* __restrict__ X, int d, in
   return acc;
 }
 
+// exact_d2 of row i against rows j[0..3], each with exact_d2's loads and summation order (so
+// bit-identical), the row-i loads shared and the four rows' loads in flight together
+__device__ __forceinline__ void exact_d2_x4(const float* __restrict__ X, int d, int64_t i, const int64_t (&j)[4],
+                                            float (&out)[4]) {
+  const float* a = X + i * d;
+  float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+  if ((d & 1) == 0 && (reinterpret_cast<uintptr_t>(X) & 7) == 0) {
+    const float2* a2 = reinterpret_cast<const float2*>(a);
+    const float2* b2[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) b2[u] = reinterpret_cast<const float2*>(X + j[u] * d);
+    for (int t = 0; t < d / 2; ++t) {
+      const float2 x = a2[t];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const float2 y = b2[u][t];
+        const float d0 = __fsub_rn(x.x, y.x);
+        acc[u] = __fadd_rn(acc[u], __fmul_rn(d0, d0));
+        const float d1 = __fsub_rn(x.y, y.y);
+        acc[u] = __fadd_rn(acc[u], __fmul_rn(d1, d1));
+      }
+    }
+  } else {
+    for (int t = 0; t < d; ++t) {
+      const float x = a[t];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const float diff = __fsub_rn(x, X[j[u] * d + t]);
+        acc[u] = __fadd_rn(acc[u], __fmul_rn(diff, diff));
+      }
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < 4; ++u) out[u] = acc[u];
+}
+
 // exact key: self -> 0, others ((d2 bits << 32) | j) + 1, invalid -> kEmpty
 __device__ __forceinline__ uint64_t exact_key(int64_t q, int64_t j, float d2) {
   if (j == q) return 0ull;
@@ -615,6 +651,35 @@ __global__ __launch_bounds__(kThreads) void finalize_kernel(
   }
 }
 
+// The 64 smallest exact keys of row q over candidates [cb, ce), ascending in lane order (one
+// wave): chunks of 64 candidates merged in candidate order, four chunks' rows loaded at once.
+__device__ __forceinline__ uint64_t brute_top64(const float* __restrict__ X, int d, int64_t q, int64_t cb,
+                                                int64_t ce) {
+  const int l = lane_id();
+  uint64_t cur = kEmpty, tail = kEmpty;
+  for (int64_t base = cb; base < ce; base += 4 * kWave) {
+    int64_t j[4];
+    bool v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int64_t jj = base + u * kWave + l;
+      v[u] = jj < ce;
+      j[u] = v[u] ? jj : cb;  // an in-range row for the load; its key is dropped
+    }
+    float d2[4];
+    exact_d2_x4(X, d, q, j, d2);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const uint64_t ek = v[u] ? exact_key(q, j[u], d2[u]) : kEmpty;
+      if (__any(ek < tail)) {
+        cur = wave_merge_min_u64(cur, wave_sort_u64(ek));
+        tail = shfl_u64(cur, kWave - 1);
+      }
+    }
+  }
+  return cur;
+}
+
 __global__ __launch_bounds__(kThreads) void fallback_kernel(
     const float* __restrict__ X, int64_t n, int d, int k, const int32_t* __restrict__ flags,
     const int64_t* __restrict__ row_range, int32_t* __restrict__ out_idx, float* __restrict__ out_dist) {
@@ -623,15 +688,7 @@ __global__ __launch_bounds__(kThreads) void fallback_kernel(
   if (q >= n || flags[q] == 0) return;
   const int64_t cb = row_range ? row_range[2 * q] : 0;
   const int64_t ce = row_range ? row_range[2 * q + 1] : n;
-  uint64_t cur = kEmpty, tail = kEmpty;
-  for (int64_t base = cb; base < ce; base += kWave) {
-    const int64_t j = base + l;
-    const uint64_t ek = (j < ce) ? exact_key(q, j, exact_d2(X, d, q, j)) : kEmpty;
-    if (__any(ek < tail)) {
-      cur = wave_merge_min_u64(cur, wave_sort_u64(ek));
-      tail = shfl_u64(cur, kWave - 1);
-    }
-  }
+  const uint64_t cur = brute_top64(X, d, q, cb, ce);
   if (l < k) {
     int64_t jj;
     float d2;
@@ -1100,15 +1157,7 @@ __global__ __launch_bounds__(kThreads) void fallback_sep_kernel(
     const int t = static_cast<int>(w / n_slots);
     const int64_t q = w - static_cast<int64_t>(t) * n_slots;
     const int64_t cb = brange[2 * t], ce = brange[2 * t + 1];
-    uint64_t cur = kEmpty, tail = kEmpty;
-    for (int64_t base = cb; base < ce; base += kWave) {
-      const int64_t j = base + l;
-      const uint64_t ek = (j < ce) ? exact_key(q, j, exact_d2(X, d, q, j)) : kEmpty;
-      if (__any(ek < tail)) {
-        cur = wave_merge_min_u64(cur, wave_sort_u64(ek));
-        tail = shfl_u64(cur, kWave - 1);
-      }
-    }
+    const uint64_t cur = brute_top64(X, d, q, cb, ce);
     if (l < kk[t]) {
       int64_t jj;
       float d2;

@@ -38,6 +38,23 @@ def test_bbknn_knn_matches_oracle(precision, nb, k):
     print(f"nb={nb} k={k} {precision}: fallback lists {info['n_fallback']}")
 
 
+@pytest.mark.parametrize("precision", ["fp32", "bf16"])
+def test_bbknn_brute_force_lists(precision):
+    """Huge offsets with tiny spreads defeat both rankings' certificates in the select passes:
+    those (batch, cell) lists are brute-forced, and the result still equals the oracle's."""
+    from metalsinglecell import _hip
+    from oracle import bbknn
+    rng = np.random.default_rng(17)
+    centers = rng.normal(0, 1e6, (10, 8))
+    X = (centers[rng.integers(0, 10, 1800)] + rng.normal(0, 1e-2, (1800, 8))).astype(np.float32)
+    batch = rng.integers(0, 3, X.shape[0])
+    idx, dist, info = _hip.bbknn(X, batch, 3, 5, precision=precision, return_info=True)
+    ri, rd, _ = bbknn.bbknn_knn(X, batch, 5)
+    assert info["n_fallback"] > 0
+    assert np.array_equal(idx.cpu().numpy(), ri)
+    assert np.array_equal(dist.cpu().numpy(), rd)
+
+
 def test_bbknn_uneven_and_tiny_batches():
     """Batches of 1, 2, 150 and 2000 cells (padding inside query blocks, kk < k), batch codes
     interleaved in the input order."""
