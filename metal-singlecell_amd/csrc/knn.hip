@@ -515,6 +515,9 @@ __device__ __forceinline__ float exact_d2(const float* __restrict__ X, int d, in
                                                                       // loads, same summation order
     const float2* a2 = reinterpret_cast<const float2*>(a);
     const float2* b2 = reinterpret_cast<const float2*>(b);
+    // unrolled so that several iterations' loads are in flight before the (unchanged,
+    // sequential) chain of adds consumes them
+#pragma unroll 8
     for (int t = 0; t < d / 2; ++t) {
       const float2 x = a2[t], y = b2[t];
       const float d0 = __fsub_rn(x.x, y.x);
@@ -524,6 +527,7 @@ __device__ __forceinline__ float exact_d2(const float* __restrict__ X, int d, in
     }
     return acc;
   }
+#pragma unroll 8
   for (int t = 0; t < d; ++t) {
     const float diff = __fsub_rn(a[t], b[t]);
     acc = __fadd_rn(acc, __fmul_rn(diff, diff));
