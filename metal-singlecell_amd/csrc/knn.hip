@@ -94,6 +94,7 @@ __global__ void colsum_partial(const float* __restrict__ X, int64_t n, int d, in
   const int64_t r1 = min(n, r0 + rows_per);
   for (int t = threadIdx.x; t < d; t += blockDim.x) {
     double s = 0.0;
+#pragma unroll 8
     for (int64_t r = r0; r < r1; ++r) s += static_cast<double>(X[r * d + t]);
     partial[static_cast<int64_t>(blockIdx.x) * d + t] = s;
   }
@@ -130,6 +131,7 @@ __global__ void prep_bf16(const float* __restrict__ X, int64_t n, int64_t n_pad,
     return;
   }
   double cn = 0.0, r2 = 0.0, hn = 0.0, hr2 = 0.0;
+#pragma unroll 8
   for (int t = 0; t < d; ++t) {
     const double xc = static_cast<double>(X[i * d + t]) - mean[t];
     const uint16_t h = f32_to_bf16_rne(static_cast<float>(xc));
@@ -172,6 +174,7 @@ __global__ void prep_f32(const float* __restrict__ X, int64_t n, int64_t n_pad, 
     return;
   }
   double cn = 0.0, r2 = 0.0;
+#pragma unroll 8
   for (int t = 0; t < d; ++t) {
     const double xc = static_cast<double>(X[i * d + t]) - mean[t];
     const float xf = static_cast<float>(xc);
@@ -189,6 +192,7 @@ __global__ void prep_f32(const float* __restrict__ X, int64_t n, int64_t n_pad, 
 __global__ void point_max(const float* __restrict__ nrm2, const float* __restrict__ res,
                           int64_t n, float* __restrict__ gstat) {
   float a = 0.0f, b = 0.0f;
+#pragma unroll 8
   for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
     a = fmaxf(a, nrm2[i]);
     b = fmaxf(b, res[i]);
