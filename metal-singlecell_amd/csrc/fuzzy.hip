@@ -40,6 +40,7 @@ __global__ void k_dist_partial(const float* __restrict__ dist, int64_t m, int k,
   const int64_t per = (mg + gridDim.x - 1) / gridDim.x;
   const int64_t b0 = blockIdx.x * per, b1 = min(mg, b0 + per);
   double s = 0.0;
+#pragma unroll 8
   for (int64_t i = b0 + threadIdx.x; i < b1; i += blockDim.x) s += static_cast<double>(dist[i]);
   s = wave_sum_f64(s);
   __shared__ double sh[kThreads / kWave];

@@ -303,6 +303,7 @@ __global__ void k_radius_emit(const float* __restrict__ X, const double* __restr
 
 __global__ void k_max_i32(const int32_t* __restrict__ v, int64_t n, int32_t* __restrict__ out) {
   int32_t m = 0;
+#pragma unroll 8
   for (int64_t i = threadIdx.x; i < n; i += blockDim.x) m = max(m, v[i]);
 #pragma unroll
   for (int s = 1; s < kWave; s <<= 1) m = max(m, __shfl_xor(m, s, kWave));

@@ -144,6 +144,7 @@ __global__ __launch_bounds__(kThreads) void k_degrees(const int64_t* __restrict_
 __global__ void k_sum_i64_max_deg(const int64_t* __restrict__ k, const int64_t* __restrict__ indptr, int64_t n,
                                   int64_t* __restrict__ out /*[0]=sum,[1]=maxdeg*/) {
   int64_t s = 0, md = 0;
+#pragma unroll 8
   for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
     s += k[i];
     md = max(md, indptr[i + 1] - indptr[i]);

@@ -36,11 +36,13 @@ __global__ void k_col_partial(const float* __restrict__ X, int64_t n, int64_t G,
     double s = 0.0;
     if (mean) {
       const double m = mean[g];
+#pragma unroll 8
       for (int64_t r = r0; r < r1; ++r) {
         const double d = static_cast<double>(X[r * G + g]) - m;
         s += d * d;
       }
     } else {
+#pragma unroll 8
       for (int64_t r = r0; r < r1; ++r) s += static_cast<double>(X[r * G + g]);
     }
     partial[blockIdx.x * G + g] = s;
@@ -51,6 +53,7 @@ __global__ void k_col_final(const double* __restrict__ partial, int nb, int64_t 
   const int64_t g = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
   if (g >= G) return;
   double s = 0.0;
+#pragma unroll 8
   for (int b = 0; b < nb; ++b) s += partial[static_cast<int64_t>(b) * G + g];
   out[g] = s * scale;
 }
@@ -73,6 +76,7 @@ constexpr int kSumBlocks = 512;
 __global__ void k_sum_f64(const double* __restrict__ v, int64_t m, int64_t per, double* __restrict__ out) {
   const int64_t b0 = blockIdx.x * per, b1 = min(m, b0 + per);
   double s = 0.0;
+#pragma unroll 8
   for (int64_t i = b0 + threadIdx.x; i < b1; i += blockDim.x) s += v[i];
   __shared__ double sh[kThreads];
   sh[threadIdx.x] = s;
@@ -564,6 +568,7 @@ __global__ void k_col_partial_f64(const double* __restrict__ M, int64_t n, int64
   for (int64_t g = threadIdx.x + static_cast<int64_t>(blockIdx.y) * blockDim.x; g < G;
        g += static_cast<int64_t>(gridDim.y) * blockDim.x) {
     double s = 0.0;
+#pragma unroll 8
     for (int64_t r = r0; r < r1; ++r) s += M[r * G + g];
     partial[blockIdx.x * G + g] = s;
   }
