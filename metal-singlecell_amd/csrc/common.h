@@ -105,6 +105,60 @@ __device__ __forceinline__ uint64_t shfl_up_u64(uint64_t x, int d) {
   return (static_cast<uint64_t>(hi) << 32) | lo;
 }
 
+// Value of lane l ^ J for a compile-time J, with all 64 lanes active.  J = 1, 2: DPP quad_perm;
+// J = 4, 8: DPP row_shl:J (lane i reads i + J) or row_shr:J (reads i - J) by lane bit J;
+// J = 16: ds_swizzle xor within 32 lanes; J = 32: ds_bpermute.  (The DPP forms are VALU
+// operands, the other two go through the LDS crossbar.)
+template <int J>
+__device__ __forceinline__ uint32_t lane_xor(uint32_t v) {
+  const int x = static_cast<int>(v);
+  if constexpr (J == 1) {
+    return static_cast<uint32_t>(__builtin_amdgcn_update_dpp(x, x, 0xB1, 0xF, 0xF, false));
+  } else if constexpr (J == 2) {
+    return static_cast<uint32_t>(__builtin_amdgcn_update_dpp(x, x, 0x4E, 0xF, 0xF, false));
+  } else if constexpr (J == 4 || J == 8) {
+    const int up = __builtin_amdgcn_update_dpp(x, x, 0x100 | J, 0xF, 0xF, false);
+    const int dn = __builtin_amdgcn_update_dpp(x, x, 0x110 | J, 0xF, 0xF, false);
+    return static_cast<uint32_t>((__lane_id() & J) ? dn : up);
+  } else if constexpr (J == 16) {
+    return static_cast<uint32_t>(__builtin_amdgcn_ds_swizzle(x, (0x10 << 10) | 0x1F));
+  } else {
+    static_assert(J == 32, "lane_xor: J must be a power of two below 64");
+    return static_cast<uint32_t>(__shfl_xor(x, 32, kWave));
+  }
+}
+template <int J>
+__device__ __forceinline__ uint64_t lane_xor_u64(uint64_t v) {
+  const uint32_t lo = lane_xor<J>(static_cast<uint32_t>(v));
+  const uint32_t hi = lane_xor<J>(static_cast<uint32_t>(v >> 32));
+  return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+
+// One DPP move of a 64-bit value (both halves with the same control); lanes whose source is
+// outside its row (bound_ctrl) or whose row is masked off read 0.
+template <int CTRL, int ROWMASK, bool BOUND>
+__device__ __forceinline__ int64_t dpp_i64(int64_t x) {
+  const uint64_t u = static_cast<uint64_t>(x);
+  const uint32_t lo = static_cast<uint32_t>(
+      __builtin_amdgcn_update_dpp(0, static_cast<int>(static_cast<uint32_t>(u)), CTRL, ROWMASK, 0xF, BOUND));
+  const uint32_t hi = static_cast<uint32_t>(
+      __builtin_amdgcn_update_dpp(0, static_cast<int>(static_cast<uint32_t>(u >> 32)), CTRL, ROWMASK, 0xF, BOUND));
+  return static_cast<int64_t>((static_cast<uint64_t>(hi) << 32) | lo);
+}
+
+// Inclusive prefix sum of an int64 across the 64 lanes (all active): Hillis-Steele inside each
+// 16-lane row by DPP row_shr 1, 2, 4, 8 (zero-filled), then row_bcast:15 adds each even row's
+// total to the next row and row_bcast:31 adds lane 31's total to rows 2 and 3.
+__device__ __forceinline__ int64_t wave_incl_sum_i64(int64_t x) {
+  x += dpp_i64<0x111, 0xF, true>(x);
+  x += dpp_i64<0x112, 0xF, true>(x);
+  x += dpp_i64<0x114, 0xF, true>(x);
+  x += dpp_i64<0x118, 0xF, true>(x);
+  x += dpp_i64<0x142, 0xA, false>(x);
+  x += dpp_i64<0x143, 0xC, false>(x);
+  return x;
+}
+
 __device__ __forceinline__ uint64_t umin64(uint64_t a, uint64_t b) { return a < b ? a : b; }
 __device__ __forceinline__ uint64_t umax64(uint64_t a, uint64_t b) { return a < b ? b : a; }
 

@@ -325,7 +325,7 @@ __device__ __forceinline__ void bitonic_stage(uint32_t (&key)[NI], int l) {
   } else {  // partner slot in lane l ^ J of the same item
 #pragma unroll
     for (int i = 0; i < NI; ++i) {
-      const uint32_t p = static_cast<uint32_t>(__shfl_xor(static_cast<int>(key[i]), J, kWave));
+      const uint32_t p = lane_xor<J>(key[i]);
       const bool asc = ((i * kWave + l) & K) == 0;
       const bool lower = (l & J) == 0;
       const uint32_t lo = key[i] < p ? key[i] : p, hi = key[i] < p ? p : key[i];
@@ -354,6 +354,18 @@ __device__ __forceinline__ int64_t sig_of(const int4& r) {
                               static_cast<uint32_t>(r.z));
 }
 
+// one butterfly step of the wave argmax over (gain, community), ties to the smaller community
+template <int M>
+__device__ __forceinline__ void argmax_step(double& g, int32_t& c) {
+  const double og = __longlong_as_double(
+      static_cast<long long>(lane_xor_u64<M>(static_cast<uint64_t>(__double_as_longlong(g)))));
+  const int32_t oc = static_cast<int32_t>(lane_xor<M>(static_cast<uint32_t>(c)));
+  if (better(og, oc, g, c)) {
+    g = og;
+    c = oc;
+  }
+}
+
 // The decision for vertex v from its eligible neighbour slots e = i * 64 + lane: key[i] =
 // (community << IB | e) or kNoKey, their weights in lw[e] and sigma_tot in lsg[e].
 template <int NI>
@@ -378,19 +390,13 @@ __device__ __forceinline__ void move_core(int64_t v, uint32_t (&key)[NI], int nv
     const int32_t ci = valid ? static_cast<int32_t>(key[i] >> IB) : -1;
     const int64_t w = valid ? lw[src] : 0;
     // inclusive prefix of w over the slots
-    int64_t x = w;
-#pragma unroll
-    for (int d = 1; d < kWave; d <<= 1) {
-      const int64_t o = static_cast<int64_t>(shfl_up_u64(static_cast<uint64_t>(x), d));
-      if (l >= d) x += o;
-    }
-    x += carry;
-    carry = static_cast<int64_t>(shfl_u64(static_cast<uint64_t>(x), kWave - 1));
+    const int64_t x = wave_incl_sum_i64(w) + carry;
+    carry = readlane_i64(x, kWave - 1);
     lp[e] = x;
     int32_t cprev = __shfl_up(ci, 1, kWave);
     if (l == 0) cprev = c_last;
     hmask[i] = __ballot(valid && (e == 0 || cprev != ci));
-    c_last = __shfl(ci, kWave - 1, kWave);
+    c_last = __builtin_amdgcn_readlane(ci, kWave - 1);
   }
 
   double best_g = -1e300, stay = 0.0;
@@ -418,14 +424,17 @@ __device__ __forceinline__ void move_core(int64_t v, uint32_t (&key)[NI], int nv
       if (better(g, ci, best_g, best_c)) { best_g = g; best_c = ci; }
     }
   }
-#pragma unroll
-  for (int m = 1; m < kWave; m <<= 1) {
-    const double og = __shfl_xor(best_g, m, kWave);
-    const int32_t oc = __shfl_xor(best_c, m, kWave);
-    if (better(og, oc, best_g, best_c)) { best_g = og; best_c = oc; }
-  }
+  argmax_step<1>(best_g, best_c);
+  argmax_step<2>(best_g, best_c);
+  argmax_step<4>(best_g, best_c);
+  argmax_step<8>(best_g, best_c);
+  argmax_step<16>(best_g, best_c);
+  argmax_step<32>(best_g, best_c);
   const uint64_t sm = __ballot(has_stay);
-  if (sm) stay = __shfl(stay, __ffsll(static_cast<long long>(sm)) - 1, kWave);
+  if (sm) {
+    const int src = __ffsll(static_cast<long long>(sm)) - 1;
+    stay = __longlong_as_double(static_cast<long long>(readlane_i64(__double_as_longlong(stay), src)));
+  }
   if (l == 0) target[v] = (best_g > stay + kTol) ? best_c : curr;
 }
 
