@@ -1,7 +1,7 @@
 // Shared device helpers and the C-ABI error state of libmsc_hip.
 //
-// Everything here is written for gfx950 (CDNA4): wave64, 64-bit lane masks, LDS-backed
-// cross-lane moves.  The library is built with -ffp-contract=off so every float operation
+// Everything here is written for gfx950 (CDNA4): wave64, 64-bit lane masks, cross-lane moves
+// by DPP where a pattern has one and through the LDS crossbar otherwise.  The library is built with -ffp-contract=off so every float operation
 // in a kernel is separately rounded unless a builtin asks otherwise; the exact-distance and
 // fixed-point parts of the neighbor-graph path rely on that.
 #pragma once
@@ -162,33 +162,37 @@ __device__ __forceinline__ int64_t wave_incl_sum_i64(int64_t x) {
 __device__ __forceinline__ uint64_t umin64(uint64_t a, uint64_t b) { return a < b ? a : b; }
 __device__ __forceinline__ uint64_t umax64(uint64_t a, uint64_t b) { return a < b ? b : a; }
 
-// Bitonic sort of one 64-bit key per lane, ascending across the 64 lanes of a wave.
-__device__ __forceinline__ uint64_t wave_sort_u64(uint64_t x) {
-  const int l = lane_id();
-#pragma unroll
-  for (int k = 2; k <= kWave; k <<= 1) {
-#pragma unroll
-    for (int j = k >> 1; j > 0; j >>= 1) {
-      uint64_t y = shfl_xor_u64(x, j);
-      bool up = (l & k) == 0;
-      bool lower = (l & j) == 0;
-      x = (lower == up) ? umin64(x, y) : umax64(x, y);
-    }
-  }
-  return x;
+// Bitonic sort of one 64-bit key per lane, ascending across the 64 lanes of a wave (all lanes
+// active).  Stages are unrolled at compile time so that each exchange is a lane_xor.
+template <int K, int J>
+__device__ __forceinline__ uint64_t wave_sort_stage(uint64_t x, int l) {
+  const uint64_t y = lane_xor_u64<J>(x);
+  const bool up = (l & K) == 0;
+  const bool lower = (l & J) == 0;
+  x = (lower == up) ? umin64(x, y) : umax64(x, y);
+  if constexpr (J > 1) return wave_sort_stage<K, J / 2>(x, l);
+  else return x;
 }
+template <int K>
+__device__ __forceinline__ uint64_t wave_sort_from(uint64_t x, int l) {
+  x = wave_sort_stage<K, K / 2>(x, l);
+  if constexpr (K < kWave) return wave_sort_from<2 * K>(x, l);
+  else return x;
+}
+__device__ __forceinline__ uint64_t wave_sort_u64(uint64_t x) { return wave_sort_from<2>(x, lane_id()); }
 
+template <int J>
+__device__ __forceinline__ uint64_t wave_merge_stage(uint64_t x, int l) {
+  const uint64_t y = lane_xor_u64<J>(x);
+  x = ((l & J) == 0) ? umin64(x, y) : umax64(x, y);
+  if constexpr (J > 1) return wave_merge_stage<J / 2>(x, l);
+  else return x;
+}
 // Merge: `cur` and `add` ascending across lanes; returns the 64 smallest of both, ascending.
 __device__ __forceinline__ uint64_t wave_merge_min_u64(uint64_t cur, uint64_t add_sorted) {
   const int l = lane_id();
-  uint64_t rev = shfl_u64(add_sorted, kWave - 1 - l);
-  uint64_t x = umin64(cur, rev);  // bitonic
-#pragma unroll
-  for (int j = kWave >> 1; j > 0; j >>= 1) {
-    uint64_t y = shfl_xor_u64(x, j);
-    x = ((l & j) == 0) ? umin64(x, y) : umax64(x, y);
-  }
-  return x;
+  const uint64_t rev = shfl_u64(add_sorted, kWave - 1 - l);
+  return wave_merge_stage<kWave / 2>(umin64(cur, rev), l);  // the minima form a bitonic sequence
 }
 
 // Order-preserving map float -> uint32 (total order; -0 < +0 is irrelevant here).
