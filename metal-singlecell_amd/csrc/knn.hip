@@ -381,8 +381,37 @@ __global__ __launch_bounds__(kThreads, 2) void select_kernel(
   auto tile_body = [&](const int buf, const int64_t t) {
 
     f32x4 acc[4][NB];
+    // BF16 with up to 4 K-steps: the next M-block's fragments are read from LDS while the
+    // current M-block's MFMAs run (two fragment sets, 8 * KS VGPRs), so the LDS latency hides
+    // behind 2 * KS MFMAs instead of two.  Same operands in the same order as below.
+    constexpr bool kAhead = BF16 && KS <= 4;
+    if constexpr (kAhead) {
+      bf16x8 af[2][KS];
 #pragma unroll
-    for (int mb = 0; mb < 4; ++mb) {
+      for (int s = 0; s < KS; ++s)
+        af[0][s] = *reinterpret_cast<const bf16x8*>(&tile[buf][lm * C::STRIDE] + 64 * s + 16 * g);
+#pragma unroll
+      for (int mb = 0; mb < 4; ++mb) {
+        if (mb + 1 < 4) {
+          const char* nrow = &tile[buf][(16 * (mb + 1) + lm) * C::STRIDE];
+#pragma unroll
+          for (int s = 0; s < KS; ++s)
+            af[(mb + 1) & 1][s] = *reinterpret_cast<const bf16x8*>(nrow + 64 * s + 16 * g);
+        }
+        // keep the next M-block's reads issued ahead of this M-block's MFMAs (the scheduler
+        // would otherwise sink each read next to its first use)
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) acc[mb][nb] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int s = 0; s < KS; ++s)
+#pragma unroll
+          for (int nb = 0; nb < NB; ++nb)
+            acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[mb & 1][s], qb[nb][s], acc[mb][nb], 0, 0, 0);
+      }
+    }
+#pragma unroll
+    for (int mb = 0; mb < (kAhead ? 0 : 4); ++mb) {
 #pragma unroll
       for (int nb = 0; nb < NB; ++nb) acc[mb][nb] = f32x4{0.f, 0.f, 0.f, 0.f};
       const char* arow = &tile[buf][(16 * mb + lm) * C::STRIDE];
