@@ -369,27 +369,28 @@ __global__ __launch_bounds__(kThreads, 2) void select_kernel(
 #define MSC_STORE_TILE(S, B) { MSC_STORE1(S, 0, B) MSC_STORE1(S, 1, B) MSC_STORE1(S, 2, B) MSC_STORE1(S, 3, B) \
                                MSC_STORE1(S, 4, B) MSC_STORE1(S, 5, B) MSC_STORE1(S, 6, B) MSC_STORE1(S, 7, B) }
 
-  if (ntiles > 0) {
-    MSC_LOAD_TILE(pb, 0);
-    MSC_STORE_TILE(pb, 0);
-  }
-  __syncthreads();
-  if (ntiles > 1) MSC_LOAD_TILE(pa, 1);
-  if (ntiles > 2) MSC_LOAD_TILE(pb, 2);
 
-  // one candidate tile: MFMA distance block from LDS buffer buf, then the fused selection
-  auto tile_body = [&](const int buf, const int64_t t) {
+  // BF16 with up to 4 K-steps: the next M-block's fragments are read from LDS while the
+  // current M-block's MFMAs run (two fragment sets, 8 * KS VGPRs), so the LDS latency hides
+  // behind 2 * KS MFMAs instead of two.  A tile's first M-block is read right after the barrier
+  // that publishes it, before the previous tile's selection.  Same operands in the same order
+  // as the plain loop below.
+  constexpr bool kAhead = BF16 && KS <= 4;
+  bf16x8 af0[kAhead ? KS : 1];
+  auto read_first = [&](const int buf) {
+    if constexpr (kAhead) {
+#pragma unroll
+      for (int s = 0; s < KS; ++s)
+        af0[s] = *reinterpret_cast<const bf16x8*>(&tile[buf][lm * C::STRIDE] + 64 * s + 16 * g);
+    }
+  };
 
-    f32x4 acc[4][NB];
-    // BF16 with up to 4 K-steps: the next M-block's fragments are read from LDS while the
-    // current M-block's MFMAs run (two fragment sets, 8 * KS VGPRs), so the LDS latency hides
-    // behind 2 * KS MFMAs instead of two.  Same operands in the same order as below.
-    constexpr bool kAhead = BF16 && KS <= 4;
+  // one candidate tile's MFMA distance block from LDS buffer buf
+  auto tile_mfma = [&](const int buf, f32x4 (&acc)[4][NB]) {
     if constexpr (kAhead) {
       bf16x8 af[2][KS];
 #pragma unroll
-      for (int s = 0; s < KS; ++s)
-        af[0][s] = *reinterpret_cast<const bf16x8*>(&tile[buf][lm * C::STRIDE] + 64 * s + 16 * g);
+      for (int s = 0; s < KS; ++s) af[0][s] = af0[s];
 #pragma unroll
       for (int mb = 0; mb < 4; ++mb) {
         if (mb + 1 < 4) {
@@ -436,7 +437,10 @@ __global__ __launch_bounds__(kThreads, 2) void select_kernel(
       }
     }
 
-    // ---- fused selection ----
+  };
+
+  // the fused selection over one tile's distance block
+  auto tile_select = [&](const f32x4 (&acc)[4][NB], const int64_t t) {
     // Hits (acc <= the lane's list tail) are appended to the lane's LDS buffer; buffers are
     // merged into the sorted register lists when any lane's buffer could overflow on the
     // next M-block, so a rare hit costs one LDS store instead of a wave-wide insertion.
@@ -487,24 +491,49 @@ __global__ __launch_bounds__(kThreads, 2) void select_kernel(
 
   };
 
-  // Tiles in pairs: tile t reads buffer 0 while tile t+1 (set A, loaded two tiles earlier) is
-  // stored to buffer 1, whose last reader (tile t-1) finished before the previous barrier;
-  // then the same with the roles swapped.  One barrier per tile publishes the stored tile.
-  // The compiler fences keep each store after its tile's compute, so the load latency hides
-  // behind two tiles of work.
-  for (int64_t t = 0; t < ntiles; t += 2) {
-    tile_body(0, t);
-    asm volatile("" ::: "memory");
-    if (t + 1 < ntiles) MSC_STORE_TILE(pa, 1);
+  // Tiles in pairs: tile t's MFMAs read buffer 0, then tile t+1 (set A, loaded two tiles
+  // earlier) is stored to buffer 1, whose last reader (tile t-1's MFMAs) finished before the
+  // previous barrier; then the same with the roles swapped.  One barrier per tile publishes the
+  // stored tile; tile t's selection runs after it, behind the first reads of tile t+1, so the
+  // MFMA drain overlaps the barrier and the LDS latency overlaps the selection.  The compiler
+  // fences keep each store after its tile's MFMAs and each read-ahead before the selection, so
+  // the global load latency hides behind two tiles of work.
+  // Every load and store is unconditional: a tile index past the end is clamped to the last
+  // tile (already read in full, so the repeat stays in bounds) and its store goes to the buffer
+  // that no later tile reads.  Conditional loads would make the compiler's wait counts assume
+  // the worst at every join, so a tile's store would also wait for the next tile's loads.
+#define MSC_CLAMP(T) ((T) < ntiles ? (T) : ntiles - 1)
+  if (ntiles > 0) {  // block-uniform
+    MSC_LOAD_TILE(pb, 0);
+    MSC_STORE_TILE(pb, 0);
     __syncthreads();
-    if (t + 3 < ntiles) MSC_LOAD_TILE(pa, t + 3);
-    if (t + 1 >= ntiles) break;
-    tile_body(1, t + 1);
-    asm volatile("" ::: "memory");
-    if (t + 2 < ntiles) MSC_STORE_TILE(pb, 0);
-    __syncthreads();
-    if (t + 4 < ntiles) MSC_LOAD_TILE(pb, t + 4);
+    MSC_LOAD_TILE(pa, MSC_CLAMP(1));
+    asm volatile("" ::: "memory");  // set A's loads before set B's, the order the loop keeps
+    MSC_LOAD_TILE(pb, MSC_CLAMP(2));
+    read_first(0);
+    f32x4 acc[4][NB];
+    for (int64_t t = 0;; t += 2) {
+      tile_mfma(0, acc);
+      asm volatile("" ::: "memory");
+      MSC_STORE_TILE(pa, 1);
+      __syncthreads();
+      read_first(1);
+      MSC_LOAD_TILE(pa, MSC_CLAMP(t + 3));
+      asm volatile("" ::: "memory");
+      tile_select(acc, t);
+      if (t + 1 >= ntiles) break;
+      tile_mfma(1, acc);
+      asm volatile("" ::: "memory");
+      MSC_STORE_TILE(pb, 0);
+      __syncthreads();
+      read_first(0);
+      MSC_LOAD_TILE(pb, MSC_CLAMP(t + 4));
+      asm volatile("" ::: "memory");
+      tile_select(acc, t + 1);
+      if (t + 2 >= ntiles) break;
+    }
   }
+#undef MSC_CLAMP
 #undef MSC_LOAD_TILE
 #undef MSC_STORE_TILE
 #undef MSC_LOAD1
