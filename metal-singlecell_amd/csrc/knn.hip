@@ -512,7 +512,11 @@ __global__ __launch_bounds__(kThreads, 2) void select_kernel(
     MSC_LOAD_TILE(pb, MSC_CLAMP(2));
     read_first(0);
     f32x4 acc[4][NB];
-    for (int64_t t = 0;; t += 2) {
+    // The pair loop has its only exit at the bottom (an exit between the two halves would give
+    // the loop header a path on which set A's loads are the newest, and the wait counts would
+    // follow it).  An odd last tile is in buffer 0 with its first fragments read.
+    int64_t t = 0;
+    for (; t + 1 < ntiles; t += 2) {
       tile_mfma(0, acc);
       asm volatile("" ::: "memory");
       MSC_STORE_TILE(pa, 1);
@@ -521,7 +525,6 @@ __global__ __launch_bounds__(kThreads, 2) void select_kernel(
       MSC_LOAD_TILE(pa, MSC_CLAMP(t + 3));
       asm volatile("" ::: "memory");
       tile_select(acc, t);
-      if (t + 1 >= ntiles) break;
       tile_mfma(1, acc);
       asm volatile("" ::: "memory");
       MSC_STORE_TILE(pb, 0);
@@ -530,7 +533,10 @@ __global__ __launch_bounds__(kThreads, 2) void select_kernel(
       MSC_LOAD_TILE(pb, MSC_CLAMP(t + 4));
       asm volatile("" ::: "memory");
       tile_select(acc, t + 1);
-      if (t + 2 >= ntiles) break;
+    }
+    if (t < ntiles) {
+      tile_mfma(0, acc);
+      tile_select(acc, t);
     }
   }
 #undef MSC_CLAMP
