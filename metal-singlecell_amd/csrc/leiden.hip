@@ -553,11 +553,11 @@ __global__ __launch_bounds__(kThreads) void k_move_light(
           wr[i] = wq[e0 + l];
         }
       }
+      // Unconditional gathers (lanes without a neighbour read record 0, which `ok` below never
+      // uses): a conditional load lets the compiler move the first use of its value into the
+      // branch, and the wait it then needs there serializes the eight gathers.
 #pragma unroll
-      for (int i = 0; i < kMoveGroup; ++i) {
-        r[i] = make_int4(0, 0, 0, 0);
-        if (u[i] >= 0) r[i] = reinterpret_cast<const int4*>(cs)[u[i]];
-      }
+      for (int i = 0; i < kMoveGroup; ++i) r[i] = reinterpret_cast<const int4*>(cs)[u[i] >= 0 ? u[i] : 0];
 #pragma unroll
       for (int i = 0; i < kMoveGroup; ++i) {
         if ((small >> i) & 1) {
