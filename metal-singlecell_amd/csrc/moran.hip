@@ -204,15 +204,16 @@ __global__ __launch_bounds__(kThreads) void k_perm_partial(
     for (int rr = 0; rr < nr; ++rr) {
       float y[VEC];
       load_vec<VEC>(Y + (rb + rr) * G + g0, y);
+      // All kPermBatch gathers are issued before the first use, unconditionally: a permutation
+      // past P reads row 0 (its jtab entry) into an accumulator that is never written.  A
+      // branch or a use between the loads would make each gather wait for the previous one.
+      float x[kPermBatch][VEC];
 #pragma unroll
-      for (int b = 0; b < kPermBatch; ++b) {
-        if (p0 + b < P) {
-          float x[VEC];
-          load_vec<VEC>(X + static_cast<int64_t>(jtab[b][rr]) * G + g0, x);
+      for (int b = 0; b < kPermBatch; ++b) load_vec<VEC>(X + static_cast<int64_t>(jtab[b][rr]) * G + g0, x[b]);
 #pragma unroll
-          for (int v = 0; v < VEC; ++v) acc[b][v] += (static_cast<double>(x[v]) - m[v]) * static_cast<double>(y[v]);
-        }
-      }
+      for (int b = 0; b < kPermBatch; ++b)
+#pragma unroll
+        for (int v = 0; v < VEC; ++v) acc[b][v] += (static_cast<double>(x[b][v]) - m[v]) * static_cast<double>(y[v]);
     }
   }
   if (!gok) return;
@@ -617,17 +618,17 @@ __global__ __launch_bounds__(kThreads) void k_geary_partial(
       double y[VEC];
 #pragma unroll
       for (int v = 0; v < VEC; ++v) y[v] = Y64[r * G + g0 + v];
+      // every gather issued before the first use, unconditionally (as in k_perm_partial)
+      float x[kPermBatch][VEC];
+#pragma unroll
+      for (int b = 0; b < kPermBatch; ++b) load_vec<VEC>(X + static_cast<int64_t>(jtab[b][rr]) * G + g0, x[b]);
 #pragma unroll
       for (int b = 0; b < kPermBatch; ++b) {
-        if (p0 + b < P) {
-          float x[VEC];
-          load_vec<VEC>(X + static_cast<int64_t>(jtab[b][rr]) * G + g0, x);
 #pragma unroll
-          for (int v = 0; v < VEC; ++v) {
-            const double a = static_cast<double>(x[v]) - m[v];
-            accA[b][v] += (a * a) * Rr;
-            accB[b][v] += a * y[v];
-          }
+        for (int v = 0; v < VEC; ++v) {
+          const double a = static_cast<double>(x[b][v]) - m[v];
+          accA[b][v] += (a * a) * Rr;
+          accB[b][v] += a * y[v];
         }
       }
     }
