@@ -45,6 +45,19 @@ def test_knn_gaussian_500x20_all_k(precision, k):
 
 
 @pytest.mark.parametrize("precision", ["fp32", "bf16"])
+@pytest.mark.parametrize("d", [12, 50])
+@pytest.mark.parametrize("n", [8, 64, 65, 129, 192, 193, 257])
+def test_knn_candidate_tile_counts(precision, d, n):
+    """1 to 5 candidate tiles of 64 rows: a single (odd) tile, whole and partial last tiles, the
+    pair loop with and without the odd last tile, and tile loads clamped past the end."""
+    rng = np.random.default_rng(n + d)
+    X = rng.standard_normal((n, d)).astype(np.float32)
+    idx, dist, _ = _run(X, 5, precision)
+    ri, rd = _oracle(X, 5)
+    assert np.array_equal(idx, ri) and np.array_equal(dist, rd)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16"])
 def test_knn_duplicates_self_first(precision):
     rng = np.random.default_rng(0)
     X = rng.standard_normal((60, 10)).astype(np.float32)
