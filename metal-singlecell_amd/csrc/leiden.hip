@@ -303,7 +303,7 @@ constexpr int kMidDeg = 1024;                        // k_move_block<256>: 4 slo
 constexpr int64_t kMidMaxN = int64_t(1) << 22;      // community ids below 2^(32 - 10)
 constexpr int kBigDeg = 4096;                        // k_move_block<1024>
 constexpr int64_t kBigMaxN = int64_t(1) << 20;      // community ids below 2^(32 - 12)
-constexpr int kMoveGroup = 8;  // consecutive vertices per wave in k_move_light / k_frontier_push
+constexpr int kMoveGroup = 8;  // consecutive vertices per wave in k_move_light
 
 // Bitonic network over the 64 * NI slots e = i * 64 + l of a wave (keys in registers).  The
 // stage sequence is unrolled by template recursion so that every register index is a
@@ -1116,10 +1116,12 @@ __global__ __launch_bounds__(kThreads) void k_frontier_push(const int64_t* __res
                                                             const int32_t* __restrict__ part,
                                                             const int64_t* __restrict__ sigtot,
                                                             CommSig* __restrict__ cs) {
+  // one vertex per lane for the record rewrite and the commit scan; each committed vertex's
+  // neighbours are then marked by the whole wave
   const int l = lane_id();
-  const int64_t v0 = (static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + threadIdx.x / kWave) * kMoveGroup;
-  if (refresh && l < kMoveGroup && v0 + l < n) refresh_one(v0 + l, comm, part, sigtot, cs);
-  uint64_t pending = __ballot(l < kMoveGroup && v0 + l < n && commit[v0 + l]);
+  const int64_t v0 = (static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + threadIdx.x / kWave) * kWave;
+  if (refresh && v0 + l < n) refresh_one(v0 + l, comm, part, sigtot, cs);
+  uint64_t pending = __ballot(v0 + l < n && commit[v0 + l]);
   while (pending) {
     const int64_t v = v0 + __ffsll(static_cast<long long>(pending)) - 1;
     pending &= pending - 1;
@@ -1502,7 +1504,7 @@ static int local_moving(Ctx& c, const Graph& g, int64_t seed, const int32_t* par
     sig_valid = inc;
     n_inc += inc ? 1 : 0;
     // after an incremental update the push also rewrites cs for the next pass
-    k_frontier_push<<<grid_for(ceil_div(g.n, kMoveGroup) * kWave), kThreads, 0, st>>>(
+    k_frontier_push<<<grid_for(g.n), kThreads, 0, st>>>(
         g.indptr, g.indices, g.n, c.commit, c.active, inc ? 1 : 0, c.comm, part, c.sigtot, c.cs);
     MSC_LAUNCH_CHECK();
     if (sync) {
