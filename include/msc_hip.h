@@ -46,16 +46,17 @@ const char* msc_last_error(void);
  * out_idx n x k int32, out_dist n x k float32 (= sqrt of that squared distance).
  * precision: MSC_PRECISION_FP32 ranks candidates with FP32 MFMA; MSC_PRECISION_BF16_RERANK
  * ranks them with BF16 MFMA (FP32 accumulation).  Both re-rank 64 admitted candidates
- * exactly and certify each row; BF16 rows that fail their certificate are re-selected with
- * FP32 MFMA, and rows no pass certifies are recomputed by exact brute force, so the two
- * precisions return identical results.  Limits: 1 <= k <= 64, k <= n < 2^31,
+ * exactly and certify each row; rows that fail their certificate are selected again over
+ * all candidates by a wide pass of the same precision (16 queries per wave, 128 admitted
+ * candidates per query), and rows no pass certifies are recomputed by exact brute force, so
+ * the two precisions return identical results.  Limits: 1 <= k <= 64, k <= n < 2^31,
  * d <= 127 (FP32) / 125 (BF16).
  * ------------------------------------------------------------------------------------- */
 typedef struct {
   int timing;           /* in: nonzero -> record per-stage times (adds one stream sync) */
   int n_splits;         /* out: candidate splits used by the select kernel */
   int list_len;         /* out: admitted candidates per query */
-  int64_t n_second_pass;/* out: BF16 rows re-selected with FP32 MFMA (certificate failed) */
+  int64_t n_second_pass;/* out: rows re-selected by the wide pass (certificate failed) */
   int64_t n_fallback;   /* out: rows recomputed by exact brute force */
   double prep_ms, select_ms, finalize_ms, second_pass_ms, fallback_ms; /* out (timing only) */
 } msc_knn_info;

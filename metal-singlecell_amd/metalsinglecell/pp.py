@@ -23,16 +23,25 @@ def _dense(X):
 
 
 def _choose_representation(adata, use_rep, n_pcs):
-    if use_rep is None and n_pcs == 0:
+    """scanpy's representation rule (reference pp.py:722-745).
+
+    ``use_rep=None``: a narrow object (at most 50 variables) is clustered on ``.X`` whether or
+    not it holds ``X_pca``; a wide one on ``obsm['X_pca']``, which must have at least
+    ``n_pcs or 50`` columns.  Where the reference would compute that PCA itself, this build
+    raises: the PCA front-end is not part of the neighbour-graph path.
+    """
+    wide = adata.n_vars > _N_PCS_DEFAULT
+    if use_rep is None and (n_pcs == 0 or not wide):
         return np.asarray(_dense(adata.X), dtype=np.float32), None
     if use_rep is None:
-        if "X_pca" in adata.obsm and np.asarray(adata.obsm["X_pca"]).shape[1] >= (n_pcs or 0):
-            return np.asarray(adata.obsm["X_pca"], dtype=np.float32), "X_pca"
-        if adata.n_vars > _N_PCS_DEFAULT:
+        need = n_pcs or _N_PCS_DEFAULT
+        have = np.asarray(adata.obsm["X_pca"]).shape[1] if "X_pca" in adata.obsm else 0
+        if have < need:
             raise NotImplementedError(
-                "no usable obsm['X_pca'] and more than 50 variables: compute the PCA first "
-                "(the PCA front-end is not part of this MI355X neighbour-graph build)")
-        return np.asarray(_dense(adata.X), dtype=np.float32), None
+                f"{adata.n_vars} variables and obsm['X_pca'] has {have} < {need} columns: "
+                "compute the PCA first (the PCA front-end is not part of this MI355X "
+                "neighbour-graph build)")
+        return np.asarray(adata.obsm["X_pca"], dtype=np.float32), "X_pca"
     if use_rep == "X":
         return np.asarray(_dense(adata.X), dtype=np.float32), "X"
     if use_rep not in adata.obsm:

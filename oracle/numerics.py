@@ -1,6 +1,6 @@
 """Shared float32 / integer numerics of the oracle — TEST INFRASTRUCTURE ONLY.
 
-Two definitions are restated identically in ``metal-singlecell_amd/csrc/numerics.h`` so that
+Two definitions are restated identically in ``metal-singlecell_amd/csrc/common.h`` so that
 the HIP kernels and this oracle produce bit-identical results:
 
 ``expf32``   exp(x) in float32 by Cody-Waite range reduction (x = n ln2 + r, with the

@@ -54,3 +54,32 @@ def test_fuzzy_hub_in_degree():
     X = rng.standard_normal((3000, 3)).astype(np.float32) * 10
     X[:200] = rng.standard_normal((200, 3)).astype(np.float32) * 0.01
     _check(X, 15)
+
+
+def test_fuzzy_floor_and_duplicate_branches_on_hand_built_rows():
+    """The sigma floor with the row mean (ties at rho), the floor with the global mean (a row
+    of duplicates, rho = 0) and rho skipping zero-distance duplicates, on a hand-built kNN:
+    sigma, rho and the CSR bit-identical to the oracle, whose values these rows pin."""
+    import torch
+    from metalsinglecell import _hip
+    from oracle import fuzzy
+    k, n = 9, 24
+    pattern = np.array([[0, 2, 2, 2, 2, 5, 6, 7, 8],           # floor binds, row mean
+                        [0, 0, 0, 0, 0, 0, 0, 0, 0],           # rho = 0, global mean
+                        [0, 0, 0, 1.5, 2.0, 2.5, 3.0, 3.5, 4.0],  # rho skips duplicates
+                        [0, 1, 1.5, 2, 2.5, 3, 3.5, 4, 4.5]], np.float32)
+    dist = pattern[np.arange(n) % 4]
+    idx = (np.arange(n)[:, None] + np.arange(k)[None, :] * 5) % n
+    assert all(np.unique(r).size == k for r in idx)
+    dev = _hip.device()
+    ti = torch.from_numpy(idx.astype(np.int32)).to(dev)
+    td = torch.from_numpy(dist).to(dev)
+    (ip, ind, dat), (sig, rho) = _hip.fuzzy_connectivities(ti, td)
+    R, rsig, rrho = fuzzy.fuzzy_simplicial_set(idx, dist)
+    assert np.array_equal(rho.cpu().numpy(), rrho) and np.array_equal(sig.cpu().numpy(), rsig)
+    assert rrho[0] == 2.0 and rrho[1] == 0.0 and rrho[2] == np.float32(1.5)
+    assert rsig[0] == np.float32(1e-3 * 34 / 9)
+    assert rsig[1] == np.float32(1e-3 * float(dist.astype(np.float64).sum()) / dist.size)
+    P = sp.csr_matrix((dat.cpu().numpy(), ind.cpu().numpy(), ip.cpu().numpy()), shape=(n, n))
+    assert np.array_equal(P.indptr, R.indptr) and np.array_equal(P.indices, R.indices)
+    assert np.array_equal(P.data, R.data)

@@ -140,3 +140,32 @@ def test_autotune_cache_roundtrip(tmp_path, monkeypatch):
     assert autotune.lookup(off, 30, 15, "fp32") is None
     off2 = np.array([0, 500, 1700, 58_000])          # same buckets -> same entry
     assert autotune.lookup(off2, 30, 15, "bf16") == 4
+
+
+def test_choose_representation_follows_the_reference_rule():
+    """reference pp.py:732-739: narrow objects use .X even when X_pca exists; wide objects use
+    X_pca only when it has n_pcs (or 50) columns, else the reference computes a PCA (raises
+    here); n_pcs=0 means .X; an unknown use_rep raises ValueError."""
+    from metalsinglecell._anndata import AnnData
+    from metalsinglecell.pp import _choose_representation
+    rng = np.random.default_rng(0)
+    narrow = AnnData(rng.normal(size=(30, 40)).astype(np.float32))
+    narrow.obsm["X_pca"] = rng.normal(size=(30, 10)).astype(np.float32)
+    rep, key = _choose_representation(narrow, None, None)
+    assert key is None and rep.shape == (30, 40) and np.array_equal(rep, narrow.X)
+    rep, key = _choose_representation(narrow, "X_pca", None)
+    assert key == "X_pca" and rep.shape == (30, 10)
+    wide = AnnData(rng.normal(size=(30, 60)).astype(np.float32))
+    wide.obsm["X_pca"] = rng.normal(size=(30, 30)).astype(np.float32)
+    with pytest.raises(NotImplementedError, match="30 < 50"):
+        _choose_representation(wide, None, None)
+    rep, key = _choose_representation(wide, None, 20)
+    assert key == "X_pca" and rep.shape == (30, 30)
+    with pytest.raises(NotImplementedError, match="30 < 40"):
+        _choose_representation(wide, None, 40)
+    rep, key = _choose_representation(wide, None, 0)
+    assert key is None and rep.shape == (30, 60)
+    rep, key = _choose_representation(wide, "X", None)
+    assert key == "X" and rep.shape == (30, 60)
+    with pytest.raises(ValueError, match="Did not find"):
+        _choose_representation(wide, "X_umap", None)

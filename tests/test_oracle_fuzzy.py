@@ -77,3 +77,57 @@ def test_duplicates_no_nan():
     idx, _, dist = knn.exact_knn(X, 10)
     P, _, _ = fuzzy.fuzzy_simplicial_set(idx, dist)
     assert not np.isnan(P.data).any()
+
+
+def test_sigma_floor_uses_the_row_mean_when_rho_is_positive():
+    """Four neighbours tie at rho, so psum >= 4 > log2(8) for every sigma: the bisection halves
+    mid 64 times and the floor 1e-3 * mean(row i) (column 0 included) must bind.  A second row
+    with far larger distances makes the global mean differ from the row mean, so taking the
+    wrong mean fails; a third row that converges normally must keep its bisection sigma."""
+    k = 8
+    idx = np.array([[i] + [(i + j) % 3 + 3 * j for j in range(1, k)] for i in range(3)])
+    dist = np.array([[0, 2, 2, 2, 2, 5, 6, 7],
+                     [0, 100, 150, 200, 250, 300, 350, 400],
+                     [0, 1, 1.5, 2, 2.5, 3, 3.5, 4]], np.float32)
+    sigma, rho = fuzzy.smooth_knn_dist(dist, k)
+    assert rho[0] == 2.0 and rho[1] == 100.0 and rho[2] == 1.0
+    assert sigma[0] == np.float32(1e-3 * (0 + 2 * 4 + 5 + 6 + 7) / 8)
+    mean_all = float(dist.astype(np.float64).sum()) / dist.size
+    assert sigma[0] != np.float32(1e-3 * mean_all)
+    # rows 1 and 2 converge above their floors: sum exp(-(d - rho)/sigma) = log2 k
+    for i in (1, 2):
+        assert sigma[i] > 1e-3 * dist[i].mean()
+        s = np.exp(-np.maximum(dist[i, 1:] - rho[i], 0) / np.float64(sigma[i])).sum()
+        assert abs(s - 3.0) < 1e-4
+    v = fuzzy.membership(idx, dist, sigma, rho)
+    assert np.all(v[0, 1:5] == 1.0)              # the tied nearest neighbours weigh 1
+    assert np.all(v[0, 5:] < 1e-30)              # exp(-(3..5)/3.25e-3) underflows
+
+
+def test_sigma_floor_uses_the_global_mean_when_rho_is_zero():
+    """A row of exact duplicates has no positive distance: rho = 0, every term is 1, psum =
+    k-1 > log2 k, and the floor is 1e-3 * the mean of ALL n*k distances."""
+    k = 6
+    dist = np.array([[0, 0, 0, 0, 0, 0],
+                     [0, 1, 2, 3, 4, 5],
+                     [0, 10, 20, 30, 40, 50]], np.float32)
+    sigma, rho = fuzzy.smooth_knn_dist(dist, k)
+    assert rho[0] == 0.0
+    mean_all = float(dist.astype(np.float64).sum()) / dist.size
+    assert sigma[0] == np.float32(1e-3 * mean_all)
+    assert sigma[0] != np.float32(0.0)
+
+
+def test_rho_skips_zero_distance_duplicates():
+    """Duplicates sit at distance 0 after the point itself; rho is the first STRICTLY positive
+    distance (column 3 here), not column 1, and the duplicates get weight 1."""
+    k = 9
+    idx = np.arange(9)[None, :]
+    dist = np.array([[0, 0, 0, 1.5, 2.0, 2.5, 3.0, 3.5, 4.0]], np.float32)
+    sigma, rho = fuzzy.smooth_knn_dist(dist, k)
+    assert rho[0] == np.float32(1.5)
+    v = fuzzy.membership(idx, dist, sigma, rho)
+    assert v[0, 0] == 0.0 and np.all(v[0, 1:4] == 1.0)
+    # the five remaining terms must make up log2(9) - 3 at the converged sigma
+    s = np.exp(-(dist[0, 4:].astype(np.float64) - 1.5) / np.float64(sigma[0])).sum()
+    assert abs(3.0 + s - np.log2(9)) < 1e-4
