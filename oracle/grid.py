@@ -128,3 +128,54 @@ def radius_graph(coords, r_max, r_min=0.0, exact=None):
     A = sp.csr_matrix((np.ones(dist.size, np.float32), cols, indptr), shape=(n, n))
     D = sp.csr_matrix((dist, cols, indptr.copy()), shape=(n, n))
     return A, D
+
+
+def section_graph_dense(xy64, codes=None, n_neighs=None, radius=None, percentile=None,
+                        set_diag=False, transform=None):
+    """Dense ``(connectivities, distances)`` of a spatial graph after squidpy's tail, built
+    section by section as the reference does (gr.py:81-147, 198-252): per section (cells with
+    one library code, in code order) the directed kNN (``n_neighs``) or radius graph of that
+    section alone; edges longer than the section's ``percentile`` of edge lengths removed
+    from both matrices; ``set_diag`` puts 1.0 on the connectivity diagonal; the distances
+    keep no diagonal and no zero; ``transform`` "spectral" scales by the column degrees
+    (``A / sqrt(deg_i deg_j)``, 0 where a degree is 0), "cosine" is the cosine similarity of
+    the rows.  Dense float64 n x n matrices: a different formulation from the product's
+    sparse assembly, for a few thousand cells."""
+    X64 = np.asarray(xy64, dtype=np.float64)
+    n = X64.shape[0]
+    codes = np.zeros(n, np.int64) if codes is None else np.asarray(codes)
+    A = np.zeros((n, n))
+    D = np.zeros((n, n))
+    for c in np.unique(codes):
+        ids = np.flatnonzero(codes == c)
+        sub = X64[ids]
+        m = ids.size
+        E = np.zeros((m, m), bool)
+        if n_neighs is not None:
+            k = min(int(n_neighs), m - 1)
+            if k > 0:
+                idx, _ = spatial_knn(sub.astype(np.float32), k)
+                E[np.repeat(np.arange(m), k), idx.ravel()] = True
+        else:
+            lo, hi = (0.0, float(radius)) if np.isscalar(radius) else (min(radius), max(radius))
+            ip, cols, _ = radius_neighbors(sub.astype(np.float32), hi, lo)
+            E[np.repeat(np.arange(m), np.diff(ip)), cols] = True
+        L = np.sqrt(((sub[:, None, :] - sub[None, :, :]) ** 2).sum(-1))
+        if percentile is not None and E.any():
+            E &= L <= np.percentile(L[E], percentile)
+        As = E.astype(np.float64)
+        if set_diag:
+            np.fill_diagonal(As, 1.0)
+        Ds = np.where(E, L, 0.0)
+        np.fill_diagonal(Ds, 0.0)
+        A[np.ix_(ids, ids)] = As
+        D[np.ix_(ids, ids)] = Ds
+    if transform == "spectral":
+        deg = A.sum(axis=0)
+        inv = np.where(deg > 0, 1.0 / np.sqrt(np.where(deg > 0, deg, 1.0)), 0.0)
+        A = A * inv[:, None] * inv[None, :]
+    elif transform == "cosine":
+        norm = np.sqrt((A * A).sum(axis=1))
+        U = np.where(norm[:, None] > 0, A / np.where(norm > 0, norm, 1.0)[:, None], 0.0)
+        A = U @ U.T
+    return A, D

@@ -76,22 +76,75 @@ def test_moran_frame_columns_and_order():
     assert not [c for c in df.columns if "fdr" in c]
 
 
-def test_spatial_graph_tail():
+def _host_knn_edges(n_neighs):
+    """A host stand-in for the device builder (the brute-force oracle), so the assembly logic
+    runs on the CPU."""
+    from oracle import grid
+
+    def edges_of(xy32, xy64):
+        m = xy32.shape[0]
+        k = min(n_neighs, m - 1)
+        idx, _ = grid.spatial_knn(xy32, k)
+        rows = np.repeat(np.arange(m), k)
+        lens = np.sqrt(((xy64[rows] - xy64[idx.ravel()]) ** 2).sum(1))
+        return np.arange(m + 1) * k, idx.ravel(), lens
+    return edges_of
+
+
+@pytest.mark.parametrize("transform", [None, "spectral", "cosine"])
+@pytest.mark.parametrize("set_diag", [False, True])
+@pytest.mark.parametrize("percentile", [None, 60.0])
+def test_section_graph_assembly_equals_dense_oracle(transform, set_diag, percentile):
+    """Interleaved libraries (one with duplicate points, so zero lengths occur), per-library
+    percentile, set_diag and both transforms against the dense per-section oracle."""
+    import pandas as pd
     from metalsinglecell import gr
-    n = 6
-    cols = np.array([1, 2, 0, 2, 0, 1, 4, 5, 3, 5, 3, 4])
-    indptr = np.arange(n + 1) * 2
-    A = sp.csr_matrix((np.ones(12, np.float32), cols, indptr), shape=(n, n))
-    D = sp.csr_matrix((np.arange(1, 13, dtype=np.float64), cols, indptr), shape=(n, n))
-    a, d = gr._finalize(A, D, None, True, None)
-    assert np.all(a.diagonal() == 1.0) and d.diagonal().sum() == 0
-    a, d = gr._finalize(A, D, None, False, 50)
-    assert d.data.max() <= np.percentile(np.arange(1, 13), 50)
-    a, _ = gr._finalize(A, D, "spectral", False, None)
+    from metalsinglecell._anndata import AnnData
+    from oracle import grid
+    rng = np.random.default_rng(8)
+    n = 300
+    codes = rng.integers(0, 3, n)
+    xy = rng.uniform(0, 100, (n, 2)) + codes[:, None] * 30.0      # sections overlap in space
+    dup = np.flatnonzero(codes == 1)[:6]
+    xy[dup[3:]] = xy[dup[:3]]
+    a = AnnData(np.zeros((n, 2), np.float32))
+    a.obsm["spatial"] = xy
+    a.obs["lib"] = pd.Categorical(np.array(["s0", "s1", "s2", "unused"])[codes],
+                                  categories=["s2", "unused", "s0", "s1"])
+    adj, dist = gr._section_graphs(a, "spatial", "lib", _host_knn_edges(6), transform,
+                                   set_diag, percentile)
+    A, D = grid.section_graph_dense(xy, codes, n_neighs=6, percentile=percentile,
+                                    set_diag=set_diag, transform=transform)
+    assert adj.shape == dist.shape == (n, n)
+    assert dist.dtype == np.float64 and not (dist.data == 0).any() and dist.diagonal().sum() == 0
+    assert np.array_equal(dist.toarray(), D)
+    if transform is None:
+        assert adj.dtype == np.float32 and np.array_equal(adj.toarray(), A)
+    else:
+        # sklearn's cosine keeps the float32 of the binary matrix (as the reference's call)
+        tol = 1e-12 if transform == "spectral" else 1e-6
+        np.testing.assert_allclose(adj.toarray(), A, rtol=tol, atol=tol * 1e-3)
+    # no edge crosses between libraries
+    r, c = adj.nonzero()
+    assert np.array_equal(codes[r], codes[c])
+    if percentile is None and transform is None:
+        # each row lists its neighbours nearest first (self-edge first with set_diag)
+        row = adj[10]
+        d = np.sqrt(((xy[row.indices] - xy[10]) ** 2).sum(1))
+        assert np.all(np.diff(d) >= 0)
+
+
+def test_spatial_graph_helpers():
+    from metalsinglecell import gr
+    A = gr._edges_to_csr(4, np.array([2, 0, 2, 1]), np.array([1, 3, 0, 2]),
+                         np.ones(4, np.float32), self_value=np.float32(1.0))
+    assert A.indptr.tolist() == [0, 2, 4, 7, 8]
+    assert A.indices.tolist() == [0, 3, 1, 2, 2, 1, 0, 3]    # self first, then list order
+    S = gr._spectral(A)
     deg = np.asarray(A.sum(axis=0)).ravel()
-    np.testing.assert_allclose(a.toarray(), A.toarray() / np.sqrt(np.outer(deg, deg)))
+    np.testing.assert_allclose(S.toarray(), A.toarray() / np.sqrt(np.outer(deg, deg)))
     with pytest.raises(NotImplementedError, match="transform"):
-        gr._transform_adj(A, "banana")
+        gr._require_transform("banana")
 
 
 def test_api_errors_before_any_gpu_work():
@@ -169,3 +222,55 @@ def test_choose_representation_follows_the_reference_rule():
     assert key == "X" and rep.shape == (30, 60)
     with pytest.raises(ValueError, match="Did not find"):
         _choose_representation(wide, "X_umap", None)
+
+
+def test_autocorr_value_sources():
+    """squidpy's extraction modes: var genes (HVG default, explicit names, layers, .raw with
+    names missing from .raw dropped), numeric obs columns, obsm columns by position."""
+    import types
+    import pandas as pd
+    from metalsinglecell import gr
+    from metalsinglecell._anndata import AnnData
+    rng = np.random.default_rng(1)
+    X = sp.csr_matrix(rng.poisson(1.0, (12, 5)).astype(np.float32))
+    a = AnnData(X)
+    a.var_names = ["a", "b", "c", "d", "e"]
+    a.var["highly_variable"] = [True, False, True, True, False]
+    vals, names = gr._values_from_var(a, None, None, False)
+    assert names == ["a", "c", "d"] and np.array_equal(vals, X.toarray()[:, [0, 2, 3]])
+    vals, names = gr._values_from_var(a, "e", None, False)
+    assert names == ["e"] and vals.shape == (12, 1)
+    a.layers["lay"] = X * 2
+    vals, _ = gr._values_from_var(a, ["b", "a"], "lay", False)
+    assert np.array_equal(vals, 2 * X.toarray()[:, [1, 0]])
+    with pytest.raises(AttributeError, match="raw"):
+        gr._values_from_var(a, None, None, True)
+    raw = types.SimpleNamespace(X=np.arange(36, dtype=np.float32).reshape(12, 3),
+                                var_names=pd.Index(["d", "a", "z"]))
+    a.raw = raw
+    vals, names = gr._values_from_var(a, None, None, True)
+    assert names == ["a", "d"] and np.array_equal(vals, raw.X[:, [1, 0]])
+    a.obs["num"] = np.arange(12.0)
+    a.obs["txt"] = ["x"] * 12
+    a.obs["num2"] = np.ones(12)
+    vals, names = gr._values_from_obs(a, None, None, False)
+    assert names == ["num", "num2"] and vals.shape == (12, 2)
+    vals, names = gr._values_from_obs(a, "num2", None, False)
+    assert names == ["num2"]
+    a.obsm["emb"] = rng.normal(size=(12, 4))
+    vals, names = gr._values_from_obsm(a, [3, 1], "emb", False)
+    assert names == [3, 1] and np.allclose(vals, a.obsm["emb"][:, [3, 1]].astype(np.float32))
+    with pytest.raises(KeyError, match="obsm"):
+        gr._values_from_obsm(a, None, "nope", False)
+    with pytest.raises(NotImplementedError, match="layers"):
+        gr.spatial_autocorr(a, attr="layers")
+
+
+def test_autocorr_frame_sorting_with_ties_and_nan():
+    from metalsinglecell import gr
+    W = sp.random(40, 40, density=0.1, random_state=2, format="csr")
+    I = np.array([0.2, np.nan, 0.5, 0.2, -0.1])
+    df = gr.moran_frame(I, None, W, ["a", "b", "c", "d", "e"])
+    assert df.index.tolist() == ["c", "a", "d", "e", "b"]
+    df = gr.moran_frame(I, None, W, ["a", "b", "c", "d", "e"], mode="geary")
+    assert df.index.tolist() == ["e", "a", "d", "c", "b"]
