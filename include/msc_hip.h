@@ -49,8 +49,11 @@ const char* msc_last_error(void);
  * exactly and certify each row; rows that fail their certificate are selected again over
  * all candidates by a wide pass of the same precision (16 queries per wave, 128 admitted
  * candidates per query), and rows no pass certifies are recomputed by exact brute force, so
- * the two precisions return identical results.  Limits: 1 <= k <= 64, k <= n < 2^31,
- * d <= 127 (FP32) / 125 (BF16).
+ * the two precisions return identical results.  Shapes beyond the MFMA select (k > 64, or
+ * d > 127 in FP32 / d > 125 in BF16) compute every candidate's exact distance instead (the
+ * reference's argpartition fallback for K > 32, neighbors.py:66-68); the result is the same
+ * exact kNN and the workspace query returns a token size.  Limits: 1 <= k <= 512,
+ * k <= n < 2^31, d >= 1.
  * ------------------------------------------------------------------------------------- */
 typedef struct {
   int timing;           /* in: nonzero -> record per-stage times (adds one stream sync) */
@@ -115,7 +118,7 @@ int msc_bbknn_distances_csr(const int32_t* idx, const float* dist, int64_t n, in
                             void* workspace, size_t workspace_bytes, void* stream);
 
 /* obsp['distances'] as CSR from a self-inclusive kNN (neighbors.py:755-760): k-1 entries per
- * row (self column dropped), columns ascending, explicit zeros kept.
+ * row (self column dropped), columns ascending, explicit zeros kept.  1 <= k <= 513.
  * out_indptr n+1 int64, out_indices n*(k-1) int32, out_data n*(k-1) float32. */
 int msc_knn_distances_csr(const int32_t* idx, const float* dist, int64_t n, int k,
                           int64_t* out_indptr, int32_t* out_indices, float* out_data,
@@ -128,7 +131,8 @@ int msc_knn_distances_csr(const int32_t* idx, const float* dist, int64_t n, int 
  * knn_idx/knn_dist: n x k self-inclusive kNN (rows ascending, distinct indices per row).
  * Output: symmetric canonical CSR (columns ascending, no zeros, no duplicates) in
  * out_indptr (n+1 int64), out_indices / out_data (capacity entries; 2*n*(k-1) always
- * suffices), per-row sigma and rho (n float32 each), and *out_nnz (host).  Synchronises. */
+ * suffices), per-row sigma and rho (n float32 each), and *out_nnz (host).  1 <= k <= 1024,
+ * n*k < 2^31.  Synchronises. */
 int msc_fuzzy_workspace_size(int64_t n, int k, size_t* bytes /*host*/);
 int msc_fuzzy_connectivities(const int32_t* knn_idx, const float* knn_dist, int64_t n, int k,
                              int64_t* out_indptr, int32_t* out_indices, float* out_data,
@@ -219,7 +223,9 @@ int msc_louvain(const int64_t* indptr, const int32_t* indices, const float* weig
  * 333-417, 143-196) as used by gr._knn_edges / gr.spatial_neighbors_knn (gr.py:171-225).
  * coords: n x dim float32.  out_idx / out_dist: n x k, ordered by (float32 distance, index).
  * coords64 / out_len64 (optional, together): n x dim float64 coordinates -> n x k float64
- * edge lengths (obsp['spatial_distances'] values, gr.py:32-34).  1 <= k <= 32, k < n. */
+ * edge lengths (obsp['spatial_distances'] values, gr.py:32-34).  1 <= k <= 511, k < n,
+ * dim >= 1.  k > 32 or dim outside {2, 3} compares every pair exactly (the reference's
+ * brute-force fallback, neighbors.py:350-355), with the same order and distances. */
 typedef struct {
   int32_t max_shell;     /* out: largest Chebyshev cell shell any query needed */
   int32_t grid_x, grid_y, grid_z;

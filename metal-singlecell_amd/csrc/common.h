@@ -263,4 +263,66 @@ __device__ __forceinline__ uint32_t coin_u32(uint64_t pkey, uint64_t v) {
   return static_cast<uint32_t>(mix64(pkey ^ (v * 0x9E3779B97F4A7C15ull)) >> 32);
 }
 
+// ---- wave-cooperative sorts of keys held in LDS -------------------------------------------
+// For key sets wider than a wave (64 lanes): one wave sorts KP (a power of two >= 2) u64 keys
+// in its own LDS region.  A wave's LDS operations complete in issue order, so between stages
+// only a compiler-level ordering point is needed, not a block barrier.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// index of the lower element of compare pair p in a stage of the given stride
+__device__ __forceinline__ int bitonic_lower(int p, int stride) {
+  return ((p & ~(stride - 1)) << 1) | (p & (stride - 1));
+}
+
+// ascending bitonic sort of a[0..KP)
+__device__ inline void wave_lds_sort_u64(uint64_t* a, int KP) {
+  const int l = lane_id();
+  for (int size = 2; size <= KP; size <<= 1) {
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      for (int p = l; p < (KP >> 1); p += kWave) {
+        const int i = bitonic_lower(p, stride), j = i + stride;
+        const uint64_t x = a[i], y = a[j];
+        const bool up = (i & size) == 0;
+        if ((x > y) == up) {
+          a[i] = y;
+          a[j] = x;
+        }
+      }
+      wave_lds_sync();
+    }
+  }
+}
+
+// list[0..KP) <- the KP smallest keys of list and buf (both ascending), ascending: the
+// elementwise minimum of list and reversed buf is bitonic, and half-cleaners sort it.
+__device__ inline void wave_lds_merge_min_u64(uint64_t* list, const uint64_t* buf, int KP) {
+  const int l = lane_id();
+  for (int i = l; i < KP; i += kWave) {
+    const uint64_t y = buf[KP - 1 - i];
+    if (y < list[i]) list[i] = y;
+  }
+  wave_lds_sync();
+  for (int stride = KP >> 1; stride > 0; stride >>= 1) {
+    for (int p = l; p < (KP >> 1); p += kWave) {
+      const int i = bitonic_lower(p, stride), j = i + stride;
+      const uint64_t x = list[i], y = list[j];
+      if (x > y) {
+        list[i] = y;
+        list[j] = x;
+      }
+    }
+    wave_lds_sync();
+  }
+}
+
+inline int pow2_at_least(int64_t x, int floor_value) {
+  int p = floor_value;
+  while (p < x) p <<= 1;
+  return p;
+}
+
 }  // namespace msc

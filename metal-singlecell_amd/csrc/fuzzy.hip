@@ -64,21 +64,25 @@ __global__ void k_dist_mean(const double* __restrict__ partial, int nb, int64_t 
   }
 }
 
-// one thread per row; the row's distances staged in LDS (k <= 64)
+// one thread per row; STAGED: the row's distances staged in LDS (k <= 64), else read from
+// global memory (k > 64; the row stays in the caches over the bisection)
+template <bool STAGED>
 __global__ __launch_bounds__(64) void k_smooth_membership(
     const int32_t* __restrict__ idx, const float* __restrict__ dist, int64_t n, int k,
     double target, const double* __restrict__ mean_all, const int64_t* __restrict__ goff, int n_groups,
     float* __restrict__ sigma_out,
     float* __restrict__ rho_out, float* __restrict__ vals) {
-  __shared__ float sd[64 * 65];
+  __shared__ float sd[STAGED ? 64 * 65 : 1];
   const int64_t r0 = static_cast<int64_t>(blockIdx.x) * 64;
   const int rows = static_cast<int>(min<int64_t>(64, n - r0));
-  for (int e = threadIdx.x; e < rows * k; e += 64) sd[(e / k) * 65 + (e % k)] = dist[r0 * k + e];
-  __syncthreads();
+  if constexpr (STAGED) {
+    for (int e = threadIdx.x; e < rows * k; e += 64) sd[(e / k) * 65 + (e % k)] = dist[r0 * k + e];
+    __syncthreads();
+  }
   const int t = threadIdx.x;
   if (t >= rows) return;
   const int64_t i = r0 + t;
-  const float* D = &sd[t * 65];
+  const float* D = STAGED ? &sd[t * 65] : dist + i * k;
 
   float rho = 0.0f;
   for (int j = 0; j < k; ++j)
@@ -244,6 +248,98 @@ __global__ __launch_bounds__(kThreads) void k_union(
   }
 }
 
+// k > 64: the same union with the out-list F of up to k - 1 entries sorted in the wave's LDS
+// region (KP >= k keys) and the found-prefix counts kept beside it
+template <bool FILL>
+__global__ __launch_bounds__(kThreads) void k_union_wide(
+    const int32_t* __restrict__ idx, const float* __restrict__ vals, int64_t n, int k, int KP,
+    const int64_t* __restrict__ in_start, const uint32_t* __restrict__ in_eid,
+    int64_t* __restrict__ row_nnz, const int64_t* __restrict__ indptr,
+    int32_t* __restrict__ out_idx, float* __restrict__ out_val, int64_t cap) {
+  extern __shared__ uint64_t ushared[];
+  const int w = threadIdx.x / kWave, l = lane_id();
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * (kThreads / kWave) + w;
+  if (i >= n) return;
+  uint64_t* F = ushared + static_cast<size_t>(w) * KP;
+  int32_t* fpre = reinterpret_cast<int32_t*>(ushared + static_cast<size_t>(kThreads / kWave) * KP) + w * (KP + 1);
+
+  int nF = 0;
+  for (int p = l; p < KP; p += kWave) {
+    uint64_t key = ~0ull;
+    if (p < k) {
+      const int32_t c = idx[i * k + p];
+      if (c >= 0 && c != static_cast<int32_t>(i) && vals[i * k + p] != 0.0f)
+        key = (static_cast<uint64_t>(static_cast<uint32_t>(c)) << 32) | static_cast<uint32_t>(p);
+    }
+    nF += __popcll(__ballot(key != ~0ull));
+    F[p] = key;
+  }
+  wave_lds_sync();
+  wave_lds_sort_u64(F, KP);
+  const int64_t s0 = in_start[i], s1 = in_start[i + 1];
+  const int64_t nS = s1 - s0;
+  const int64_t base = FILL ? indptr[i] : 0;
+
+  // F entries in column order: found in the in-list or not, and their output positions
+  int nBoth = 0;
+  for (int f0 = 0; f0 < KP; f0 += kWave) {
+    const int f = f0 + l;
+    const bool inF = f < nF;
+    const uint64_t key = F[f];
+    const int64_t col = inF ? static_cast<int64_t>(key >> 32) : INT64_MAX;
+    int64_t lb = s0;
+    bool found = false;
+    if (inF) {
+      lb = in_lower_bound(in_eid, s0, s1, k, col);
+      found = lb < s1 && static_cast<int64_t>(in_eid[lb] / k) == col;
+    }
+    const uint64_t fmask = __ballot(found);
+    const int pre = nBoth + __popcll(fmask & ((1ull << l) - 1ull));
+    fpre[f] = pre;
+    if (FILL && inF) {
+      const int slot = static_cast<int>(key & 0xFFFFFFFFu);
+      const float a = vals[i * k + slot];
+      float v = a;
+      if (found) {
+        const float b = vals[in_eid[lb]];
+        v = __fsub_rn(__fadd_rn(a, b), __fmul_rn(a, b));
+      }
+      const int64_t pos = base + f + (lb - s0) - pre;
+      if (pos < cap) {
+        out_idx[pos] = static_cast<int32_t>(col);
+        out_val[pos] = v;
+      }
+    }
+    nBoth += __popcll(fmask);
+  }
+  if (l == 0) fpre[KP] = nBoth;
+  if (!FILL) {
+    if (l == 0) row_nnz[i] = nF + nS - nBoth;
+    if (i == 0 && l == 0) row_nnz[n] = 0;
+    return;
+  }
+  wave_lds_sync();
+  // in-only entries
+  for (int64_t p = s0 + l; p < s1; p += kWave) {
+    const uint32_t e = in_eid[p];
+    const int64_t src = static_cast<int64_t>(e / k);
+    int lo = 0, hi = nF;
+    while (lo < hi) {
+      const int m = (lo + hi) >> 1;
+      if (static_cast<int64_t>(F[m] >> 32) < src) lo = m + 1;
+      else hi = m;
+    }
+    const bool dup = lo < nF && static_cast<int64_t>(F[lo] >> 32) == src;
+    if (!dup) {
+      const int64_t pos = base + lo + ((p - s0) - fpre[lo]);
+      if (pos < cap) {
+        out_idx[pos] = static_cast<int32_t>(src);
+        out_val[pos] = vals[e];
+      }
+    }
+  }
+}
+
 struct Bufs {
   double* partial;
   double* mean;
@@ -282,7 +378,7 @@ static Bufs carve(Carve& cv, int64_t n, int k, int n_groups = 1) {
 }
 
 static int validate(int64_t n, int k) {
-  MSC_REQUIRE(n >= 1 && k >= 1 && k <= 64, "fuzzy: need n >= 1 and 1 <= k <= 64");
+  MSC_REQUIRE(n >= 1 && k >= 1 && k <= 1024, "fuzzy: need n >= 1 and 1 <= k <= 1024");
   MSC_REQUIRE(n * k < (int64_t(1) << 31), "fuzzy: n*k must be < 2^31");
   return kOk;
 }
@@ -393,8 +489,12 @@ static int fuzzy_kernels(const int32_t* knn_idx, const float* knn_dist, int64_t 
   k_dist_mean<<<n_groups, 64, 0, st>>>(b.partial, kMeanBlocks, m, k, goff, b.mean);
   MSC_LAUNCH_CHECK();
   const double target = std::log2(static_cast<double>(k));
-  k_smooth_membership<<<static_cast<unsigned>(ceil_div(n, 64)), 64, 0, st>>>(
-      knn_idx, knn_dist, n, k, target, b.mean, goff, n_groups, out_sigma, out_rho, b.vals);
+  if (k <= 64)
+    k_smooth_membership<true><<<static_cast<unsigned>(ceil_div(n, 64)), 64, 0, st>>>(
+        knn_idx, knn_dist, n, k, target, b.mean, goff, n_groups, out_sigma, out_rho, b.vals);
+  else
+    k_smooth_membership<false><<<static_cast<unsigned>(ceil_div(n, 64)), 64, 0, st>>>(
+        knn_idx, knn_dist, n, k, target, b.mean, goff, n_groups, out_sigma, out_rho, b.vals);
   MSC_LAUNCH_CHECK();
   k_edge_keys<<<static_cast<unsigned>(ceil_div(m, kThreads)), kThreads, 0, st>>>(knn_idx, b.vals, n, k,
                                                                                  b.key, b.eid);
@@ -403,8 +503,14 @@ static int fuzzy_kernels(const int32_t* knn_idx, const float* knn_dist, int64_t 
                                   b.sort_ws_bytes, st));
   MSC_TRY(prims::segment_starts(b.skey, m, n, b.in_start, st));
   const unsigned wb = static_cast<unsigned>(ceil_div(n, kThreads / kWave));
-  k_union<false><<<wb, kThreads, 0, st>>>(knn_idx, b.vals, n, k, b.in_start, b.seid, b.row_nnz,
-                                          nullptr, nullptr, nullptr, 0);
+  const int KP = pow2_at_least(k, kWave);
+  const size_t ulds = (kThreads / kWave) * (KP * sizeof(int64_t) + (KP + 1) * sizeof(int32_t));
+  if (k <= kWave)
+    k_union<false><<<wb, kThreads, 0, st>>>(knn_idx, b.vals, n, k, b.in_start, b.seid, b.row_nnz,
+                                            nullptr, nullptr, nullptr, 0);
+  else
+    k_union_wide<false><<<wb, kThreads, ulds, st>>>(knn_idx, b.vals, n, k, KP, b.in_start, b.seid, b.row_nnz,
+                                                    nullptr, nullptr, nullptr, 0);
   MSC_LAUNCH_CHECK();
   MSC_TRY(prims::exclusive_scan<int64_t>(b.row_nnz, out_indptr, n + 1, b.scan_ws, b.scan_ws_bytes, st));
   if (host_nnz) {
@@ -417,8 +523,12 @@ static int fuzzy_kernels(const int32_t* knn_idx, const float* knn_dist, int64_t 
     }
     *host_nnz = nnz;
   }
-  k_union<true><<<wb, kThreads, 0, st>>>(knn_idx, b.vals, n, k, b.in_start, b.seid, nullptr,
-                                         out_indptr, out_indices, out_data, capacity);
+  if (k <= kWave)
+    k_union<true><<<wb, kThreads, 0, st>>>(knn_idx, b.vals, n, k, b.in_start, b.seid, nullptr,
+                                           out_indptr, out_indices, out_data, capacity);
+  else
+    k_union_wide<true><<<wb, kThreads, ulds, st>>>(knn_idx, b.vals, n, k, KP, b.in_start, b.seid, nullptr,
+                                                   out_indptr, out_indices, out_data, capacity);
   MSC_LAUNCH_CHECK();
   return kOk;
 }

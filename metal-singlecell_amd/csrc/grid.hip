@@ -16,6 +16,7 @@
 #include "common.h"
 #include "msc_hip.h"
 #include "prims.h"
+#include "wide.h"
 
 namespace msc {
 namespace grid {
@@ -460,9 +461,14 @@ static int sort_cells(const float* coords, int64_t n, const Geo& geo, const Bufs
   return prims::segment_starts(b.skey, n, ncells, b.cell_start, st);
 }
 
+// The grid query holds at most 32 neighbours in registers and bins 2-D / 3-D cells; other
+// shapes take the exact all-pairs path (the reference's brute-force fallback for k > 32 or
+// d not in {2, 3}, neighbors.py:350-355).
+static bool grid_shape(int dim, int k) { return (dim == 2 || dim == 3) && k <= 32; }
+
 static int validate(int64_t n, int dim, int k) {
-  MSC_REQUIRE(dim == 2 || dim == 3, "spatial_knn: dim must be 2 or 3");
-  MSC_REQUIRE(k >= 1 && k <= 32, "spatial_knn: k must be in [1, 32]");
+  MSC_REQUIRE(dim >= 1, "spatial_knn: dim must be >= 1");
+  MSC_REQUIRE(k >= 1 && k < wide::kMaxK, "spatial_knn: k must be in [1, %d]", wide::kMaxK - 1);
   MSC_REQUIRE(n > k && n < (int64_t(1) << 31) / 4, "spatial_knn: need k < n < 2^29");
   return kOk;
 }
@@ -476,6 +482,10 @@ using namespace msc::grid;
 extern "C" int msc_spatial_knn_workspace_size(int64_t n, int dim, int k, size_t* bytes) {
   MSC_TRY(validate(n, dim, k));
   MSC_REQUIRE(bytes, "spatial_knn: bytes is null");
+  if (!grid_shape(dim, k)) {
+    *bytes = 256;
+    return kOk;
+  }
   Carve cv(nullptr, 0);
   carve(cv, n);
   *bytes = cv.used + 256;
@@ -489,6 +499,21 @@ extern "C" int msc_spatial_knn(const float* coords, int64_t n, int dim, int k, i
   MSC_REQUIRE(coords && out_idx && out_dist && workspace, "spatial_knn: null pointer argument");
   MSC_REQUIRE((coords64 == nullptr) == (out_len64 == nullptr), "spatial_knn: coords64 and out_len64 go together");
   hipStream_t st = static_cast<hipStream_t>(stream);
+  if (!grid_shape(dim, k)) {
+    // k + 1 keys per row by (float32 distance, index), the point itself (key 0) dropped
+    MSC_TRY(wide::knn(coords, n, dim, k + 1, true, 1, out_idx, out_dist, st));
+    if (coords64) {
+      k_edge_len<<<static_cast<unsigned>(ceil_div(n * k, kThreads)), kThreads, 0, st>>>(coords64, n, dim, k,
+                                                                                        out_idx, out_len64);
+      MSC_LAUNCH_CHECK();
+    }
+    if (info) {
+      info->max_shell = -1;
+      info->grid_x = info->grid_y = info->grid_z = 0;
+      info->cell = 0.0f;
+    }
+    return kOk;
+  }
   Carve cv(workspace, workspace_bytes);
   Bufs b = carve(cv, n);
   if (!cv.ok()) {

@@ -43,6 +43,7 @@
 #include "common.h"
 #include "msc_hip.h"
 #include "prims.h"
+#include "wide.h"
 
 namespace msc {
 namespace knn {
@@ -792,6 +793,29 @@ __global__ __launch_bounds__(kThreads) void distances_csr_kernel(
   }
 }
 
+// the same rows for k > 65: the k - 1 keys sorted in the wave's LDS region (KP >= k - 1)
+__global__ __launch_bounds__(kThreads) void distances_csr_wide_kernel(
+    const int32_t* __restrict__ idx, const float* __restrict__ dist, int64_t n, int k, int KP,
+    int64_t* __restrict__ indptr, int32_t* __restrict__ indices, float* __restrict__ data) {
+  extern __shared__ uint64_t skeys[];
+  const int w = threadIdx.x / kWave, l = lane_id();
+  const int64_t q = static_cast<int64_t>(blockIdx.x) * (kThreads / kWave) + w;
+  if (q > n) return;
+  if (l == 0) indptr[q] = q * (k - 1);
+  if (q == n) return;
+  uint64_t* a = skeys + static_cast<size_t>(w) * KP;
+  for (int p = l; p < KP; p += kWave)
+    a[p] = p < k - 1 ? (static_cast<uint64_t>(static_cast<uint32_t>(idx[q * k + 1 + p])) << 32) |
+                           __float_as_uint(dist[q * k + 1 + p])
+                     : kEmpty;
+  wave_lds_sync();
+  wave_lds_sort_u64(a, KP);
+  for (int p = l; p < k - 1; p += kWave) {
+    indices[q * (k - 1) + p] = static_cast<int32_t>(a[p] >> 32);
+    data[q * (k - 1) + p] = __uint_as_float(static_cast<uint32_t>(a[p]));
+  }
+}
+
 // ------------------------------------------------------------------------------------------
 // host orchestration
 // ------------------------------------------------------------------------------------------
@@ -962,15 +986,19 @@ static Bound make_bound(bool bf16, int ks) {
   return bnd;
 }
 
+// Shapes the MFMA select takes: at most 64 neighbours (its 64 admitted candidates per query)
+// and d within its K padding (4 K-steps of 32 bf16 / 8 floats, after the norm terms).
+static bool select_shape(int d, int k, int precision) {
+  return k <= 64 && (precision == MSC_PRECISION_FP32 ? d <= 127 : d <= 125);
+}
+
 static int validate(int64_t n, int d, int k, int precision) {
   MSC_REQUIRE(n >= 1 && n < (int64_t(1) << 31) - kQueriesPerBlock, "knn: n=%lld out of range", (long long)n);
   MSC_REQUIRE(d >= 1, "knn: d must be >= 1");
-  MSC_REQUIRE(k >= 1 && k <= 64, "knn: k=%d must be in [1, 64]", k);
+  MSC_REQUIRE(k >= 1 && k <= wide::kMaxK, "knn: k=%d must be in [1, %d]", k, wide::kMaxK);
   MSC_REQUIRE(k <= n, "knn: k=%d exceeds n=%lld", k, (long long)n);
   MSC_REQUIRE(precision == MSC_PRECISION_FP32 || precision == MSC_PRECISION_BF16_RERANK,
               "knn: unknown precision %d", precision);
-  MSC_REQUIRE(precision == MSC_PRECISION_FP32 ? d <= 127 : d <= 125,
-              "knn: d=%d unsupported for this precision", d);
   return kOk;
 }
 
@@ -1031,6 +1059,10 @@ using namespace msc::knn;
 extern "C" int msc_knn_workspace_size(int64_t n, int d, int k, int precision, size_t* bytes) {
   MSC_TRY(validate(n, d, k, precision));
   MSC_REQUIRE(bytes != nullptr, "knn: bytes is null");
+  if (!select_shape(d, k, precision)) {  // the wide path keeps its lists in LDS
+    *bytes = 256;
+    return kOk;
+  }
   Plan p = make_plan(n, d, k, precision);
   Carve cv(nullptr, 0);
   carve(cv, p);
@@ -1044,6 +1076,35 @@ extern "C" int msc_knn_exact(const float* X, int64_t n, int d, int k, int precis
   MSC_TRY(validate(n, d, k, precision));
   MSC_REQUIRE(X && out_idx && out_dist && workspace, "knn: null pointer argument");
   hipStream_t st = static_cast<hipStream_t>(stream);
+  if (!select_shape(d, k, precision)) {
+    // every candidate's exact distance (both precisions: the result is the same exact kNN)
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    const bool timing = info && info->timing;
+    if (timing) {
+      MSC_HIP(hipEventCreate(&e0));
+      MSC_HIP(hipEventCreate(&e1));
+      MSC_HIP(hipEventRecord(e0, st));
+    }
+    MSC_TRY(wide::knn(X, n, d, k, false, 0, out_idx, out_dist, st));
+    if (info) {
+      info->n_splits = 0;
+      info->list_len = k;
+      info->n_second_pass = 0;
+      info->n_fallback = n;
+      info->prep_ms = info->select_ms = info->finalize_ms = info->second_pass_ms = 0.0;
+      info->fallback_ms = 0.0;
+    }
+    if (timing) {
+      MSC_HIP(hipEventRecord(e1, st));
+      MSC_HIP(hipEventSynchronize(e1));
+      float ms = 0.0f;
+      MSC_HIP(hipEventElapsedTime(&ms, e0, e1));
+      info->fallback_ms = ms;
+      MSC_HIP(hipEventDestroy(e0));
+      MSC_HIP(hipEventDestroy(e1));
+    }
+    return kOk;
+  }
   Plan p = make_plan(n, d, k, precision);
   Carve cv(workspace, workspace_bytes);
   Buffers b = carve(cv, p);
@@ -1124,10 +1185,16 @@ extern "C" int msc_knn_distances_csr(const int32_t* idx, const float* dist, int6
                                      void* stream) {
   MSC_REQUIRE(idx && dist && out_indptr && (k == 1 || (out_indices && out_data)),
               "distances_csr: null pointer argument");
-  MSC_REQUIRE(k >= 1 && k <= 65 && n >= 1, "distances_csr: bad n/k");
+  MSC_REQUIRE(k >= 1 && k <= wide::kMaxK + 1 && n >= 1, "distances_csr: bad n/k");
   hipStream_t st = static_cast<hipStream_t>(stream);
   const unsigned wb = static_cast<unsigned>(ceil_div(n + 1, kThreads / kWave));
-  distances_csr_kernel<<<wb, kThreads, 0, st>>>(idx, dist, n, k, out_indptr, out_indices, out_data);
+  if (k <= kWave + 1) {
+    distances_csr_kernel<<<wb, kThreads, 0, st>>>(idx, dist, n, k, out_indptr, out_indices, out_data);
+  } else {
+    const int KP = pow2_at_least(k - 1, kWave);
+    distances_csr_wide_kernel<<<wb, kThreads, (kThreads / kWave) * KP * sizeof(uint64_t), st>>>(
+        idx, dist, n, k, KP, out_indptr, out_indices, out_data);
+  }
   MSC_LAUNCH_CHECK();
   return kOk;
 }

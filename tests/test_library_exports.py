@@ -41,12 +41,18 @@ def test_validation_errors_without_gpu():
     assert lib.msc_knn_workspace_size(1000, 50, 30, 1, ctypes.byref(nb)) == 0 and nb.value > 0
     assert lib.msc_knn_workspace_size(10, 50, 30, 1, ctypes.byref(nb)) != 0
     assert b"exceeds" in lib.msc_last_error()
-    assert lib.msc_knn_workspace_size(1000, 50, 65, 0, ctypes.byref(nb)) != 0
-    assert lib.msc_knn_workspace_size(1000, 200, 15, 1, ctypes.byref(nb)) != 0
+    # wide shapes (k > 64, d beyond the MFMA K padding) take the LDS-list path: token workspace
+    assert lib.msc_knn_workspace_size(1000, 50, 65, 0, ctypes.byref(nb)) == 0 and nb.value == 256
+    assert lib.msc_knn_workspace_size(1000, 200, 15, 1, ctypes.byref(nb)) == 0 and nb.value == 256
+    assert lib.msc_knn_workspace_size(1000, 50, 513, 1, ctypes.byref(nb)) != 0
+    assert b"k=513" in lib.msc_last_error()
     assert lib.msc_fuzzy_workspace_size(1000, 15, ctypes.byref(nb)) == 0
+    assert lib.msc_fuzzy_workspace_size(1000, 1025, ctypes.byref(nb)) != 0
     assert lib.msc_leiden_workspace_size(1000, 30000, ctypes.byref(nb)) == 0
     assert lib.msc_spatial_knn_workspace_size(1000, 2, 6, ctypes.byref(nb)) == 0
-    assert lib.msc_spatial_knn_workspace_size(1000, 4, 6, ctypes.byref(nb)) != 0
+    assert lib.msc_spatial_knn_workspace_size(1000, 4, 6, ctypes.byref(nb)) == 0 and nb.value == 256
+    assert lib.msc_spatial_knn_workspace_size(1000, 2, 40, ctypes.byref(nb)) == 0 and nb.value == 256
+    assert lib.msc_spatial_knn_workspace_size(1000, 0, 6, ctypes.byref(nb)) != 0
     assert lib.msc_moran_workspace_size(1000, 500, 6000, 100, ctypes.byref(nb)) == 0
 
 
