@@ -154,6 +154,20 @@ int msc_fuzzy_connectivities_batched(const int32_t* knn_idx, const float* knn_di
                                      float* out_sigma, float* out_rho, void* workspace,
                                      size_t workspace_bytes, void* stream);
 
+/* msc_fuzzy_connectivities_batched without a workspace argument.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Scratch is a static 64 MiB device
+ * array per GPU, held until the process ends, so sizes whose workspace would exceed it
+ * (msc_fuzzy_batched_workspace_size > 64 MiB, about n*k > 2.5M) are rejected with
+ * MSC_EINVAL.  Calls that overlap on one GPU share that array and race: serialise them (one
+ * stream, or events).  Every other argument, the result and the graph-capture property are
+ * those of msc_fuzzy_connectivities_batched; the call allocates nothing and does not
+ * synchronise. */
+int msc_fuzzy_connectivities_batched_noalloc(const int32_t* knn_idx, const float* knn_dist, int64_t n, int k,
+                                             const int64_t* group_offsets, int n_groups, int64_t* out_indptr,
+                                             int32_t* out_indices, float* out_data, int64_t capacity,
+                                             float* out_sigma, float* out_rho, void* stream);
+
 /* Connectivity trimming of bbknn: neighbors._trim_graph (src/metalsinglecell/neighbors.py:
  * 465-488) on a symmetric canonical CSR.  Row i's threshold is its trim-th largest value
  * when it holds more than trim entries, else 0; entry (i, j) is kept when nonzero and >= the

@@ -10,6 +10,8 @@
 //   4. per row: union of the out-list (sorted by column) and the in-list (sources ascending),
 //      P_ij = (a + b) - a*b in float32; count, scan, fill -> canonical CSR
 // exp is the project's float32 expf32 (common.h), identical to the oracle's.
+#include <mutex>
+
 #include "common.h"
 #include "msc_hip.h"
 #include "prims.h"
@@ -581,6 +583,59 @@ extern "C" int msc_fuzzy_connectivities_batched(const int32_t* knn_idx, const fl
   }
   return fuzzy_kernels(knn_idx, knn_dist, n, k, group_offsets, n_groups, out_indptr, out_indices, out_data, capacity,
                        out_sigma, out_rho, b, st, nullptr);
+}
+
+// ---- the batched connectivities without a workspace --------------------------------------
+// Scratch for callers that cannot allocate: one static device array per GPU, held until the
+// process ends.  Host code uses the device address from hipGetSymbolAddress (looked up once
+// per device, on the first call there), never the array's name.  Calls that overlap on one
+// GPU share the array and race; every byte a call reads is written by that call first.
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+int scratch_on_current_device(void** out) {
+  int dev = 0;
+  MSC_HIP(hipGetDevice(&dev));
+  MSC_REQUIRE(dev >= 0 && dev < kMaxDevices, "fuzzy_batched_noalloc: device %d beyond %d", dev, kMaxDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (g_scratch_ptr[dev] == nullptr) {
+    void* p = nullptr;
+    MSC_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    MSC_REQUIRE(p != nullptr, "fuzzy_batched_noalloc: no device address for the scratch array");
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return kOk;
+}
+}  // namespace
+
+extern "C" int msc_fuzzy_connectivities_batched_noalloc(const int32_t* knn_idx, const float* knn_dist, int64_t n,
+                                                        int k, const int64_t* group_offsets, int n_groups,
+                                                        int64_t* out_indptr, int32_t* out_indices, float* out_data,
+                                                        int64_t capacity, float* out_sigma, float* out_rho,
+                                                        void* stream) {
+  MSC_TRY(validate(n, k));
+  MSC_REQUIRE(knn_idx && knn_dist && group_offsets && n_groups >= 1 && out_indptr && out_indices && out_data &&
+                  out_sigma && out_rho,
+              "fuzzy_batched_noalloc: null pointer argument");
+  MSC_REQUIRE(capacity >= 2 * n * (k - 1), "fuzzy_batched_noalloc: capacity must be >= 2*n*(k-1)");
+  Carve need(nullptr, 0);
+  carve(need, n, k, n_groups);
+  MSC_REQUIRE(need.used <= kNoAllocScratchBytes,
+              "fuzzy_batched_noalloc: n=%lld k=%d needs %zu scratch bytes; the static scratch holds %zu "
+              "(use msc_fuzzy_connectivities_batched with a workspace)",
+              (long long)n, k, need.used, kNoAllocScratchBytes);
+  void* ws = nullptr;
+  MSC_TRY(scratch_on_current_device(&ws));
+  Carve cv(ws, kNoAllocScratchBytes);
+  Bufs b = carve(cv, n, k, n_groups);
+  return fuzzy_kernels(knn_idx, knn_dist, n, k, group_offsets, n_groups, out_indptr, out_indices, out_data, capacity,
+                       out_sigma, out_rho, b, static_cast<hipStream_t>(stream), nullptr);
 }
 
 extern "C" int msc_trim_graph_workspace_size(int64_t n, size_t* bytes) {

@@ -79,6 +79,8 @@ SIGNATURES = {
     "msc_fuzzy_batched_workspace_size": (c_int, [c_i64, c_int, c_int, ctypes.POINTER(c_sz)]),
     "msc_fuzzy_connectivities_batched": (c_int, [c_vp, c_vp, c_i64, c_int, c_vp, c_int, c_vp, c_vp, c_vp,
                                                  c_i64, c_vp, c_vp, c_vp, c_sz, c_vp]),
+    "msc_fuzzy_connectivities_batched_noalloc": (c_int, [c_vp, c_vp, c_i64, c_int, c_vp, c_int, c_vp, c_vp,
+                                                         c_vp, c_i64, c_vp, c_vp, c_vp]),
     "msc_spatial_knn_workspace_size": (c_int, [c_i64, c_int, c_int, ctypes.POINTER(c_sz)]),
     "msc_spatial_knn": (c_int, [c_vp, c_i64, c_int, c_int, c_vp, c_vp, c_vp, c_vp,
                                 ctypes.POINTER(SpatialInfo), c_vp, c_sz, c_vp]),
