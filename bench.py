@@ -71,8 +71,10 @@ def timed(Xd, k, precision, steps, warmup):
     return total / steps, recs, labels
 
 
-def cpu_baseline(X, k, threads):
-    """CPU restatement timed on bounded samples of the same workload (extrapolated)."""
+def cpu_baseline(X, k, threads, gpu_idx=None):
+    """CPU restatement timed on bounded samples of the same workload (extrapolated), with a
+    single-thread row; the same sampled kNN rows check the device result (recall@k and
+    bit-exact rows against the C oracle)."""
     import numpy as np
     from metalsinglecell import synthetic  # noqa: F401
     from oracle import fuzzy, knn, leiden
@@ -80,10 +82,21 @@ def cpu_baseline(X, k, threads):
     rng = np.random.default_rng(0)
     # kNN: queries are independent -> time q queries against all n candidates
     q = 1000 if n >= 200_000 else min(n, 4000)
-    rows = rng.choice(n, q, replace=False)
+    rows = np.sort(rng.choice(n, q, replace=False))
     t = time.perf_counter()
-    knn.exact_knn_c(X, k, rows=rows, threads=threads)
+    ri, _, rd = knn.exact_knn_c(X, k, rows=rows, threads=threads)
     t_knn = (time.perf_counter() - t) * n / q
+    q1 = max(1, q // 10)
+    t = time.perf_counter()
+    knn.exact_knn_c(X, k, rows=rows[:q1], threads=1)
+    t_knn1 = (time.perf_counter() - t) * n / q1
+    check = None
+    if gpu_idx is not None:
+        gi, gd = gpu_idx
+        gi, gd = gi[rows], gd[rows]
+        recall = float(np.mean([np.intersect1d(a, b).size / k for a, b in zip(gi, ri)]))
+        check = {"rows": int(q), "recall_at_k": recall,
+                 "rows_bit_exact": int(np.sum(np.all(gi == ri, axis=1) & np.all(gd == rd, axis=1)))}
     # connectivities + Leiden on a subsample of m cells (cost ~ linear in cells at fixed k)
     m = min(n, 20_000)
     sub = np.ascontiguousarray(X[rng.choice(n, m, replace=False)])
@@ -95,11 +108,82 @@ def cpu_baseline(X, k, threads):
     leiden.leiden(P, resolution=1.0, random_state=0, n_iterations=2)
     t_leiden = (time.perf_counter() - t) * n / m
     total = t_knn + t_conn + t_leiden
+    total1 = t_knn1 + t_conn + t_leiden
     return {"value": n / total, "unit": "cells/s", "cores": threads, "kind": "port",
             "sample": (f"kNN: {q} query rows x all {n} candidates (C oracle, {threads} threads), "
-                       f"x{n / q:.0f}; connectivities + Leiden (NumPy oracle, n_iterations=2) on a "
-                       f"{m}-cell subsample, x{n / m:.0f}; all extrapolated linearly to {n} cells"),
-            "stage_s": {"knn": t_knn, "connectivities": t_conn, "leiden": t_leiden}}
+                       f"x{n / q:.0f}; connectivities + Leiden (NumPy oracle, one thread, n_iterations=2) "
+                       f"on a {m}-cell subsample, x{n / m:.0f}; all extrapolated linearly to {n} cells"),
+            "stage_s": {"knn": t_knn, "connectivities": t_conn, "leiden": t_leiden},
+            "single_thread": {"value": n / total1, "cores": 1, "knn_s": t_knn1,
+                              "sample": f"kNN: {q1} query rows on one thread, x{n / q1:.0f}; "
+                                        "connectivities and Leiden as above"},
+            "oracle_check": check}
+
+
+def bf16_ranking_error(X, idx, rows):
+    """Distance error of the BF16 ranking form against float64, on (row, neighbour) pairs:
+    the device ranks |q^ - c~|^2 with q^ = bf16(q - mu) and c~ = hi + lo, hi = bf16(c - mu),
+    lo = bf16((c - mu) - hi); this evaluates that squared distance exactly (float64) from the
+    rounded operands and compares it with the float64 squared distance of the float32 rows.
+    (The FP32 accumulation error is bounded separately by the certificate.)"""
+    import numpy as np
+    import torch
+    Xd = torch.from_numpy(np.ascontiguousarray(X, np.float32))
+    mu = Xd.double().mean(0).float()
+    C = Xd - mu
+    hi = C.to(torch.bfloat16).float()
+    lo = (C - hi).to(torch.bfloat16).float()
+    q_hat = hi[rows].double()
+    nb = torch.from_numpy(np.asarray(idx[rows], np.int64))
+    c_til = hi[nb].double() + lo[nb].double()
+    approx = ((q_hat[:, None, :] - c_til) ** 2).sum(-1)
+    X64 = Xd.double()
+    exact = ((X64[rows][:, None, :] - X64[nb]) ** 2).sum(-1)
+    mask = exact > 0
+    rel = ((approx - exact).abs()[mask] / exact[mask]).numpy()
+    return {"pairs": int(mask.sum()), "max_rel_d2": float(rel.max()), "median_rel_d2": float(np.median(rel)),
+            "max_abs_d": float((approx.sqrt() - exact.sqrt()).abs().max())}
+
+
+def precision_report(k_c3, idx_c3, X_c3):
+    """BF16 ranking-distance error against float64 per configuration (the device kNN's own
+    neighbours of 500 sampled rows each): the 1M Zipf mixture, the 100k anisotropic mixture
+    and the largest of the 256 batched samples."""
+    import numpy as np
+    from metalsinglecell import _hip, synthetic
+    rng = np.random.default_rng(7)
+    out = {"c3_1M_zipf": bf16_ranking_error(X_c3, idx_c3, np.sort(rng.choice(X_c3.shape[0], 500, replace=False)))}
+    X2, _ = synthetic.gaussian_mixture_anisotropic()
+    i2, _ = _hip.knn_exact(X2, 15, precision="bf16")
+    out["c2_100k_aniso"] = bf16_ranking_error(X2, i2.cpu().numpy(), np.sort(rng.choice(X2.shape[0], 500, replace=False)))
+    S = [x for x, _ in synthetic.batched_samples()]
+    big = max(S, key=lambda x: x.shape[0])
+    i4, _ = _hip.knn_exact(big, 15, precision="bf16")
+    out["c4_largest_sample"] = bf16_ranking_error(big, i4.cpu().numpy(),
+                                                  np.sort(rng.choice(big.shape[0], 500, replace=False)))
+    return out
+
+
+def end_to_end(X, k):
+    """Host embedding -> AnnData slots: pp.neighbors (upload, kNN, connectivities, CSR
+    download into obsp) then tl.leiden(backend="gpu") (CSR upload, Leiden, labels into a
+    natsorted Categorical in obs); wall time of one call each after the warm device path."""
+    import numpy as np
+    from metalsinglecell import pp, tl
+    from metalsinglecell._anndata import AnnData
+    a = AnnData(n_obs=X.shape[0])          # no variables: .X is unused with use_rep
+    a.obsm["X_pca"] = X
+    t0 = time.perf_counter()
+    pp.neighbors(a, n_neighbors=k, use_rep="X_pca")
+    t1 = time.perf_counter()
+    tl.leiden(a, backend="gpu")
+    t2 = time.perf_counter()
+    n = X.shape[0]
+    return {"value": n / (t2 - t0), "unit": "cells/s", "neighbors_s": t1 - t0, "leiden_s": t2 - t1,
+            "n_clusters": int(len(a.obs["leiden"].cat.categories)),
+            "includes": "host->device upload of X, device kNN + connectivities, CSR download into "
+                        "obsp['distances'/'connectivities'], canonical CSR upload, Leiden, labels "
+                        "download, natsorted Categorical in obs['leiden']"}
 
 
 def batched(reps=3, workers=8):
@@ -142,55 +226,66 @@ def batched(reps=3, workers=8):
             "leiden_workers": workers, "cells_per_s": X.shape[0] / (total * 1e-3)}
 
 
-def serving(duration_s, rate=20.0, genes=500, n_perms=100, max_slots=6, seed=4):
-    """Poisson stream of spatial jobs (grid kNN k=6 + Moran's I, P permutations) through the
-    continuous-batching server; arrival -> completion latency percentiles."""
+def serving(duration_s, rates=(20.0,), genes=500, n_perms=100, max_slots=6, seed=4):
+    """Poisson streams of spatial jobs (grid kNN k=6 + Moran's I, P permutations) through the
+    continuous-batching server, one stream per offered rate; arrival -> completion latency
+    percentiles per rate, and the highest offered rate whose p99 stays under 2 s."""
     import threading
     import numpy as np
-    import torch
     from metalsinglecell import synthetic
     from metalsinglecell.serve import Server, SpatialJob
 
-    n_jobs = max(1, int(duration_s * rate))
-    arrivals, sizes = synthetic.poisson_job_stream(n_jobs, rate=rate, seed=seed)
+    streams = {}
+    n_max = 0
+    for rate in rates:
+        n_jobs = max(1, int(duration_s * rate))
+        arrivals, sizes = synthetic.poisson_job_stream(n_jobs, rate=rate, seed=seed)
+        streams[rate] = (arrivals, sizes)
+        n_max = max(n_max, int(sizes.max()))
     t = time.perf_counter()
-    pool = synthetic.nb_counts(int(sizes.max()), genes, rng=np.random.default_rng(seed))
-    rng = np.random.default_rng(seed + 1)
-    jobs = []
-    for i, n in enumerate(sizes):
-        xy = synthetic.tissue_coordinates(int(n), rng=rng)
-        jobs.append(SpatialJob(coords=xy, counts=pool[: int(n)], n_perms=n_perms, seed=i, job_id=i))
-    log(f"serving: {n_jobs} jobs prepared in {time.perf_counter() - t:.1f}s "
-        f"(sizes {int(sizes.min())}..{int(sizes.max())}, mean {int(sizes.mean())})")
-    srv = Server(max_slots=max_slots)
-    futs = []
-    t0 = time.perf_counter()
+    pool = synthetic.nb_counts(n_max, genes, rng=np.random.default_rng(seed))
+    log(f"serving: gene pool {n_max} x {genes} in {time.perf_counter() - t:.1f}s")
+    per_rate = {}
+    for rate in rates:
+        arrivals, sizes = streams[rate]
+        rng = np.random.default_rng(seed + 1)
+        jobs = [SpatialJob(coords=synthetic.tissue_coordinates(int(n), rng=rng), counts=pool[: int(n)],
+                           n_perms=n_perms, seed=i, job_id=i) for i, n in enumerate(sizes)]
+        srv = Server(max_slots=max_slots)
+        futs = []
+        t0 = time.perf_counter()
 
-    def feed():
-        for j, at in zip(jobs, arrivals):
-            delay = t0 + at - time.perf_counter()
-            if delay > 0:
-                time.sleep(delay)
-            j.arrival = time.perf_counter()
-            futs.append(srv.submit(j))
+        def feed():
+            for j, at in zip(jobs, arrivals):
+                delay = t0 + at - time.perf_counter()
+                if delay > 0:
+                    time.sleep(delay)
+                j.arrival = time.perf_counter()
+                futs.append(srv.submit(j))
 
-    th = threading.Thread(target=feed)
-    th.start()
-    th.join()
-    errors = 0
-    for f in futs:
-        try:
-            f.wait(600)
-        except Exception:
-            errors += 1
-    t1 = time.perf_counter()
-    srv.close()
-    st = srv.stats(t0, t1)
-    st.update({"offered_rate_jobs_per_s": rate, "errors": errors, "genes": genes,
-               "n_perms": n_perms, "n_neighs": 6, "max_slots": max_slots,
-               "mean_cells": float(sizes.mean()), "hbm_budget_gib": srv.hbm_budget / 2**30,
-               "stream_s": float(arrivals[-1])})
-    return st
+        th = threading.Thread(target=feed)
+        th.start()
+        th.join()
+        errors = 0
+        for f in futs:
+            try:
+                f.wait(600)
+            except Exception:
+                errors += 1
+        t1 = time.perf_counter()
+        srv.close()
+        st = srv.stats(t0, t1)
+        st.update({"offered_rate_jobs_per_s": rate, "errors": errors, "genes": genes,
+                   "n_perms": n_perms, "n_neighs": 6, "max_slots": max_slots,
+                   "mean_cells": float(sizes.mean()), "hbm_budget_gib": srv.hbm_budget / 2**30,
+                   "stream_s": float(arrivals[-1])})
+        log(f"serving at {rate} jobs/s: p50 {st['p50_s']:.3f} s, p99 {st['p99_s']:.3f} s, "
+            f"{st['jobs_per_s']:.1f} jobs/s completed")
+        per_rate[str(rate)] = st
+    ok = [r for r in rates if per_rate[str(r)]["p99_s"] < 2.0 and per_rate[str(r)]["errors"] == 0
+          and per_rate[str(r)]["rejected"] == 0]
+    first = per_rate[str(rates[0])]
+    return {**first, "by_rate": per_rate, "max_offered_rate_p99_under_2s": max(ok) if ok else None}
 
 
 def _ev_ms(fn, reps=3):
@@ -375,7 +470,10 @@ def main():
     ap.add_argument("--k", type=int, default=30)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
-    ap.add_argument("--serve-seconds", type=float, default=10.0)
+    ap.add_argument("--serve-seconds", type=float, default=8.0)
+    ap.add_argument("--serve-rates", default="20,30,40,60", help="offered job rates (jobs/s), comma-separated")
+    ap.add_argument("--no-precision", action="store_true")
+    ap.add_argument("--no-e2e", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-extensions", action="store_true")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -410,10 +508,11 @@ def main():
     log(f"bf16 form: {ms_bf * 1e3:.1f} ms/step", recs_bf[-1])
     fp32 = None
     if not a.no_fp32:
-        ms_fp, recs_fp, lab_fp = timed(Xd, a.k, "fp32", 1, 0)
+        ms_fp, recs_fp, lab_fp = timed(Xd, a.k, "fp32", 3, 1)
         same = bool(torch.equal(lab_bf, lab_fp))
-        fp32 = {"value": n / ms_fp, "ms_per_step": ms_fp * 1e3, "identical_labels": same,
-                "knn_s": recs_fp[-1]["knn_s"], "select_ms": recs_fp[-1]["knn"]["select_ms"],
+        fp32 = {"value": n / ms_fp, "ms_per_step": ms_fp * 1e3, "steps": 3, "warmup": 1,
+                "identical_labels": same, "knn_s": recs_fp[-1]["knn_s"],
+                "select_ms": float(np.mean([r["knn"]["select_ms"] for r in recs_fp])),
                 "speedup_bf16_vs_fp32": ms_fp / ms_bf}
         log("fp32 form:", fp32)
 
@@ -430,12 +529,26 @@ def main():
             "peak": PEAK["bf16"], "unit": "TFLOP/s", "frac": achieved / PEAK["bf16"],
             "traffic": traffic, "flops_per_launch": flops, "avg_launch_ms": sel_ms}
 
+    from metalsinglecell import _hip
+    gi, gd = _hip.knn_exact(Xd, a.k, precision="bf16")
+    gi, gd = gi.cpu().numpy(), gd.cpu().numpy()
     cpu = None
     if not a.no_cpu:
         threads = len(os.sched_getaffinity(0))
         t = time.perf_counter()
-        cpu = cpu_baseline(X, a.k, threads)
+        cpu = cpu_baseline(X, a.k, threads, gpu_idx=(gi, gd))
         log(f"cpu baseline in {time.perf_counter() - t:.1f}s", cpu)
+    prec = None
+    if not a.no_precision:
+        t = time.perf_counter()
+        prec = precision_report(a.k, gi, X)
+        log(f"precision report in {time.perf_counter() - t:.1f}s", prec)
+    e2e = None
+    if not a.no_e2e:
+        end_to_end(X[:20_000], a.k)          # warm the host path once
+        t = time.perf_counter()
+        e2e = end_to_end(X, a.k)
+        log(f"end to end in {time.perf_counter() - t:.1f}s", e2e)
 
     batch = None
     if not a.no_batched:
@@ -459,7 +572,7 @@ def main():
     serve = None
     if a.serve_seconds > 0:
         t = time.perf_counter()
-        serve = serving(a.serve_seconds)
+        serve = serving(a.serve_seconds, rates=tuple(float(r) for r in a.serve_rates.split(",")))
         log(f"serving in {time.perf_counter() - t:.1f}s", serve)
 
     last = recs_bf[-1]
@@ -473,7 +586,8 @@ def main():
                    "cells": n, "dims": d, "k": a.k, "parallelism": "single GPU"},
         "stages_s": {"knn": last["knn_s"], "connectivities": last["conn_s"], "leiden": last["leiden_s"]},
         "knn_info": last["knn"], "modularity": last["q"], "n_clusters": last["n_clusters"],
-        "fp32_form": fp32, "batched": batch, "serving": serve, "extensions": ext, "roofline": roof,
+        "fp32_form": fp32, "precision": prec, "end_to_end": e2e, "batched": batch, "serving": serve,
+        "extensions": ext, "roofline": roof,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

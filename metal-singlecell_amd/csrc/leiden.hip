@@ -19,6 +19,7 @@
 // ranges whose community ids would not fit the sort key: 2^22 and 2^20 vertices and up) go
 // through a sort path over their edges keyed by (vertex, community).
 #include <cmath>
+#include <mutex>
 #include <vector>
 
 #include "common.h"
@@ -1235,6 +1236,7 @@ struct Ctx {
   hipStream_t side = nullptr;       // big-vertex moves run here, beside k_move_light
   hipStream_t side2 = nullptr;      // mid-vertex moves
   hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_join2 = nullptr;
+  int side_dev = 0;                 // device whose pool the side set came from
   int64_t n0, E0;
   double gamma;
   uint64_t thr;
@@ -1273,6 +1275,58 @@ struct Ctx {
   int64_t nb = 0;
   msc_leiden_info* info;
 };
+
+// Side streams (default priority) and fork/join events for the concurrent degree tiers.  They
+// are runtime objects, so a call does not create or destroy them: each device keeps a pool of
+// sets, a call takes one on first need and returns it when it ends, and concurrent calls (one
+// per host thread) hold different sets.  Sets live until the process ends.
+struct SideSet {
+  hipStream_t side, side2;
+  hipEvent_t ev_fork, ev_join, ev_join2;
+};
+static std::mutex g_side_mutex;
+static std::vector<std::vector<SideSet>> g_side_free;  // per device
+
+static int acquire_side_set(Ctx& c) {
+  int dev = 0;
+  MSC_HIP(hipGetDevice(&dev));
+  {
+    std::lock_guard<std::mutex> lock(g_side_mutex);
+    if (static_cast<int>(g_side_free.size()) <= dev) g_side_free.resize(dev + 1);
+    auto& pool = g_side_free[dev];
+    if (!pool.empty()) {
+      const SideSet s = pool.back();
+      pool.pop_back();
+      c.side = s.side;
+      c.side2 = s.side2;
+      c.ev_fork = s.ev_fork;
+      c.ev_join = s.ev_join;
+      c.ev_join2 = s.ev_join2;
+      c.side_dev = dev;
+      return kOk;
+    }
+  }
+  SideSet s{};
+  MSC_HIP(hipStreamCreateWithFlags(&s.side, hipStreamNonBlocking));
+  MSC_HIP(hipStreamCreateWithFlags(&s.side2, hipStreamNonBlocking));
+  MSC_HIP(hipEventCreateWithFlags(&s.ev_fork, hipEventDisableTiming));
+  MSC_HIP(hipEventCreateWithFlags(&s.ev_join, hipEventDisableTiming));
+  MSC_HIP(hipEventCreateWithFlags(&s.ev_join2, hipEventDisableTiming));
+  c.side = s.side;
+  c.side2 = s.side2;
+  c.ev_fork = s.ev_fork;
+  c.ev_join = s.ev_join;
+  c.ev_join2 = s.ev_join2;
+  c.side_dev = dev;
+  return kOk;
+}
+
+static void release_side_set(Ctx& c) {
+  std::lock_guard<std::mutex> lock(g_side_mutex);
+  g_side_free[c.side_dev].push_back(SideSet{c.side, c.side2, c.ev_fork, c.ev_join, c.ev_join2});
+  c.side = c.side2 = nullptr;
+  c.ev_fork = c.ev_join = c.ev_join2 = nullptr;
+}
 
 static void carve_ctx(Carve& cv, Ctx& c, int64_t n, int64_t E) {
   const int64_t n1 = n + 1, E1 = E + 1;
@@ -1455,13 +1509,7 @@ static int local_moving(Ctx& c, const Graph& g, int64_t seed, const int32_t* par
     // the three move kernels write target[] of disjoint vertex sets and only read the rest:
     // the listed (mid / big) vertices run on the side stream concurrently with the light ones
     const bool fork = c.nm > 0 || c.nb > 0;
-    if (fork && !c.side) {  // created on first need: most small graphs never list a vertex
-      MSC_HIP(hipStreamCreateWithFlags(&c.side, hipStreamNonBlocking));
-      MSC_HIP(hipStreamCreateWithFlags(&c.side2, hipStreamNonBlocking));
-      MSC_HIP(hipEventCreateWithFlags(&c.ev_fork, hipEventDisableTiming));
-      MSC_HIP(hipEventCreateWithFlags(&c.ev_join, hipEventDisableTiming));
-      MSC_HIP(hipEventCreateWithFlags(&c.ev_join2, hipEventDisableTiming));
-    }
+    if (fork && !c.side) MSC_TRY(acquire_side_set(c));  // most small graphs never list a vertex
     if (fork) {
       MSC_HIP(hipEventRecord(c.ev_fork, st));
       if (c.nb > 0) MSC_HIP(hipStreamWaitEvent(c.side, c.ev_fork, 0));
@@ -1707,18 +1755,15 @@ static int ceil_log2(double x) {
   return m == 0.5 ? e - 1 : e;
 }
 
-// side streams (default priority) and fork/join events are created on first need by
-// local_moving and released on every return path of an entry point
+// returns the call's side set to the pool on every return path of an entry point, once the
+// side streams have drained (the main stream already waits on them through the join events)
 struct SideGuard {
   Ctx& c;
   ~SideGuard() {
-    for (hipStream_t s : {c.side, c.side2})
-      if (s) {
-        (void)hipStreamSynchronize(s);
-        (void)hipStreamDestroy(s);
-      }
-    for (hipEvent_t e : {c.ev_fork, c.ev_join, c.ev_join2})
-      if (e) (void)hipEventDestroy(e);
+    if (!c.side) return;
+    (void)hipStreamSynchronize(c.side);
+    (void)hipStreamSynchronize(c.side2);
+    release_side_set(c);
   }
 };
 

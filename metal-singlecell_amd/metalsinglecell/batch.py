@@ -104,10 +104,21 @@ def leiden_samples(bg, resolution=1.0, random_state=0, n_iterations=2, workers=8
     ip_host = bg.indptr.cpu().numpy()      # one transfer: row starts of every sample
     dev = bg.indptr.device
 
+    # one stream per worker thread, created here once (default priority) and reused by every
+    # sample that thread runs
+    import threading
+    streams = [torch.cuda.Stream(device=dev) for _ in range(max(1, int(workers)))]
+    slot = threading.local()
+    taken = iter(range(len(streams)))
+    lock = threading.Lock()
+
     def one(s):
         a, b = lo_hi[s]
         e0, e1 = int(ip_host[a]), int(ip_host[b])
-        stream = torch.cuda.Stream(device=dev)
+        if not hasattr(slot, "stream"):
+            with lock:
+                slot.stream = streams[next(taken)]
+        stream = slot.stream
         with torch.cuda.stream(stream):
             ip = bg.indptr[a:b + 1] - e0
             ind = bg.indices[e0:e1] - a

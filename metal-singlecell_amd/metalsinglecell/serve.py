@@ -110,7 +110,9 @@ class Server:
         import torch
         _hip.device()
         free, total = torch.cuda.mem_get_info()
-        self.hbm_budget = int(hbm_budget if hbm_budget is not None else 0.7 * free)
+        # memory torch's caching allocator holds but no tensor uses is free for these jobs too
+        cached = torch.cuda.memory_reserved() - torch.cuda.memory_allocated()
+        self.hbm_budget = int(hbm_budget if hbm_budget is not None else 0.7 * (free + cached))
         self.max_slots = int(max_slots)
         self.max_bypass_s = max_bypass_s
         self.permutations = permutations
