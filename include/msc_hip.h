@@ -216,6 +216,22 @@ int msc_leiden(const int64_t* indptr, const int32_t* indices, const float* weigh
                double* out_modularity /*host*/, msc_leiden_info* info /*host, may be NULL*/,
                void* workspace, size_t workspace_bytes, void* stream);
 
+/* Leiden of many independent samples in one run: the samples are the diagonal blocks of one
+ * CSR (sample s = rows offsets[s]..offsets[s+1]-1, host offsets, n_samples+1 of them, column
+ * ids global, no edge between samples).  Sample s's labels (out_labels at its rows, dense
+ * 0..C_s-1) and modularity (out_modularity[s], host) equal msc_leiden on that sample alone
+ * with the same arguments: each sample is quantised with its own exponent, scored with its
+ * own total weight, and draws its coins by its vertices' indices within the sample; the
+ * passes run in lockstep and a sample whose multilevel loop ends is frozen while the rest
+ * continue.  Replaces the per-sample loop of calls of cluster.leiden (cluster.py:36-54) that
+ * a batched caller would make.  Synchronises. */
+int msc_leiden_batched_workspace_size(int64_t n, int64_t nnz, int n_samples, size_t* bytes /*host*/);
+int msc_leiden_batched(const int64_t* indptr, const int32_t* indices, const float* weights, int64_t n,
+                       int64_t nnz, const int64_t* offsets /*host*/, int n_samples, double resolution,
+                       int64_t seed, int n_iterations, double commit_prob, int32_t* out_labels,
+                       double* out_modularity /*host, n_samples*/, void* workspace, size_t workspace_bytes,
+                       void* stream);
+
 /* Synchronous parallel multilevel Louvain (the same move kernels and contraction, no
  * refinement).  Replaces tl.louvain(backend="gpu") -> graph.louvain.louvain
  * (src/metalsinglecell/tl.py:76-91, graph/louvain.py:397-453) with variant="sync".

@@ -60,6 +60,11 @@ struct GainParams {
   double inv_s;   // 2^-s
   double inv;     // 1 / (twom * 2^-s)
   double gamma;
+  // batched runs (many samples in one block-diagonal graph): each vertex takes its sample's
+  // scale and total weight; null for a single graph
+  const int32_t* vsid = nullptr;   // sample of each vertex
+  const double* s_inv_s = nullptr; // per sample
+  const double* s_inv = nullptr;   // per sample
 };
 
 __device__ __forceinline__ double gain_of(int64_t wc, int64_t sig, int64_t kv, const GainParams& gp) {
@@ -67,6 +72,17 @@ __device__ __forceinline__ double gain_of(int64_t wc, int64_t sig, int64_t kv, c
   const double S = static_cast<double>(sig) * gp.inv_s;
   const double K = static_cast<double>(kv) * gp.inv_s;
   return W - ((gp.gamma * S) * K) * gp.inv;
+}
+
+// the gain parameters of vertex v (its sample's, in a batched run)
+__device__ __forceinline__ GainParams for_vertex(const GainParams& gp, int64_t v) {
+  GainParams g = gp;
+  if (gp.vsid) {
+    const int32_t sid = gp.vsid[v];
+    g.inv_s = gp.s_inv_s[sid];
+    g.inv = gp.s_inv[sid];
+  }
+  return g;
 }
 
 // better(g1, c1) than (g2, c2): larger gain, ties to the smaller community
@@ -226,8 +242,9 @@ template <int NI>
 __device__ __forceinline__ void move_one(int64_t v, int64_t e0, int64_t deg, int32_t curr, int64_t kv,
                                          const int32_t* __restrict__ indices, const int64_t* __restrict__ wq,
                                          const int32_t* __restrict__ comm, const int32_t* __restrict__ part,
-                                         const int64_t* __restrict__ sigtot, const GainParams& gp,
+                                         const int64_t* __restrict__ sigtot, const GainParams& gp_all,
                                          int32_t* __restrict__ target) {
+  const GainParams gp = for_vertex(gp_all, v);
   const int l = lane_id();
   const int32_t pv = part ? part[v] : 0;
   bool ok[NI];
@@ -371,9 +388,10 @@ __device__ __forceinline__ void argmax_step(double& g, int32_t& c) {
 // (community << IB | e) or kNoKey, their weights in lw[e] and sigma_tot in lsg[e].
 template <int NI>
 __device__ __forceinline__ void move_core(int64_t v, uint32_t (&key)[NI], int nvalid, int32_t curr, int64_t kv,
-                                          const GainParams& gp, int32_t* __restrict__ target,
+                                          const GainParams& gp_all, int32_t* __restrict__ target,
                                           const int64_t* __restrict__ lw, const int64_t* __restrict__ lsg,
                                           int64_t* __restrict__ lp) {
+  const GainParams gp = for_vertex(gp_all, v);
   constexpr int IB = NI == 1 ? 6 : NI == 2 ? 7 : 8;
   static_assert(NI == 1 || NI == 2 || NI == 4, "NI");
   const int l = lane_id();
@@ -639,7 +657,7 @@ __global__ __launch_bounds__(THREADS) void k_move_block(
     const int32_t* __restrict__ verts, const int64_t* __restrict__ indptr, const int32_t* __restrict__ indices,
     const int64_t* __restrict__ wq, const int64_t* __restrict__ kdeg, const int32_t* __restrict__ comm,
     const int32_t* __restrict__ part, const int64_t* __restrict__ sigtot, const uint8_t* __restrict__ active,
-    GainParams gp, int32_t* __restrict__ target) {
+    GainParams gp_all, int32_t* __restrict__ target) {
   constexpr int ITEMS = 4;
   constexpr int N = THREADS * ITEMS;
   constexpr int IB = THREADS == 256 ? 10 : THREADS == 1024 ? 12 : -1;
@@ -657,6 +675,7 @@ __global__ __launch_bounds__(THREADS) void k_move_block(
   __shared__ double s_stay;
   const int t = threadIdx.x, l = lane_id(), wv = t / kWave;
   const int64_t v = verts[blockIdx.x];
+  const GainParams gp = for_vertex(gp_all, v);
   const int32_t curr = comm[v];
   if (!active[v]) {  // uniform over the block
     if (t == 0) target[v] = curr;
@@ -907,10 +926,11 @@ __global__ __launch_bounds__(kThreads) void k_heavy_argmax(
     const int32_t* __restrict__ verts, int64_t nh, const int64_t* __restrict__ hseg,
     const int64_t* __restrict__ segstart, const int64_t* __restrict__ prefix, const uint32_t* __restrict__ perm2,
     const uint32_t* __restrict__ kc_unsorted, const int64_t* __restrict__ kdeg, const int32_t* __restrict__ comm,
-    const int64_t* __restrict__ sigtot, GainParams gp, int32_t* __restrict__ target) {
+    const int64_t* __restrict__ sigtot, GainParams gp_all, int32_t* __restrict__ target) {
   const int64_t h = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + threadIdx.x / kWave;
   if (h >= nh) return;
   const int32_t v = verts[h];
+  const GainParams gp = for_vertex(gp_all, v);
   const int32_t curr = comm[v];
   const int64_t kv = kdeg[v];
   double best_g = -1e300, stay = 0.0;
@@ -949,7 +969,8 @@ constexpr int kCommitThreads = 1024;
 __global__ __launch_bounds__(kCommitThreads) void k_commit(
     int32_t* __restrict__ comm, const int32_t* __restrict__ target, int64_t n, uint64_t pkey, uint64_t thr,
     uint8_t* __restrict__ commit, uint8_t* __restrict__ active, int32_t* __restrict__ wants_flag,
-    int32_t* __restrict__ mv_v, int32_t* __restrict__ mv_old, int32_t* __restrict__ mv_cnt) {
+    int32_t* __restrict__ mv_v, int32_t* __restrict__ mv_old, int32_t* __restrict__ mv_cnt,
+    const int32_t* __restrict__ vsid, const int64_t* __restrict__ vbase) {
   __shared__ int wcnt[kCommitThreads / kWave];
   const int64_t v = static_cast<int64_t>(blockIdx.x) * kCommitThreads + threadIdx.x;
   bool cm = false;
@@ -958,7 +979,9 @@ __global__ __launch_bounds__(kCommitThreads) void k_commit(
     const int32_t t = target[v];
     c = comm[v];
     const bool wants = t != c;
-    const bool coin = static_cast<uint64_t>(coin_u32(pkey, static_cast<uint64_t>(v))) < thr;
+    // a batched run draws each sample's coins by the vertex's index within its sample
+    const int64_t vl = vsid ? v - vbase[vsid[v]] : v;
+    const bool coin = static_cast<uint64_t>(coin_u32(pkey, static_cast<uint64_t>(vl))) < thr;
     cm = wants && coin;
     if (cm) comm[v] = t;
     commit[v] = cm;
@@ -1229,6 +1252,119 @@ __global__ void k_scatter_pagg(const int32_t* __restrict__ R, const int32_t* __r
 }
 
 // ------------------------------------------------------------------------------------------
+// batched runs: many independent samples as one block-diagonal graph
+// ------------------------------------------------------------------------------------------
+// sample of vertex v: the last s with vbase[s] <= v
+__device__ __forceinline__ int32_t sample_of(const int64_t* __restrict__ vbase, int S, int64_t v) {
+  int lo = 0, hi = S - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (vbase[mid] <= v) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+__global__ void k_fill_sid(const int64_t* __restrict__ vbase, int S, int64_t n, int32_t* __restrict__ vsid) {
+  const int64_t v = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (v < n) vsid[v] = sample_of(vbase, S, v);
+}
+__global__ void k_active_alive(uint8_t* __restrict__ active, int64_t n, const int32_t* __restrict__ vsid,
+                               const uint8_t* __restrict__ alive) {
+  const int64_t v = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (v < n) active[v] = alive[vsid[v]];
+}
+// out[s] = src[idx[s]] for s <= S
+__global__ void k_gather_at(const int64_t* __restrict__ src64, const int32_t* __restrict__ src32,
+                            const int64_t* __restrict__ idx, int S, int64_t* __restrict__ out) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s <= S) out[s] = src64 ? src64[idx[s]] : static_cast<int64_t>(src32[idx[s]]);
+}
+// per-sample maximum weight over the sample's edge range (block per sample; +inf for a
+// negative or non-finite weight, as k_max_weight)
+__global__ void k_sample_max_weight(const float* __restrict__ w, const int64_t* __restrict__ eoff,
+                                    float* __restrict__ out) {
+  float m = 0.0f;
+  for (int64_t i = eoff[blockIdx.x] + threadIdx.x; i < eoff[blockIdx.x + 1]; i += blockDim.x) {
+    const float x = w[i];
+    m = fmaxf(m, (isfinite(x) && x >= 0.0f) ? x : __uint_as_float(0x7F800000u));
+  }
+  m = wave_max_f32(m);
+  __shared__ float sh[kWavesPerBlock];
+  if (lane_id() == 0) sh[threadIdx.x / kWave] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int i = 1; i < kWavesPerBlock; ++i) m = fmaxf(m, sh[i]);
+    out[blockIdx.x] = m;
+  }
+}
+__global__ void k_sample_quantize(const float* __restrict__ w, const int64_t* __restrict__ eoff,
+                                  const int32_t* __restrict__ sexp, int64_t* __restrict__ wq) {
+  const int se = sexp[blockIdx.x];
+  for (int64_t i = eoff[blockIdx.x] + threadIdx.x; i < eoff[blockIdx.x + 1]; i += blockDim.x)
+    wq[i] = static_cast<int64_t>(rint(ldexp(static_cast<double>(w[i]), se)));
+}
+// per-sample sum of k over the sample's vertex range (block per sample; exact)
+__global__ void k_sample_sum(const int64_t* __restrict__ k, const int64_t* __restrict__ vbase,
+                             int64_t* __restrict__ out) {
+  int64_t t = 0;
+  for (int64_t v = vbase[blockIdx.x] + threadIdx.x; v < vbase[blockIdx.x + 1]; v += blockDim.x) t += k[v];
+  t = wave_sum_i64(t);
+  __shared__ int64_t sh[kWavesPerBlock];
+  if (lane_id() == 0) sh[threadIdx.x / kWave] = t;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int i = 1; i < kWavesPerBlock; ++i) t += sh[i];
+    out[blockIdx.x] = t;
+  }
+}
+// out[o] = lab[orig2node[o]] - base[s] for the original vertices o of the samples s with
+// mask[s] set (labels relative to the sample's own range)
+__global__ void k_local_labels(const int32_t* __restrict__ lab, const int32_t* __restrict__ orig2node,
+                               const int32_t* __restrict__ vsid0, const int64_t* __restrict__ base,
+                               const uint8_t* __restrict__ mask, int64_t n0, int32_t* __restrict__ out) {
+  const int64_t o = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (o >= n0) return;
+  const int32_t sid = vsid0[o];
+  if (mask == nullptr || mask[sid])
+    out[o] = static_cast<int32_t>(static_cast<int64_t>(lab[orig2node ? orig2node[o] : o]) - base[sid]);
+}
+__global__ void k_copy_masked(const int32_t* __restrict__ src, const int32_t* __restrict__ vsid0,
+                              const uint8_t* __restrict__ mask, int64_t n0, int32_t* __restrict__ dst) {
+  const int64_t o = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (o < n0 && mask[vsid0[o]]) dst[o] = src[o];
+}
+// Pagg holds this level's move labels (sample s: [cb[s], cb[s+1])); the next level's vertices
+// of sample s are [rb[s], rb[s+1]).  Community ids must lie in their own sample's vertex range,
+// so they move to rb[s] + (label - cb[s]) (order within the sample kept).
+__global__ void k_rebase_labels(const int32_t* __restrict__ src, const int32_t* __restrict__ vsid, int64_t n,
+                                const int64_t* __restrict__ from, const int64_t* __restrict__ to,
+                                int32_t* __restrict__ dst) {
+  const int64_t v = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (v < n) {
+    const int32_t sid = vsid[v];
+    dst[v] = static_cast<int32_t>(static_cast<int64_t>(src[v]) - from[sid] + to[sid]);
+  }
+}
+// out[o] = local[o] + vbase0[s]: per-sample labels into the level-0 id space
+__global__ void k_global_labels(const int32_t* __restrict__ local, const int32_t* __restrict__ vsid0,
+                                const int64_t* __restrict__ vbase0, int64_t n0, int32_t* __restrict__ out) {
+  const int64_t o = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (o < n0) out[o] = static_cast<int32_t>(local[o] + vbase0[vsid0[o]]);
+}
+// modularity terms with each community's sample scale and total weight (k_q_terms per sample)
+__global__ void k_q_terms_batched(const int64_t* __restrict__ tot, const int64_t* __restrict__ in, int64_t C,
+                                  const int64_t* __restrict__ cbase, int S, const double* __restrict__ s_inv_s,
+                                  const double* __restrict__ s_twom_real, double gamma, double* __restrict__ terms) {
+  const int64_t c = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (c < C) {
+    const int32_t sid = sample_of(cbase, S, c);
+    const double inv_s = s_inv_s[sid], twom_real = s_twom_real[sid];
+    const double a = (static_cast<double>(tot[c]) * inv_s) / twom_real;
+    terms[c] = (static_cast<double>(in[c]) * inv_s) / twom_real - gamma * (a * a);
+  }
+}
+
+// ------------------------------------------------------------------------------------------
 // the driver
 // ------------------------------------------------------------------------------------------
 struct Ctx {
@@ -1274,6 +1410,12 @@ struct Ctx {
   int32_t *bflag, *bpos, *bverts;  // big-degree vertices (one-wave sorted path, 96 KiB LDS)
   int64_t nb = 0;
   msc_leiden_info* info;
+  // batched run (samples laid out block-diagonally); null for a single graph
+  int32_t* vsid = nullptr;          // sample of each vertex of the current level graph
+  int64_t* d_vbase = nullptr;       // sample vertex offsets of the current level graph (S + 1)
+  double* s_inv_s = nullptr;        // per-sample 2^-s
+  double* s_inv = nullptr;          // per-sample 1 / (twom * 2^-s)
+  uint8_t* s_alive = nullptr;       // per sample: 0 once its run has ended (its vertices stay put)
 };
 
 // Side streams (default priority) and fork/join events for the concurrent degree tiers.  They
@@ -1487,8 +1629,12 @@ static int local_moving(Ctx& c, const Graph& g, int64_t seed, const int32_t* par
   gp.inv_s = c.inv_s;
   gp.inv = 1.0 / (static_cast<double>(g.twom) * c.inv_s);
   gp.gamma = c.gamma;
+  gp.vsid = c.vsid;
+  gp.s_inv_s = c.s_inv_s;
+  gp.s_inv = c.s_inv;
   const uint64_t key = stream_key(seed, part ? 1 : 0);
-  k_fill_u8<<<grid_for(g.n), kThreads, 0, st>>>(c.active, g.n, 1);
+  if (c.vsid) k_active_alive<<<grid_for(g.n), kThreads, 0, st>>>(c.active, g.n, c.vsid, c.s_alive);
+  else k_fill_u8<<<grid_for(g.n), kThreads, 0, st>>>(c.active, g.n, 1);
   k_fill_i32<<<grid_for(2 * kMaxPasses), kThreads, 0, st>>>(c.pass_info, 2 * kMaxPasses, 0);
   MSC_LAUNCH_CHECK();
   // sigma_tot is rebuilt from scratch (sort + segmented sums) at a pass start unless the
@@ -1540,7 +1686,7 @@ static int local_moving(Ctx& c, const Graph& g, int64_t seed, const int32_t* par
     MSC_TRY(heavy_moves(c, g, c.comm, part, gp));
     k_commit<<<static_cast<unsigned>(nblk), kCommitThreads, 0, st>>>(
         c.comm, c.target, g.n, pass_key(key, p), c.thr, c.commit, c.active, c.pass_info + 2 * p, c.mv_v, c.mv_old,
-        c.mv_cnt);
+        c.mv_cnt, c.vsid, c.d_vbase);
     MSC_LAUNCH_CHECK();
     const bool sync = (p % kSyncEvery == kSyncEvery - 1) || p == kMaxPasses - 1;
     // incremental passes update sigma_tot; a full-mode sync pass only counts the moves
@@ -1835,6 +1981,279 @@ static int end_run(Ctx& c, const Graph& g0, const int32_t* out_labels, double* o
   return kOk;
 }
 
+// ---- batched run ---------------------------------------------------------------------
+// Many independent samples as one block-diagonal graph.  Every sample follows exactly the
+// steps msc_leiden takes on it alone: its weights are quantised with its own exponent, its
+// gains use its own total weight, its coins are drawn by its vertices' indices within the
+// sample, and the pass loop runs in lockstep (a sample whose vertices stopped wanting to
+// move makes no further change, which is what ending its own loop there would give).  The
+// dense relabels keep each sample's communities contiguous and in their own order, so level
+// graphs, labels and modularity are per sample what a single run produces.  A sample whose
+// multilevel loop ends (no modularity gain, no merge, or no refinement) is frozen: its
+// vertices stay inactive and its labels fixed while the others continue.
+struct BatchRun {
+  int S = 0;
+  std::vector<int64_t> vb0, vb;       // sample vertex offsets: level 0 and the current level
+  std::vector<double> twom_real;      // per sample
+  std::vector<double> q_prev;
+  std::vector<uint8_t> alive;
+  int64_t* d_vb0 = nullptr;           // device copies
+  int32_t* d_vsid0 = nullptr;         // sample of each original vertex
+  int64_t* d_base = nullptr;          // per-sample label base of the current P numbering
+  int64_t* d_tmp = nullptr;           // S + 1 gathered values
+  int64_t* d_eoff = nullptr;
+  double* d_twom_real = nullptr;
+  uint8_t* d_mask = nullptr;          // per-sample mask for the label kernels
+  float* d_wmax = nullptr;
+  int32_t* d_sexp = nullptr;
+  int64_t* d_twom = nullptr;
+};
+
+static void carve_batch(Carve& cv, Ctx& c, BatchRun& b, int64_t n, int S) {
+  const size_t S1 = static_cast<size_t>(S) + 1;
+  c.vsid = cv.take<int32_t>(n + 1);
+  c.d_vbase = cv.take<int64_t>(S1);
+  c.s_inv_s = cv.take<double>(S1);
+  c.s_inv = cv.take<double>(S1);
+  c.s_alive = cv.take<uint8_t>(S1);
+  b.d_vb0 = cv.take<int64_t>(S1);
+  b.d_vsid0 = cv.take<int32_t>(n + 1);
+  b.d_base = cv.take<int64_t>(S1);
+  b.d_tmp = cv.take<int64_t>(S1);
+  b.d_eoff = cv.take<int64_t>(S1);
+  b.d_twom_real = cv.take<double>(S1);
+  b.d_mask = cv.take<uint8_t>(S1);
+  b.d_wmax = cv.take<float>(S1);
+  b.d_sexp = cv.take<int32_t>(S1);
+  b.d_twom = cv.take<int64_t>(S1);
+}
+
+template <typename T>
+static int upload(T* dev, const std::vector<T>& host, hipStream_t st) {
+  MSC_HIP(hipMemcpyAsync(dev, host.data(), host.size() * sizeof(T), hipMemcpyHostToDevice, st));
+  // host vectors may change right after: the copy must have consumed them
+  MSC_HIP(hipStreamSynchronize(st));
+  return kOk;
+}
+
+// per-sample first index of a dense relabel: map[vb[s]] for s <= S (map = c.map of the last relabel)
+static int sample_bases(Ctx& c, BatchRun& b, std::vector<int64_t>& out) {
+  hipStream_t st = c.st;
+  k_gather_at<<<static_cast<unsigned>(ceil_div(b.S + 1, kThreads)), kThreads, 0, st>>>(nullptr, c.map, c.d_vbase, b.S,
+                                                                                      b.d_tmp);
+  MSC_LAUNCH_CHECK();
+  out.resize(b.S + 1);
+  MSC_HIP(hipMemcpyAsync(out.data(), b.d_tmp, (b.S + 1) * sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  MSC_HIP(hipStreamSynchronize(st));
+  return kOk;
+}
+
+static int set_mask(BatchRun& b, const std::vector<uint8_t>& m, hipStream_t st) { return upload(b.d_mask, m, st); }
+
+// modularity of dense labels lab (C labels, sample s owning [cb[s], cb[s+1])) per sample
+static int modularity_batched(Ctx& c, BatchRun& b, const Graph& g, const int32_t* lab, int64_t C,
+                              const std::vector<int64_t>& cb, std::vector<double>& q) {
+  hipStream_t st = c.st;
+  k_internal<<<grid_for(g.n * kWave), kThreads, 0, st>>>(g.indptr, g.indices, g.wq, lab, g.n, c.iw);
+  MSC_LAUNCH_CHECK();
+  MSC_TRY(group_sort(lab, g.n, C, c.grp, st));
+  MSC_TRY(group_sum(g.k, g.n, C, c.grp, c.tot, st));
+  MSC_TRY(group_sum(c.iw, g.n, C, c.grp, c.inw, st));
+  MSC_TRY(upload(b.d_tmp, cb, st));
+  k_q_terms_batched<<<grid_for(C), kThreads, 0, st>>>(c.tot, c.inw, C, b.d_tmp, b.S, c.s_inv_s, b.d_twom_real,
+                                                       c.gamma, c.terms);
+  MSC_LAUNCH_CHECK();
+  std::vector<double> terms(static_cast<size_t>(C));
+  MSC_HIP(hipMemcpyAsync(terms.data(), c.terms, C * sizeof(double), hipMemcpyDeviceToHost, st));
+  MSC_HIP(hipStreamSynchronize(st));
+  q.assign(b.S, 0.0);
+  for (int s = 0; s < b.S; ++s) {
+    double acc = 0.0;
+    for (int64_t t = cb[s]; t < cb[s + 1]; ++t) acc += terms[t];  // sequential, label order
+    q[s] = acc;
+  }
+  return kOk;
+}
+
+static bool any_alive(const BatchRun& b) {
+  for (uint8_t a : b.alive)
+    if (a) return true;
+  return false;
+}
+
+// one multilevel pass of every sample; init (global level-0 ids) may be null; the result,
+// per-sample dense labels, in c.final_lab
+static int leiden_pass_batched(Ctx& c, Graph& g0, int64_t seed, const int32_t* init, BatchRun& b) {
+  hipStream_t st = c.st;
+  const int64_t n0 = g0.n;
+  const int S = b.S;
+  Graph g = g0;
+  int slot = 0;
+  k_iota_i32<<<grid_for(n0), kThreads, 0, st>>>(c.orig2node, n0);
+  if (init) k_copy_i32<<<grid_for(n0), kThreads, 0, st>>>(init, c.P, n0);
+  else k_iota_i32<<<grid_for(n0), kThreads, 0, st>>>(c.P, n0);
+  MSC_LAUNCH_CHECK();
+  // the current level's offsets, sample ids and label base (level 0: the vertex offsets)
+  b.vb = b.vb0;
+  MSC_TRY(upload(c.d_vbase, b.vb, st));
+  MSC_TRY(upload(b.d_base, b.vb0, st));
+  k_copy_i32<<<grid_for(n0), kThreads, 0, st>>>(b.d_vsid0, c.vsid, n0);
+  k_local_labels<<<grid_for(n0), kThreads, 0, st>>>(c.P, nullptr, b.d_vsid0, b.d_base, nullptr, n0, c.final_lab);
+  MSC_LAUNCH_CHECK();
+  b.q_prev.assign(S, -1.0);
+  b.alive.assign(S + 1, 1);
+  b.alive[S] = 0;
+  MSC_TRY(upload(c.s_alive, b.alive, st));
+  std::vector<int64_t> cb, rb;
+  std::vector<double> q;
+  std::vector<uint8_t> accept(S + 1, 0), reject(S + 1, 0);
+  for (int level = 0; level < kMaxLevels; ++level) {
+    // prev labels of the running samples (relative to their own P range)
+    MSC_TRY(set_mask(b, b.alive, st));
+    k_local_labels<<<grid_for(n0), kThreads, 0, st>>>(c.P, c.orig2node, b.d_vsid0, b.d_base, b.d_mask, n0,
+                                                       c.prev_lab);
+    k_copy_i32<<<grid_for(g.n), kThreads, 0, st>>>(c.P, c.comm, g.n);
+    MSC_LAUNCH_CHECK();
+    int mp = 0, md = 0;
+    MSC_TRY(local_moving(c, g, seed + level, nullptr, &mp, &md));
+    int64_t ncomm;
+    MSC_TRY(dense_relabel(c, c.comm, g.n, c.P, &ncomm));
+    MSC_TRY(sample_bases(c, b, cb));
+    MSC_TRY(modularity_batched(c, b, g, c.P, ncomm, cb, q));
+    std::fill(accept.begin(), accept.end(), 0);
+    std::fill(reject.begin(), reject.end(), 0);
+    for (int s = 0; s < S; ++s) {
+      if (!b.alive[s]) continue;
+      if (q[s] <= b.q_prev[s] + kTol) {
+        reject[s] = 1;
+        b.alive[s] = 0;
+        continue;
+      }
+      b.q_prev[s] = q[s];
+      accept[s] = 1;
+      if (cb[s + 1] - cb[s] == b.vb[s + 1] - b.vb[s]) b.alive[s] = 0;
+    }
+    MSC_TRY(set_mask(b, reject, st));
+    k_copy_masked<<<grid_for(n0), kThreads, 0, st>>>(c.prev_lab, b.d_vsid0, b.d_mask, n0, c.final_lab);
+    MSC_LAUNCH_CHECK();
+    MSC_TRY(set_mask(b, accept, st));
+    MSC_TRY(upload(b.d_base, cb, st));
+    k_local_labels<<<grid_for(n0), kThreads, 0, st>>>(c.P, c.orig2node, b.d_vsid0, b.d_base, b.d_mask, n0,
+                                                       c.final_lab);
+    MSC_LAUNCH_CHECK();
+    if (!any_alive(b)) break;
+    MSC_TRY(upload(c.s_alive, b.alive, st));
+    k_iota_i32<<<grid_for(g.n), kThreads, 0, st>>>(c.comm, g.n);
+    MSC_LAUNCH_CHECK();
+    int rp = 0, rd = 0;
+    MSC_TRY(local_moving(c, g, seed + level, c.P, &rp, &rd));
+    int64_t nR;
+    MSC_TRY(dense_relabel(c, c.comm, g.n, c.R, &nR));
+    MSC_TRY(sample_bases(c, b, rb));
+    for (int s = 0; s < S; ++s)
+      if (b.alive[s] && rb[s + 1] - rb[s] == b.vb[s + 1] - b.vb[s]) b.alive[s] = 0;
+    if (!any_alive(b)) break;
+    MSC_TRY(upload(c.s_alive, b.alive, st));
+    k_scatter_pagg<<<grid_for(g.n), kThreads, 0, st>>>(c.R, c.P, g.n, c.Pagg);
+    k_gather_i32<<<grid_for(n0), kThreads, 0, st>>>(c.R, c.orig2node, c.tmp_lab, n0);
+    k_copy_i32<<<grid_for(n0), kThreads, 0, st>>>(c.tmp_lab, c.orig2node, n0);
+    MSC_LAUNCH_CHECK();
+    Graph ng;
+    MSC_TRY(contract(c, g, c.R, nR, slot, &ng));
+    slot ^= 1;
+    g = ng;
+    // the next level's vertices are the refined communities (offsets rb); its starting
+    // communities, this level's move labels, move into each sample's own vertex range
+    b.vb = rb;
+    MSC_TRY(upload(c.d_vbase, b.vb, st));
+    k_fill_sid<<<grid_for(g.n), kThreads, 0, st>>>(c.d_vbase, S, g.n, c.vsid);
+    MSC_LAUNCH_CHECK();
+    MSC_TRY(upload(b.d_base, cb, st));
+    k_rebase_labels<<<grid_for(nR), kThreads, 0, st>>>(c.Pagg, c.vsid, nR, b.d_base, c.d_vbase, c.P);
+    MSC_LAUNCH_CHECK();
+    MSC_TRY(upload(b.d_base, rb, st));
+  }
+  return kOk;
+}
+
+static int begin_batched(Ctx& c, BatchRun& b, Graph& g0, const int64_t* indptr, const int32_t* indices,
+                         const float* weights, int64_t n, int64_t nnz, const int64_t* offsets, int S,
+                         double resolution, double commit_prob, int32_t* out_labels, void* workspace,
+                         size_t workspace_bytes, void* stream) {
+  MSC_TRY(validate(n, nnz));
+  MSC_REQUIRE(indptr && indices && weights && out_labels && workspace && offsets, "leiden_batched: null pointer");
+  MSC_REQUIRE(S >= 1 && offsets[0] == 0 && offsets[S] == n, "leiden_batched: offsets must run from 0 to n");
+  for (int s = 0; s < S; ++s)
+    MSC_REQUIRE(offsets[s + 1] > offsets[s], "leiden_batched: sample %d is empty", s);
+  MSC_REQUIRE(commit_prob > 0.0 && commit_prob <= 1.0, "leiden_batched: commit_prob must be in (0, 1]");
+  c.st = static_cast<hipStream_t>(stream);
+  hipStream_t st = c.st;
+  c.n0 = n;
+  c.E0 = nnz;
+  c.gamma = resolution;
+  c.thr = static_cast<uint64_t>(std::min(std::floor(commit_prob * 4294967296.0), 4294967296.0));
+  c.info = nullptr;
+  Carve cv(workspace, workspace_bytes);
+  carve_ctx(cv, c, n, nnz);
+  int64_t* wq0 = cv.take<int64_t>(nnz + 1);
+  int64_t* k0 = cv.take<int64_t>(n + 1);
+  carve_batch(cv, c, b, n, S);
+  if (!cv.ok()) {
+    set_error("leiden_batched: workspace too small (%zu < %zu)", workspace_bytes, cv.used);
+    return kWorkspaceTooSmall;
+  }
+  b.S = S;
+  b.vb0.assign(offsets, offsets + S + 1);
+  MSC_TRY(upload(b.d_vb0, b.vb0, st));
+  k_fill_sid<<<grid_for(n), kThreads, 0, st>>>(b.d_vb0, S, n, b.d_vsid0);
+  k_gather_at<<<static_cast<unsigned>(ceil_div(S + 1, kThreads)), kThreads, 0, st>>>(indptr, nullptr, b.d_vb0, S,
+                                                                                    b.d_eoff);
+  k_sample_max_weight<<<S, kThreads, 0, st>>>(weights, b.d_eoff, b.d_wmax);
+  MSC_LAUNCH_CHECK();
+  std::vector<int64_t> eoff(S + 1);
+  std::vector<float> wmax(S);
+  MSC_HIP(hipMemcpyAsync(eoff.data(), b.d_eoff, (S + 1) * sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  MSC_HIP(hipMemcpyAsync(wmax.data(), b.d_wmax, S * sizeof(float), hipMemcpyDeviceToHost, st));
+  MSC_HIP(hipStreamSynchronize(st));
+  MSC_REQUIRE(eoff[0] == 0 && eoff[S] == nnz, "leiden_batched: indptr does not match the offsets");
+  std::vector<int32_t> sexp(S);
+  std::vector<double> inv_s(S);
+  for (int s = 0; s < S; ++s) {
+    MSC_REQUIRE(std::isfinite(wmax[s]), "leiden_batched: weights must be finite and >= 0 (sample %d)", s);
+    const int64_t Es = eoff[s + 1] - eoff[s];
+    sexp[s] = (Es > 0 && wmax[s] > 0.0f) ? 52 - ceil_log2(wmax[s]) - ceil_log2(static_cast<double>(Es)) : 0;
+    inv_s[s] = std::ldexp(1.0, -sexp[s]);
+  }
+  MSC_TRY(upload(b.d_sexp, sexp, st));
+  k_sample_quantize<<<S, kThreads, 0, st>>>(weights, b.d_eoff, b.d_sexp, wq0);
+  MSC_LAUNCH_CHECK();
+  g0.n = n;
+  g0.E = nnz;
+  g0.indptr = const_cast<int64_t*>(indptr);
+  g0.indices = const_cast<int32_t*>(indices);
+  g0.wq = wq0;
+  g0.k = k0;
+  MSC_TRY(graph_setup(c, g0));
+  k_sample_sum<<<S, kThreads, 0, st>>>(g0.k, b.d_vb0, b.d_twom);
+  MSC_LAUNCH_CHECK();
+  std::vector<int64_t> twom(S);
+  MSC_HIP(hipMemcpyAsync(twom.data(), b.d_twom, S * sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  MSC_HIP(hipStreamSynchronize(st));
+  std::vector<double> s_inv(S);
+  b.twom_real.resize(S);
+  for (int s = 0; s < S; ++s) {
+    MSC_REQUIRE(twom[s] > 0, "leiden_batched: sample %d has no positive edge weight", s);
+    s_inv[s] = 1.0 / (static_cast<double>(twom[s]) * inv_s[s]);
+    b.twom_real[s] = static_cast<double>(twom[s]) * inv_s[s];
+  }
+  MSC_TRY(upload(c.s_inv_s, inv_s, st));
+  MSC_TRY(upload(c.s_inv, s_inv, st));
+  MSC_TRY(upload(b.d_twom_real, b.twom_real, st));
+  c.s = 0;
+  c.inv_s = 1.0;  // unused: every vertex takes its sample's scale
+  return kOk;
+}
+
 }  // namespace leiden
 }  // namespace msc
 
@@ -1900,4 +2319,59 @@ extern "C" int msc_louvain(const int64_t* indptr, const int32_t* indices, const 
   k_copy_i32<<<grid_for(n), kThreads, 0, c.st>>>(c.final_lab, out_labels, n);
   MSC_LAUNCH_CHECK();
   return end_run(c, g0, out_labels, out_modularity, info);
+}
+
+extern "C" int msc_leiden_batched_workspace_size(int64_t n, int64_t nnz, int n_samples, size_t* bytes) {
+  MSC_TRY(validate(n, nnz));
+  MSC_REQUIRE(bytes && n_samples >= 1, "leiden_batched: bad arguments");
+  Carve cv(nullptr, 0);
+  Ctx c{};
+  BatchRun b;
+  carve_ctx(cv, c, n, nnz);
+  cv.take<int64_t>(nnz + 1);
+  cv.take<int64_t>(n + 1);
+  carve_batch(cv, c, b, n, n_samples);
+  *bytes = cv.used + 256;
+  return kOk;
+}
+
+extern "C" int msc_leiden_batched(const int64_t* indptr, const int32_t* indices, const float* weights, int64_t n,
+                                  int64_t nnz, const int64_t* offsets, int n_samples, double resolution, int64_t seed,
+                                  int n_iterations, double commit_prob, int32_t* out_labels, double* out_modularity,
+                                  void* workspace, size_t workspace_bytes, void* stream) {
+  Ctx c{};
+  SideGuard side_guard{c};
+  BatchRun b;
+  Graph g0;
+  MSC_TRY(begin_batched(c, b, g0, indptr, indices, weights, n, nnz, offsets, n_samples, resolution, commit_prob,
+                        out_labels, workspace, workspace_bytes, stream));
+  hipStream_t st = c.st;
+  const int iters = std::max(1, n_iterations);
+  for (int it = 0; it < iters; ++it) {
+    const int32_t* init = nullptr;
+    if (it > 0) {  // the previous labels, in the level-0 id space
+      k_global_labels<<<grid_for(n), kThreads, 0, st>>>(out_labels, b.d_vsid0, b.d_vb0, n, c.tmp_lab);
+      MSC_LAUNCH_CHECK();
+      init = c.tmp_lab;
+    }
+    MSC_TRY(graph_setup(c, g0));
+    MSC_TRY(leiden_pass_batched(c, g0, seed + 17 * it, init, b));
+    k_copy_i32<<<grid_for(n), kThreads, 0, st>>>(c.final_lab, out_labels, n);
+    MSC_LAUNCH_CHECK();
+  }
+  if (out_modularity) {
+    k_global_labels<<<grid_for(n), kThreads, 0, st>>>(out_labels, b.d_vsid0, b.d_vb0, n, c.tmp_lab);
+    MSC_LAUNCH_CHECK();
+    int64_t C;
+    MSC_TRY(dense_relabel(c, c.tmp_lab, n, c.tmp_lab, &C));
+    b.vb = b.vb0;
+    MSC_TRY(upload(c.d_vbase, b.vb, st));
+    std::vector<int64_t> cb;
+    MSC_TRY(sample_bases(c, b, cb));
+    std::vector<double> q;
+    MSC_TRY(modularity_batched(c, b, g0, c.tmp_lab, C, cb, q));
+    for (int s = 0; s < n_samples; ++s) out_modularity[s] = q[s];
+  }
+  MSC_HIP(hipStreamSynchronize(st));
+  return kOk;
 }

@@ -103,6 +103,9 @@ SIGNATURES = {
     "msc_leiden_workspace_size": (c_int, [c_i64, c_i64, ctypes.POINTER(c_sz)]),
     "msc_leiden": (c_int, [c_vp, c_vp, c_vp, c_i64, c_i64, c_dbl, c_i64, c_int, c_dbl, c_vp, c_vp,
                            ctypes.POINTER(c_dbl), ctypes.POINTER(LeidenInfo), c_vp, c_sz, c_vp]),
+    "msc_leiden_batched_workspace_size": (c_int, [c_i64, c_i64, c_int, ctypes.POINTER(c_sz)]),
+    "msc_leiden_batched": (c_int, [c_vp, c_vp, c_vp, c_i64, c_i64, c_vp, c_int, c_dbl, c_i64, c_int, c_dbl,
+                                   c_vp, c_vp, c_vp, c_sz, c_vp]),
     "msc_louvain_workspace_size": (c_int, [c_i64, c_i64, ctypes.POINTER(c_sz)]),
     "msc_louvain": (c_int, [c_vp, c_vp, c_vp, c_i64, c_i64, c_dbl, c_i64, c_int, c_dbl, c_vp,
                             ctypes.POINTER(c_dbl), ctypes.POINTER(LeidenInfo), c_vp, c_sz, c_vp]),
@@ -365,6 +368,31 @@ def leiden(indptr, indices, data, n, resolution=1.0, random_state=0, n_iteration
               for i in range(info.n_levels)]
     return labels, q.value, {"scale_exponent": info.scale_exponent,
                              "twom_fixed": info.twom_fixed, "levels": levels}
+
+
+def leiden_batched(indptr, indices, data, offsets, resolution=1.0, random_state=0, n_iterations=2,
+                   commit_prob=0.9):
+    """Leiden of every sample of a block-diagonal CSR (samples at row ``offsets[s]:offsets[s+1]``,
+    global column ids) in one run: ``(labels int32 device (n,), modularity per sample (numpy))``;
+    sample s's labels equal ``leiden`` on that sample alone."""
+    import torch
+    lib = load_library()
+    indptr = to_device(indptr, torch.int64)
+    indices = to_device(indices, torch.int32)
+    data = to_device(data, torch.float32)
+    off = np.ascontiguousarray(offsets, dtype=np.int64)
+    S = off.size - 1
+    n = int(off[-1])
+    nnz = int(indptr[n].item())           # indices / data may be longer (capacity-sized)
+    nbytes = c_sz(0)
+    check(lib.msc_leiden_batched_workspace_size(n, nnz, S, ctypes.byref(nbytes)))
+    ws = workspace(nbytes.value)
+    labels = torch.empty(n, dtype=torch.int32, device=indptr.device)
+    q = np.zeros(S, dtype=np.float64)
+    check(lib.msc_leiden_batched(ptr(indptr), ptr(indices), ptr(data), n, nnz, off.ctypes.data, S,
+                                 float(resolution), int(random_state), int(n_iterations), float(commit_prob),
+                                 ptr(labels), q.ctypes.data, ptr(ws), ws.numel(), stream_ptr()))
+    return labels, q
 
 
 def louvain(indptr, indices, data, n, resolution=1.0, random_state=0, max_levels=20, commit_prob=0.9):

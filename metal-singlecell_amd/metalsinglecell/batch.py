@@ -8,9 +8,11 @@ Samples of uneven cell counts are concatenated (rows of sample s at ``offsets[s]
   (``msc_fuzzy_connectivities_batched``; block s of the block-diagonal result equals the
   single-sample result).  Both enqueue kernels only, so the stage is captured once into a
   HIP graph (``torch.cuda.CUDAGraph``) and later batches of the same shape replay it.
-* Clustering stage: Leiden per sample (its modularity is normalised by the sample's own
-  total weight, and its level/pass loops branch on device results, so it is not captured),
-  several samples at a time, each on its own stream (default priority).
+* Clustering stage: the Leiden of every sample in one run over the block-diagonal graph
+  (``msc_leiden_batched``): one pass loop in lockstep for all samples, each sample with its own
+  fixed-point scale, total weight and coin stream, frozen when its own multilevel loop ends,
+  so that its labels equal a single-sample run.  (``method="per_sample"`` instead calls
+  ``msc_leiden`` per sample from several host threads, one stream each.)
 
 Results are identical to per-sample calls of the single-sample entry points, whatever the
 split count the autotuner picks, and identical on every graph replay.
@@ -97,10 +99,17 @@ class BatchGraph:
         return self.idx, self.dist, (self.indptr, self.indices, self.data)
 
 
-def leiden_samples(bg, resolution=1.0, random_state=0, n_iterations=2, workers=8):
-    """Leiden of every sample of a BatchGraph's last result: list of int32 label tensors."""
+def leiden_samples(bg, resolution=1.0, random_state=0, n_iterations=2, workers=8, method="joint"):
+    """Leiden of every sample of a BatchGraph's last result: ``(labels, modularities)``, a list of
+    int32 label tensors (dense per sample) and the per-sample modularity."""
     import torch
     lo_hi = [(int(bg.offsets[s]), int(bg.offsets[s + 1])) for s in range(bg.sizes.size)]
+    if method == "joint":
+        lab, q = _hip.leiden_batched(bg.indptr, bg.indices, bg.data, bg.offsets, resolution=resolution,
+                                     random_state=random_state, n_iterations=n_iterations)
+        return [lab[a:b] for a, b in lo_hi], [float(x) for x in q]
+    if method != "per_sample":
+        raise ValueError(f"method must be 'joint' or 'per_sample', got {method!r}")
     ip_host = bg.indptr.cpu().numpy()      # one transfer: row starts of every sample
     dev = bg.indptr.device
 
@@ -134,7 +143,7 @@ def leiden_samples(bg, resolution=1.0, random_state=0, n_iterations=2, workers=8
 
 
 def neighbors_leiden_batched(samples, k=15, precision="bf16", resolution=1.0, random_state=0, n_iterations=2,
-                             workers=8, splits=None, autotune_splits=True):
+                             workers=8, splits=None, autotune_splits=True, method="joint"):
     """kNN + connectivities + Leiden for a list of (n_s x d) float32 samples.
 
     Returns ``(labels, modularities, graph)``: per-sample int32 label tensors and Q values,
@@ -144,5 +153,5 @@ def neighbors_leiden_batched(samples, k=15, precision="bf16", resolution=1.0, ra
     bg = BatchGraph(sizes, X.shape[1], k=k, precision=precision, splits=splits,
                     tune_on=X if autotune_splits else None)
     bg.run(X)
-    labels, qs = leiden_samples(bg, resolution, random_state, n_iterations, workers)
+    labels, qs = leiden_samples(bg, resolution, random_state, n_iterations, workers, method=method)
     return labels, qs, bg

@@ -171,15 +171,18 @@ def stage_batched_c4(reps):
     X = np.concatenate(samples)
     bg = BatchGraph(sizes, 30, k=15, precision="bf16", splits=1)
     graph_ms, _ = _events_ms(lambda: bg.run(X), reps)
-    leiden_samples(bg)
-    torch.cuda.synchronize()
-    t = time.perf_counter()
-    for _ in range(reps):
-        leiden_samples(bg)
-    torch.cuda.synchronize()
-    leiden_ms = (time.perf_counter() - t) * 1e3 / reps
-    return {"graph_stage_ms": graph_ms, "leiden_ms": leiden_ms, "cells": int(X.shape[0]),
-            "cells_per_s": X.shape[0] / ((graph_ms + leiden_ms) * 1e-3)}
+    out = {}
+    for method in ("joint", "per_sample"):
+        leiden_samples(bg, method=method)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(reps):
+            leiden_samples(bg, method=method)
+        torch.cuda.synchronize()
+        out[method] = (time.perf_counter() - t) * 1e3 / reps
+    leiden_ms = out["joint"]
+    return {"graph_stage_ms": graph_ms, "leiden_ms": leiden_ms, "leiden_per_sample_calls_ms": out["per_sample"],
+            "cells": int(X.shape[0]), "cells_per_s": X.shape[0] / ((graph_ms + leiden_ms) * 1e-3)}
 
 
 STAGES = {"select-c3": stage_select_c3, "conn-c3": stage_conn_c3, "leiden-c3": stage_leiden_c3,
@@ -190,9 +193,17 @@ def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("stage", choices=sorted(STAGES))
     ap.add_argument("--reps", type=int, default=2)
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="first run one plain device copy of this many MiB (counter calibration)")
     a = ap.parse_args(argv)
     import torch
     torch.cuda.set_device(0)
+    if a.copy_probe_mib > 0:
+        src = torch.ones(a.copy_probe_mib << 18, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     print(f"[stage] {a.stage}", file=sys.stderr, flush=True)
     res = STAGES[a.stage](a.reps)
     print(json.dumps({"stage": a.stage, **res}), flush=True)
