@@ -186,7 +186,7 @@ def end_to_end(X, k):
                         "download, natsorted Categorical in obs['leiden']"}
 
 
-def batched(reps=3, workers=8):
+def batched(reps=3, workers=8, per_sample=True):
     """256 samples, 500-60,000 cells log-uniform, d = 30, k = 15: the kNN + connectivities stage
     as one HIP-graph replay (split count from the autotuner), then per-sample Leiden."""
     import numpy as np
@@ -213,11 +213,13 @@ def batched(reps=3, workers=8):
         t2 = time.perf_counter()
         g_ms.append(1e3 * (t1 - t0))
         l_ms.append(1e3 * (t2 - t1))
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    leiden_samples(bg, workers=workers, method="per_sample")   # the per-sample form, for comparison
-    torch.cuda.synchronize()
-    per_sample_leiden_ms = 1e3 * (time.perf_counter() - t0)
+    per_sample_leiden_ms = None
+    if per_sample:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        leiden_samples(bg, workers=workers, method="per_sample")   # the per-sample form, for comparison
+        torch.cuda.synchronize()
+        per_sample_leiden_ms = 1e3 * (time.perf_counter() - t0)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for x in S:                                   # the same stage as per-sample calls
@@ -483,6 +485,8 @@ def main():
     ap.add_argument("--no-precision", action="store_true")
     ap.add_argument("--no-e2e", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-per-sample", action="store_true",
+                    help="skip the per-sample (8 host threads) Leiden comparison of the batched section")
     ap.add_argument("--no-extensions", action="store_true")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="first run one plain device copy of this many MiB (counter calibration)")
@@ -561,7 +565,7 @@ def main():
     batch = None
     if not a.no_batched:
         t = time.perf_counter()
-        batch = batched()
+        batch = batched(per_sample=not a.no_per_sample)
         log(f"batched in {time.perf_counter() - t:.1f}s", batch)
 
     ext = None
