@@ -229,7 +229,7 @@ def batched(reps=3, workers=8, per_sample=True):
     per_sample_ms = 1e3 * (time.perf_counter() - t0)
     total = float(np.median(g_ms) + np.median(l_ms))
     return {"samples": len(S), "cells": int(X.shape[0]), "dims": int(X.shape[1]), "k": 15,
-            "splits": bg.splits, "setup_s": setup_s, "graph_stage_ms": float(np.median(g_ms)),
+            "splits": bg.splits, "list_len": bg.list_len or 16, "setup_s": setup_s, "graph_stage_ms": float(np.median(g_ms)),
             "graph_stage_per_sample_calls_ms": per_sample_ms, "leiden_ms": float(np.median(l_ms)),
             "leiden_method": "one block-diagonal run of all samples (msc_leiden_batched)",
             "leiden_per_sample_calls_ms": per_sample_leiden_ms, "leiden_workers_per_sample": workers,

@@ -73,24 +73,27 @@ int msc_knn_exact(const float* X, int64_t n, int d, int k, int precision, int32_
  * only its own sample).  X: total x d rows, samples contiguous; offsets (host): n_samples+1
  * row offsets.  out_idx / out_dist: total x k, column ids GLOBAL (offset of the sample
  * included), so the result is the block-diagonal kNN of the concatenation.  Every sample
- * needs >= k cells; d <= 125.  splits (1..16, 0 = 1): each sample's candidates are scanned
- * in that many ranges by separate blocks (a tile-shape knob; results do not depend on it).
+ * needs >= k cells; d <= 125.  The tile shape has two knobs, neither of which changes the
+ * results: splits (1..16, 0 = 1): each sample's candidates are scanned in that many ranges
+ * by separate blocks; list_len (0 = 16, 16 or 8; 8 needs k <= 32): keys each lane keeps per
+ * query (64 or 32 admitted candidates per query; a shorter list costs less upkeep on short
+ * candidate streams and may send more rows to the exact fallback).
  * Replaces per-sample calls of neighbors._knn (src/metalsinglecell/neighbors.py:698-724),
  * as bbknn's per-batch loop does (neighbors.py:491-559).
  *   _prepare uploads the layout tables into the workspace (synchronises);
  *   _launch enqueues the kernels only: no allocation, no host synchronisation, so it may be
  *   captured in a HIP graph; it needs a prior _prepare with the same offsets, d, k,
- *   precision, splits and workspace;
+ *   precision, splits, list_len and workspace;
  *   msc_knn_batched = _prepare + _launch (+ the fallback count in info). */
 int msc_knn_batched_workspace_size(const int64_t* offsets /*host*/, int n_samples, int d, int k,
-                                   int precision, int splits, size_t* bytes /*host*/);
+                                   int precision, int splits, int list_len, size_t* bytes /*host*/);
 int msc_knn_batched_prepare(const int64_t* offsets /*host*/, int n_samples, int d, int k, int precision,
-                            int splits, void* workspace, size_t workspace_bytes, void* stream);
+                            int splits, int list_len, void* workspace, size_t workspace_bytes, void* stream);
 int msc_knn_batched_launch(const float* X, const int64_t* offsets /*host*/, int n_samples, int d, int k,
-                           int precision, int splits, int32_t* out_idx, float* out_dist, void* workspace,
-                           size_t workspace_bytes, void* stream);
+                           int precision, int splits, int list_len, int32_t* out_idx, float* out_dist,
+                           void* workspace, size_t workspace_bytes, void* stream);
 int msc_knn_batched(const float* X, const int64_t* offsets /*host*/, int n_samples, int d, int k,
-                    int precision, int splits, int32_t* out_idx, float* out_dist, void* workspace,
+                    int precision, int splits, int list_len, int32_t* out_idx, float* out_dist, void* workspace,
                     size_t workspace_bytes, void* stream, msc_knn_info* info /*host, may be NULL*/);
 
 /* Batch-balanced kNN: for every cell, its min(k, n_t) exact nearest cells within each batch

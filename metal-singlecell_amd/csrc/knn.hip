@@ -1448,7 +1448,10 @@ BatchBufs carve_batched(msc::Carve& cv, const BatchLayout& L, int n_samples, int
   return t;
 }
 
-int batched_common(const int64_t* offsets, int n_samples, int d, int k, int precision, int splits, BatchLayout* L) {
+int batched_common(const int64_t* offsets, int n_samples, int d, int k, int precision, int splits, int list_len,
+                   BatchLayout* L) {
+  MSC_REQUIRE(list_len == 0 || list_len == kL || (list_len == kLShort && k <= 4 * kLShort),
+              "knn_batched: list_len must be 0, %d or %d (the latter for k <= %d)", kL, kLShort, 4 * kLShort);
   MSC_REQUIRE(offsets && n_samples >= 1, "knn_batched: bad arguments");
   MSC_REQUIRE(precision == MSC_PRECISION_FP32 || precision == MSC_PRECISION_BF16_RERANK, "knn_batched: precision");
   MSC_REQUIRE(k >= 1 && k <= 64 && d >= 1 && d <= 125, "knn_batched: bad d/k");
@@ -1457,10 +1460,10 @@ int batched_common(const int64_t* offsets, int n_samples, int d, int k, int prec
 }  // namespace
 
 extern "C" int msc_knn_batched_workspace_size(const int64_t* offsets, int n_samples, int d, int k, int precision,
-                                              int splits, size_t* bytes) {
+                                              int splits, int list_len, size_t* bytes) {
   MSC_REQUIRE(bytes, "knn_batched: bytes is null");
   BatchLayout L;
-  MSC_TRY(batched_common(offsets, n_samples, d, k, precision, splits, &L));
+  MSC_TRY(batched_common(offsets, n_samples, d, k, precision, splits, list_len, &L));
   Carve cv(nullptr, 0);
   carve_batched(cv, L, n_samples, d, k, precision);
   *bytes = cv.used + 256;
@@ -1468,9 +1471,10 @@ extern "C" int msc_knn_batched_workspace_size(const int64_t* offsets, int n_samp
 }
 
 extern "C" int msc_knn_batched_prepare(const int64_t* offsets, int n_samples, int d, int k, int precision, int splits,
+                                       int list_len,
                                        void* workspace, size_t workspace_bytes, void* stream) {
   BatchLayout L;
-  MSC_TRY(batched_common(offsets, n_samples, d, k, precision, splits, &L));
+  MSC_TRY(batched_common(offsets, n_samples, d, k, precision, splits, list_len, &L));
   MSC_REQUIRE(workspace, "knn_batched: null workspace");
   hipStream_t st = static_cast<hipStream_t>(stream);
   Carve cv(workspace, workspace_bytes);
@@ -1490,10 +1494,11 @@ extern "C" int msc_knn_batched_prepare(const int64_t* offsets, int n_samples, in
 }
 
 extern "C" int msc_knn_batched_launch(const float* X, const int64_t* offsets, int n_samples, int d, int k,
-                                      int precision, int splits, int32_t* out_idx, float* out_dist, void* workspace,
+                                      int precision, int splits, int list_len, int32_t* out_idx, float* out_dist,
+                                      void* workspace,
                                       size_t workspace_bytes, void* stream) {
   BatchLayout L;
-  MSC_TRY(batched_common(offsets, n_samples, d, k, precision, splits, &L));
+  MSC_TRY(batched_common(offsets, n_samples, d, k, precision, splits, list_len, &L));
   MSC_REQUIRE(X && out_idx && out_dist && workspace, "knn_batched: null pointer argument");
   hipStream_t st = static_cast<hipStream_t>(stream);
   Carve cv(workspace, workspace_bytes);
@@ -1510,7 +1515,8 @@ extern "C" int msc_knn_batched_launch(const float* X, const int64_t* offsets, in
   const bool bf16 = precision == MSC_PRECISION_BF16_RERANK;
   Operands op = prepare(p, b, t.Xpad, n, bf16, st);
   MSC_LAUNCH_CHECK();
-  MSC_TRY(launch_select(bf16, false, op.ks, op.X, n, n, d, n, L.S, nullptr, t.blk_range, b.keys, b.taus, st));
+  MSC_TRY(launch_select(bf16, false, op.ks, op.X, n, n, d, n, L.S, nullptr, t.blk_range, b.keys, b.taus, st,
+                        list_len == kLShort));
   const unsigned wb = static_cast<unsigned>(ceil_div(n, kThreads / kWave));
   finalize_kernel<<<wb, kThreads, 0, st>>>(t.Xpad, n, d, k, L.S, n, nullptr, b.keys, b.taus, op.qn2, op.qres,
                                            op.gstat, op.bnd, t.row_range, t.idx_pad, t.dist_pad, b.flags);
@@ -1523,14 +1529,16 @@ extern "C" int msc_knn_batched_launch(const float* X, const int64_t* offsets, in
 }
 
 extern "C" int msc_knn_batched(const float* X, const int64_t* offsets, int n_samples, int d, int k, int precision,
-                               int splits, int32_t* out_idx, float* out_dist, void* workspace, size_t workspace_bytes,
+                               int splits, int list_len, int32_t* out_idx, float* out_dist, void* workspace,
+                               size_t workspace_bytes,
                                void* stream, msc_knn_info* info) {
-  MSC_TRY(msc_knn_batched_prepare(offsets, n_samples, d, k, precision, splits, workspace, workspace_bytes, stream));
-  MSC_TRY(msc_knn_batched_launch(X, offsets, n_samples, d, k, precision, splits, out_idx, out_dist, workspace,
+  MSC_TRY(msc_knn_batched_prepare(offsets, n_samples, d, k, precision, splits, list_len, workspace, workspace_bytes,
+                                  stream));
+  MSC_TRY(msc_knn_batched_launch(X, offsets, n_samples, d, k, precision, splits, list_len, out_idx, out_dist, workspace,
                                  workspace_bytes, stream));
   if (info) {
     BatchLayout L;
-    MSC_TRY(batched_common(offsets, n_samples, d, k, precision, splits, &L));
+    MSC_TRY(batched_common(offsets, n_samples, d, k, precision, splits, list_len, &L));
     Carve cv(workspace, workspace_bytes);
     BatchBufs t = carve_batched(cv, L, n_samples, d, k, precision);
     int64_t m = 0;
@@ -1538,7 +1546,7 @@ extern "C" int msc_knn_batched(const float* X, const int64_t* offsets, int n_sam
     info->n_fallback = m;
     info->n_second_pass = 0;
     info->n_splits = L.S;
-    info->list_len = 4 * kL;
+    info->list_len = 4 * (list_len == kLShort ? kLShort : kL);
   }
   return kOk;
 }

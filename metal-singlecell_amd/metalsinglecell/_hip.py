@@ -57,12 +57,12 @@ SIGNATURES = {
     "msc_knn_workspace_size": (c_int, [c_i64, c_int, c_int, c_int, ctypes.POINTER(c_sz)]),
     "msc_knn_exact": (c_int, [c_vp, c_i64, c_int, c_int, c_int, c_vp, c_vp, c_vp, c_sz, c_vp,
                               ctypes.POINTER(KnnInfo)]),
-    "msc_knn_batched_workspace_size": (c_int, [c_vp, c_int, c_int, c_int, c_int, c_int,
+    "msc_knn_batched_workspace_size": (c_int, [c_vp, c_int, c_int, c_int, c_int, c_int, c_int,
                                                ctypes.POINTER(c_sz)]),
-    "msc_knn_batched_prepare": (c_int, [c_vp, c_int, c_int, c_int, c_int, c_int, c_vp, c_sz, c_vp]),
-    "msc_knn_batched_launch": (c_int, [c_vp, c_vp, c_int, c_int, c_int, c_int, c_int, c_vp, c_vp, c_vp,
+    "msc_knn_batched_prepare": (c_int, [c_vp, c_int, c_int, c_int, c_int, c_int, c_int, c_vp, c_sz, c_vp]),
+    "msc_knn_batched_launch": (c_int, [c_vp, c_vp, c_int, c_int, c_int, c_int, c_int, c_int, c_vp, c_vp, c_vp,
                                        c_sz, c_vp]),
-    "msc_knn_batched": (c_int, [c_vp, c_vp, c_int, c_int, c_int, c_int, c_int, c_vp, c_vp, c_vp, c_sz,
+    "msc_knn_batched": (c_int, [c_vp, c_vp, c_int, c_int, c_int, c_int, c_int, c_int, c_vp, c_vp, c_vp, c_sz,
                                 c_vp, ctypes.POINTER(KnnInfo)]),
     "msc_knn_distances_csr": (c_int, [c_vp, c_vp, c_i64, c_int, c_vp, c_vp, c_vp, c_vp]),
     "msc_bbknn_workspace_size": (c_int, [c_vp, c_i64, c_int, c_int, c_int, c_int, ctypes.POINTER(c_sz)]),
@@ -196,17 +196,20 @@ def knn_exact(X, k, precision="fp32", timing=False, return_info=False):
     return idx, dist
 
 
-def knn_batched_workspace_bytes(offsets, d, k, precision="bf16", splits=1):
+def knn_batched_workspace_bytes(offsets, d, k, precision="bf16", splits=1, list_len=0):
     lib = load_library()
     off = np.ascontiguousarray(offsets, dtype=np.int64)
     nbytes = c_sz(0)
     check(lib.msc_knn_batched_workspace_size(off.ctypes.data, off.size - 1, int(d), int(k), PRECISION[precision],
-                                             int(splits), ctypes.byref(nbytes)))
+                                             int(splits), int(list_len), ctypes.byref(nbytes)))
     return nbytes.value
 
 
-def knn_batched(X, offsets, k, precision="bf16", splits=1, return_info=False):
-    """Block-diagonal exact kNN of concatenated samples: ``(idx int32 global, dist float32)``."""
+def knn_batched(X, offsets, k, precision="bf16", splits=1, list_len=0, return_info=False):
+    """Block-diagonal exact kNN of concatenated samples: ``(idx int32 global, dist float32)``.
+
+    ``list_len`` is the select tile's per-lane list length: 0 (default, 16), 16, or 8 (k <= 32
+    only: half the list registers, more waves per CU).  Results do not depend on it."""
     import torch
     lib = load_library()
     Xd = to_device(X, torch.float32)
@@ -214,13 +217,13 @@ def knn_batched(X, offsets, k, precision="bf16", splits=1, return_info=False):
     S = off.size - 1
     d = Xd.shape[1]
     p = PRECISION[precision]
-    ws = workspace(knn_batched_workspace_bytes(off, d, k, precision, splits))
+    ws = workspace(knn_batched_workspace_bytes(off, d, k, precision, splits, list_len))
     n = Xd.shape[0]
     idx = torch.empty((n, k), dtype=torch.int32, device=Xd.device)
     dist = torch.empty((n, k), dtype=torch.float32, device=Xd.device)
     info = KnnInfo()
-    check(lib.msc_knn_batched(ptr(Xd), off.ctypes.data, S, d, int(k), p, int(splits), ptr(idx), ptr(dist),
-                              ptr(ws), ws.numel(), stream_ptr(), ctypes.byref(info)))
+    check(lib.msc_knn_batched(ptr(Xd), off.ctypes.data, S, d, int(k), p, int(splits), int(list_len), ptr(idx),
+                              ptr(dist), ptr(ws), ws.numel(), stream_ptr(), ctypes.byref(info)))
     if return_info:
         return idx, dist, {f: getattr(info, f) for f, _ in KnnInfo._fields_}
     return idx, dist

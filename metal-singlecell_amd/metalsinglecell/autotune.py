@@ -1,15 +1,23 @@
 """Tile-shape autotuner for the batched neighbour-graph launch, with an on-disk cache.
 
-The tunable is the number of candidate splits of the batched select launch
-(``msc_knn_batched_*(..., splits, ...)``): with one split every query block of a sample
-scans the whole sample, so the largest samples form a long tail; more splits give more,
-shorter blocks at the cost of merging more partial lists.  Results never depend on the
-choice (the exact re-rank and the certificate fix them), only the time does.
+The tile shape of the batched select launch (``msc_knn_batched_*(..., splits, list_len, ...)``)
+has two knobs:
+
+* ``splits``: with one split every query block of a sample scans the whole sample, so the
+  largest samples form a long tail; more splits give more, shorter blocks at the cost of
+  merging more partial lists;
+* ``list_len``: keys each lane keeps per query, 16 (64 admitted candidates per query) or 8
+  (32, for k <= 32): the short list halves the list registers and the upkeep of each
+  admission, so more waves fit per CU, but its certificate fails more often and more rows
+  take the exact fallback.
+
+Results never depend on the choice (the exact re-rank and the certificate fix them), only
+the time does.
 
 Cache: a JSON object mapping keys ``"knn_batched|<precision>|d<d>|k<k>|max<2^a>|tot<2^b>"``
 (power-of-two buckets of the largest sample and of the total cell count) to
-``{"splits": s, "ms": {candidate: median ms}}``.  Location: ``$MSC_AUTOTUNE_CACHE`` or
-``~/.cache/metalsinglecell/autotune.json``.
+``{"splits": s, "list_len": L, "ms": {"<s>x<L>": median ms}}``.  Location:
+``$MSC_AUTOTUNE_CACHE`` or ``~/.cache/metalsinglecell/autotune.json``.
 """
 
 from __future__ import annotations
@@ -20,7 +28,8 @@ import threading
 
 import numpy as np
 
-CANDIDATES = (1, 2, 4, 8)
+CANDIDATES = (1, 2, 4, 8)          # split counts
+LIST_LENS = (16, 8)                 # per-lane list lengths (8 only for k <= 32)
 _lock = threading.Lock()
 
 
@@ -60,13 +69,21 @@ def store(key, entry):
 
 
 def lookup(offsets, d, k, precision):
+    """The cached ``(splits, list_len)`` for this shape bucket, or None."""
     entry = load().get(key_for(offsets, d, k, precision))
-    return None if entry is None else int(entry["splits"])
+    if entry is None:
+        return None
+    return int(entry["splits"]), int(entry.get("list_len", 0))
 
 
-def tune_batched_splits(X, offsets, k, precision="bf16", candidates=CANDIDATES, reps=3):
-    """Time the batched select launch for each candidate split count on ``X`` (device rows of
-    the concatenated samples), cache and return the fastest."""
+def candidates_for(k, splits=CANDIDATES, list_lens=LIST_LENS):
+    return [(int(s), int(L)) for L in list_lens if L == 16 or k <= 4 * L for s in splits]
+
+
+def tune_batched_tile(X, offsets, k, precision="bf16", candidates=None, reps=3):
+    """Time the batched select launch for each candidate tile shape ``(splits, list_len)`` on
+    ``X`` (device rows of the concatenated samples), cache and return the fastest as
+    ``((splits, list_len), {"<s>x<L>": ms})``."""
     import torch
     from . import _hip
     lib = _hip.load_library()
@@ -77,22 +94,23 @@ def tune_batched_splits(X, offsets, k, precision="bf16", candidates=CANDIDATES, 
     idx = torch.empty((n, k), dtype=torch.int32, device=X.device)
     dist = torch.empty((n, k), dtype=torch.float32, device=X.device)
     timings = {}
-    for s in candidates:
-        ws = _hip.workspace(_hip.knn_batched_workspace_bytes(off, d, k, precision, s))
-        _hip.check(lib.msc_knn_batched_prepare(off.ctypes.data, S, d, int(k), p, int(s), _hip.ptr(ws), ws.numel(),
-                                               _hip.stream_ptr()))
+    for s, L in (candidates if candidates is not None else candidates_for(k)):
+        ws = _hip.workspace(_hip.knn_batched_workspace_bytes(off, d, k, precision, s, L))
+        _hip.check(lib.msc_knn_batched_prepare(off.ctypes.data, S, d, int(k), p, int(s), int(L), _hip.ptr(ws),
+                                               ws.numel(), _hip.stream_ptr()))
         ms = []
         for _ in range(reps + 1):
             e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             e0.record()
-            _hip.check(lib.msc_knn_batched_launch(_hip.ptr(X), off.ctypes.data, S, d, int(k), p, int(s),
+            _hip.check(lib.msc_knn_batched_launch(_hip.ptr(X), off.ctypes.data, S, d, int(k), p, int(s), int(L),
                                                   _hip.ptr(idx), _hip.ptr(dist), _hip.ptr(ws), ws.numel(),
                                                   _hip.stream_ptr()))
             e1.record()
             e1.synchronize()
             ms.append(e0.elapsed_time(e1))
-        timings[s] = float(np.median(ms[1:]))
+        timings[(s, L)] = float(np.median(ms[1:]))
         del ws
     best = min(timings, key=timings.get)
-    store(key_for(off, d, k, precision), {"splits": int(best), "ms": {str(c): t for c, t in timings.items()}})
-    return best, timings
+    table = {f"{s}x{L}": t for (s, L), t in timings.items()}
+    store(key_for(off, d, k, precision), {"splits": int(best[0]), "list_len": int(best[1]), "ms": table})
+    return best, table

@@ -15,7 +15,7 @@ Samples of uneven cell counts are concatenated (rows of sample s at ``offsets[s]
   ``msc_leiden`` per sample from several host threads, one stream each.)
 
 Results are identical to per-sample calls of the single-sample entry points, whatever the
-split count the autotuner picks, and identical on every graph replay.
+tile shape (split count, list length) the autotuner picks, and identical on every graph replay.
 """
 
 from __future__ import annotations
@@ -30,7 +30,7 @@ from . import _hip, autotune
 class BatchGraph:
     """Captured kNN + connectivities stage for a fixed set of sample sizes."""
 
-    def __init__(self, sizes, d, k=15, precision="bf16", splits=None, capture=True, tune_on=None):
+    def __init__(self, sizes, d, k=15, precision="bf16", splits=None, capture=True, tune_on=None, list_len=None):
         import torch
         lib = _hip.load_library()
         self.sizes = np.asarray(sizes, dtype=np.int64)
@@ -42,13 +42,20 @@ class BatchGraph:
         dev = _hip.device()
         self.X = torch.empty((self.n, self.d), dtype=torch.float32, device=dev)   # static input
         if splits is None:
-            splits = autotune.lookup(self.offsets, d, k, precision)
-        if splits is None and tune_on is not None:
-            self.X.copy_(_hip.to_device(tune_on, torch.float32))
-            splits, _ = autotune.tune_batched_splits(self.X, self.offsets, k, precision)
+            # tile shape: the cached (splits, list_len) of this shape bucket, else tuned on
+            # `tune_on`, else one split with the default list
+            shape = autotune.lookup(self.offsets, d, k, precision)
+            if shape is None and tune_on is not None:
+                self.X.copy_(_hip.to_device(tune_on, torch.float32))
+                shape, _ = autotune.tune_batched_tile(self.X, self.offsets, k, precision)
+            if shape is not None:
+                splits, tuned_len = shape
+                list_len = tuned_len if list_len is None else list_len
         self.splits = int(splits or 1)
+        self.list_len = int(list_len or 0)
         S = self.sizes.size
-        self.ws_knn = _hip.workspace(_hip.knn_batched_workspace_bytes(self.offsets, d, k, precision, self.splits))
+        self.ws_knn = _hip.workspace(_hip.knn_batched_workspace_bytes(self.offsets, d, k, precision, self.splits,
+                                                                      self.list_len))
         self.ws_fz = _hip.workspace(_hip.fuzzy_batched_workspace_bytes(self.n, k, S))
         self.goff = torch.from_numpy(self.offsets).to(dev)
         self.idx = torch.empty((self.n, k), dtype=torch.int32, device=dev)
@@ -61,7 +68,8 @@ class BatchGraph:
         self.rho = torch.empty(self.n, dtype=torch.float32, device=dev)
         self._cap = cap
         _hip.check(lib.msc_knn_batched_prepare(self.offsets.ctypes.data, S, self.d, self.k,
-                                               _hip.PRECISION[precision], self.splits, _hip.ptr(self.ws_knn),
+                                               _hip.PRECISION[precision], self.splits, self.list_len,
+                                               _hip.ptr(self.ws_knn),
                                                self.ws_knn.numel(), _hip.stream_ptr()))
         self.graph = None
         if capture:
@@ -78,7 +86,8 @@ class BatchGraph:
         st = _hip.stream_ptr()
         S = self.sizes.size
         _hip.check(lib.msc_knn_batched_launch(_hip.ptr(self.X), self.offsets.ctypes.data, S, self.d, self.k,
-                                              _hip.PRECISION[self.precision], self.splits, _hip.ptr(self.idx),
+                                              _hip.PRECISION[self.precision], self.splits, self.list_len,
+                                              _hip.ptr(self.idx),
                                               _hip.ptr(self.dist), _hip.ptr(self.ws_knn), self.ws_knn.numel(), st))
         _hip.check(lib.msc_fuzzy_connectivities_batched(
             _hip.ptr(self.idx), _hip.ptr(self.dist), self.n, self.k, _hip.ptr(self.goff), S, _hip.ptr(self.indptr),

@@ -17,7 +17,9 @@ Stages (sizes are the workloads' real sizes):
   moran-2m    Moran's I of 2M cells x 500 genes, kNN(6) weights, 100 device permutations;
               bytes = 4 n G (P + 3) + 8 nnz(W) + 4 n P
   batched-c4  256 samples of 500-60,000 cells x 30 (seed 3), k = 15: the captured kNN +
-              connectivities graph stage and the per-sample Leiden of every sample
+              connectivities graph stage at the autotuned tile shape (every candidate's time
+              reported), the joint Leiden of every sample, and (unless --no-per-sample) the
+              per-sample Leiden calls for comparison
 """
 
 from __future__ import annotations
@@ -160,19 +162,25 @@ def stage_moran_2m(reps):
     return {"ms": ms, "bytes": b, "gbps": b / ms / 1e6, "frac": b / (ms * 1e-3) / HBM_PEAK}
 
 
-def stage_batched_c4(reps):
+def stage_batched_c4(reps, per_sample=True):
     import time
 
     import torch
-    from . import synthetic
+    from . import autotune, synthetic
     from .batch import BatchGraph, leiden_samples
     samples = [x for x, _ in synthetic.batched_samples(n_samples=256, lo=500, hi=60_000, d=30, seed=3)]
     sizes = [x.shape[0] for x in samples]
     X = np.concatenate(samples)
-    bg = BatchGraph(sizes, 30, k=15, precision="bf16", splits=1)
+    offsets = np.concatenate([[0], np.cumsum(sizes)])
+    shape = autotune.lookup(offsets, 30, 15, "bf16")
+    Xd = torch.from_numpy(X).to(torch.device("cuda"))
+    if shape is None:
+        shape, _ = autotune.tune_batched_tile(Xd, offsets, 15, "bf16")
+    tile_ms = autotune.load()[autotune.key_for(offsets, 30, 15, "bf16")]["ms"]
+    bg = BatchGraph(sizes, 30, k=15, precision="bf16", splits=shape[0], list_len=shape[1])
     graph_ms, _ = _events_ms(lambda: bg.run(X), reps)
-    out = {}
-    for method in ("joint", "per_sample"):
+    out = {"per_sample": None}
+    for method in (("joint", "per_sample") if per_sample else ("joint",)):
         leiden_samples(bg, method=method)
         torch.cuda.synchronize()
         t = time.perf_counter()
@@ -181,7 +189,8 @@ def stage_batched_c4(reps):
         torch.cuda.synchronize()
         out[method] = (time.perf_counter() - t) * 1e3 / reps
     leiden_ms = out["joint"]
-    return {"graph_stage_ms": graph_ms, "leiden_ms": leiden_ms, "leiden_per_sample_calls_ms": out["per_sample"],
+    return {"splits": bg.splits, "list_len": bg.list_len or 16, "select_ms_by_tile": tile_ms,
+            "graph_stage_ms": graph_ms, "leiden_ms": leiden_ms, "leiden_per_sample_calls_ms": out["per_sample"],
             "cells": int(X.shape[0]), "cells_per_s": X.shape[0] / ((graph_ms + leiden_ms) * 1e-3)}
 
 
@@ -193,6 +202,7 @@ def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("stage", choices=sorted(STAGES))
     ap.add_argument("--reps", type=int, default=2)
+    ap.add_argument("--no-per-sample", action="store_true", help="batched-c4: skip the per-sample Leiden calls")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="first run one plain device copy of this many MiB (counter calibration)")
     a = ap.parse_args(argv)
@@ -205,7 +215,8 @@ def main(argv=None):
         torch.cuda.synchronize()
         del src, dst
     print(f"[stage] {a.stage}", file=sys.stderr, flush=True)
-    res = STAGES[a.stage](a.reps)
+    kw = {"per_sample": not a.no_per_sample} if a.stage == "batched-c4" else {}
+    res = STAGES[a.stage](a.reps, **kw)
     print(json.dumps({"stage": a.stage, **res}), flush=True)
 
 

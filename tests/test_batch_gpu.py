@@ -18,15 +18,16 @@ def _samples(seed=3, sizes=(200, 1000, 128, 3000, 777, 64), d=30):
 
 
 @pytest.mark.parametrize("precision", ["bf16", "fp32"])
-@pytest.mark.parametrize("splits", [1, 4])
-def test_batched_knn_equals_per_sample(precision, splits):
+@pytest.mark.parametrize("splits,list_len", [(1, 0), (4, 0), (1, 8), (4, 8)])
+def test_batched_knn_equals_per_sample(precision, splits, list_len):
     from metalsinglecell import _hip
     S = _samples()
     X = np.concatenate(S)
     off = np.concatenate([[0], np.cumsum([s.shape[0] for s in S])])
-    idx, dist, info = _hip.knn_batched(X, off, 15, precision=precision, splits=splits, return_info=True)
+    idx, dist, info = _hip.knn_batched(X, off, 15, precision=precision, splits=splits, list_len=list_len,
+                                       return_info=True)
     idx, dist = idx.cpu().numpy(), dist.cpu().numpy()
-    assert info["n_splits"] == splits
+    assert info["n_splits"] == splits and info["list_len"] == 4 * (list_len or 16)
     for s, x in enumerate(S):
         ri, rd = _hip.knn_exact(x, 15, precision=precision)
         a, b = off[s], off[s + 1]
@@ -101,7 +102,8 @@ def test_autotuner_picks_and_caches(tmp_path, monkeypatch):
     X = np.concatenate(S)
     sizes = [s.shape[0] for s in S]
     bg = BatchGraph(sizes, 30, k=15, precision="bf16", tune_on=X, capture=False)
-    assert bg.splits in autotune.CANDIDATES
-    assert autotune.lookup(bg.offsets, 30, 15, "bf16") == bg.splits
+    assert (bg.splits, bg.list_len) in autotune.candidates_for(15)
+    assert autotune.lookup(bg.offsets, 30, 15, "bf16") == (bg.splits, bg.list_len)
+    assert len(autotune.load()[autotune.key_for(bg.offsets, 30, 15, "bf16")]["ms"]) == 8
     bg2 = BatchGraph(sizes, 30, k=15, precision="bf16", capture=False)   # from the cache
-    assert bg2.splits == bg.splits
+    assert (bg2.splits, bg2.list_len) == (bg.splits, bg.list_len)

@@ -188,11 +188,17 @@ def test_autotune_cache_roundtrip(tmp_path, monkeypatch):
     off = np.array([0, 500, 1700, 60_000])
     assert autotune.key_for(off, 30, 15, "bf16") == "knn_batched|bf16|d30|k15|max65536|tot65536"
     assert autotune.lookup(off, 30, 15, "bf16") is None
-    autotune.store(autotune.key_for(off, 30, 15, "bf16"), {"splits": 4, "ms": {"1": 2.0, "4": 1.0}})
-    assert autotune.lookup(off, 30, 15, "bf16") == 4
+    autotune.store(autotune.key_for(off, 30, 15, "bf16"),
+                   {"splits": 4, "list_len": 8, "ms": {"1x16": 2.0, "4x8": 1.0}})
+    assert autotune.lookup(off, 30, 15, "bf16") == (4, 8)
     assert autotune.lookup(off, 30, 15, "fp32") is None
     off2 = np.array([0, 500, 1700, 58_000])          # same buckets -> same entry
-    assert autotune.lookup(off2, 30, 15, "bf16") == 4
+    assert autotune.lookup(off2, 30, 15, "bf16") == (4, 8)
+    autotune.store(autotune.key_for(off, 30, 40, "bf16"), {"splits": 2, "ms": {"2": 1.0}})
+    assert autotune.lookup(off, 30, 40, "bf16") == (2, 0)      # entries without a list length
+    # the short list is a candidate only where it holds k (4 lanes x 8 keys)
+    assert autotune.candidates_for(32) == [(s, L) for L in (16, 8) for s in (1, 2, 4, 8)]
+    assert autotune.candidates_for(33) == [(s, 16) for s in (1, 2, 4, 8)]
 
 
 def test_choose_representation_follows_the_reference_rule():
