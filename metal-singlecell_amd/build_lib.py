@@ -7,6 +7,7 @@ Each ``csrc/*.hip`` file is compiled to an object with hipcc, then linked into
 * ``-ffp-contract=off``: no implicit fused multiply-add; the exact-distance and fixed-point
   arithmetic of the path is defined with separately rounded operations.
 * ``-Xclang -target-feature -Xclang -packed-fp32-ops``: no packed FP32 instructions.
+* ``knn.hip`` only: ``-mno-amdgpu-ieee -fno-honor-nans`` (see ``FILE_FLAGS``).
 """
 
 from __future__ import annotations
@@ -32,9 +33,14 @@ FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950", "-ffp-contract=o
          "-Wno-unused-result", f"-I{os.path.join(ROOT, 'include')}", f"-I{CSRC}"]
 
 
+# knn.hip: min/max without NaN canonicalisation (the select's gate takes the minimum of 16
+# MFMA results per M-block; distances are never NaN).  Arithmetic is unchanged.
+FILE_FLAGS = {"knn.hip": ["-mno-amdgpu-ieee", "-fno-honor-nans"]}
+
+
 def _compile(src):
     obj = os.path.join(OBJ_DIR, os.path.basename(src) + ".o")
-    cmd = [HIPCC, *FLAGS, "-c", src, "-o", obj]
+    cmd = [HIPCC, *FLAGS, *FILE_FLAGS.get(os.path.basename(src), []), "-c", src, "-o", obj]
     p = subprocess.run(cmd, capture_output=True, text=True)
     err = "\n".join(line for line in p.stderr.splitlines()
                     if "packed-fp32-ops" not in line and "not a recognized feature" not in line)

@@ -574,6 +574,208 @@ __global__ __launch_bounds__(kThreads, 2) void select_kernel(
 }
 
 // ------------------------------------------------------------------------------------------
+// 3b. select, BF16 main pass on v_mfma_f32_32x32x16_bf16
+// ------------------------------------------------------------------------------------------
+// Each wave holds 32 queries as ONE N-block of the 32x32x16 MFMA and streams 64-candidate
+// tiles as two M-blocks of 32.  K runs in steps of 16, so d = 50 (2d + 3 = 103 operand
+// terms) takes 7 steps (K = 112) instead of the 4 x 32 = 128 of the 16x16x32 form.
+//   A (candidates, LDS): lane l reads row (l mod 32) of the M-block, 8 terms at 16 s + 8 (l div 32)
+//   B (queries, registers): lane l holds query (l mod 32), the same 8 terms
+//   C: lane l holds query (l mod 32) against rows 8 (i div 4) + 4 (l div 32) + (i mod 4), i < 16
+// Two lanes share a query; each keeps a sorted register list of its L = 32 smallest keys, so 64
+// candidates are admitted per query, as in the 16x16x32 form, and a query's admission threshold
+// is the smaller of its two lanes' tails (about its 60th candidate, against about the 40th
+// with four lanes of 16: far fewer rows fail the certificate).  Hits go to a per-lane LDS
+// buffer merged into the list when a lane's buffer could overflow.  The M-blocks are
+// software pipelined: the selection of one M-block runs while the next one's MFMAs execute.
+//
+// Tiles are copied global -> LDS by the load unit itself (global_load_lds_dwordx4: a wave's
+// 64 lanes fill 1 KB of LDS), three tile buffers in flight, no staging registers.  A tile is
+// SC = CH | 1 wave loads: row r of the tile sits at r * 16 SC bytes, its 16-byte chunk c at
+// + 16 c; the one pad chunk of a row (SC > CH) loads a copy of the row's last chunk.  The odd
+// row stride in 16-byte groups puts the 32 rows of a half-wave's fragment read on 32 distinct
+// bank groups.
+
+// one 16-byte global -> LDS copy per lane through the load unit (LDS address: the wave-uniform
+// base + 16 x lane); the builtin exists only in device compilation
+__device__ __forceinline__ void lds_load16(const void* src, void* lds_base) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  __builtin_amdgcn_global_load_lds(src, (__attribute__((address_space(3))) void*)lds_base, 16, 0, 0);
+#endif
+}
+
+template <int KT>
+struct Sel32Cfg {
+  static constexpr int CH = 2 * KT;     // 16-byte chunks of a row used (K = 16 KT bf16)
+  static constexpr int SC = CH | 1;     // chunks per LDS row: an odd number of 16 B groups
+  static constexpr int STRIDE = 16 * SC;
+  static constexpr int TB = kTile * STRIDE;  // = 1 KB x SC
+  static constexpr int NI = SC;              // wave loads per tile
+  static constexpr int PERW = (NI + 3) / 4;  // per wave (the last one repeated where short)
+  static constexpr int FIT = (kLdsHalfCU - 3 * TB) / ((kThreads / kWave) * kWave * 8);
+  static constexpr int CAP = FIT < 24 ? FIT : 24;
+  static_assert(CAP >= 12, "hit buffer");
+};
+
+template <int KT>
+__global__ __launch_bounds__(kThreads, 2) void select32_kernel(
+    const void* __restrict__ Xp, int64_t n, int64_t n_slots, int d, int rowb, int64_t split_len,
+    const int64_t* __restrict__ blk_range, uint64_t* __restrict__ keys_out, float* __restrict__ tau_out) {
+  using C = Sel32Cfg<KT>;
+  typedef __attribute__((__vector_size__(16 * sizeof(float)))) float f32x16;
+  constexpr int L = 32;
+  __shared__ __attribute__((aligned(1024))) char tA[C::TB];
+  __shared__ __attribute__((aligned(1024))) char tB[C::TB];
+  __shared__ __attribute__((aligned(1024))) char tC[C::TB];
+  __shared__ uint64_t hitbuf[kThreads / kWave][C::CAP][kWave];
+
+  const int l = lane_id();
+  const int wv = threadIdx.x / kWave;
+  const int h = l >> 5;
+  const int lq = l & 31;
+  const int split = blockIdx.y;
+  const int64_t br = 2 * (static_cast<int64_t>(split) * gridDim.x + blockIdx.x);
+  const int64_t c_begin = blk_range ? blk_range[br] : static_cast<int64_t>(split) * split_len;
+  const int64_t c_end = blk_range ? blk_range[br + 1] : min(n, c_begin + split_len);
+  const int64_t slot = static_cast<int64_t>(blockIdx.x) * kQueriesPerBlock + wv * 32 + lq;
+  const char* Xb = static_cast<const char*>(Xp);
+
+  bf16x8 qb[KT];
+  {
+    const uint16_t* src = reinterpret_cast<const uint16_t*>(Xb + slot * rowb);
+#pragma unroll
+    for (int s = 0; s < KT; ++s) {
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const int k = 16 * s + 8 * h + e;
+        float v;
+        if (k < d) v = -2.0f * bf16_to_f32(src[k]);
+        else if (k < 2 * d) v = -2.0f * bf16_to_f32(src[k - d]);
+        else if (k < 2 * d + 3) v = 1.0f;
+        else v = 0.0f;
+        qb[s][e] = static_cast<__bf16>(v);
+      }
+    }
+  }
+
+  uint64_t lst[L];
+#pragma unroll
+  for (int i = 0; i < L; ++i) lst[i] = kEmpty;
+  float tq = __uint_as_float(0x7F800000u);
+  int cnt = 0;
+  const int64_t ntiles = (c_end > c_begin) ? ceil_div(c_end - c_begin, kTile) : 0;
+
+  // this wave's loads of a tile: wave load u = wv + 4 j (clamped), lane l fills LDS chunk
+  // 64 u + l = (row, chunk) of the tile
+  int lds_off[C::PERW], src_off[C::PERW];
+#pragma unroll
+  for (int j = 0; j < C::PERW; ++j) {
+    const int u = min(wv + 4 * j, C::NI - 1);
+    const int pch = 64 * u + l;
+    const int row = pch / C::SC, col = pch % C::SC;
+    lds_off[j] = 1024 * u;
+    src_off[j] = row * rowb + min(col, C::CH - 1) * 16;
+  }
+  auto load_tile = [&](char* buf, const int64_t T) __attribute__((always_inline)) {
+    const char* base = Xb + (c_begin + (T < ntiles ? T : ntiles - 1) * kTile) * rowb;
+#pragma unroll
+    for (int j = 0; j < C::PERW; ++j)
+      lds_load16(base + src_off[j], buf + lds_off[j]);
+  };
+  // wait until at most `later` of this wave's tile loads are outstanding (vmcnt; expcnt and
+  // lgkmcnt left at their maxima)
+  constexpr int kWaitNext = (C::PERW & 15) | (7 << 4) | (15 << 8) | (((C::PERW >> 4) & 3) << 14);
+
+  bf16x8 af[2][KT];
+  auto read_mb = [&](const char* buf, const int mb, bf16x8 (&a)[KT]) {
+    const char* row = buf + (32 * mb + lq) * C::STRIDE + 16 * h;
+#pragma unroll
+    for (int s = 0; s < KT; ++s) a[s] = *reinterpret_cast<const bf16x8*>(row + 32 * s);
+  };
+  auto mfma_mb = [&](const bf16x8 (&a)[KT], f32x16& acc) {
+    acc = f32x16{};
+#pragma unroll
+    for (int s = 0; s < KT; ++s) acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[s], qb[s], acc, 0, 0, 0);
+  };
+  auto flush = [&]() __attribute__((always_inline)) {
+    for (int j = 0; j < cnt; ++j) {
+      const uint64_t key = hitbuf[wv][j][l];
+      if (key < lst[L - 1]) list_insert<L>(lst, key);
+    }
+    cnt = 0;
+    const float t = tail_value(lst[L - 1]);
+    tq = fminf(t, __shfl_xor(t, 32, kWave));
+  };
+  // gate + hits of one M-block: element i = 4 g + j holds candidate c0 + 8 g + 4 h + j; a
+  // wave-uniform test per M-block, then per group of four
+  auto select_mb = [&](const f32x16& acc, const int64_t c0, const bool tail_tile) __attribute__((always_inline)) {
+    float gm[4];
+#pragma unroll
+    for (int g = 0; g < 4; ++g) gm[g] = fminf(fminf(acc[4 * g], acc[4 * g + 1]), fminf(acc[4 * g + 2], acc[4 * g + 3]));
+    if (__builtin_amdgcn_ballot_w64(fminf(fminf(gm[0], gm[1]), fminf(gm[2], gm[3])) <= tq) == 0ull) return;
+    const uint32_t cg = static_cast<uint32_t>(c0) + 4 * h;  // candidate ids < 2^31
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+      if (__builtin_amdgcn_ballot_w64(gm[g] <= tq) == 0ull) continue;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const float v = acc[4 * g + j];
+        const uint32_t c = cg + 8 * g + j;
+        if (v <= tq && (!tail_tile || static_cast<int64_t>(c) < c_end)) {
+          hitbuf[wv][cnt][l] = (static_cast<uint64_t>(float_ord(v)) << 32) | c;
+          ++cnt;
+        }
+      }
+      if (__builtin_amdgcn_ballot_w64(cnt > C::CAP - 4) != 0ull) flush();
+    }
+  };
+
+  if (ntiles > 0) {  // block-uniform
+    load_tile(tA, 0);
+    load_tile(tB, 1);
+    __builtin_amdgcn_s_waitcnt(kWaitNext);
+    __syncthreads();
+    read_mb(tA, 0, af[0]);
+    f32x16 acc0, acc1;
+    bool have1 = false;
+    // tile t in X, t+1 in Y (loading), Z free: start t+2 into Z; M-block 1 of t read, M-block 0
+    // multiplied, the previous M-block 1 selected, M-block 1 multiplied; wait for t+1, barrier,
+    // M-block 0 of t+1 read; M-block 0 of t selected
+    auto step = [&](char* X, char* Y, char* Z, const int64_t t) __attribute__((always_inline)) {
+      load_tile(Z, t + 2);
+      read_mb(X, 1, af[1]);
+      mfma_mb(af[0], acc0);
+      if (have1) select_mb(acc1, c_begin + (t - 1) * kTile + 32, false);
+      mfma_mb(af[1], acc1);
+      // this wave's loads of tile t+1 are done; the barrier publishes everyone's.  A plain
+      // barrier: __syncthreads' release fence would also wait for tile t+2's loads.
+      __builtin_amdgcn_s_waitcnt(kWaitNext);
+      asm volatile("" ::: "memory");
+      __builtin_amdgcn_s_barrier();
+      asm volatile("" ::: "memory");
+      read_mb(Y, 0, af[0]);
+      select_mb(acc0, c_begin + t * kTile, t + 1 == ntiles);
+      have1 = true;
+    };
+    int64_t t = 0;
+    for (; t + 3 <= ntiles; t += 3) {
+      step(tA, tB, tC, t);
+      step(tB, tC, tA, t + 1);
+      step(tC, tA, tB, t + 2);
+    }
+    if (t < ntiles) step(tA, tB, tC, t);
+    if (t + 1 < ntiles) step(tB, tC, tA, t + 1);
+    select_mb(acc1, c_begin + (ntiles - 1) * kTile + 32, true);
+  }
+
+  flush();
+  uint64_t* out = keys_out + (static_cast<int64_t>(split) * n_slots + slot) * kListStride;
+#pragma unroll
+  for (int i = 0; i < L; ++i) out[h * L + i] = lst[i];
+  if (h == 0) tau_out[static_cast<int64_t>(split) * n_slots + slot] = tq;
+}
+
+// ------------------------------------------------------------------------------------------
 // 4./5. exact re-rank, certificate, fallback
 // ------------------------------------------------------------------------------------------
 __device__ __forceinline__ float exact_d2(const float* __restrict__ X, int d, int64_t i, int64_t j) {
@@ -927,12 +1129,40 @@ static void launch_select_t(const void* Xp, int64_t n, int64_t n_slots, int d, i
                                                               taus);
 }
 
+template <int KT>
+static void launch_select32_t(const void* Xp, int64_t n, int64_t n_slots, int d, int rowb, int64_t split_len, int S,
+                              const int64_t* blk_range, uint64_t* keys, float* taus, hipStream_t st) {
+  dim3 grid(static_cast<unsigned>(n_slots / kQueriesPerBlock), static_cast<unsigned>(S));
+  select32_kernel<KT><<<grid, kThreads, 0, st>>>(Xp, n, n_slots, d, rowb, split_len, blk_range, keys, taus);
+}
+
+// K-steps of 16 of the 32x32x16 form: the 2d + 3 operand terms, rounded up
+static inline int bf16_kt(int d) { return (2 * d + 3 + 15) / 16; }
+
 // n_slots must be a multiple of kQueriesPerBlock; the wide form writes 2 * S key sets.
 // short_list keeps kLShort keys per lane (4 kLShort per query) instead of kL: for small k,
 // where many short candidate ranges would otherwise spend their time filling long lists.
 static int launch_select(bool bf16, bool wide, int ks, const void* Xp, int64_t n, int64_t n_slots, int d,
                          int64_t split_len, int S, const int32_t* qrows, const int64_t* blk_range, uint64_t* keys,
                          float* taus, hipStream_t st, bool short_list = false) {
+  // the BF16 main pass (no row list, 64-key lists) runs on the 32x32x16 form up to 8 K-steps
+  if (bf16 && !wide && !short_list && qrows == nullptr && bf16_kt(d) <= 8) {
+    const int rowb = 64 * ks;
+    switch (bf16_kt(d)) {
+#define MSC_SEL32(KT) launch_select32_t<KT>(Xp, n, n_slots, d, rowb, split_len, S, blk_range, keys, taus, st)
+      case 1: MSC_SEL32(1); break;
+      case 2: MSC_SEL32(2); break;
+      case 3: MSC_SEL32(3); break;
+      case 4: MSC_SEL32(4); break;
+      case 5: MSC_SEL32(5); break;
+      case 6: MSC_SEL32(6); break;
+      case 7: MSC_SEL32(7); break;
+      case 8: MSC_SEL32(8); break;
+#undef MSC_SEL32
+    }
+    MSC_LAUNCH_CHECK();
+    return kOk;
+  }
 #define MSC_SEL(B, K)                                                                                             \
   (wide ? launch_select_t<B, K, true, kL>(Xp, n, n_slots, d, split_len, S, qrows, blk_range, keys, taus, st)     \
         : (short_list                                                                                             \

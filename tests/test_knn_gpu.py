@@ -129,41 +129,48 @@ def test_knn_certificate_rescue_paths():
 
 @pytest.mark.parametrize("precision", ["fp32", "bf16"])
 def test_knn_lane_partition_rescued_by_wide_pass(precision):
-    """Candidate j reaches list (j mod 16) div 4 of its query in the first select pass, so
-    a query whose 20 nearest neighbours all have j mod 16 < 4 overflows one list of 16 and
-    fails the certificate.  The wide pass splits those candidates again by (j mod 64) div 32
-    (about 10 per list) and must certify every such row without brute force.
+    """In the first select pass a query's candidates are split over lists by index: FP32
+    (16x16 tiles) keeps 4 lists of 16, candidate j in list (j mod 16) div 4; BF16 (32x32
+    tiles) keeps 2 lists of 32, candidate j in list (j mod 8) div 4.  A query whose k nearest
+    neighbours all fall in one list overflows it (FP32: k = 20 with j mod 16 < 4; BF16:
+    k = 40 with j mod 8 < 4) and fails the certificate.  The wide pass splits those candidates
+    again over 8 lists of 16 by (j mod 16) div 4 and (j mod 64) div 32 and must certify every
+    such row without brute force.
 
-    Every point sits in a tight cluster of 21 (its 20 neighbours are its cluster), so no
-    near-ties exist across clusters: 32 clusters use only indices with j mod 16 < 4, the
-    other 96 only the remaining indices."""
+    Every point sits in a tight cluster of k + 1 (its k neighbours are its cluster), so no
+    near-ties exist across clusters: one group of clusters uses only the indices of one list,
+    the other group the remaining indices."""
     from oracle import knn
     rng = np.random.default_rng(11)
-    n, dim, k = 2688, 8, 20
+    if precision == "fp32":
+        n, dim, k, period, width = 2688, 8, 20, 16, 4
+    else:
+        n, dim, k, period, width = 3280, 8, 40, 8, 4
     j = np.arange(n)
-    groups = [j[j % 16 < 4], j[j % 16 >= 4]]
-    n_cl = [len(g) // 21 for g in groups]
-    assert n_cl == [32, 96]
+    groups = [j[j % period < width], j[j % period >= width]]
+    n_cl = [len(g) // (k + 1) for g in groups]
+    assert all(len(g) == m * (k + 1) for g, m in zip(groups, n_cl))
     centers = rng.normal(0.0, 100.0, (sum(n_cl), dim))
     X = np.empty((n, dim))
     c = 0
     for g, m in zip(groups, n_cl):
         for t in range(m):
-            X[g[21 * t: 21 * (t + 1)]] = centers[c] + rng.normal(0.0, 1.0, (21, dim))
+            X[g[(k + 1) * t: (k + 1) * (t + 1)]] = centers[c] + rng.normal(0.0, 1.0, (k + 1, dim))
             c += 1
     X = X.astype(np.float32)
     idx, dist, info = _run(X, k, precision)
     ri, _, rd = knn.exact_knn_c(X, k)
-    assert info["n_second_pass"] >= 21 * n_cl[0]
+    assert info["n_second_pass"] >= (k + 1) * n_cl[0]
     assert info["n_fallback"] == 0
     assert np.array_equal(idx, ri) and np.array_equal(dist, rd)
 
 
-@pytest.mark.parametrize("d", [1, 6, 7, 14, 22, 30, 46, 50, 61, 77, 93, 109, 125])
+@pytest.mark.parametrize("d", [1, 6, 7, 14, 22, 30, 38, 46, 50, 61, 62, 63, 77, 93, 109, 125])
 def test_knn_bf16_every_k_step_shape(d):
-    """BF16 operand rows of 2d + 3 values padded to 32-element MFMA steps (d = 1..125 covers
-    every step count 1..8, several at a step's last full d) and FP32 rows of d + 1
-    values in 8-element steps; both precisions equal the exact oracle."""
+    """BF16 operand rows of 2d + 3 values: up to d = 62 in 16-element steps of the 32x32x16
+    MFMA (d = 1..62 covers every step count 1..8, several at a step's last full d), beyond in
+    32-element steps of the 16x16x32 MFMA (5..8 steps); FP32 rows of d + 1 values in
+    8-element steps; both precisions equal the exact oracle."""
     from metalsinglecell import _hip
     rng = np.random.default_rng(d)
     centers = rng.normal(0, 4.0, (12, d))
