@@ -537,7 +537,7 @@ def main():
     if os.path.exists(tpath):
         with open(tpath) as f:
             traffic = json.load(f).get("bytes_per_launch")
-    roof = {"bound": "mfma", "kernel": "select_kernel<bf16>", "achieved": achieved,
+    roof = {"bound": "mfma", "kernel": "select32_kernel<7> (BF16, 32x32x16 MFMA)", "achieved": achieved,
             "peak": PEAK["bf16"], "unit": "TFLOP/s", "frac": achieved / PEAK["bf16"],
             "traffic": traffic, "flops_per_launch": flops, "avg_launch_ms": sel_ms}
 

@@ -101,7 +101,8 @@ int msc_knn_batched(const float* X, const int64_t* offsets /*host*/, int n_sampl
  * position in the concatenation).  Replaces the per-batch loop of neighbors.bbknn
  * (src/metalsinglecell/neighbors.py:491-541) as called by pp.bbknn (pp.py:809-833).
  * X: n x d float32 (device); batch (host): n codes in [0, n_batches), every code present,
- * n_batches <= 64.  Width K = sum_t min(k, n_t) <= 64 (returned in *out_width, host).
+ * n_batches <= 1024, k <= 64.  Width K = sum_t min(k, n_t) <= 1024 (returned in *out_width,
+ * host; rows wider than 64 are combined by a sort in LDS).
  * out_idx / out_dist: n x K.  Within a batch the order is the kNN key (j != i, d2, j), so a
  * cell's own batch lists the cell itself first at distance 0.  d <= 125 (BF16) / 127.
  * Workspace grows as n_batches x n x 512 B (one candidate list per cell and batch). */
@@ -114,7 +115,8 @@ int msc_bbknn(const float* X, int64_t n, int d, const int32_t* batch /*host*/, i
 
 /* obsp['distances'] of bbknn (neighbors.py:554-557): the K combined neighbours per row
  * without the row itself and without zero distances, columns ascending.  out_indptr n+1
- * int64 (nnz = out_indptr[n], on the device); out_indices / out_data capacity n*K. */
+ * int64 (nnz = out_indptr[n], on the device); out_indices / out_data capacity n*K.
+ * 1 <= K <= 1024. */
 int msc_bbknn_distances_workspace_size(int64_t n, size_t* bytes /*host*/);
 int msc_bbknn_distances_csr(const int32_t* idx, const float* dist, int64_t n, int K,
                             int64_t* out_indptr, int32_t* out_indices, float* out_data,

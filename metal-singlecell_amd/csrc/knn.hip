@@ -1557,6 +1557,29 @@ __global__ __launch_bounds__(kThreads) void k_bbknn_combine(
   }
 }
 
+// The same combine for K > 64: the row's K keys sorted in the wave's LDS region (KP >= K)
+__global__ __launch_bounds__(kThreads) void k_bbknn_combine_wide(
+    const int32_t* __restrict__ idx_pad, const float* __restrict__ dist_pad, int64_t n_slots, int K, int KP,
+    const int32_t* __restrict__ prow2row, int32_t* __restrict__ out_idx, float* __restrict__ out_dist) {
+  extern __shared__ uint64_t skeys[];
+  const int w = threadIdx.x / kWave, l = lane_id();
+  const int64_t q = static_cast<int64_t>(blockIdx.x) * (kThreads / kWave) + w;
+  if (q >= n_slots) return;
+  const int32_t row = prow2row[q];
+  if (row < 0) return;
+  uint64_t* a = skeys + static_cast<size_t>(w) * KP;
+  for (int p = l; p < KP; p += kWave)
+    a[p] = p < K ? (static_cast<uint64_t>(__float_as_uint(dist_pad[q * K + p])) << 32) | static_cast<uint32_t>(p)
+                 : kEmpty;
+  wave_lds_sync();
+  wave_lds_sort_u64(a, KP);
+  for (int p = l; p < K; p += kWave) {
+    const int64_t e = q * K + static_cast<int64_t>(a[p] & 0xFFFFFFFFull);
+    out_idx[static_cast<int64_t>(row) * K + p] = prow2row[idx_pad[e]];
+    out_dist[static_cast<int64_t>(row) * K + p] = dist_pad[e];
+  }
+}
+
 // obsp['distances'] of the combined lists: the self column and zero distances dropped,
 // columns ascending (scipy's coo->csr, setdiag(0), eliminate_zeros)
 __global__ __launch_bounds__(kThreads) void k_bbknn_dcount(const int32_t* __restrict__ idx,
@@ -1569,9 +1592,12 @@ __global__ __launch_bounds__(kThreads) void k_bbknn_dcount(const int32_t* __rest
     if (l == 0) cnt[n] = 0;
     return;
   }
-  const bool keep = l < K && idx[q * K + l] != static_cast<int32_t>(q) && dist[q * K + l] != 0.0f;
-  const uint64_t m = __ballot(keep);
-  if (l == 0) cnt[q] = __popcll(m);
+  int64_t c = 0;
+  for (int p0 = 0; p0 < K; p0 += kWave) {
+    const int p = p0 + l;
+    c += __popcll(__ballot(p < K && idx[q * K + p] != static_cast<int32_t>(q) && dist[q * K + p] != 0.0f));
+  }
+  if (l == 0) cnt[q] = c;
 }
 __global__ __launch_bounds__(kThreads) void k_bbknn_dfill(const int32_t* __restrict__ idx,
                                                           const float* __restrict__ dist, int64_t n, int K,
@@ -1588,6 +1614,32 @@ __global__ __launch_bounds__(kThreads) void k_bbknn_dfill(const int32_t* __restr
   if (l < m) {
     indices[b + l] = static_cast<int32_t>(key >> 32);
     data[b + l] = __uint_as_float(static_cast<uint32_t>(key));
+  }
+}
+
+// the same fill for K > 64: the kept (column, distance) keys sorted in the wave's LDS region
+__global__ __launch_bounds__(kThreads) void k_bbknn_dfill_wide(const int32_t* __restrict__ idx,
+                                                               const float* __restrict__ dist, int64_t n, int K,
+                                                               int KP, const int64_t* __restrict__ indptr,
+                                                               int32_t* __restrict__ indices,
+                                                               float* __restrict__ data) {
+  extern __shared__ uint64_t skeys[];
+  const int w = threadIdx.x / kWave, l = lane_id();
+  const int64_t q = static_cast<int64_t>(blockIdx.x) * (kThreads / kWave) + w;
+  if (q >= n) return;
+  uint64_t* a = skeys + static_cast<size_t>(w) * KP;
+  for (int p = l; p < KP; p += kWave) {
+    uint64_t key = kEmpty;
+    if (p < K && idx[q * K + p] != static_cast<int32_t>(q) && dist[q * K + p] != 0.0f)
+      key = (static_cast<uint64_t>(static_cast<uint32_t>(idx[q * K + p])) << 32) | __float_as_uint(dist[q * K + p]);
+    a[p] = key;
+  }
+  wave_lds_sync();
+  wave_lds_sort_u64(a, KP);
+  const int64_t b = indptr[q], m = indptr[q + 1] - b;
+  for (int64_t p = l; p < m; p += kWave) {
+    indices[b + p] = static_cast<int32_t>(a[p] >> 32);
+    data[b + p] = __uint_as_float(static_cast<uint32_t>(a[p]));
   }
 }
 
@@ -1789,6 +1841,9 @@ namespace {
 // P_t % 128 == 0; every query block of every batch runs one select split per batch t, over
 // exactly batch t's rows.  Results: kk_t = min(k, n_t) columns per batch at offset coff_t of
 // a K = sum kk_t wide row.
+// combined neighbours per row (the K-wide rows are sorted in LDS beyond 64)
+constexpr int kBbMaxWidth = 1024;
+
 struct BbLayout {
   int64_t n_tot = 0, nblk = 0;
   int nb = 0, K = 0;
@@ -1799,7 +1854,7 @@ struct BbLayout {
 int make_bb_layout(const int32_t* batch, int64_t n, int n_batches, int k, BbLayout* L) {
   using msc::knn::kQueriesPerBlock;
   MSC_REQUIRE(batch && n >= 1 && n < (int64_t(1) << 31) - 8192, "bbknn: bad n or null batch");
-  MSC_REQUIRE(n_batches >= 1 && n_batches <= 64, "bbknn: n_batches must be in [1, 64]");
+  MSC_REQUIRE(n_batches >= 1 && n_batches <= kBbMaxWidth, "bbknn: n_batches must be in [1, %d]", kBbMaxWidth);
   MSC_REQUIRE(k >= 1 && k <= 64, "bbknn: neighbors_within_batch must be in [1, 64]");
   std::vector<int64_t> cnt(n_batches, 0);
   for (int64_t i = 0; i < n; ++i) {
@@ -1825,7 +1880,8 @@ int make_bb_layout(const int32_t* batch, int64_t n, int n_batches, int k, BbLayo
     L->brange[2 * t + 1] = base + cnt[t];
     base += msc::ceil_div(cnt[t], kQueriesPerBlock) * kQueriesPerBlock;
   }
-  MSC_REQUIRE(K <= 64, "bbknn: %d combined neighbours (sum over batches of min(k, batch size)) exceed 64", K);
+  MSC_REQUIRE(K <= kBbMaxWidth, "bbknn: %d combined neighbours (sum over batches of min(k, batch size)) exceed %d",
+              K, kBbMaxWidth);
   L->K = K;
   MSC_REQUIRE(base * n_batches < (int64_t(1) << 31) - 1, "bbknn: n_batches x padded cells must stay below 2^31");
   L->n_tot = base;
@@ -1945,8 +2001,14 @@ extern "C" int msc_bbknn(const float* X, int64_t n, int d, const int32_t* batch,
   fallback_sep_kernel<<<kFallbackBlocks, kThreads, 0, st>>>(t.Xpad, nt, d, t.flist, t.fpos + M, t.brange, t.kk,
                                                             t.coff, L.K, t.idx_pad, t.dist_pad);
   MSC_LAUNCH_CHECK();
-  k_bbknn_combine<<<static_cast<unsigned>(ceil_div(nt, kThreads / kWave)), kThreads, 0, st>>>(
-      t.idx_pad, t.dist_pad, nt, L.K, t.prow2row, out_idx, out_dist);
+  const unsigned cb = static_cast<unsigned>(ceil_div(nt, kThreads / kWave));
+  if (L.K <= kWave) {
+    k_bbknn_combine<<<cb, kThreads, 0, st>>>(t.idx_pad, t.dist_pad, nt, L.K, t.prow2row, out_idx, out_dist);
+  } else {
+    const int KP = pow2_at_least(L.K, 128);
+    k_bbknn_combine_wide<<<cb, kThreads, (kThreads / kWave) * KP * sizeof(uint64_t), st>>>(
+        t.idx_pad, t.dist_pad, nt, L.K, KP, t.prow2row, out_idx, out_dist);
+  }
   MSC_LAUNCH_CHECK();
   if (out_width) *out_width = L.K;
   if (info) {
@@ -1971,13 +2033,19 @@ extern "C" int msc_bbknn_distances_csr(const int32_t* idx, const float* dist, in
                                        int32_t* out_indices, float* out_data, void* workspace,
                                        size_t workspace_bytes, void* stream) {
   MSC_REQUIRE(idx && dist && out_indptr && out_indices && out_data && workspace, "bbknn_distances: null pointer");
-  MSC_REQUIRE(n >= 1 && K >= 1 && K <= 64, "bbknn_distances: need n >= 1 and 1 <= K <= 64");
+  MSC_REQUIRE(n >= 1 && K >= 1 && K <= kBbMaxWidth, "bbknn_distances: need n >= 1 and 1 <= K <= %d", kBbMaxWidth);
   hipStream_t st = static_cast<hipStream_t>(stream);
   const unsigned wb = static_cast<unsigned>(ceil_div(n + 1, kThreads / kWave));
   k_bbknn_dcount<<<wb, kThreads, 0, st>>>(idx, dist, n, K, out_indptr);
   MSC_LAUNCH_CHECK();
   MSC_TRY(prims::exclusive_scan<int64_t>(out_indptr, out_indptr, n + 1, workspace, workspace_bytes, st));
-  k_bbknn_dfill<<<wb, kThreads, 0, st>>>(idx, dist, n, K, out_indptr, out_indices, out_data);
+  if (K <= kWave) {
+    k_bbknn_dfill<<<wb, kThreads, 0, st>>>(idx, dist, n, K, out_indptr, out_indices, out_data);
+  } else {
+    const int KP = pow2_at_least(K, 128);
+    k_bbknn_dfill_wide<<<wb, kThreads, (kThreads / kWave) * KP * sizeof(uint64_t), st>>>(
+        idx, dist, n, K, KP, out_indptr, out_indices, out_data);
+  }
   MSC_LAUNCH_CHECK();
   return kOk;
 }

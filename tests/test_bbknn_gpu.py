@@ -102,6 +102,40 @@ def test_bbknn_graph_matches_oracle(trim):
     assert _same_csr(C, RC)
 
 
+@pytest.mark.parametrize("nb,k", [(80, 3), (3, 30), (40, 20)])
+def test_bbknn_wide_rows_match_oracle(nb, k):
+    """More than 64 combined neighbours per row (K = sum_t min(k, n_t): 80 batches x 3, 3 x 30,
+    40 x 20 = 800) and more than 64 batches: the combined rows, the distances CSR and the
+    trimmed connectivities equal the oracle's (the reference loops over any number of
+    batches, neighbors.py:491-541)."""
+    from metalsinglecell import _hip
+    from metalsinglecell.neighbors import bbknn as hip_bbknn
+    from oracle import bbknn
+    n = 2400 if nb < 40 else 3000
+    X, _ = _mixture(n, 12, nb + k)
+    batch = np.random.default_rng(nb * k).integers(0, nb, n)
+    codes, m = bbknn.batch_codes(batch)
+    idx, dist = _hip.bbknn(X, codes, m, k, precision="bf16")
+    ri, rd, K = bbknn.bbknn_knn(X, batch, k)
+    assert K > 64 and idx.shape[1] == K
+    assert np.array_equal(idx.cpu().numpy(), ri) and np.array_equal(dist.cpu().numpy(), rd)
+    ip, ind, dat = _hip.bbknn_distances_csr(idx, dist)
+    D = sp.csr_matrix((dat.cpu().numpy(), ind.cpu().numpy(), ip.cpu().numpy()), shape=(n, n))
+    assert _same_csr(D, bbknn.distances_csr(ri, rd, n))
+    HD, HC, HK = hip_bbknn(X, batch, neighbors_within_batch=k, n_pcs=None)
+    RD, RC, RK = bbknn.bbknn(X, batch, k)
+    assert HK == RK == K
+    assert _same_csr(HD, RD) and _same_csr(HC, RC)
+
+
+def test_bbknn_width_limit():
+    from metalsinglecell import _hip
+    X, _ = _mixture(1100, 8, 2)
+    batch = np.arange(1100) % 1025
+    with pytest.raises(RuntimeError, match="n_batches"):
+        _hip.bbknn(X, batch, 1025, 1)
+
+
 def test_trim_graph_hubs():
     """Rows far longer than trim (hubs) and tied values: thresholds by the bitwise search."""
     from metalsinglecell import _hip
