@@ -604,6 +604,13 @@ __device__ __forceinline__ void lds_load16(const void* src, void* lds_base) {
 #endif
 }
 
+// block shape of the 32x32x16 form: kSelWaves waves of 32 queries share every tile
+constexpr int kSelWaves = 4;
+constexpr int kSelThreads = kSelWaves * kWave;
+constexpr int kSelQueries = 32 * kSelWaves;
+constexpr int kSelBlocksPerCU = 8 / kSelWaves;           // two waves per SIMD
+constexpr int kSelLds = 163840 / kSelBlocksPerCU;          // LDS per block (160 KB per CU)
+
 template <int KT>
 struct Sel32Cfg {
   static constexpr int CH = 2 * KT;     // 16-byte chunks of a row used (K = 16 KT bf16)
@@ -611,14 +618,14 @@ struct Sel32Cfg {
   static constexpr int STRIDE = 16 * SC;
   static constexpr int TB = kTile * STRIDE;  // = 1 KB x SC
   static constexpr int NI = SC;              // wave loads per tile
-  static constexpr int PERW = (NI + 3) / 4;  // per wave (the last one repeated where short)
-  static constexpr int FIT = (kLdsHalfCU - 3 * TB) / ((kThreads / kWave) * kWave * 8);
+  static constexpr int PERW = (NI + kSelWaves - 1) / kSelWaves;  // per wave (the last one repeated)
+  static constexpr int FIT = (kSelLds - 3 * TB) / (kSelWaves * kWave * 8);
   static constexpr int CAP = FIT < 24 ? FIT : 24;
   static_assert(CAP >= 12, "hit buffer");
 };
 
 template <int KT>
-__global__ __launch_bounds__(kThreads, 2) void select32_kernel(
+__global__ __launch_bounds__(kSelThreads, kSelBlocksPerCU) void select32_kernel(
     const void* __restrict__ Xp, int64_t n, int64_t n_slots, int d, int rowb, int64_t split_len,
     const int64_t* __restrict__ blk_range, uint64_t* __restrict__ keys_out, float* __restrict__ tau_out) {
   using C = Sel32Cfg<KT>;
@@ -627,7 +634,7 @@ __global__ __launch_bounds__(kThreads, 2) void select32_kernel(
   __shared__ __attribute__((aligned(1024))) char tA[C::TB];
   __shared__ __attribute__((aligned(1024))) char tB[C::TB];
   __shared__ __attribute__((aligned(1024))) char tC[C::TB];
-  __shared__ uint64_t hitbuf[kThreads / kWave][C::CAP][kWave];
+  __shared__ uint64_t hitbuf[kSelWaves][C::CAP][kWave];
 
   const int l = lane_id();
   const int wv = threadIdx.x / kWave;
@@ -637,12 +644,13 @@ __global__ __launch_bounds__(kThreads, 2) void select32_kernel(
   const int64_t br = 2 * (static_cast<int64_t>(split) * gridDim.x + blockIdx.x);
   const int64_t c_begin = blk_range ? blk_range[br] : static_cast<int64_t>(split) * split_len;
   const int64_t c_end = blk_range ? blk_range[br + 1] : min(n, c_begin + split_len);
-  const int64_t slot = static_cast<int64_t>(blockIdx.x) * kQueriesPerBlock + wv * 32 + lq;
+  const int64_t slot = static_cast<int64_t>(blockIdx.x) * kSelQueries + wv * 32 + lq;
+  const bool own = slot < n_slots;  // a wave past the last slot only helps load the tiles
   const char* Xb = static_cast<const char*>(Xp);
 
   bf16x8 qb[KT];
   {
-    const uint16_t* src = reinterpret_cast<const uint16_t*>(Xb + slot * rowb);
+    const uint16_t* src = reinterpret_cast<const uint16_t*>(Xb + (own ? slot : 0) * rowb);
 #pragma unroll
     for (int s = 0; s < KT; ++s) {
 #pragma unroll
@@ -665,12 +673,12 @@ __global__ __launch_bounds__(kThreads, 2) void select32_kernel(
   int cnt = 0;
   const int64_t ntiles = (c_end > c_begin) ? ceil_div(c_end - c_begin, kTile) : 0;
 
-  // this wave's loads of a tile: wave load u = wv + 4 j (clamped), lane l fills LDS chunk
+  // this wave's loads of a tile: wave load u = wv + kSelWaves j (clamped), lane l fills LDS chunk
   // 64 u + l = (row, chunk) of the tile
   int lds_off[C::PERW], src_off[C::PERW];
 #pragma unroll
   for (int j = 0; j < C::PERW; ++j) {
-    const int u = min(wv + 4 * j, C::NI - 1);
+    const int u = min(wv + kSelWaves * j, C::NI - 1);
     const int pch = 64 * u + l;
     const int row = pch / C::SC, col = pch % C::SC;
     lds_off[j] = 1024 * u;
@@ -769,6 +777,7 @@ __global__ __launch_bounds__(kThreads, 2) void select32_kernel(
   }
 
   flush();
+  if (!own) return;
   uint64_t* out = keys_out + (static_cast<int64_t>(split) * n_slots + slot) * kListStride;
 #pragma unroll
   for (int i = 0; i < L; ++i) out[h * L + i] = lst[i];
@@ -1132,8 +1141,8 @@ static void launch_select_t(const void* Xp, int64_t n, int64_t n_slots, int d, i
 template <int KT>
 static void launch_select32_t(const void* Xp, int64_t n, int64_t n_slots, int d, int rowb, int64_t split_len, int S,
                               const int64_t* blk_range, uint64_t* keys, float* taus, hipStream_t st) {
-  dim3 grid(static_cast<unsigned>(n_slots / kQueriesPerBlock), static_cast<unsigned>(S));
-  select32_kernel<KT><<<grid, kThreads, 0, st>>>(Xp, n, n_slots, d, rowb, split_len, blk_range, keys, taus);
+  dim3 grid(static_cast<unsigned>(ceil_div(n_slots, kSelQueries)), static_cast<unsigned>(S));
+  select32_kernel<KT><<<grid, kSelThreads, 0, st>>>(Xp, n, n_slots, d, rowb, split_len, blk_range, keys, taus);
 }
 
 // K-steps of 16 of the 32x32x16 form: the 2d + 3 operand terms, rounded up

@@ -77,7 +77,11 @@ def lookup(offsets, d, k, precision):
 
 
 def candidates_for(k, splits=CANDIDATES, list_lens=LIST_LENS):
-    return [(int(s), int(L)) for L in list_lens if L == 16 or k <= 4 * L for s in splits]
+    """Tile shapes worth timing: every split count with the 16-key lists, and the 8-key lists
+    where they hold k (k <= 32) with at least 4k admitted candidates per query over the splits
+    (32 per split); with fewer, most rows fail the certificate and take the exact fallback."""
+    return [(int(s), int(L)) for L in list_lens for s in splits
+            if L == 16 or (k <= 4 * L and 4 * L * s >= 4 * k)]
 
 
 def tune_batched_tile(X, offsets, k, precision="bf16", candidates=None, reps=3):

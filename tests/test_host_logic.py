@@ -196,8 +196,10 @@ def test_autotune_cache_roundtrip(tmp_path, monkeypatch):
     assert autotune.lookup(off2, 30, 15, "bf16") == (4, 8)
     autotune.store(autotune.key_for(off, 30, 40, "bf16"), {"splits": 2, "ms": {"2": 1.0}})
     assert autotune.lookup(off, 30, 40, "bf16") == (2, 0)      # entries without a list length
-    # the short list is a candidate only where it holds k (4 lanes x 8 keys)
-    assert autotune.candidates_for(32) == [(s, L) for L in (16, 8) for s in (1, 2, 4, 8)]
+    # the short list is a candidate only where it holds k (4 lanes x 8 keys) and the splits
+    # admit at least 4k candidates per query
+    assert autotune.candidates_for(32) == [(s, 16) for s in (1, 2, 4, 8)] + [(4, 8), (8, 8)]
+    assert autotune.candidates_for(15) == [(s, 16) for s in (1, 2, 4, 8)] + [(2, 8), (4, 8), (8, 8)]
     assert autotune.candidates_for(33) == [(s, 16) for s in (1, 2, 4, 8)]
 
 
