@@ -10,8 +10,9 @@ One step = one full pass from the device-resident embedding to device labels:
 ``value`` = cells / (mean step time) for that BF16+re-rank form.  The plain FP32 GPU form
 (FP32 MFMA ranking) is timed beside it on the same input and reported in ``fp32_form``;
 both forms return identical graphs and labels.  ``roofline`` covers the dominant MFMA
-kernel (the kNN select kernel): algorithmic flops 2 n^2 d per launch over its HIP-event
-duration, against the 2.5 PFLOP/s dense BF16 peak.  ``cpu_baseline`` times the project's
+kernel (the kNN select kernel): algorithmic flops 2 d per scanned (query, candidate) pair
+per launch -- s n^2 pairs, s the scanned fraction of the partition-pruned layout, 1 without
+it -- over its HIP-event duration, against the 2.5 PFLOP/s dense BF16 peak.  ``cpu_baseline`` times the project's
 CPU restatement (oracle/) of the same three stages on bounded samples on this host.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N] [--no-cpu]
@@ -530,7 +531,10 @@ def main():
 
     sel = [r["knn"]["select_ms"] for r in recs_bf]
     sel_ms = float(np.mean(sel))
-    flops = 2.0 * n * n * d
+    # algorithmic flops of the launch: 2 d per (query, candidate) pair it scans (the pruned
+    # layout scans `scanned` of the n^2 pairs); the brute-force equivalent is 2 n^2 d
+    scanned = float(recs_bf[-1]["knn"].get("scanned", 1.0))
+    flops = 2.0 * scanned * n * n * d
     achieved = flops / (sel_ms * 1e-3) / 1e12
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "knn_select_traffic.json")
@@ -539,7 +543,9 @@ def main():
             traffic = json.load(f).get("bytes_per_launch")
     roof = {"bound": "mfma", "kernel": "select32_kernel<7> (BF16, 32x32x16 MFMA)", "achieved": achieved,
             "peak": PEAK["bf16"], "unit": "TFLOP/s", "frac": achieved / PEAK["bf16"],
-            "traffic": traffic, "flops_per_launch": flops, "avg_launch_ms": sel_ms}
+            "traffic": traffic, "flops_per_launch": flops, "avg_launch_ms": sel_ms,
+            "scanned_fraction": scanned,
+            "brute_force_equivalent_tflops": 2.0 * n * n * d / (sel_ms * 1e-3) / 1e12}
 
     from metalsinglecell import _hip
     gi, gd = _hip.knn_exact(Xd, a.k, precision="bf16")

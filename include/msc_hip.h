@@ -54,7 +54,17 @@ const char* msc_last_error(void);
  * reference's argpartition fallback for K > 32, neighbors.py:66-68); the result is the same
  * exact kNN and the workspace query returns a token size.  Limits: 1 <= k <= 512,
  * k <= n < 2^31, d >= 1.
+ * Pruned candidate ranges (d <= 64, k <= 64, n >= 256): the rows are grouped into partitions
+ * (balls around sampled centres) and every block of 128 queries scans only the partitions the
+ * triangle inequality cannot rule out for any of its queries (a partition's ball against an upper
+ * bound on each query's k-th neighbour distance, with float32 rounding margins).  The result is
+ * the same exact kNN; only the work changes (about the sum of squared cluster fractions of n^2 on
+ * well-separated clusters, all of n^2 on data without cluster structure).  info->prune: AUTO uses
+ * them from 65,536 rows on, OFF never, ON whenever the shape allows (otherwise MSC_EINVAL).
+ * info == NULL means AUTO.
  * ------------------------------------------------------------------------------------- */
+enum { MSC_PRUNE_AUTO = 0, MSC_PRUNE_OFF = 1, MSC_PRUNE_ON = 2 };
+
 typedef struct {
   int timing;           /* in: nonzero -> record per-stage times (adds one stream sync) */
   int n_splits;         /* out: candidate splits used by the select kernel */
@@ -62,6 +72,12 @@ typedef struct {
   int64_t n_second_pass;/* out: rows re-selected by the wide pass (certificate failed) */
   int64_t n_fallback;   /* out: rows recomputed by exact brute force */
   double prep_ms, select_ms, finalize_ms, second_pass_ms, fallback_ms; /* out (timing only) */
+  int prune;            /* in (msc_knn_exact): MSC_PRUNE_AUTO / _OFF / _ON, see below */
+  int n_partitions;     /* out: partitions of the pruned layout (0: every candidate scanned) */
+  int n_groups;         /* out: groups of partitions with query blocks of their own */
+  int n_ranges;         /* out: candidate row ranges per query block, at most */
+  double scanned;       /* out: fraction of the n^2 (query, candidate) pairs the select scans */
+  double partition_ms;  /* out (timing only): building the pruned layout */
 } msc_knn_info;
 
 int msc_knn_workspace_size(int64_t n, int d, int k, int precision, size_t* bytes /*host*/);

@@ -43,6 +43,7 @@
 #include "common.h"
 #include "msc_hip.h"
 #include "prims.h"
+#include "part.h"
 #include "wide.h"
 
 namespace msc {
@@ -252,6 +253,44 @@ struct SelCfg {
   static constexpr int NB_FRAG = BF16 ? KS : KS;             // B fragments per N-block
 };
 
+// a block-uniform 64-bit value, declared uniform (kept in scalar registers)
+__device__ __forceinline__ int64_t uniform_i64(int64_t v) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(v));
+  const uint32_t hi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(static_cast<uint64_t>(v) >> 32));
+  return static_cast<int64_t>((static_cast<uint64_t>(hi) << 32) | lo);
+}
+
+// Candidate tiles of one block: one range [c_begin, c_end) (a split of [0, n), or a sample's
+// own rows), or -- with rng4 -- up to four row ranges of the partition-pruned layout
+// (rng4[8 b + 2 r], rng4[8 b + 2 r + 1] = begin, end of range r of block b), walked in order as
+// one stream of tiles.  Tile T starts at row MSC_TROW(T); rows of its range end at MSC_TEND(T).
+// Everything is block-uniform.
+#define MSC_TILE_MAP(RNG4, BLK, CB, CE)                                                              \
+  const int64_t* tm_r = (RNG4) ? (RNG4) + static_cast<int64_t>(BLK) * 8 : nullptr;                  \
+  const int64_t tm_b0 = tm_r ? tm_r[0] : (CB), tm_e0 = tm_r ? tm_r[1] : (CE);                       \
+  const int64_t tm_b1 = tm_r ? tm_r[2] : 0, tm_e1 = tm_r ? tm_r[3] : 0;                             \
+  const int64_t tm_b2 = tm_r ? tm_r[4] : 0, tm_e2 = tm_r ? tm_r[5] : 0;                             \
+  const int64_t tm_b3 = tm_r ? tm_r[6] : 0, tm_e3 = tm_r ? tm_r[7] : 0;                             \
+  const int64_t tm_t1 = tm_e0 > tm_b0 ? ceil_div(tm_e0 - tm_b0, kTile) : 0;                         \
+  const int64_t tm_t2 = tm_t1 + (tm_e1 > tm_b1 ? ceil_div(tm_e1 - tm_b1, kTile) : 0);               \
+  const int64_t tm_t3 = tm_t2 + (tm_e2 > tm_b2 ? ceil_div(tm_e2 - tm_b2, kTile) : 0);               \
+  const int64_t tm_n = tm_t3 + (tm_e3 > tm_b3 ? ceil_div(tm_e3 - tm_b3, kTile) : 0);                \
+  /* row(T) = b_r + 64 (T - t_r) in range r, as increments over range 0 */                         \
+  const int64_t tm_d1 = uniform_i64((tm_b1 - tm_t1 * kTile) - tm_b0);                              \
+  const int64_t tm_d2 = uniform_i64((tm_b2 - tm_t2 * kTile) - (tm_b1 - tm_t1 * kTile));            \
+  const int64_t tm_d3 = uniform_i64((tm_b3 - tm_t3 * kTile) - (tm_b2 - tm_t2 * kTile));            \
+  const int64_t tm_f1 = uniform_i64(tm_e1 - tm_e0), tm_f2 = uniform_i64(tm_e2 - tm_e1),             \
+                tm_f3 = uniform_i64(tm_e3 - tm_e2);                                                 \
+  const int64_t tm_u0 = uniform_i64(tm_b0), tm_v0 = uniform_i64(tm_e0), tm_u1 = uniform_i64(tm_t1), \
+                tm_u2 = uniform_i64(tm_t2), tm_u3 = uniform_i64(tm_t3);
+// (sums of independent selects: a chain of selects over four values is turned into a lookup
+// in a private array, i.e. scratch memory)
+#define MSC_TROW(T)                                                                        \
+  (tm_u0 + (T) * kTile + ((T) >= tm_u1 ? tm_d1 : int64_t(0)) + ((T) >= tm_u2 ? tm_d2 : int64_t(0)) + \
+   ((T) >= tm_u3 ? tm_d3 : int64_t(0)))
+#define MSC_TEND(T) \
+  (tm_v0 + ((T) >= tm_u1 ? tm_f1 : int64_t(0)) + ((T) >= tm_u2 ? tm_f2 : int64_t(0)) + ((T) >= tm_u3 ? tm_f3 : int64_t(0)))
+
 // WIDE = false: each wave holds 32 queries (2 N-blocks), one list of L keys per lane and
 //   query; a query's 64 candidates of every tile spread over 4 lanes (lists) -> 4L keys.
 // WIDE = true: each wave holds 16 queries (1 N-block); a lane keeps two lists for its
@@ -261,7 +300,7 @@ template <bool BF16, int KS, int L, bool WIDE>
 __global__ __launch_bounds__(kThreads, 2) void select_kernel(
     const void* __restrict__ Xp, int64_t n, int64_t n_slots, int d, int64_t split_len,
     const int32_t* __restrict__ qrows, const int64_t* __restrict__ blk_range,
-    uint64_t* __restrict__ keys_out, float* __restrict__ tau_out) {
+    const int64_t* __restrict__ rng4, uint64_t* __restrict__ keys_out, float* __restrict__ tau_out) {
   using C = SelCfg<BF16, KS>;
   constexpr int NB = WIDE ? 1 : 2;         // N-blocks (of 16 queries) per wave
   constexpr int QPB = 4 * 16 * NB;         // queries per block
@@ -283,7 +322,9 @@ __global__ __launch_bounds__(kThreads, 2) void select_kernel(
   const int64_t br = 2 * (static_cast<int64_t>(split) * gridDim.x + blockIdx.x);
   const int64_t c_begin = blk_range ? blk_range[br] : static_cast<int64_t>(split) * split_len;
   const int64_t c_end = blk_range ? blk_range[br + 1] : min(n, c_begin + split_len);
-  // query slots: slot s is row s, or row qrows[s] when a row list is given
+  MSC_TILE_MAP(rng4, blockIdx.x, c_begin, c_end)
+  // query slots: slot s is row s, or row qrows[s] when a row list is given (a padding slot, -1,
+  // computes row 0's lists, which nothing reads)
   const int64_t sbase = static_cast<int64_t>(blockIdx.x) * QPB + wv * (16 * NB);
   const char* Xb = static_cast<const char*>(Xp);
 
@@ -293,7 +334,7 @@ __global__ __launch_bounds__(kThreads, 2) void select_kernel(
 #pragma unroll
   for (int nb = 0; nb < NB; ++nb) {
     const int64_t slot = sbase + 16 * nb + lm;
-    const int64_t q = qrows ? static_cast<int64_t>(qrows[slot]) : slot;
+    const int64_t q = qrows ? max(static_cast<int64_t>(qrows[slot]), int64_t(0)) : slot;
     const char* qrow = Xb + q * C::ROWB;
     if constexpr (BF16) {
       // [-2 hi, -2 hi, 1, 1, 1, 0 ...] against the candidate row [hi, lo, cn_hi, cn_mid, cn_lo, 0 ...]
@@ -337,7 +378,7 @@ __global__ __launch_bounds__(kThreads, 2) void select_kernel(
     tq[nb] = tau[nb];
   }
 
-  const int64_t ntiles = (c_end > c_begin) ? ceil_div(c_end - c_begin, kTile) : 0;
+  const int64_t ntiles = tm_n;
 
   // Prefetch registers for one tile: up to eight named 16-byte registers chosen at compile
   // time (an array here gets demoted to LDS or scratch by the compiler).  Global reads and
@@ -355,7 +396,7 @@ __global__ __launch_bounds__(kThreads, 2) void select_kernel(
   if constexpr (kPer > i) {                                                                     \
     const int c_ = threadIdx.x + i * kThreads;                                                  \
     if (kNChunk % kThreads == 0 || c_ < kNChunk)                                                \
-      MSC_PF(S, i) = *reinterpret_cast<const uint4*>(Xb + (c_begin + (T) * kTile) * C::ROWB + static_cast<int64_t>(c_) * 16); \
+      MSC_PF(S, i) = *reinterpret_cast<const uint4*>(Xb + MSC_TROW(T) * C::ROWB + static_cast<int64_t>(c_) * 16); \
   }
 #define MSC_STORE1(S, i, B)                                                                     \
   if constexpr (kPer > i) {                                                                     \
@@ -378,7 +419,7 @@ __global__ __launch_bounds__(kThreads, 2) void select_kernel(
   // as the plain loop below.
   constexpr bool kAhead = BF16 && KS <= 4;
   bf16x8 af0[kAhead ? KS : 1];
-  auto read_first = [&](const int buf) {
+  auto read_first = [&](const int buf) __attribute__((always_inline)) {
     if constexpr (kAhead) {
 #pragma unroll
       for (int s = 0; s < KS; ++s)
@@ -387,7 +428,7 @@ __global__ __launch_bounds__(kThreads, 2) void select_kernel(
   };
 
   // one candidate tile's MFMA distance block from LDS buffer buf
-  auto tile_mfma = [&](const int buf, f32x4 (&acc)[4][NB]) {
+  auto tile_mfma = [&](const int buf, f32x4 (&acc)[4][NB]) __attribute__((always_inline)) {
     if constexpr (kAhead) {
       bf16x8 af[2][KS];
 #pragma unroll
@@ -441,14 +482,15 @@ __global__ __launch_bounds__(kThreads, 2) void select_kernel(
   };
 
   // the fused selection over one tile's distance block
-  auto tile_select = [&](const f32x4 (&acc)[4][NB], const int64_t t) {
+  auto tile_select = [&](const f32x4 (&acc)[4][NB], const int64_t t) __attribute__((always_inline)) {
     // Hits (acc <= the lane's list tail) are appended to the lane's LDS buffer; buffers are
     // merged into the sorted register lists when any lane's buffer could overflow on the
     // next M-block, so a rare hit costs one LDS store instead of a wave-wide insertion.
     // Each (M-block, N-block) group is gated by a wave-uniform test: after the first few
     // hundred tiles almost every group has no hit and costs four compares.
-    const int64_t c0 = c_begin + t * kTile;
-    const bool tail_tile = (t + 1 == ntiles);
+    const int64_t c0 = MSC_TROW(t);
+    const int64_t cend = MSC_TEND(t);
+    const bool tail_tile = c0 + kTile > cend;  // the tile's range ends inside it
 #pragma unroll
     for (int nb = 0; nb < NB; ++nb) {
 #pragma unroll
@@ -462,7 +504,7 @@ __global__ __launch_bounds__(kThreads, 2) void select_kernel(
         for (int i = 0; i < 4; ++i) {
           const float v = a4[i];
           const uint32_t c = cg + i;
-          if (v <= tq[li] && (!tail_tile || static_cast<int64_t>(c) < c_end)) {
+          if (v <= tq[li] && (!tail_tile || static_cast<int64_t>(c) < cend)) {
             hitbuf[wv][li][cnt[li]][l] = (static_cast<uint64_t>(float_ord(v)) << 32) | c;
             ++cnt[li];
           }
@@ -627,7 +669,8 @@ struct Sel32Cfg {
 template <int KT>
 __global__ __launch_bounds__(kSelThreads, kSelBlocksPerCU) void select32_kernel(
     const void* __restrict__ Xp, int64_t n, int64_t n_slots, int d, int rowb, int64_t split_len,
-    const int64_t* __restrict__ blk_range, uint64_t* __restrict__ keys_out, float* __restrict__ tau_out) {
+    const int32_t* __restrict__ qrows, const int64_t* __restrict__ blk_range, const int64_t* __restrict__ rng4,
+    uint64_t* __restrict__ keys_out, float* __restrict__ tau_out) {
   using C = Sel32Cfg<KT>;
   typedef __attribute__((__vector_size__(16 * sizeof(float)))) float f32x16;
   constexpr int L = 32;
@@ -644,13 +687,17 @@ __global__ __launch_bounds__(kSelThreads, kSelBlocksPerCU) void select32_kernel(
   const int64_t br = 2 * (static_cast<int64_t>(split) * gridDim.x + blockIdx.x);
   const int64_t c_begin = blk_range ? blk_range[br] : static_cast<int64_t>(split) * split_len;
   const int64_t c_end = blk_range ? blk_range[br + 1] : min(n, c_begin + split_len);
+  MSC_TILE_MAP(rng4, blockIdx.x, c_begin, c_end)
   const int64_t slot = static_cast<int64_t>(blockIdx.x) * kSelQueries + wv * 32 + lq;
-  const bool own = slot < n_slots;  // a wave past the last slot only helps load the tiles
+  // slot s is row s, or row qrows[s] (-1: a padding slot); a lane without a query only helps
+  // load the tiles
+  const int64_t qrow = slot < n_slots ? (qrows ? static_cast<int64_t>(qrows[slot]) : slot) : -1;
+  const bool own = qrow >= 0;
   const char* Xb = static_cast<const char*>(Xp);
 
   bf16x8 qb[KT];
   {
-    const uint16_t* src = reinterpret_cast<const uint16_t*>(Xb + (own ? slot : 0) * rowb);
+    const uint16_t* src = reinterpret_cast<const uint16_t*>(Xb + (own ? qrow : 0) * rowb);
 #pragma unroll
     for (int s = 0; s < KT; ++s) {
 #pragma unroll
@@ -671,7 +718,7 @@ __global__ __launch_bounds__(kSelThreads, kSelBlocksPerCU) void select32_kernel(
   for (int i = 0; i < L; ++i) lst[i] = kEmpty;
   float tq = __uint_as_float(0x7F800000u);
   int cnt = 0;
-  const int64_t ntiles = (c_end > c_begin) ? ceil_div(c_end - c_begin, kTile) : 0;
+  const int64_t ntiles = tm_n;
 
   // this wave's loads of a tile: wave load u = wv + kSelWaves j (clamped), lane l fills LDS chunk
   // 64 u + l = (row, chunk) of the tile
@@ -685,7 +732,7 @@ __global__ __launch_bounds__(kSelThreads, kSelBlocksPerCU) void select32_kernel(
     src_off[j] = row * rowb + min(col, C::CH - 1) * 16;
   }
   auto load_tile = [&](char* buf, const int64_t T) __attribute__((always_inline)) {
-    const char* base = Xb + (c_begin + (T < ntiles ? T : ntiles - 1) * kTile) * rowb;
+    const char* base = Xb + MSC_TROW(T < ntiles ? T : ntiles - 1) * rowb;
 #pragma unroll
     for (int j = 0; j < C::PERW; ++j)
       lds_load16(base + src_off[j], buf + lds_off[j]);
@@ -695,12 +742,12 @@ __global__ __launch_bounds__(kSelThreads, kSelBlocksPerCU) void select32_kernel(
   constexpr int kWaitNext = (C::PERW & 15) | (7 << 4) | (15 << 8) | (((C::PERW >> 4) & 3) << 14);
 
   bf16x8 af[2][KT];
-  auto read_mb = [&](const char* buf, const int mb, bf16x8 (&a)[KT]) {
+  auto read_mb = [&](const char* buf, const int mb, bf16x8 (&a)[KT]) __attribute__((always_inline)) {
     const char* row = buf + (32 * mb + lq) * C::STRIDE + 16 * h;
 #pragma unroll
     for (int s = 0; s < KT; ++s) a[s] = *reinterpret_cast<const bf16x8*>(row + 32 * s);
   };
-  auto mfma_mb = [&](const bf16x8 (&a)[KT], f32x16& acc) {
+  auto mfma_mb = [&](const bf16x8 (&a)[KT], f32x16& acc) __attribute__((always_inline)) {
     acc = f32x16{};
 #pragma unroll
     for (int s = 0; s < KT; ++s) acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[s], qb[s], acc, 0, 0, 0);
@@ -716,7 +763,8 @@ __global__ __launch_bounds__(kSelThreads, kSelBlocksPerCU) void select32_kernel(
   };
   // gate + hits of one M-block: element i = 4 g + j holds candidate c0 + 8 g + 4 h + j; a
   // wave-uniform test per M-block, then per group of four
-  auto select_mb = [&](const f32x16& acc, const int64_t c0, const bool tail_tile) __attribute__((always_inline)) {
+  auto select_mb = [&](const f32x16& acc, const int64_t c0, const int64_t cend) __attribute__((always_inline)) {
+    const bool tail_tile = c0 + 32 > cend;
     float gm[4];
 #pragma unroll
     for (int g = 0; g < 4; ++g) gm[g] = fminf(fminf(acc[4 * g], acc[4 * g + 1]), fminf(acc[4 * g + 2], acc[4 * g + 3]));
@@ -729,7 +777,7 @@ __global__ __launch_bounds__(kSelThreads, kSelBlocksPerCU) void select32_kernel(
       for (int j = 0; j < 4; ++j) {
         const float v = acc[4 * g + j];
         const uint32_t c = cg + 8 * g + j;
-        if (v <= tq && (!tail_tile || static_cast<int64_t>(c) < c_end)) {
+        if (v <= tq && (!tail_tile || static_cast<int64_t>(c) < cend)) {
           hitbuf[wv][cnt][l] = (static_cast<uint64_t>(float_ord(v)) << 32) | c;
           ++cnt;
         }
@@ -753,7 +801,7 @@ __global__ __launch_bounds__(kSelThreads, kSelBlocksPerCU) void select32_kernel(
       load_tile(Z, t + 2);
       read_mb(X, 1, af[1]);
       mfma_mb(af[0], acc0);
-      if (have1) select_mb(acc1, c_begin + (t - 1) * kTile + 32, false);
+      if (have1) select_mb(acc1, MSC_TROW(t - 1) + 32, MSC_TEND(t - 1));
       mfma_mb(af[1], acc1);
       // this wave's loads of tile t+1 are done; the barrier publishes everyone's.  A plain
       // barrier: __syncthreads' release fence would also wait for tile t+2's loads.
@@ -762,7 +810,7 @@ __global__ __launch_bounds__(kSelThreads, kSelBlocksPerCU) void select32_kernel(
       __builtin_amdgcn_s_barrier();
       asm volatile("" ::: "memory");
       read_mb(Y, 0, af[0]);
-      select_mb(acc0, c_begin + t * kTile, t + 1 == ntiles);
+      select_mb(acc0, MSC_TROW(t), MSC_TEND(t));
       have1 = true;
     };
     int64_t t = 0;
@@ -773,7 +821,7 @@ __global__ __launch_bounds__(kSelThreads, kSelBlocksPerCU) void select32_kernel(
     }
     if (t < ntiles) step(tA, tB, tC, t);
     if (t + 1 < ntiles) step(tB, tC, tA, t + 1);
-    select_mb(acc1, c_begin + (ntiles - 1) * kTile + 32, true);
+    select_mb(acc1, MSC_TROW(ntiles - 1) + 32, MSC_TEND(ntiles - 1));
   }
 
   flush();
@@ -891,12 +939,16 @@ __global__ __launch_bounds__(kThreads) void finalize_kernel(
     const float* __restrict__ X, int64_t nq, int d, int k, int S, int64_t n_slots,
     const int32_t* __restrict__ qrows, const uint64_t* __restrict__ keys, const float* __restrict__ taus,
     const float* __restrict__ nrm2, const float* __restrict__ res, const float* __restrict__ gstat,
-    Bound bnd, const int64_t* __restrict__ row_range, int32_t* __restrict__ out_idx,
-    float* __restrict__ out_dist, int32_t* __restrict__ flags) {
+    Bound bnd, const int64_t* __restrict__ row_range, const int32_t* __restrict__ perm,
+    int32_t* __restrict__ out_idx, float* __restrict__ out_dist, int32_t* __restrict__ flags) {
+  // Rows of X (and of keys, flags) are in the order the select used; with perm, row r of X is
+  // row perm[r] of the caller's data: exact keys and outputs use the caller's row ids.
   const int l = lane_id();
   const int64_t slot = static_cast<int64_t>(blockIdx.x) * (kThreads / kWave) + threadIdx.x / kWave;
   if (slot >= nq) return;
   const int64_t q = qrows ? static_cast<int64_t>(qrows[slot]) : slot;
+  if (q < 0) return;  // padding slot of a query-block layout
+  const int64_t qo = perm ? static_cast<int64_t>(perm[q]) : q;
   if (row_range && row_range[2 * q + 1] == 0) {  // padding row of a batched layout
     if (l == 0) flags[q] = 0;
     return;
@@ -914,7 +966,7 @@ __global__ __launch_bounds__(kThreads) void finalize_kernel(
   const bool valid = cur != kEmpty;
   const int64_t j = static_cast<int64_t>(cur & 0xFFFFFFFFull);
   uint64_t ek = kEmpty;
-  if (valid) ek = exact_key(q, j, exact_d2(X, d, q, j));
+  if (valid) ek = exact_key(qo, perm ? static_cast<int64_t>(perm[j]) : j, exact_d2(X, d, q, j));
   ek = wave_sort_u64(ek);
 
   const uint64_t kth = shfl_u64(ek, k - 1);
@@ -925,9 +977,9 @@ __global__ __launch_bounds__(kThreads) void finalize_kernel(
     if (l < k) {
       int64_t jj;
       float d2;
-      decode_exact(ek, q, &jj, &d2);
-      out_idx[q * k + l] = static_cast<int32_t>(jj);
-      out_dist[q * k + l] = sqrtf(d2);
+      decode_exact(ek, qo, &jj, &d2);
+      out_idx[qo * k + l] = static_cast<int32_t>(jj);
+      out_dist[qo * k + l] = sqrtf(d2);
     }
     if (l == 0) flags[q] = 0;
   } else if (l == 0) {
@@ -938,7 +990,8 @@ __global__ __launch_bounds__(kThreads) void finalize_kernel(
 // The 64 smallest exact keys of row q over candidates [cb, ce), ascending in lane order (one
 // wave): chunks of 64 candidates merged in candidate order, four chunks' rows loaded at once.
 __device__ __forceinline__ uint64_t brute_top64(const float* __restrict__ X, int d, int64_t q, int64_t cb,
-                                                int64_t ce) {
+                                                int64_t ce, const int32_t* __restrict__ perm = nullptr) {
+  const int64_t qo = perm ? static_cast<int64_t>(perm[q]) : q;
   const int l = lane_id();
   uint64_t cur = kEmpty, tail = kEmpty;
   for (int64_t base = cb; base < ce; base += 4 * kWave) {
@@ -954,7 +1007,7 @@ __device__ __forceinline__ uint64_t brute_top64(const float* __restrict__ X, int
     exact_d2_x4(X, d, q, j, d2);
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      const uint64_t ek = v[u] ? exact_key(q, j[u], d2[u]) : kEmpty;
+      const uint64_t ek = v[u] ? exact_key(qo, perm ? static_cast<int64_t>(perm[j[u]]) : j[u], d2[u]) : kEmpty;
       if (__any(ek < tail)) {
         cur = wave_merge_min_u64(cur, wave_sort_u64(ek));
         tail = shfl_u64(cur, kWave - 1);
@@ -966,19 +1019,57 @@ __device__ __forceinline__ uint64_t brute_top64(const float* __restrict__ X, int
 
 __global__ __launch_bounds__(kThreads) void fallback_kernel(
     const float* __restrict__ X, int64_t n, int d, int k, const int32_t* __restrict__ flags,
-    const int64_t* __restrict__ row_range, int32_t* __restrict__ out_idx, float* __restrict__ out_dist) {
+    const int64_t* __restrict__ row_range, const int32_t* __restrict__ perm, int32_t* __restrict__ out_idx,
+    float* __restrict__ out_dist) {
   const int l = lane_id();
   const int64_t q = static_cast<int64_t>(blockIdx.x) * (kThreads / kWave) + threadIdx.x / kWave;
   if (q >= n || flags[q] == 0) return;
   const int64_t cb = row_range ? row_range[2 * q] : 0;
   const int64_t ce = row_range ? row_range[2 * q + 1] : n;
-  const uint64_t cur = brute_top64(X, d, q, cb, ce);
+  const uint64_t cur = brute_top64(X, d, q, cb, ce, perm);
+  const int64_t qo = perm ? static_cast<int64_t>(perm[q]) : q;
   if (l < k) {
     int64_t jj;
     float d2;
-    decode_exact(cur, q, &jj, &d2);
-    out_idx[q * k + l] = static_cast<int32_t>(jj);
-    out_dist[q * k + l] = sqrtf(d2);
+    decode_exact(cur, qo, &jj, &d2);
+    out_idx[qo * k + l] = static_cast<int32_t>(jj);
+    out_dist[qo * k + l] = sqrtf(d2);
+  }
+}
+
+// Fallback for the few rows no pass certifies, split across waves: wave (r, s) brute-forces
+// chunk s of row rows[r]'s candidates into a sorted list of 64 exact keys; the merge kernel then
+// keeps the 64 smallest over the chunks (lists sorted ascending, merged in chunk order: the
+// same keys as one scan of all candidates, so the same result).
+__global__ __launch_bounds__(kThreads) void fallback_chunk_kernel(
+    const float* __restrict__ X, int64_t n, int d, const int32_t* __restrict__ rows, int64_t m, int S,
+    int64_t chunk, const int32_t* __restrict__ perm, uint64_t* __restrict__ lists) {
+  const int l = lane_id();
+  const int64_t w = static_cast<int64_t>(blockIdx.x) * (kThreads / kWave) + threadIdx.x / kWave;
+  if (w >= m * S) return;
+  const int64_t r = w / S;
+  const int s = static_cast<int>(w % S);
+  const int64_t cb = static_cast<int64_t>(s) * chunk;
+  const int64_t ce = min(n, cb + chunk);
+  lists[w * kWave + l] = cb < ce ? brute_top64(X, d, rows[r], cb, ce, perm) : kEmpty;
+}
+
+__global__ __launch_bounds__(kThreads) void fallback_merge_kernel(
+    const int32_t* __restrict__ rows, int64_t m, int S, int k, const int32_t* __restrict__ perm,
+    const uint64_t* __restrict__ lists, int32_t* __restrict__ out_idx, float* __restrict__ out_dist) {
+  const int l = lane_id();
+  const int64_t r = static_cast<int64_t>(blockIdx.x) * (kThreads / kWave) + threadIdx.x / kWave;
+  if (r >= m) return;
+  uint64_t cur = lists[r * S * kWave + l];
+  for (int s = 1; s < S; ++s) cur = wave_merge_min_u64(cur, lists[(r * S + s) * kWave + l]);
+  const int64_t q = rows[r];
+  const int64_t qo = perm ? static_cast<int64_t>(perm[q]) : q;
+  if (l < k) {
+    int64_t jj;
+    float d2;
+    decode_exact(cur, qo, &jj, &d2);
+    out_idx[qo * k + l] = static_cast<int32_t>(jj);
+    out_dist[qo * k + l] = sqrtf(d2);
   }
 }
 
@@ -1048,6 +1139,8 @@ __global__ void k_pad_rows(int32_t* __restrict__ rows, int64_t m, int64_t m_pad)
 
 struct Plan {
   int64_t n, n_pad, split_len;
+  int64_t n_ops;  // operand rows: n_pad plus one tile of padding, so that a tile starting at any
+                  // row below n stays inside the array
   int d, k, S, precision;
   int ks_bf16, ks_f32;
 };
@@ -1059,6 +1152,7 @@ static Plan make_plan(int64_t n, int d, int k, int precision) {
   p.k = k;
   p.precision = precision;
   p.n_pad = ceil_div(n, kQueriesPerBlock) * kQueriesPerBlock;
+  p.n_ops = p.n_pad + kTile;
   p.ks_bf16 = bf16_ks(d);
   p.ks_f32 = f32_ks(d);
   const int64_t qblocks = p.n_pad / kQueriesPerBlock;
@@ -1105,14 +1199,14 @@ static Buffers carve(Carve& cv, const Plan& p) {
   const bool bf = p.precision == MSC_PRECISION_BF16_RERANK;
   b.partial = cv.take<double>(static_cast<size_t>(kColBlocks) * p.d);
   b.mean = cv.take<double>(p.d);
-  b.Xb = bf ? cv.take<char>(static_cast<size_t>(p.n_pad) * 64 * p.ks_bf16) : nullptr;
-  b.Xf = bf ? nullptr : cv.take<char>(static_cast<size_t>(p.n_pad) * 32 * p.ks_f32);
-  b.nrm2b = cv.take<float>(p.n_pad);
-  b.resb = cv.take<float>(p.n_pad);
-  b.qn2b = cv.take<float>(p.n_pad);
-  b.qresb = cv.take<float>(p.n_pad);
-  b.nrm2f = cv.take<float>(p.n_pad);
-  b.resf = cv.take<float>(p.n_pad);
+  b.Xb = bf ? cv.take<char>(static_cast<size_t>(p.n_ops) * 64 * p.ks_bf16) : nullptr;
+  b.Xf = bf ? nullptr : cv.take<char>(static_cast<size_t>(p.n_ops) * 32 * p.ks_f32);
+  b.nrm2b = cv.take<float>(p.n_ops);
+  b.resb = cv.take<float>(p.n_ops);
+  b.qn2b = cv.take<float>(p.n_ops);
+  b.qresb = cv.take<float>(p.n_ops);
+  b.nrm2f = cv.take<float>(p.n_ops);
+  b.resf = cv.take<float>(p.n_ops);
   b.gstatb = cv.take<float>(4);
   b.gstatf = cv.take<float>(4);
   // the wide second pass over m rows writes 2 * S2 * m_pad <= second_pass_slots(n_pad) slots
@@ -1130,19 +1224,21 @@ static Buffers carve(Carve& cv, const Plan& p) {
 
 template <bool BF16, int KS, bool WIDE, int L>
 static void launch_select_t(const void* Xp, int64_t n, int64_t n_slots, int d, int64_t split_len, int S,
-                            const int32_t* qrows, const int64_t* blk_range, uint64_t* keys, float* taus,
-                            hipStream_t st) {
+                            const int32_t* qrows, const int64_t* blk_range, const int64_t* rng4, uint64_t* keys,
+                            float* taus, hipStream_t st) {
   const int64_t qpb = WIDE ? kQueriesPerBlock / 2 : kQueriesPerBlock;
   dim3 grid(static_cast<unsigned>(n_slots / qpb), static_cast<unsigned>(S));
-  select_kernel<BF16, KS, L, WIDE><<<grid, kThreads, 0, st>>>(Xp, n, n_slots, d, split_len, qrows, blk_range, keys,
-                                                              taus);
+  select_kernel<BF16, KS, L, WIDE><<<grid, kThreads, 0, st>>>(Xp, n, n_slots, d, split_len, qrows, blk_range, rng4,
+                                                              keys, taus);
 }
 
 template <int KT>
 static void launch_select32_t(const void* Xp, int64_t n, int64_t n_slots, int d, int rowb, int64_t split_len, int S,
-                              const int64_t* blk_range, uint64_t* keys, float* taus, hipStream_t st) {
+                              const int32_t* qrows, const int64_t* blk_range, const int64_t* rng4, uint64_t* keys,
+                              float* taus, hipStream_t st) {
   dim3 grid(static_cast<unsigned>(ceil_div(n_slots, kSelQueries)), static_cast<unsigned>(S));
-  select32_kernel<KT><<<grid, kSelThreads, 0, st>>>(Xp, n, n_slots, d, rowb, split_len, blk_range, keys, taus);
+  select32_kernel<KT><<<grid, kSelThreads, 0, st>>>(Xp, n, n_slots, d, rowb, split_len, qrows, blk_range, rng4, keys,
+                                                     taus);
 }
 
 // K-steps of 16 of the 32x32x16 form: the 2d + 3 operand terms, rounded up
@@ -1153,12 +1249,13 @@ static inline int bf16_kt(int d) { return (2 * d + 3 + 15) / 16; }
 // where many short candidate ranges would otherwise spend their time filling long lists.
 static int launch_select(bool bf16, bool wide, int ks, const void* Xp, int64_t n, int64_t n_slots, int d,
                          int64_t split_len, int S, const int32_t* qrows, const int64_t* blk_range, uint64_t* keys,
-                         float* taus, hipStream_t st, bool short_list = false) {
-  // the BF16 main pass (no row list, 64-key lists) runs on the 32x32x16 form up to 8 K-steps
-  if (bf16 && !wide && !short_list && qrows == nullptr && bf16_kt(d) <= 8) {
+                         float* taus, hipStream_t st, bool short_list = false, const int64_t* rng4 = nullptr) {
+  // the BF16 main pass (64-key lists) runs on the 32x32x16 form up to 8 K-steps
+  if (bf16 && !wide && !short_list && bf16_kt(d) <= 8) {
     const int rowb = 64 * ks;
     switch (bf16_kt(d)) {
-#define MSC_SEL32(KT) launch_select32_t<KT>(Xp, n, n_slots, d, rowb, split_len, S, blk_range, keys, taus, st)
+#define MSC_SEL32(KT) \
+  launch_select32_t<KT>(Xp, n, n_slots, d, rowb, split_len, S, qrows, blk_range, rng4, keys, taus, st)
       case 1: MSC_SEL32(1); break;
       case 2: MSC_SEL32(2); break;
       case 3: MSC_SEL32(3); break;
@@ -1173,11 +1270,12 @@ static int launch_select(bool bf16, bool wide, int ks, const void* Xp, int64_t n
     return kOk;
   }
 #define MSC_SEL(B, K)                                                                                             \
-  (wide ? launch_select_t<B, K, true, kL>(Xp, n, n_slots, d, split_len, S, qrows, blk_range, keys, taus, st)     \
-        : (short_list                                                                                             \
-               ? launch_select_t<B, K, false, kLShort>(Xp, n, n_slots, d, split_len, S, qrows, blk_range, keys,   \
-                                                       taus, st)                                                  \
-               : launch_select_t<B, K, false, kL>(Xp, n, n_slots, d, split_len, S, qrows, blk_range, keys, taus, st)))
+  (wide ? launch_select_t<B, K, true, kL>(Xp, n, n_slots, d, split_len, S, qrows, blk_range, rng4, keys, taus, st) \
+        : (short_list                                                                                               \
+               ? launch_select_t<B, K, false, kLShort>(Xp, n, n_slots, d, split_len, S, qrows, blk_range, rng4,     \
+                                                       keys, taus, st)                                              \
+               : launch_select_t<B, K, false, kL>(Xp, n, n_slots, d, split_len, S, qrows, blk_range, rng4, keys,    \
+                                                  taus, st)))
   if (bf16) {
     switch (ks) {
       case 1: MSC_SEL(true, 1); break;
@@ -1273,15 +1371,15 @@ static Operands prepare(const Plan& p, const Buffers& b, const float* X, int64_t
   const int64_t rows_per = ceil_div(n, kColBlocks);
   colsum_partial<<<kColBlocks, 64, 0, st>>>(X, n, d, rows_per, b.partial);
   colmean_final<<<1, 64, 0, st>>>(b.partial, kColBlocks, d, n, b.mean);
-  const unsigned pb = static_cast<unsigned>(ceil_div(p.n_pad, kThreads));
+  const unsigned pb = static_cast<unsigned>(ceil_div(p.n_ops, kThreads));
   Operands op{};
   if (bf16) {
-    prep_bf16<<<pb, kThreads, 0, st>>>(X, n, p.n_pad, d, 32 * p.ks_bf16, b.mean, static_cast<uint16_t*>(b.Xb),
+    prep_bf16<<<pb, kThreads, 0, st>>>(X, n, p.n_ops, d, 32 * p.ks_bf16, b.mean, static_cast<uint16_t*>(b.Xb),
                                        b.nrm2b, b.resb, b.qn2b, b.qresb);
     point_max<<<1, kThreads, 0, st>>>(b.nrm2b, b.resb, n, b.gstatb);
     op = Operands{b.Xb, b.qn2b, b.qresb, b.gstatb, p.ks_bf16, make_bound(true, p.ks_bf16)};
   } else {
-    prep_f32<<<pb, kThreads, 0, st>>>(X, n, p.n_pad, d, 8 * p.ks_f32, b.mean, static_cast<float*>(b.Xf), b.nrm2f,
+    prep_f32<<<pb, kThreads, 0, st>>>(X, n, p.n_ops, d, 8 * p.ks_f32, b.mean, static_cast<float*>(b.Xf), b.nrm2f,
                                       b.resf);
     point_max<<<1, kThreads, 0, st>>>(b.nrm2f, b.resf, n, b.gstatf);
     op = Operands{b.Xf, b.nrm2f, b.resf, b.gstatf, p.ks_f32, make_bound(false, p.ks_f32)};
@@ -1295,6 +1393,19 @@ static Operands prepare(const Plan& p, const Buffers& b, const float* X, int64_t
 using namespace msc;
 using namespace msc::knn;
 
+namespace msc {
+namespace knn {
+// The partition-pruned candidate ranges (part.h) apply to the MFMA select shapes with d <= 64;
+// they are used from kPruneMinRows rows on unless msc_knn_info.prune asks otherwise.
+constexpr int64_t kPruneMinRows = 65536;
+
+static bool prune_shape(int64_t n, int d, int k, int precision) {
+  return select_shape(d, k, precision) && d <= part::kDMax && n >= 2 * kQueriesPerBlock;
+}
+
+}  // namespace knn
+}  // namespace msc
+
 extern "C" int msc_knn_workspace_size(int64_t n, int d, int k, int precision, size_t* bytes) {
   MSC_TRY(validate(n, d, k, precision));
   MSC_REQUIRE(bytes != nullptr, "knn: bytes is null");
@@ -1305,6 +1416,7 @@ extern "C" int msc_knn_workspace_size(int64_t n, int d, int k, int precision, si
   Plan p = make_plan(n, d, k, precision);
   Carve cv(nullptr, 0);
   carve(cv, p);
+  if (prune_shape(n, d, k, precision)) cv.take<char>(part::workspace(n, d));
   *bytes = cv.used + 256;
   return kOk;
 }
@@ -1314,7 +1426,15 @@ extern "C" int msc_knn_exact(const float* X, int64_t n, int d, int k, int precis
                              size_t workspace_bytes, void* stream, msc_knn_info* info) {
   MSC_TRY(validate(n, d, k, precision));
   MSC_REQUIRE(X && out_idx && out_dist && workspace, "knn: null pointer argument");
+  const int mode = info ? info->prune : MSC_PRUNE_AUTO;
+  MSC_REQUIRE(mode == MSC_PRUNE_AUTO || mode == MSC_PRUNE_OFF || mode == MSC_PRUNE_ON, "knn: unknown prune mode %d",
+              mode);
   hipStream_t st = static_cast<hipStream_t>(stream);
+  if (info) {
+    info->n_partitions = info->n_groups = info->n_ranges = 0;
+    info->scanned = 1.0;
+    info->partition_ms = 0.0;
+  }
   if (!select_shape(d, k, precision)) {
     // every candidate's exact distance (both precisions: the result is the same exact kNN)
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -1345,8 +1465,15 @@ extern "C" int msc_knn_exact(const float* X, int64_t n, int d, int k, int precis
     return kOk;
   }
   Plan p = make_plan(n, d, k, precision);
+  const bool prune_ok = prune_shape(n, d, k, precision);
+  MSC_REQUIRE(mode != MSC_PRUNE_ON || prune_ok,
+              "knn: the pruned candidate ranges need d <= %d, k <= 64 and n >= %d (d=%d, k=%d, n=%lld)", part::kDMax,
+              2 * kQueriesPerBlock, d, k, (long long)n);
+  const bool pruned = prune_ok && (mode == MSC_PRUNE_ON || (mode == MSC_PRUNE_AUTO && n >= kPruneMinRows));
   Carve cv(workspace, workspace_bytes);
   Buffers b = carve(cv, p);
+  const size_t pbytes = prune_ok ? part::workspace(n, d) : 0;
+  char* pws = prune_ok ? cv.take<char>(pbytes) : nullptr;
   if (!cv.ok()) {
     set_error("knn: workspace too small (%zu < %zu)", workspace_bytes, cv.used);
     return kWorkspaceTooSmall;
@@ -1358,21 +1485,45 @@ extern "C" int msc_knn_exact(const float* X, int64_t n, int d, int k, int precis
     for (auto& e : ev) MSC_HIP(hipEventCreate(&e));
   if (timing) MSC_HIP(hipEventRecord(ev[0], st));
 
-  Operands op = prepare(p, b, X, n, bf16, st);
-  MSC_LAUNCH_CHECK();
+  // Rows in select order: X itself, or its rows in partition order (Xw[r] = X[perm[r]]) with
+  // every query block given its candidate ranges.
+  part::Layout L{};
+  const float* Xw = X;
+  const int32_t* perm = nullptr;
+  if (pruned) {
+    MSC_TRY(part::build(X, n, d, k, pws, pbytes, &L, st));
+    Xw = L.Xs;
+    perm = L.perm;
+  }
   if (timing) MSC_HIP(hipEventRecord(ev[1], st));
 
-  MSC_TRY(launch_select(bf16, false, op.ks, op.X, n, p.n_pad, d, p.split_len, p.S, nullptr, nullptr, b.keys, b.taus,
-                        st));
+  Operands op = prepare(p, b, Xw, n, bf16, st);
+  MSC_LAUNCH_CHECK();
   if (timing) MSC_HIP(hipEventRecord(ev[2], st));
 
-  const unsigned wb = static_cast<unsigned>(ceil_div(n, kThreads / kWave));
-  finalize_kernel<<<wb, kThreads, 0, st>>>(X, n, d, k, p.S, p.n_pad, nullptr, b.keys, b.taus, op.qn2, op.qres,
-                                           op.gstat, op.bnd, nullptr, out_idx, out_dist, b.flags);
-  MSC_LAUNCH_CHECK();
+  if (pruned) {
+    MSC_TRY(launch_select(bf16, false, op.ks, op.X, n, L.n_slots, d, n, 1, L.qrows, nullptr, b.keys, b.taus, st,
+                          false, L.blk_range));
+  } else {
+    MSC_TRY(launch_select(bf16, false, op.ks, op.X, n, p.n_pad, d, p.split_len, p.S, nullptr, nullptr, b.keys,
+                          b.taus, st));
+  }
   if (timing) MSC_HIP(hipEventRecord(ev[3], st));
 
-  // second pass: rows the certificate could not clear are selected again with the wide list
+  const unsigned wb = static_cast<unsigned>(ceil_div(n, kThreads / kWave));
+  if (pruned) {
+    finalize_kernel<<<static_cast<unsigned>(ceil_div(L.n_slots, kThreads / kWave)), kThreads, 0, st>>>(
+        Xw, L.n_slots, d, k, 1, L.n_slots, L.qrows, b.keys, b.taus, op.qn2, op.qres, op.gstat, op.bnd, nullptr, perm,
+        out_idx, out_dist, b.flags);
+  } else {
+    finalize_kernel<<<wb, kThreads, 0, st>>>(X, n, d, k, p.S, p.n_pad, nullptr, b.keys, b.taus, op.qn2, op.qres,
+                                             op.gstat, op.bnd, nullptr, nullptr, out_idx, out_dist, b.flags);
+  }
+  MSC_LAUNCH_CHECK();
+  if (timing) MSC_HIP(hipEventRecord(ev[4], st));
+
+  // second pass: rows the certificate could not clear are selected again, over all candidates,
+  // with the wide list
   int64_t m1 = 0, m2 = 0;
   MSC_TRY(compact_flags(p, b, &m1, st));
   if (m1 > 0) {
@@ -1388,30 +1539,48 @@ extern "C" int msc_knn_exact(const float* X, int64_t n, int d, int k, int precis
     MSC_TRY(launch_select(bf16, true, op.ks, op.X, n, m_pad, d, split2, static_cast<int>(S2), b.rows, nullptr, b.keys,
                           b.taus, st));
     finalize_kernel<<<static_cast<unsigned>(ceil_div(m1, kThreads / kWave)), kThreads, 0, st>>>(
-        X, m1, d, k, static_cast<int>(2 * S2), m_pad, b.rows, b.keys, b.taus, op.qn2, op.qres, op.gstat, op.bnd,
-        nullptr, out_idx, out_dist, b.flags);
-    MSC_LAUNCH_CHECK();
-  }
-  if (timing) MSC_HIP(hipEventRecord(ev[4], st));
-  if (m1 > 0) {
-    fallback_kernel<<<wb, kThreads, 0, st>>>(X, n, d, k, b.flags, nullptr, out_idx, out_dist);
+        Xw, m1, d, k, static_cast<int>(2 * S2), m_pad, b.rows, b.keys, b.taus, op.qn2, op.qres, op.gstat, op.bnd,
+        nullptr, perm, out_idx, out_dist, b.flags);
     MSC_LAUNCH_CHECK();
   }
   if (timing) MSC_HIP(hipEventRecord(ev[5], st));
+  if (m1 > 0) {
+    // rows still uncertified: each row's candidates in S chunks of >= 16,384 rows, one wave per
+    // (row, chunk), then a merge per row
+    MSC_TRY(compact_flags(p, b, &m2, st));
+    if (m2 > 0) {
+      const int64_t slots = static_cast<int64_t>(second_pass_slots(p.n_pad));
+      int64_t S3 = std::max<int64_t>(1, std::min<int64_t>({ceil_div(n, 16384), 256, slots / m2}));
+      const int64_t chunk = ceil_div(n, S3);
+      S3 = ceil_div(n, chunk);
+      fallback_chunk_kernel<<<static_cast<unsigned>(ceil_div(m2 * S3, kThreads / kWave)), kThreads, 0, st>>>(
+          Xw, n, d, b.rows, m2, static_cast<int>(S3), chunk, perm, b.keys);
+      fallback_merge_kernel<<<static_cast<unsigned>(ceil_div(m2, kThreads / kWave)), kThreads, 0, st>>>(
+          b.rows, m2, static_cast<int>(S3), k, perm, b.keys, out_idx, out_dist);
+      MSC_LAUNCH_CHECK();
+    }
+  }
+  if (timing) MSC_HIP(hipEventRecord(ev[6], st));
   if (info) {
-    if (m1 > 0) MSC_TRY(compact_flags(p, b, &m2, st));
     info->n_fallback = m2;
     info->n_second_pass = m1;
-    info->n_splits = p.S;
+    info->n_splits = pruned ? 1 : p.S;
     info->list_len = 4 * kL;
+    if (pruned) {
+      info->n_partitions = L.n_parts;
+      info->n_groups = L.n_groups;
+      info->n_ranges = L.S;
+      info->scanned = L.pairs / (static_cast<double>(n) * static_cast<double>(n));
+    }
     if (timing) {
-      MSC_HIP(hipEventSynchronize(ev[5]));
+      MSC_HIP(hipEventSynchronize(ev[6]));
       float ms;
-      MSC_HIP(hipEventElapsedTime(&ms, ev[0], ev[1])); info->prep_ms = ms;
-      MSC_HIP(hipEventElapsedTime(&ms, ev[1], ev[2])); info->select_ms = ms;
-      MSC_HIP(hipEventElapsedTime(&ms, ev[2], ev[3])); info->finalize_ms = ms;
-      MSC_HIP(hipEventElapsedTime(&ms, ev[3], ev[4])); info->second_pass_ms = ms;
-      MSC_HIP(hipEventElapsedTime(&ms, ev[4], ev[5])); info->fallback_ms = ms;
+      MSC_HIP(hipEventElapsedTime(&ms, ev[0], ev[1])); info->partition_ms = ms;
+      MSC_HIP(hipEventElapsedTime(&ms, ev[1], ev[2])); info->prep_ms = ms;
+      MSC_HIP(hipEventElapsedTime(&ms, ev[2], ev[3])); info->select_ms = ms;
+      MSC_HIP(hipEventElapsedTime(&ms, ev[3], ev[4])); info->finalize_ms = ms;
+      MSC_HIP(hipEventElapsedTime(&ms, ev[4], ev[5])); info->second_pass_ms = ms;
+      MSC_HIP(hipEventElapsedTime(&ms, ev[5], ev[6])); info->fallback_ms = ms;
     }
   }
   if (timing)
@@ -1810,9 +1979,9 @@ extern "C" int msc_knn_batched_launch(const float* X, const int64_t* offsets, in
                         list_len == kLShort));
   const unsigned wb = static_cast<unsigned>(ceil_div(n, kThreads / kWave));
   finalize_kernel<<<wb, kThreads, 0, st>>>(t.Xpad, n, d, k, L.S, n, nullptr, b.keys, b.taus, op.qn2, op.qres,
-                                           op.gstat, op.bnd, t.row_range, t.idx_pad, t.dist_pad, b.flags);
+                                           op.gstat, op.bnd, t.row_range, nullptr, t.idx_pad, t.dist_pad, b.flags);
   MSC_LAUNCH_CHECK();
-  fallback_kernel<<<wb, kThreads, 0, st>>>(t.Xpad, n, d, k, b.flags, t.row_range, t.idx_pad, t.dist_pad);
+  fallback_kernel<<<wb, kThreads, 0, st>>>(t.Xpad, n, d, k, b.flags, t.row_range, nullptr, t.idx_pad, t.dist_pad);
   k_unpad<<<static_cast<unsigned>(ceil_div(n * k, kThreads)), kThreads, 0, st>>>(
       t.idx_pad, t.dist_pad, t.prow2row, t.row_range, t.in_off, t.prow_sample, n, k, out_idx, out_dist);
   MSC_LAUNCH_CHECK();

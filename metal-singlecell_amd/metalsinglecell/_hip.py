@@ -47,7 +47,11 @@ class KnnInfo(ctypes.Structure):
     _fields_ = [("timing", c_int), ("n_splits", c_int), ("list_len", c_int),
                 ("n_second_pass", c_i64), ("n_fallback", c_i64), ("prep_ms", c_dbl),
                 ("select_ms", c_dbl), ("finalize_ms", c_dbl), ("second_pass_ms", c_dbl),
-                ("fallback_ms", c_dbl)]
+                ("fallback_ms", c_dbl), ("prune", c_int), ("n_partitions", c_int), ("n_groups", c_int),
+                ("n_ranges", c_int), ("scanned", c_dbl), ("partition_ms", c_dbl)]
+
+
+PRUNE = {None: 0, "auto": 0, False: 1, "off": 1, True: 2, "on": 2}
 
 
 # name -> (restype, argtypes); the declarations of include/msc_hip.h
@@ -173,8 +177,12 @@ def workspace(nbytes):
 # ----------------------------------------------------------------------------------------
 # (a) exact kNN
 # ----------------------------------------------------------------------------------------
-def knn_exact(X, k, precision="fp32", timing=False, return_info=False):
-    """Exact self-inclusive kNN on the device: ``(idx int32 (n,k), dist float32 (n,k))``."""
+def knn_exact(X, k, precision="fp32", timing=False, return_info=False, prune=None):
+    """Exact self-inclusive kNN on the device: ``(idx int32 (n,k), dist float32 (n,k))``.
+
+    ``prune``: None / "auto" (partition-pruned candidate ranges from 65,536 rows on), False /
+    "off" (every query scans every candidate) or True / "on"; the result is the same either way.
+    """
     import torch
     lib = load_library()
     Xd = to_device(X, torch.float32)
@@ -189,8 +197,9 @@ def knn_exact(X, k, precision="fp32", timing=False, return_info=False):
     dist = torch.empty((n, k), dtype=torch.float32, device=Xd.device)
     info = KnnInfo()
     info.timing = int(bool(timing))
+    info.prune = PRUNE[prune]
     check(lib.msc_knn_exact(ptr(Xd), n, d, int(k), p, ptr(idx), ptr(dist), ptr(ws),
-                            ws.numel(), stream_ptr(), ctypes.byref(info) if (timing or return_info) else None))
+                            ws.numel(), stream_ptr(), ctypes.byref(info)))
     if return_info or timing:
         return idx, dist, {f: getattr(info, f) for f, _ in KnnInfo._fields_}
     return idx, dist

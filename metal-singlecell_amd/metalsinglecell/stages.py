@@ -7,8 +7,9 @@ algorithmic bytes (or flops) per call by the formulas below, and the achieved ra
 the MI355X peaks (8 TB/s HBM, 2.5 PFLOP/s dense BF16 MFMA).
 
 Stages (sizes are the workloads' real sizes):
-  select-c3   exact kNN of 1M x 50 (200 Zipf(1.5) clusters, seed 2), k = 30, BF16 + re-rank;
-              flops = 2 n^2 d for the select kernel (its own event time)
+  select-c3   exact kNN of 1M x 50 (200 Zipf(1.5) clusters, seed 2), k = 30, BF16 + re-rank,
+              with the partition-pruned candidate ranges and with the full scan; flops = 2 d per
+              scanned (query, candidate) pair for the select kernel (its own event time)
   conn-c3     UMAP connectivities of that kNN; bytes = 8 n k + 8 E + 8 (n + 1)
   leiden-c3   Leiden (res 1.0, seed 0, 2 iterations) on those connectivities;
               bytes = sum over levels of (passes (8 E_l + 24 n_l) + 12 (E_l + E_l+1))
@@ -62,15 +63,26 @@ def stage_select_c3(reps):
     from . import _hip
     Xd = _c3()
     n, d = Xd.shape
-    infos = []
-    for _ in range(reps + 1):
-        _, _, info = _hip.knn_exact(Xd, 30, precision="bf16", timing=True)
-        infos.append(info)
-    sel = float(np.mean([i["select_ms"] for i in infos[1:]]))
-    flops = 2.0 * n * n * d
-    return {"ms": sel, "flops": flops, "tflops": flops / sel / 1e9, "frac": flops / (sel * 1e-3) / BF16_PEAK,
+    out = {}
+    for mode in ("auto", "off"):
+        infos = []
+        for _ in range(reps + 1):
+            _, _, info = _hip.knn_exact(Xd, 30, precision="bf16", timing=True, prune=mode)
+            infos.append(info)
+        sel = float(np.mean([i["select_ms"] for i in infos[1:]]))
+        scanned = float(infos[-1]["scanned"])
+        flops = 2.0 * scanned * n * n * d
+        out["pruned" if mode == "auto" else "full_scan"] = {
+            "ms": sel, "flops": flops, "tflops": flops / sel / 1e9, "frac": flops / (sel * 1e-3) / BF16_PEAK,
+            "scanned": scanned, "brute_force_equivalent_tflops": 2.0 * n * n * d / sel / 1e9,
+            "partition_ms": float(np.mean([i["partition_ms"] for i in infos[1:]])),
+            "prep_ms": float(np.mean([i["prep_ms"] for i in infos[1:]])),
+            "finalize_ms": float(np.mean([i["finalize_ms"] for i in infos[1:]])),
             "second_pass_ms": float(np.mean([i["second_pass_ms"] for i in infos[1:]])),
-            "n_second_pass": infos[-1]["n_second_pass"]}
+            "n_second_pass": infos[-1]["n_second_pass"], "n_fallback": infos[-1]["n_fallback"],
+            "n_partitions": infos[-1]["n_partitions"], "n_groups": infos[-1]["n_groups"],
+            "n_ranges": infos[-1]["n_ranges"]}
+    return out
 
 
 def _c3_graph():
