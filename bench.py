@@ -4,7 +4,8 @@ Workload (default): 1,000,000 cells x 50 PCs float32, 200-cluster Gaussian mixtu
 Zipf(1.5) cluster sizes (seed 2), k = 30, Leiden resolution 1.0, seed 0, n_iterations 2.
 One step = one full pass from the device-resident embedding to device labels:
 
-    exact kNN (BF16 MFMA ranking, 64 admitted candidates, exact FP32 re-rank)
+    exact kNN (partition-pruned candidate ranges, BF16 MFMA ranking, 64 admitted candidates,
+    exact FP32 re-rank)
     -> UMAP fuzzy connectivities -> synchronous parallel Leiden.
 
 ``value`` = cells / (mean step time) for that BF16+re-rank form.  The plain FP32 GPU form
@@ -600,7 +601,8 @@ def main():
         "ms_per_step": ms_bf * 1e3, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "bf16", "data": "synthetic",
         "config": {"workload": f"gaussian_mixture_zipf n={n} d={d} k={a.k} k'=64 "
-                               "BF16 MFMA + exact FP32 re-rank; Leiden res=1.0 seed=0 n_iter=2",
+                               "BF16 MFMA + exact FP32 re-rank over partition-pruned candidate ranges; "
+                               "Leiden res=1.0 seed=0 n_iter=2",
                    "cells": n, "dims": d, "k": a.k, "parallelism": "single GPU"},
         "stages_s": {"knn": last["knn_s"], "connectivities": last["conn_s"], "leiden": last["leiden_s"]},
         "knn_info": last["knn"], "modularity": last["q"], "n_clusters": last["n_clusters"],

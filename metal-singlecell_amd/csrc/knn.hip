@@ -253,43 +253,44 @@ struct SelCfg {
   static constexpr int NB_FRAG = BF16 ? KS : KS;             // B fragments per N-block
 };
 
-// a block-uniform 64-bit value, declared uniform (kept in scalar registers)
-__device__ __forceinline__ int64_t uniform_i64(int64_t v) {
-  const uint32_t lo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(v));
-  const uint32_t hi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(static_cast<uint64_t>(v) >> 32));
-  return static_cast<int64_t>((static_cast<uint64_t>(hi) << 32) | lo);
+// a block-uniform value, declared uniform (kept in a scalar register)
+__device__ __forceinline__ int32_t uniform_i32(int64_t v) {
+  return static_cast<int32_t>(__builtin_amdgcn_readfirstlane(static_cast<uint32_t>(v)));
 }
 
 // Candidate tiles of one block: one range [c_begin, c_end) (a split of [0, n), or a sample's
 // own rows), or -- with rng4 -- up to four row ranges of the partition-pruned layout
 // (rng4[8 b + 2 r], rng4[8 b + 2 r + 1] = begin, end of range r of block b), walked in order as
 // one stream of tiles.  Tile T starts at row MSC_TROW(T); rows of its range end at MSC_TEND(T).
-// Everything is block-uniform.
+// Everything is block-uniform and 32-bit (row ids and tile counts stay below 2^31), so the
+// per-tile arithmetic runs on the scalar unit.
 #define MSC_TILE_MAP(RNG4, BLK, CB, CE)                                                              \
   const int64_t* tm_r = (RNG4) ? (RNG4) + static_cast<int64_t>(BLK) * 8 : nullptr;                  \
-  const int64_t tm_b0 = tm_r ? tm_r[0] : (CB), tm_e0 = tm_r ? tm_r[1] : (CE);                       \
-  const int64_t tm_b1 = tm_r ? tm_r[2] : 0, tm_e1 = tm_r ? tm_r[3] : 0;                             \
-  const int64_t tm_b2 = tm_r ? tm_r[4] : 0, tm_e2 = tm_r ? tm_r[5] : 0;                             \
-  const int64_t tm_b3 = tm_r ? tm_r[6] : 0, tm_e3 = tm_r ? tm_r[7] : 0;                             \
-  const int64_t tm_t1 = tm_e0 > tm_b0 ? ceil_div(tm_e0 - tm_b0, kTile) : 0;                         \
-  const int64_t tm_t2 = tm_t1 + (tm_e1 > tm_b1 ? ceil_div(tm_e1 - tm_b1, kTile) : 0);               \
-  const int64_t tm_t3 = tm_t2 + (tm_e2 > tm_b2 ? ceil_div(tm_e2 - tm_b2, kTile) : 0);               \
-  const int64_t tm_n = tm_t3 + (tm_e3 > tm_b3 ? ceil_div(tm_e3 - tm_b3, kTile) : 0);                \
+  const int32_t tm_b0 = static_cast<int32_t>(tm_r ? tm_r[0] : (CB));                               \
+  const int32_t tm_e0 = static_cast<int32_t>(tm_r ? tm_r[1] : (CE));                               \
+  const int32_t tm_b1 = static_cast<int32_t>(tm_r ? tm_r[2] : 0), tm_e1 = static_cast<int32_t>(tm_r ? tm_r[3] : 0); \
+  const int32_t tm_b2 = static_cast<int32_t>(tm_r ? tm_r[4] : 0), tm_e2 = static_cast<int32_t>(tm_r ? tm_r[5] : 0); \
+  const int32_t tm_b3 = static_cast<int32_t>(tm_r ? tm_r[6] : 0), tm_e3 = static_cast<int32_t>(tm_r ? tm_r[7] : 0); \
+  const int32_t tm_t1 = tm_e0 > tm_b0 ? (tm_e0 - tm_b0 + kTile - 1) / kTile : 0;                    \
+  const int32_t tm_t2 = tm_t1 + (tm_e1 > tm_b1 ? (tm_e1 - tm_b1 + kTile - 1) / kTile : 0);          \
+  const int32_t tm_t3 = tm_t2 + (tm_e2 > tm_b2 ? (tm_e2 - tm_b2 + kTile - 1) / kTile : 0);          \
+  const int64_t tm_n = tm_t3 + (tm_e3 > tm_b3 ? (tm_e3 - tm_b3 + kTile - 1) / kTile : 0);           \
   /* row(T) = b_r + 64 (T - t_r) in range r, as increments over range 0 */                         \
-  const int64_t tm_d1 = uniform_i64((tm_b1 - tm_t1 * kTile) - tm_b0);                              \
-  const int64_t tm_d2 = uniform_i64((tm_b2 - tm_t2 * kTile) - (tm_b1 - tm_t1 * kTile));            \
-  const int64_t tm_d3 = uniform_i64((tm_b3 - tm_t3 * kTile) - (tm_b2 - tm_t2 * kTile));            \
-  const int64_t tm_f1 = uniform_i64(tm_e1 - tm_e0), tm_f2 = uniform_i64(tm_e2 - tm_e1),             \
-                tm_f3 = uniform_i64(tm_e3 - tm_e2);                                                 \
-  const int64_t tm_u0 = uniform_i64(tm_b0), tm_v0 = uniform_i64(tm_e0), tm_u1 = uniform_i64(tm_t1), \
-                tm_u2 = uniform_i64(tm_t2), tm_u3 = uniform_i64(tm_t3);
+  const int32_t tm_d1 = uniform_i32((tm_b1 - tm_t1 * kTile) - tm_b0);                              \
+  const int32_t tm_d2 = uniform_i32((tm_b2 - tm_t2 * kTile) - (tm_b1 - tm_t1 * kTile));            \
+  const int32_t tm_d3 = uniform_i32((tm_b3 - tm_t3 * kTile) - (tm_b2 - tm_t2 * kTile));            \
+  const int32_t tm_f1 = uniform_i32(tm_e1 - tm_e0), tm_f2 = uniform_i32(tm_e2 - tm_e1),             \
+                tm_f3 = uniform_i32(tm_e3 - tm_e2);                                                 \
+  const int32_t tm_u0 = uniform_i32(tm_b0), tm_v0 = uniform_i32(tm_e0), tm_u1 = uniform_i32(tm_t1), \
+                tm_u2 = uniform_i32(tm_t2), tm_u3 = uniform_i32(tm_t3);
 // (sums of independent selects: a chain of selects over four values is turned into a lookup
 // in a private array, i.e. scratch memory)
-#define MSC_TROW(T)                                                                        \
-  (tm_u0 + (T) * kTile + ((T) >= tm_u1 ? tm_d1 : int64_t(0)) + ((T) >= tm_u2 ? tm_d2 : int64_t(0)) + \
-   ((T) >= tm_u3 ? tm_d3 : int64_t(0)))
-#define MSC_TEND(T) \
-  (tm_v0 + ((T) >= tm_u1 ? tm_f1 : int64_t(0)) + ((T) >= tm_u2 ? tm_f2 : int64_t(0)) + ((T) >= tm_u3 ? tm_f3 : int64_t(0)))
+#define MSC_TROW(T)                                                                                 \
+  (tm_u0 + static_cast<int32_t>(T) * kTile + (static_cast<int32_t>(T) >= tm_u1 ? tm_d1 : 0) +       \
+   (static_cast<int32_t>(T) >= tm_u2 ? tm_d2 : 0) + (static_cast<int32_t>(T) >= tm_u3 ? tm_d3 : 0))
+#define MSC_TEND(T)                                                                                 \
+  (tm_v0 + (static_cast<int32_t>(T) >= tm_u1 ? tm_f1 : 0) + (static_cast<int32_t>(T) >= tm_u2 ? tm_f2 : 0) + \
+   (static_cast<int32_t>(T) >= tm_u3 ? tm_f3 : 0))
 
 // WIDE = false: each wave holds 32 queries (2 N-blocks), one list of L keys per lane and
 //   query; a query's 64 candidates of every tile spread over 4 lanes (lists) -> 4L keys.
@@ -396,7 +397,7 @@ __global__ __launch_bounds__(kThreads, 2) void select_kernel(
   if constexpr (kPer > i) {                                                                     \
     const int c_ = threadIdx.x + i * kThreads;                                                  \
     if (kNChunk % kThreads == 0 || c_ < kNChunk)                                                \
-      MSC_PF(S, i) = *reinterpret_cast<const uint4*>(Xb + MSC_TROW(T) * C::ROWB + static_cast<int64_t>(c_) * 16); \
+      MSC_PF(S, i) = *reinterpret_cast<const uint4*>(Xb + static_cast<int64_t>(MSC_TROW(T)) * C::ROWB + static_cast<int64_t>(c_) * 16); \
   }
 #define MSC_STORE1(S, i, B)                                                                     \
   if constexpr (kPer > i) {                                                                     \
@@ -488,8 +489,8 @@ __global__ __launch_bounds__(kThreads, 2) void select_kernel(
     // next M-block, so a rare hit costs one LDS store instead of a wave-wide insertion.
     // Each (M-block, N-block) group is gated by a wave-uniform test: after the first few
     // hundred tiles almost every group has no hit and costs four compares.
-    const int64_t c0 = MSC_TROW(t);
-    const int64_t cend = MSC_TEND(t);
+    const int32_t c0 = MSC_TROW(t);
+    const int32_t cend = MSC_TEND(t);
     const bool tail_tile = c0 + kTile > cend;  // the tile's range ends inside it
 #pragma unroll
     for (int nb = 0; nb < NB; ++nb) {
@@ -504,7 +505,7 @@ __global__ __launch_bounds__(kThreads, 2) void select_kernel(
         for (int i = 0; i < 4; ++i) {
           const float v = a4[i];
           const uint32_t c = cg + i;
-          if (v <= tq[li] && (!tail_tile || static_cast<int64_t>(c) < cend)) {
+          if (v <= tq[li] && (!tail_tile || c < static_cast<uint32_t>(cend))) {
             hitbuf[wv][li][cnt[li]][l] = (static_cast<uint64_t>(float_ord(v)) << 32) | c;
             ++cnt[li];
           }
@@ -732,7 +733,7 @@ __global__ __launch_bounds__(kSelThreads, kSelBlocksPerCU) void select32_kernel(
     src_off[j] = row * rowb + min(col, C::CH - 1) * 16;
   }
   auto load_tile = [&](char* buf, const int64_t T) __attribute__((always_inline)) {
-    const char* base = Xb + MSC_TROW(T < ntiles ? T : ntiles - 1) * rowb;
+    const char* base = Xb + static_cast<int64_t>(MSC_TROW(T < ntiles ? T : ntiles - 1)) * rowb;
 #pragma unroll
     for (int j = 0; j < C::PERW; ++j)
       lds_load16(base + src_off[j], buf + lds_off[j]);
@@ -763,8 +764,7 @@ __global__ __launch_bounds__(kSelThreads, kSelBlocksPerCU) void select32_kernel(
   };
   // gate + hits of one M-block: element i = 4 g + j holds candidate c0 + 8 g + 4 h + j; a
   // wave-uniform test per M-block, then per group of four
-  auto select_mb = [&](const f32x16& acc, const int64_t c0, const int64_t cend) __attribute__((always_inline)) {
-    const bool tail_tile = c0 + 32 > cend;
+  auto select_mb = [&](const f32x16& acc, const int32_t c0, const int32_t cend) __attribute__((always_inline)) {
     float gm[4];
 #pragma unroll
     for (int g = 0; g < 4; ++g) gm[g] = fminf(fminf(acc[4 * g], acc[4 * g + 1]), fminf(acc[4 * g + 2], acc[4 * g + 3]));
@@ -777,7 +777,7 @@ __global__ __launch_bounds__(kSelThreads, kSelBlocksPerCU) void select32_kernel(
       for (int j = 0; j < 4; ++j) {
         const float v = acc[4 * g + j];
         const uint32_t c = cg + 8 * g + j;
-        if (v <= tq && (!tail_tile || static_cast<int64_t>(c) < cend)) {
+        if (v <= tq && c < static_cast<uint32_t>(cend)) {  // rows past the range end are not candidates
           hitbuf[wv][cnt][l] = (static_cast<uint64_t>(float_ord(v)) << 32) | c;
           ++cnt;
         }

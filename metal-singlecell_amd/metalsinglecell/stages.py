@@ -10,6 +10,8 @@ Stages (sizes are the workloads' real sizes):
   select-c3   exact kNN of 1M x 50 (200 Zipf(1.5) clusters, seed 2), k = 30, BF16 + re-rank,
               with the partition-pruned candidate ranges and with the full scan; flops = 2 d per
               scanned (query, candidate) pair for the select kernel (its own event time)
+  knn-c3      the whole exact kNN call of that workload (partition layout, prep, select, finalize,
+              second pass, fallback), per-stage event times and the layout's statistics
   conn-c3     UMAP connectivities of that kNN; bytes = 8 n k + 8 E + 8 (n + 1)
   leiden-c3   Leiden (res 1.0, seed 0, 2 iterations) on those connectivities;
               bytes = sum over levels of (passes (8 E_l + 24 n_l) + 12 (E_l + E_l+1))
@@ -82,6 +84,28 @@ def stage_select_c3(reps):
             "n_second_pass": infos[-1]["n_second_pass"], "n_fallback": infos[-1]["n_fallback"],
             "n_partitions": infos[-1]["n_partitions"], "n_groups": infos[-1]["n_groups"],
             "n_ranges": infos[-1]["n_ranges"]}
+    return out
+
+
+def stage_knn_c3(reps):
+    """The whole exact kNN call at C3 (BF16, pruned candidate ranges): per-stage event times."""
+    from . import _hip
+    Xd = _c3()
+    infos, walls = [], []
+    import time
+    import torch
+    for _ in range(reps + 1):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        _, _, info = _hip.knn_exact(Xd, 30, precision="bf16", timing=True)
+        torch.cuda.synchronize()
+        walls.append((time.perf_counter() - t0) * 1e3)
+        infos.append(info)
+    keys = ("partition_ms", "prep_ms", "select_ms", "finalize_ms", "second_pass_ms", "fallback_ms")
+    out = {k: float(np.mean([i[k] for i in infos[1:]])) for k in keys}
+    out["wall_ms"] = float(np.mean(walls[1:]))
+    for k in ("scanned", "n_partitions", "n_groups", "n_ranges", "n_second_pass", "n_fallback"):
+        out[k] = infos[-1][k]
     return out
 
 
@@ -206,7 +230,7 @@ def stage_batched_c4(reps, per_sample=True):
             "cells": int(X.shape[0]), "cells_per_s": X.shape[0] / ((graph_ms + leiden_ms) * 1e-3)}
 
 
-STAGES = {"select-c3": stage_select_c3, "conn-c3": stage_conn_c3, "leiden-c3": stage_leiden_c3,
+STAGES = {"select-c3": stage_select_c3, "knn-c3": stage_knn_c3, "conn-c3": stage_conn_c3, "leiden-c3": stage_leiden_c3,
           "grid-2m": stage_grid_2m, "moran-2m": stage_moran_2m, "batched-c4": stage_batched_c4}
 
 
