@@ -10,7 +10,8 @@ One step = one full pass from the device-resident embedding to device labels:
 
 ``value`` = cells / (mean step time) for that BF16+re-rank form.  The plain FP32 GPU form
 (FP32 MFMA ranking) is timed beside it on the same input and reported in ``fp32_form``;
-both forms return identical graphs and labels.  ``roofline`` covers the dominant MFMA
+both forms return identical graphs and labels; ``fp32_form.full_scan`` is the FP32 form without
+the partition-pruned candidate ranges (every query scans all n rows).  ``roofline`` covers the dominant MFMA
 kernel (the kNN select kernel): algorithmic flops 2 d per scanned (query, candidate) pair
 per launch -- s n^2 pairs, s the scanned fraction of the partition-pruned layout, 1 without
 it -- over its HIP-event duration, against the 2.5 PFLOP/s dense BF16 peak.  ``cpu_baseline`` times the project's
@@ -38,11 +39,11 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def gpu_step(Xd, k, precision, resolution=1.0, seed=0, n_iterations=2):
+def gpu_step(Xd, k, precision, resolution=1.0, seed=0, n_iterations=2, prune=None):
     from metalsinglecell import _hip
     import torch
     t0 = time.perf_counter()
-    idx, dist, kinfo = _hip.knn_exact(Xd, k, precision=precision, timing=True)
+    idx, dist, kinfo = _hip.knn_exact(Xd, k, precision=precision, timing=True, prune=prune)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     (ip, ind, dat), _ = _hip.fuzzy_connectivities(idx, dist)
@@ -57,16 +58,16 @@ def gpu_step(Xd, k, precision, resolution=1.0, seed=0, n_iterations=2):
                     "n_clusters": int(labels.max().item()) + 1}
 
 
-def timed(Xd, k, precision, steps, warmup):
+def timed(Xd, k, precision, steps, warmup, prune=None):
     import torch
     for _ in range(warmup):
-        gpu_step(Xd, k, precision)
+        gpu_step(Xd, k, precision, prune=prune)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     recs = []
     labels = None
     for _ in range(steps):
-        labels, r = gpu_step(Xd, k, precision)
+        labels, r = gpu_step(Xd, k, precision, prune=prune)
         recs.append(r)
     torch.cuda.synchronize()
     total = time.perf_counter() - t0
@@ -529,6 +530,14 @@ def main():
                 "select_ms": float(np.mean([r["knn"]["select_ms"] for r in recs_fp])),
                 "speedup_bf16_vs_fp32": ms_fp / ms_bf}
         log("fp32 form:", fp32)
+        # the plain FP32 form without the pruned candidate ranges (every query scans every row)
+        ms_pl, recs_pl, lab_pl = timed(Xd, a.k, "fp32", 3, 1, prune=False)
+        fp32["full_scan"] = {"value": n / ms_pl, "ms_per_step": ms_pl * 1e3, "steps": 3, "warmup": 1,
+                             "identical_labels": bool(torch.equal(lab_bf, lab_pl)),
+                             "knn_s": recs_pl[-1]["knn_s"],
+                             "select_ms": float(np.mean([r["knn"]["select_ms"] for r in recs_pl])),
+                             "speedup_bf16_pruned_vs_this": ms_pl / ms_bf}
+        log("fp32 full-scan form:", fp32["full_scan"])
 
     sel = [r["knn"]["select_ms"] for r in recs_bf]
     sel_ms = float(np.mean(sel))

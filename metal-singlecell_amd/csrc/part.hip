@@ -14,10 +14,11 @@
 //                does not stretch a partition's ball over other clusters.
 //   4. groups    connected components of the partitions whose balls (c_p, R_p) overlap, R_p
 //                the largest radius kept in p, among the partitions with R_p <= 3 x the median
-//                R (a small partition that straddles two clusters links nothing: as a
+//                R over the rows (a small partition that straddles two clusters links nothing: as a
 //                candidate it costs little, as a link it would merge the clusters' groups);
-//                the groups of >= kMinGroup points, largest first (at most kMaxGroups), then
-//                one pooled group for the rest.
+//                one pooled group for the groups under kMinGroup points (placed first: its
+//                scattered rows need the most candidates), then the others, largest first (at most
+//                kMaxGroups).
 //   5. order     rows sorted by (group, partition, r); query blocks of <= 128 rows inside one
 //                group.
 //   6. bounds    UB(q) = (r(q) + rk(p(q))) (1 + eps), rk(p) the largest radius among the k
@@ -297,24 +298,34 @@ __global__ void k_peel(const float* __restrict__ X, int64_t n, int d, const uint
 // ---- 4. groups ---------------------------------------------------------------------------------
 __device__ void block_bitonic_u64(uint64_t* a, int np2);
 
-// rlim[0] = 3 x the median R over the non-empty partitions (one workgroup)
+// rlim[0] = 3 x the median of R over the rows (each partition's R weighted by its size, so the
+// singleton partitions, R = 0, barely move it); one workgroup
 __global__ __launch_bounds__(kCompThreads) void k_radius_limit(const float* __restrict__ Rp,
                                                                const int32_t* __restrict__ size, int P,
                                                                float* __restrict__ rlim) {
   __shared__ uint64_t keys[kSortCap];
-  __shared__ int m;
   int np2 = 2;
   while (np2 < P) np2 <<= 1;
-  if (threadIdx.x == 0) m = 0;
-  __syncthreads();
   for (int i = threadIdx.x; i < np2; i += blockDim.x) {
     const bool ok = i < P && size[i] > 0;
     keys[i] = ok ? ((static_cast<uint64_t>(__float_as_uint(Rp[i])) << 32) | static_cast<uint32_t>(i)) : ~0ull;
-    if (ok) atomicAdd(&m, 1);
   }
   __syncthreads();
   block_bitonic_u64(keys, np2);
-  if (threadIdx.x == 0) rlim[0] = m > 0 ? 3.0f * __uint_as_float(static_cast<uint32_t>(keys[m / 2] >> 32)) : 0.0f;
+  if (threadIdx.x == 0) {
+    int64_t total = 0;
+    for (int i = 0; i < P && keys[i] != ~0ull; ++i) total += size[static_cast<uint32_t>(keys[i])];
+    int64_t acc = 0;
+    float med = 0.0f;
+    for (int i = 0; i < P && keys[i] != ~0ull; ++i) {
+      acc += size[static_cast<uint32_t>(keys[i])];
+      if (2 * acc >= total) {
+        med = __uint_as_float(static_cast<uint32_t>(keys[i] >> 32));
+        break;
+      }
+    }
+    rlim[0] = 3.0f * med;
+  }
 }
 
 // adj[p][w] bit b: partitions p and q = 32 w + b both non-empty, both with R <= rlim, with
@@ -360,9 +371,9 @@ __device__ void block_bitonic_u64(uint64_t* a, int np2) {
 }
 
 // One workgroup: component labels (the smallest partition id of each component, by min-label
-// sweeps with pointer jumping until a sweep changes nothing), group sizes, the group order
-// (>= kMinGroup points, largest first, then the id; at most kMaxGroups; the rest pooled as group
-// rank G), and the partition order (group rank, id).
+// sweeps with pointer jumping until a sweep changes nothing), group sizes, the group order (rank
+// 0: the pooled group; then the groups of >= kMinGroup points, largest first, then the id, at
+// most kMaxGroups), and the partition order (group rank, id).
 __global__ __launch_bounds__(kCompThreads) void k_components(const uint32_t* __restrict__ adj, int W, int P,
                                                            const int32_t* __restrict__ size,
                                                            int32_t* __restrict__ ppos, int32_t* __restrict__ porder,
@@ -424,10 +435,11 @@ __global__ __launch_bounds__(kCompThreads) void k_components(const uint32_t* __r
   }
   __syncthreads();
   const int G = nbig;
-  // gsz[root] <- group rank (pooled: G)
-  for (int p = threadIdx.x; p < P; p += blockDim.x) gsz[p] = G;
+  // gsz[root] <- group rank: the pooled group first (rank 0: its queries are scattered rows
+  // whose blocks scan the most candidates, so they start first), then the groups by size
+  for (int p = threadIdx.x; p < P; p += blockDim.x) gsz[p] = 0;
   __syncthreads();
-  for (int i = threadIdx.x; i < G; i += blockDim.x) gsz[static_cast<uint32_t>(keys[i])] = i;
+  for (int i = threadIdx.x; i < G; i += blockDim.x) gsz[static_cast<uint32_t>(keys[i])] = i + 1;
   __syncthreads();
   for (int i = threadIdx.x; i < np2; i += blockDim.x)
     keys[i] = i < P ? ((static_cast<uint64_t>(gsz[lab[i]]) << 32) | static_cast<uint32_t>(i)) : ~0ull;
@@ -486,7 +498,7 @@ __global__ void k_perm_parts(const int32_t* __restrict__ porder, int P, const fl
   }
 }
 
-// group g's rows start at fstart[gpstart[g]] (g = 0..G+1; the pooled group G may be empty)
+// group g's rows start at fstart[gpstart[g]] (g = 0..G+1; the pooled group 0 may be empty)
 __global__ void k_group_rows(const int64_t* __restrict__ fstart, const int64_t* __restrict__ gpstart, int ng,
                              int64_t* __restrict__ grow) {
   const int g = blockIdx.x * kT + threadIdx.x;
@@ -1019,7 +1031,7 @@ int build(const float* X, int64_t n, int d, int k, void* ws, size_t ws_bytes, La
   MSC_LAUNCH_CHECK();
   MSC_TRY(prims::segment_starts(fk, n, P2, B.fstart, st));
   MSC_TRY(prims::segment_starts(B.pgr, P2, G + 1, B.gpstart, st));
-  const int ng = static_cast<int>(G) + 1;  // the pooled group last (possibly empty)
+  const int ng = static_cast<int>(G) + 1;  // the pooled group first (possibly empty)
   k_group_rows<<<grid_for(ng + 1), kT, 0, st>>>(B.fstart, B.gpstart, ng + 1, B.grow);
   k_block_offsets<<<1, kCompThreads, 0, st>>>(B.grow, ng, B.boff, B.counts);
   MSC_LAUNCH_CHECK();

@@ -107,7 +107,7 @@ def test_pruned_equals_unpruned_200k(precision):
     b_i, b_d = _hip.knn_exact(X, 30, precision=precision, prune=False)
     print(f"[pruned 200k] scanned {info['scanned']:.4f} partitions {info['n_partitions']} "
           f"groups {info['n_groups']}", flush=True)
-    assert info["scanned"] < 0.5
+    assert info["scanned"] < 0.3  # 200 separated clusters: about sum_c p_c^2 = 0.19
     assert np.array_equal(a_i.cpu().numpy(), b_i.cpu().numpy())
     assert np.array_equal(a_d.cpu().numpy(), b_d.cpu().numpy())
     rows = np.random.default_rng(0).choice(X.shape[0], 1000, replace=False)
