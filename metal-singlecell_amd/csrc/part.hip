@@ -160,6 +160,98 @@ __global__ __launch_bounds__(kT) void k_assign(const float* __restrict__ X, int6
   }
 }
 
+// The same choice on the 32x32x16 BF16 MFMA (d <= 62): scores |c|^2 - 2 x.c from one-term
+// bf16 operands, |c|^2 carried as two bf16 terms against constant 1s in the query operand.  A
+// block holds 128 rows (4 waves x 32, the B operand in registers) and streams the centres
+// through LDS 32 at a time (the A operand).  Output layout of the MFMA: lane l holds row
+// (l mod 32) against centres 8 (i div 4) + 4 (l div 32) + (i mod 4) of the tile, i < 16.  The
+// rounding only changes which centre a row joins: every bound uses the exact float32 radius of the
+// row to the centre it actually joined.
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(16 * sizeof(float)))) float f32x16;
+constexpr int kAsgStride = 72;  // bf16 per LDS centre row: 64 terms + 8 (an odd number of 16-B groups)
+
+__global__ __launch_bounds__(kT) void k_assign_mfma(const float* __restrict__ X, int64_t n, int d,
+                                                    const float* __restrict__ C, const float* __restrict__ cn2,
+                                                    int cb, int ce, int first, int32_t* __restrict__ part,
+                                                    float* __restrict__ dmin2) {
+  __shared__ __attribute__((aligned(16))) __bf16 ct[32][kAsgStride];
+  const int l = lane_id(), wv = threadIdx.x / kWave, h = l >> 5, lq = l & 31;
+  const int64_t q = static_cast<int64_t>(blockIdx.x) * (4 * 32) + wv * 32 + lq;
+  const bool valid = q < n;
+  bf16x8 qb[4];
+  float xn = 0.0f;
+#pragma unroll
+  for (int st = 0; st < 4; ++st) {
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const int k = 16 * st + 8 * h + e;
+      float v = 0.0f;
+      if (k < d) {
+        const float x = valid ? X[q * d + k] : 0.0f;
+        xn = fmaf(x, x, xn);
+        v = -2.0f * x;
+      } else if (k == d || k == d + 1) {
+        v = 1.0f;
+      }
+      qb[st][e] = static_cast<__bf16>(v);
+    }
+  }
+  xn += __shfl_xor(xn, 32, kWave);  // the other half of the row's terms
+  float best = inf_f();
+  int bp = 0;
+  if (!first && valid) {
+    best = dmin2[q];
+    bp = part[q];
+  }
+  for (int c0 = cb; c0 < ce; c0 += 32) {
+    __syncthreads();
+    {  // thread t converts 8 terms of centre t / 8: [c (d terms), hi(|c|^2), lo(|c|^2), 0 ...]
+      const int m = threadIdx.x >> 3, j = threadIdx.x & 7, c = c0 + m;
+      bf16x8 v;
+      const float nrm = c < ce ? cn2[c] : 0.0f;
+      const float hi = static_cast<float>(static_cast<__bf16>(nrm));
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const int k = 8 * j + e;
+        float x = 0.0f;
+        if (c >= ce) x = (k == d) ? inf_f() : 0.0f;  // past the last centre: score +inf
+        else if (k < d) x = C[static_cast<int64_t>(c) * kDMax + k];
+        else if (k == d) x = hi;
+        else if (k == d + 1) x = nrm - hi;
+        v[e] = static_cast<__bf16>(x);
+      }
+      *reinterpret_cast<bf16x8*>(&ct[m][8 * j]) = v;
+    }
+    __syncthreads();
+    f32x16 acc = f32x16{};
+#pragma unroll
+    for (int st = 0; st < 4; ++st) {
+      const bf16x8 a = *reinterpret_cast<const bf16x8*>(&ct[lq][16 * st + 8 * h]);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, qb[st], acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      const float sc = acc[i] + xn;  // D^2 = |c|^2 - 2 x.c + |x|^2
+      if (sc < best) {  // centres ascend within a lane: ties keep the lower index
+        best = sc;
+        bp = c0 + 8 * (i >> 2) + 4 * h + (i & 3);
+      }
+    }
+  }
+  // the two lanes of a row: the smaller (D^2, centre)
+  const float ob = __shfl_xor(best, 32, kWave);
+  const int op = __shfl_xor(bp, 32, kWave);
+  if (ob < best || (ob == best && op < bp)) {
+    best = ob;
+    bp = op;
+  }
+  if (valid && h == 0) {
+    part[q] = bp;
+    dmin2[q] = fmaxf(best, 0.0f);
+  }
+}
+
 constexpr int kSumBlocks = 256;
 // deterministic float64 sum of n floats: fixed grid, fixed per-thread stride, butterfly waves
 __global__ __launch_bounds__(kT) void k_sum_partial(const float* __restrict__ v, int64_t n, double* __restrict__ partial) {
@@ -905,6 +997,12 @@ unsigned grid_for(int64_t m) { return static_cast<unsigned>(ceil_div(std::max<in
 
 int launch_assign(const float* X, int64_t n, int d, const float* C, const float* cn2, int cb, int ce, int first,
                   int32_t* part, float* dmin2, hipStream_t st) {
+  if (d + 2 <= 64) {
+    k_assign_mfma<<<static_cast<unsigned>(ceil_div(n, 4 * 32)), kT, 0, st>>>(X, n, d, C, cn2, cb, ce, first, part,
+                                                                             dmin2);
+    MSC_LAUNCH_CHECK();
+    return kOk;
+  }
   const int D4 = (d + 3) / 4;
   const unsigned g = grid_for(n);
   switch (D4) {
