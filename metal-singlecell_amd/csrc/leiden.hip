@@ -13,13 +13,13 @@
 // wave sum: no atomics anywhere, so labels are bit-identical to the oracle and independent
 // of scheduling.
 //
-// Per-vertex best move: vertices of degree <= 256 take one wave (up to four neighbours per
-// lane, sorted inside the wave); degree 257..1024 and 1025..4096 take one workgroup each from
-// a list (256 / 1024 threads, sorted across the workgroup).  Heavier vertices (and the list
-// ranges whose community ids would not fit the sort key: 2^22 and 2^20 vertices and up) go
-// through a sort path over their edges keyed by (vertex, community).
+// Per-vertex best move, one launch per pass (k_move): vertices of degree <= 256 take one wave
+// (up to four neighbours per lane, sorted inside the wave); degree 257..1024 and 1025..4096
+// take one 256-thread block each from a list (4 / 16 neighbours per thread, sorted across the
+// block).  Heavier vertices (and the list ranges whose community ids would not fit the sort
+// key: 2^22 and 2^20 vertices and up) go through a sort path over their edges keyed by
+// (vertex, community).
 #include <cmath>
-#include <mutex>
 #include <vector>
 
 #include "common.h"
@@ -317,11 +317,14 @@ __device__ __forceinline__ void move_one(int64_t v, int64_t e0, int64_t deg, int
 // once at the end of its run and takes the wave argmax (ties to the smaller community).
 // Requires community ids < 2^(32 - IB) so that a key fits 32 bits.
 constexpr int64_t kSortedMaxN = int64_t(1) << 24;  // NI <= 4
-constexpr int kMidDeg = 1024;                        // k_move_block<256>: 4 slots per thread
-constexpr int64_t kMidMaxN = int64_t(1) << 22;      // community ids below 2^(32 - 10)
-constexpr int kBigDeg = 4096;                        // k_move_block<1024>
-constexpr int64_t kBigMaxN = int64_t(1) << 20;      // community ids below 2^(32 - 12)
-constexpr int kMoveGroup = 8;  // consecutive vertices per wave in k_move_light
+// Listed vertices of degree 257..4096 take one 256-thread block each (k_move), in four
+// classes: class i holds degrees (256 << i, 512 << i], sorted as N = 512 << i keys of
+// (community << (9 + i) | slot), ITEMS = 2 << i per thread.  Class i needs community ids below
+// 2^(23 - i) (strictly below, so that no key equals kNoKey), so it is used for graphs of fewer
+// than hub_class_max_n(i) vertices.
+constexpr int kHubClasses = 4;
+__host__ __device__ constexpr int64_t hub_class_max_n(int i) { return int64_t(1) << (23 - i); }
+constexpr int kMoveGroup = 8;  // consecutive vertices per wave on the light path of k_move
 
 // Bitonic network over the 64 * NI slots e = i * 64 + l of a wave (keys in registers).  The
 // stage sequence is unrolled by template recursion so that every register index is a
@@ -525,16 +528,17 @@ __device__ __forceinline__ void move_vertex(int64_t v, const int64_t* __restrict
   }
 }
 
-__global__ __launch_bounds__(kThreads) void k_move_light(
-    const int64_t* __restrict__ indptr, const int32_t* __restrict__ indices, const int64_t* __restrict__ wq,
-    const int64_t* __restrict__ kdeg, int64_t n, const int32_t* __restrict__ comm,
+// The light vertices of light block lb (degree <= 256): each wave owns kMoveGroup consecutive
+// vertices; inactive ones keep their community (one coalesced store), active ones are moved
+// one after another.  lds: 3 x kLightDeg int64 per wave.
+__device__ __forceinline__ void move_light_block(
+    int64_t lb, const int64_t* __restrict__ indptr, const int32_t* __restrict__ indices,
+    const int64_t* __restrict__ wq, const int64_t* __restrict__ kdeg, int64_t n, const int32_t* __restrict__ comm,
     const int32_t* __restrict__ part, const int64_t* __restrict__ sigtot, const CommSig* __restrict__ cs,
-    const uint8_t* __restrict__ active, GainParams gp, int32_t* __restrict__ target) {
-  __shared__ int64_t lds[kWavesPerBlock][3][kLightDeg];
-  // each wave owns kMoveGroup consecutive vertices: inactive ones keep their community
-  // (one coalesced store), active light ones are moved one after another
+    const uint8_t* __restrict__ active, const GainParams& gp, int32_t* __restrict__ target,
+    int64_t (*lds)[3][kLightDeg]) {
   const int l = lane_id();
-  const int64_t v0 = (static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + threadIdx.x / kWave) * kMoveGroup;
+  const int64_t v0 = (lb * kWavesPerBlock + threadIdx.x / kWave) * kMoveGroup;
   bool todo = false;
   int64_t e0v = 0, degv = 0, kvv = 0;
   int32_t currv = 0, pvv = 0;
@@ -601,48 +605,50 @@ __global__ __launch_bounds__(kThreads) void k_move_light(
   }
 }
 
-// Degree 257..1024 and 1025..4096: one workgroup of THREADS threads per listed vertex, 4
-// neighbour slots per thread (N = 4 THREADS).  The same decision as move_sorted, atomics-free:
-// keys (community << IB | slot) sorted by a bitonic network whose stages run where the
-// partner lives — inside the thread (stride < 4), across the wave by shuffles (stride < 256)
-// or through LDS with barriers (stride >= 256: 3 of the 55 stages at N = 1024, 10 of 78 at
-// 4096) — then the weights of each run of equal community summed by a block int64 prefix
-// scan in sorted order, each run's head found by a block max-scan, each community scored
+// Degree 257..1024 (ITEMS = 4) and 1025..4096 (ITEMS = 16): one 256-thread block per listed
+// vertex, ITEMS neighbour slots per thread (N = 256 ITEMS keys).  The same decision as
+// move_sorted, atomics-free: keys (community << IB | slot) sorted by a bitonic network whose
+// stages run where the partner lives -- inside the thread (stride < ITEMS), across the wave by
+// shuffles (stride < 64 ITEMS) or through LDS with barriers (the two or three widest strides)
+// -- then the weights of each run of equal community summed in sorted order: inside a thread
+// by a running sum that restarts at each run head, across threads by a block segmented scan
+// (a run that starts in an earlier thread takes the scan's carry).  Each community is scored
 // once at the end of its run and the block argmax taken (ties to the smaller community).
-// Weights and sigma_tot are re-read by slot from global memory, so the LDS holds only the
-// keys and the prefix (12 N bytes) and the blocks share CUs with the light kernel.
-template <int THREADS>
-__device__ __forceinline__ int64_t block_incl_sum_i64(int64_t x, int64_t* __restrict__ wtot) {
+// Weights and sigma_tot are re-read by slot from global memory; the LDS holds only the keys.
+
+// Exclusive segmented sum over the threads of the block.  Thread t gives (f, x): f = it holds
+// a run head, x = the sum of its weights after its last head (all of them when f is false).
+// Returns, for thread t, the sum of the weights from the last run head before its first
+// position up to that position (exclusive): (f1, x1) . (f2, x2) = (f1 | f2, f2 ? x2 : x1 + x2).
+__device__ __forceinline__ int64_t block_excl_segsum_i64(bool f, int64_t x, int64_t* __restrict__ wsum,
+                                                         int32_t* __restrict__ wflag) {
   const int l = lane_id(), wv = threadIdx.x / kWave;
+  int fi = f ? 1 : 0;
+  int64_t xi = x;
 #pragma unroll
   for (int d = 1; d < kWave; d <<= 1) {
-    const int64_t o = static_cast<int64_t>(shfl_up_u64(static_cast<uint64_t>(x), d));
-    if (l >= d) x += o;
+    const int of = __shfl_up(fi, d, kWave);
+    const int64_t ox = static_cast<int64_t>(shfl_up_u64(static_cast<uint64_t>(xi), d));
+    if (l >= d) {
+      if (!fi) xi += ox;
+      fi |= of;
+    }
   }
-  if (l == kWave - 1) wtot[wv] = x;
-  __syncthreads();
-  int64_t off = 0;
-  for (int w = 0; w < wv; ++w) off += wtot[w];
-  __syncthreads();  // wtot is free for the next scan
-  return x + off;
-}
-template <int THREADS>
-__device__ __forceinline__ int32_t block_excl_max_i32(int32_t x, int32_t* __restrict__ wtot) {
-  const int l = lane_id(), wv = threadIdx.x / kWave;
-  int32_t inc = x;
-#pragma unroll
-  for (int d = 1; d < kWave; d <<= 1) {
-    const int32_t o = __shfl_up(inc, d, kWave);
-    if (l >= d) inc = max(inc, o);
+  if (l == kWave - 1) {
+    wsum[wv] = xi;
+    wflag[wv] = fi;
   }
-  if (l == kWave - 1) wtot[wv] = inc;
   __syncthreads();
-  int32_t off = -1;
-  for (int w = 0; w < wv; ++w) off = max(off, wtot[w]);
-  __syncthreads();
-  int32_t ex = __shfl_up(inc, 1, kWave);
-  if (l == 0) ex = -1;
-  return max(ex, off);
+  int64_t cx = 0;  // carry of the earlier waves
+  for (int w = 0; w < wv; ++w) cx = wflag[w] ? wsum[w] : cx + wsum[w];
+  __syncthreads();  // wsum / wflag are free for the next scan
+  int ef = __shfl_up(fi, 1, kWave);
+  int64_t ex = static_cast<int64_t>(shfl_up_u64(static_cast<uint64_t>(xi), 1));
+  if (l == 0) {
+    ef = 0;
+    ex = 0;
+  }
+  return ef ? ex : cx + ex;
 }
 
 // one compare-exchange of position e (value mine) with its partner p (value other) in the
@@ -652,29 +658,44 @@ __device__ __forceinline__ uint32_t bitonic_pick(uint32_t mine, uint32_t other, 
   return ((e < p) == ((e & k) == 0)) ? lo : hi;
 }
 
-template <int THREADS>
-__global__ __launch_bounds__(THREADS) void k_move_block(
-    const int32_t* __restrict__ verts, const int64_t* __restrict__ indptr, const int32_t* __restrict__ indices,
-    const int64_t* __restrict__ wq, const int64_t* __restrict__ kdeg, const int32_t* __restrict__ comm,
-    const int32_t* __restrict__ part, const int64_t* __restrict__ sigtot, const uint8_t* __restrict__ active,
-    GainParams gp_all, int32_t* __restrict__ target) {
-  constexpr int ITEMS = 4;
-  constexpr int N = THREADS * ITEMS;
-  constexpr int IB = THREADS == 256 ? 10 : THREADS == 1024 ? 12 : -1;
-  static_assert(IB > 0, "THREADS must be 256 or 1024");
-  constexpr int NW = THREADS / kWave;
+// bitonic stage of stride J < ITEMS: the partner of item s is item s ^ J of the same thread
+template <int ITEMS, int J>
+__device__ __forceinline__ void stage_in_thread(uint32_t (&key)[ITEMS], int eb, int k) {
+#pragma unroll
+  for (int s = 0; s < ITEMS; ++s) {
+    if ((s & J) == 0) {
+      const bool asc = ((eb + s) & k) == 0;
+      const uint32_t a = key[s], b = key[s | J];
+      const uint32_t lo = a < b ? a : b, hi = a < b ? b : a;
+      key[s] = asc ? lo : hi;
+      key[s | J] = asc ? hi : lo;
+    }
+  }
+}
+
+template <int ITEMS>
+struct BlockMoveSmem {
+  uint32_t skey[kThreads * ITEMS];
+  int64_t wsum[kWavesPerBlock];
+  int32_t wflag[kWavesPerBlock];
+  double wbest_g[kWavesPerBlock];
+  int32_t wbest_c[kWavesPerBlock];
+  double stay;
+};
+
+template <int ITEMS>
+__device__ __forceinline__ void block_move(int64_t v, const int64_t* __restrict__ indptr,
+                                           const int32_t* __restrict__ indices, const int64_t* __restrict__ wq,
+                                           const int64_t* __restrict__ kdeg, const int32_t* __restrict__ comm,
+                                           const int32_t* __restrict__ part, const int64_t* __restrict__ sigtot,
+                                           const uint8_t* __restrict__ active, const GainParams& gp_all,
+                                           int32_t* __restrict__ target, BlockMoveSmem<ITEMS>& sm) {
+  static_assert(ITEMS == 2 || ITEMS == 4 || ITEMS == 8 || ITEMS == 16, "ITEMS");
+  constexpr int N = kThreads * ITEMS;
+  constexpr int IB = ITEMS == 2 ? 9 : ITEMS == 4 ? 10 : ITEMS == 8 ? 11 : 12;  // log2 N
   constexpr int kWaveSpan = ITEMS * kWave;  // positions held by one wave
-  constexpr uint32_t kNone = 0xFFFFFFFFu;
   constexpr uint32_t kSlot = (1u << IB) - 1u;
-  __shared__ uint32_t skey[N];
-  __shared__ int64_t lp[N];    // inclusive weight prefix, by sorted position
-  __shared__ int64_t wsum[NW];
-  __shared__ int32_t wmax[NW];
-  __shared__ double wbest_g[NW];
-  __shared__ int32_t wbest_c[NW];
-  __shared__ double s_stay;
   const int t = threadIdx.x, l = lane_id(), wv = t / kWave;
-  const int64_t v = verts[blockIdx.x];
   const GainParams gp = for_vertex(gp_all, v);
   const int32_t curr = comm[v];
   if (!active[v]) {  // uniform over the block
@@ -685,81 +706,52 @@ __global__ __launch_bounds__(THREADS) void k_move_block(
   const int64_t kv = kdeg[v];
   const int32_t pv = part ? part[v] : 0;
   const int eb = ITEMS * t;  // this thread's first position
+  int32_t u[ITEMS];
+#pragma unroll
+  for (int s = 0; s < ITEMS; ++s) u[s] = eb + s < deg ? indices[e0 + eb + s] : -1;
   uint32_t key[ITEMS];
 #pragma unroll
   for (int s = 0; s < ITEMS; ++s) {
-    const int e = eb + s;
-    key[s] = kNone;
-    if (e < deg) {
-      const int32_t u = indices[e0 + e];
-      if (u != static_cast<int32_t>(v) && (!part || part[u] == pv))
-        key[s] = (static_cast<uint32_t>(comm[u]) << IB) | static_cast<uint32_t>(e);
-    }
+    key[s] = kNoKey;
+    if (u[s] >= 0 && u[s] != static_cast<int32_t>(v) && (!part || part[u[s]] == pv))
+      key[s] = (static_cast<uint32_t>(comm[u[s]]) << IB) | static_cast<uint32_t>(eb + s);
   }
-  if (t == 0) s_stay = 0.0;
+  if (t == 0) sm.stay = 0.0;
   for (int k = 2; k <= N; k <<= 1)
     for (int j = k >> 1; j > 0; j >>= 1) {
       if (j >= kWaveSpan) {  // partner in another wave
         __syncthreads();     // earlier readers of skey are done
 #pragma unroll
-        for (int s = 0; s < ITEMS; ++s) skey[eb + s] = key[s];
+        for (int s = 0; s < ITEMS; ++s) sm.skey[eb + s] = key[s];
         __syncthreads();
 #pragma unroll
-        for (int s = 0; s < ITEMS; ++s) key[s] = bitonic_pick(key[s], skey[(eb + s) ^ j], eb + s, (eb + s) ^ j, k);
+        for (int s = 0; s < ITEMS; ++s)
+          key[s] = bitonic_pick(key[s], sm.skey[(eb + s) ^ j], eb + s, (eb + s) ^ j, k);
       } else if (j >= ITEMS) {  // partner in lane l ^ (j / ITEMS) of this wave
 #pragma unroll
         for (int s = 0; s < ITEMS; ++s)
           key[s] = bitonic_pick(key[s], __shfl_xor(key[s], j / ITEMS, kWave), eb + s, (eb + s) ^ j, k);
-      } else if (j == 2) {
-        const uint32_t a0 = key[0], a1 = key[1], a2 = key[2], a3 = key[3];
-        key[0] = bitonic_pick(a0, a2, eb, eb + 2, k);
-        key[2] = bitonic_pick(a2, a0, eb + 2, eb, k);
-        key[1] = bitonic_pick(a1, a3, eb + 1, eb + 3, k);
-        key[3] = bitonic_pick(a3, a1, eb + 3, eb + 1, k);
-      } else {  // j == 1
-        const uint32_t a0 = key[0], a1 = key[1], a2 = key[2], a3 = key[3];
-        key[0] = bitonic_pick(a0, a1, eb, eb + 1, k);
-        key[1] = bitonic_pick(a1, a0, eb + 1, eb, k);
-        key[2] = bitonic_pick(a2, a3, eb + 2, eb + 3, k);
-        key[3] = bitonic_pick(a3, a2, eb + 3, eb + 2, k);
+      } else if (j == 1) {
+        stage_in_thread<ITEMS, 1>(key, eb, k);
+      } else if constexpr (ITEMS > 2) {
+        if (j == 2) stage_in_thread<ITEMS, 2>(key, eb, k);
+        else if constexpr (ITEMS > 4) {
+          if (j == 4) stage_in_thread<ITEMS, 4>(key, eb, k);
+          else if constexpr (ITEMS > 8) stage_in_thread<ITEMS, 8>(key, eb, k);
+        }
       }
     }
   __syncthreads();  // the last LDS stage's readers are done
 #pragma unroll
-  for (int s = 0; s < ITEMS; ++s) skey[eb + s] = key[s];
+  for (int s = 0; s < ITEMS; ++s) sm.skey[eb + s] = key[s];
   __syncthreads();
-  // thread t owns the sorted positions eb .. eb + 3
-  int64_t pre[ITEMS];
-  int32_t head[ITEMS];  // last run head at or before the position within this thread, or -1
-  int64_t run = 0;
-  int32_t hrun = -1;
-  const uint32_t before = t > 0 ? skey[eb - 1] : kNone;
-  const uint32_t after = eb + ITEMS < N ? skey[eb + ITEMS] : kNone;
-#pragma unroll
-  for (int s = 0; s < ITEMS; ++s) {
-    const bool valid = key[s] != kNone;
-    run += valid ? wq[e0 + (key[s] & kSlot)] : 0;
-    pre[s] = run;
-    const uint32_t prev = s > 0 ? key[s > 0 ? s - 1 : 0] : before;
-    if (valid && (eb + s == 0 || prev == kNone || (prev >> IB) != (key[s] >> IB))) hrun = eb + s;
-    head[s] = hrun;
-  }
-  const int64_t base = block_incl_sum_i64<THREADS>(run, wsum) - run;
-  const int32_t hprev = block_excl_max_i32<THREADS>(hrun, wmax);  // last head of earlier threads
-#pragma unroll
-  for (int s = 0; s < ITEMS; ++s) lp[eb + s] = base + pre[s];
-  __syncthreads();
+  const uint32_t before = t > 0 ? sm.skey[eb - 1] : kNoKey;
+  const uint32_t after = eb + ITEMS < N ? sm.skey[eb + ITEMS] : kNoKey;
   double best_g = -1e300, stay = 0.0;
   int32_t best_c = 0x7FFFFFFF;
   bool has_stay = false;
-#pragma unroll
-  for (int s = 0; s < ITEMS; ++s) {
-    if (key[s] == kNone) continue;
-    const uint32_t next = s + 1 < ITEMS ? key[s + 1 < ITEMS ? s + 1 : s] : after;
-    const int32_t ci = static_cast<int32_t>(key[s] >> IB);
-    if (next != kNone && static_cast<int32_t>(next >> IB) == ci) continue;  // not the end of its run
-    const int32_t h = head[s] >= 0 ? head[s] : hprev;
-    const int64_t wc = lp[eb + s] - (h > 0 ? lp[h - 1] : 0);
+  auto score = [&](int64_t wc, uint32_t kk) {
+    const int32_t ci = static_cast<int32_t>(kk >> IB);
     const int64_t sig = sigtot[ci] - (ci == curr ? kv : 0);
     const double g = gain_of(wc, sig, kv, gp);
     if (ci == curr) {
@@ -770,6 +762,37 @@ __global__ __launch_bounds__(THREADS) void k_move_block(
       best_g = g;
       best_c = ci;
     }
+  };
+  // running sum from the last run head; a run end whose head lies in an earlier thread (at
+  // most one per thread: the first run end) waits for the block scan's carry
+  int64_t acc = 0, open_acc = 0;
+  bool seen_head = false;
+  int open_s = -1;
+#pragma unroll
+  for (int s = 0; s < ITEMS; ++s) {
+    if (key[s] == kNoKey) continue;  // valid keys are a prefix of the sorted order
+    const uint32_t prev = s > 0 ? key[s > 0 ? s - 1 : 0] : before;
+    if (eb + s == 0 || prev == kNoKey || (prev >> IB) != (key[s] >> IB)) {
+      acc = 0;
+      seen_head = true;
+    }
+    acc += wq[e0 + (key[s] & kSlot)];
+    const uint32_t next = s + 1 < ITEMS ? key[s + 1 < ITEMS ? s + 1 : s] : after;
+    if (next == kNoKey || (next >> IB) != (key[s] >> IB)) {  // end of its run
+      if (seen_head) score(acc, key[s]);
+      else {
+        open_acc = acc;
+        open_s = s;
+      }
+    }
+  }
+  const int64_t carry = block_excl_segsum_i64(seen_head, acc, sm.wsum, sm.wflag);
+  if (open_s >= 0) {
+    uint32_t kk = key[0];
+#pragma unroll
+    for (int s = 1; s < ITEMS; ++s)
+      if (s == open_s) kk = key[s];
+    score(carry + open_acc, kk);
   }
 #pragma unroll
   for (int m = 1; m < kWave; m <<= 1) {
@@ -781,21 +804,60 @@ __global__ __launch_bounds__(THREADS) void k_move_block(
     }
   }
   if (l == 0) {
-    wbest_g[wv] = best_g;
-    wbest_c[wv] = best_c;
+    sm.wbest_g[wv] = best_g;
+    sm.wbest_c[wv] = best_c;
   }
-  if (has_stay) s_stay = stay;  // one run holds the current community
+  if (has_stay) sm.stay = stay;  // one run holds the current community
   __syncthreads();
   if (t == 0) {
-    double bg = wbest_g[0];
-    int32_t bc = wbest_c[0];
-    for (int w = 1; w < NW; ++w)
-      if (better(wbest_g[w], wbest_c[w], bg, bc)) {
-        bg = wbest_g[w];
-        bc = wbest_c[w];
+    double bg = sm.wbest_g[0];
+    int32_t bc = sm.wbest_c[0];
+    for (int w = 1; w < kWavesPerBlock; ++w)
+      if (better(sm.wbest_g[w], sm.wbest_c[w], bg, bc)) {
+        bg = sm.wbest_g[w];
+        bc = sm.wbest_c[w];
       }
-    target[v] = (bg > s_stay + kTol) ? bc : curr;
+    target[v] = (bg > sm.stay + kTol) ? bc : curr;
   }
+}
+
+constexpr size_t kLightSmem = sizeof(int64_t) * kWavesPerBlock * 3 * kLightDeg;
+constexpr size_t kMoveSmem = kLightSmem > sizeof(BlockMoveSmem<16>) ? kLightSmem : sizeof(BlockMoveSmem<16>);
+
+// The listed vertices of the four hub classes: one array, the largest class first.
+struct HubLists {
+  const int32_t* verts;
+  int64_t count[kHubClasses];  // per class i; verts holds class 3's vertices, then 2, 1, 0
+};
+
+// Best move of every vertex of degree <= 4096 in ONE launch (256-thread blocks): the first
+// blocks take the listed hub vertices, one block each, largest class first, so that they start
+// while the light blocks are still being dispatched; the rest take the light vertices in
+// groups of kMoveGroup per wave.  Each vertex's target[] is written by exactly one block;
+// heavier vertices are left to the sort path.
+__global__ __launch_bounds__(kThreads) void k_move(
+    HubLists hubs, const int64_t* __restrict__ indptr, const int32_t* __restrict__ indices,
+    const int64_t* __restrict__ wq, const int64_t* __restrict__ kdeg, int64_t n, const int32_t* __restrict__ comm,
+    const int32_t* __restrict__ part, const int64_t* __restrict__ sigtot, const CommSig* __restrict__ cs,
+    const uint8_t* __restrict__ active, GainParams gp, int32_t* __restrict__ target) {
+  __shared__ __align__(16) unsigned char smem[kMoveSmem];
+  int64_t b = blockIdx.x;
+  const int32_t* verts = hubs.verts;
+#define MSC_HUB_CLASS(I, ITEMS)                                                                  \
+  if (b < hubs.count[I]) {                                                                       \
+    block_move<ITEMS>(verts[b], indptr, indices, wq, kdeg, comm, part, sigtot, active, gp, target, \
+                      *reinterpret_cast<BlockMoveSmem<ITEMS>*>(smem));                           \
+    return;                                                                                      \
+  }                                                                                              \
+  b -= hubs.count[I];                                                                            \
+  verts += hubs.count[I];
+  MSC_HUB_CLASS(3, 16)
+  MSC_HUB_CLASS(2, 8)
+  MSC_HUB_CLASS(1, 4)
+  MSC_HUB_CLASS(0, 2)
+#undef MSC_HUB_CLASS
+  move_light_block(b, indptr, indices, wq, kdeg, n, comm, part, sigtot, cs, active, gp, target,
+                   reinterpret_cast<int64_t(*)[3][kLightDeg]>(smem));
 }
 
 // ------------------------------------------------------------------------------------------
@@ -820,24 +882,28 @@ struct Heavy {
   int64_t* hstart;     // nh + 1: first sorted heavy edge per heavy vertex
 };
 
-// mid: kLightDeg < degree <= mid_max; big: mid_max < degree <= big_max; heavy: above
-__global__ void k_heavy_flags(const int64_t* __restrict__ indptr, int64_t n, int64_t mid_max, int64_t big_max,
-                              int32_t* __restrict__ flag, int64_t* __restrict__ deg_h,
-                              int32_t* __restrict__ mflag, int32_t* __restrict__ bflag) {
+// hub class of each vertex (kLightDeg < degree <= hub_max: class i for degrees in
+// (256 << i, 512 << i], else -1); heavy: degree above hub_max
+__global__ void k_heavy_flags(const int64_t* __restrict__ indptr, int64_t n, int64_t hub_max,
+                              int32_t* __restrict__ flag, int64_t* __restrict__ deg_h, int32_t* __restrict__ hclass) {
   const int64_t v = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
   if (v < n) {
     const int64_t d = indptr[v + 1] - indptr[v];
-    flag[v] = d > big_max ? 1 : 0;
-    deg_h[v] = d > big_max ? d : 0;
-    mflag[v] = (d > kLightDeg && d <= mid_max) ? 1 : 0;
-    bflag[v] = (d > mid_max && d <= big_max) ? 1 : 0;
+    flag[v] = d > hub_max ? 1 : 0;
+    deg_h[v] = d > hub_max ? d : 0;
+    int cl = -1;
+    if (d > kLightDeg && d <= hub_max) cl = d <= 512 ? 0 : d <= 1024 ? 1 : d <= 2048 ? 2 : 3;
+    hclass[v] = cl;
   }
   if (v == n) {
     flag[n] = 0;
     deg_h[n] = 0;
-    mflag[n] = 0;
-    bflag[n] = 0;
+    hclass[n] = -1;
   }
+}
+__global__ void k_class_flag(const int32_t* __restrict__ hclass, int64_t n, int cl, int32_t* __restrict__ flag) {
+  const int64_t v = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (v <= n) flag[v] = (v < n && hclass[v] == cl) ? 1 : 0;
 }
 __global__ void k_heavy_compact(const int32_t* __restrict__ flag, const int32_t* __restrict__ pos, int64_t n,
                                 int32_t* __restrict__ verts) {
@@ -1369,10 +1435,6 @@ __global__ void k_q_terms_batched(const int64_t* __restrict__ tot, const int64_t
 // ------------------------------------------------------------------------------------------
 struct Ctx {
   hipStream_t st;
-  hipStream_t side = nullptr;       // big-vertex moves run here, beside k_move_light
-  hipStream_t side2 = nullptr;      // mid-vertex moves
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_join2 = nullptr;
-  int side_dev = 0;                 // device whose pool the side set came from
   int64_t n0, E0;
   double gamma;
   uint64_t thr;
@@ -1385,7 +1447,7 @@ struct Ctx {
   double* terms;
   int32_t* pass_info;       // 2 * kMaxPasses: per pass (any vertex wanted to move, moved count)
   int32_t *mv_v, *mv_old, *mv_cnt;  // committed vertices per commit block (k_commit)
-  CommSig* cs;              // (comm, part, sigtot[comm]) per vertex for k_move_light
+  CommSig* cs;              // (comm, part, sigtot[comm]) per vertex for the light path of k_move
   int64_t* red;             // small reduction outputs
   float* fred;
   // graph slots
@@ -1405,10 +1467,8 @@ struct Ctx {
   Heavy hv;
   int32_t *hflag, *hpos;
   int64_t *hdeg, *hdegoff;
-  int32_t *mflag, *mpos, *mverts;  // mid-degree vertices (one-wave sorted path)
-  int64_t nm = 0;
-  int32_t *bflag, *bpos, *bverts;  // big-degree vertices (one-wave sorted path, 96 KiB LDS)
-  int64_t nb = 0;
+  int32_t *hub_class, *hub_flag, *hub_pos, *hub_verts;  // listed hub vertices (k_move), by class
+  int64_t n_hub[kHubClasses] = {0, 0, 0, 0};
   msc_leiden_info* info;
   // batched run (samples laid out block-diagonally); null for a single graph
   int32_t* vsid = nullptr;          // sample of each vertex of the current level graph
@@ -1417,58 +1477,6 @@ struct Ctx {
   double* s_inv = nullptr;          // per-sample 1 / (twom * 2^-s)
   uint8_t* s_alive = nullptr;       // per sample: 0 once its run has ended (its vertices stay put)
 };
-
-// Side streams (default priority) and fork/join events for the concurrent degree tiers.  They
-// are runtime objects, so a call does not create or destroy them: each device keeps a pool of
-// sets, a call takes one on first need and returns it when it ends, and concurrent calls (one
-// per host thread) hold different sets.  Sets live until the process ends.
-struct SideSet {
-  hipStream_t side, side2;
-  hipEvent_t ev_fork, ev_join, ev_join2;
-};
-static std::mutex g_side_mutex;
-static std::vector<std::vector<SideSet>> g_side_free;  // per device
-
-static int acquire_side_set(Ctx& c) {
-  int dev = 0;
-  MSC_HIP(hipGetDevice(&dev));
-  {
-    std::lock_guard<std::mutex> lock(g_side_mutex);
-    if (static_cast<int>(g_side_free.size()) <= dev) g_side_free.resize(dev + 1);
-    auto& pool = g_side_free[dev];
-    if (!pool.empty()) {
-      const SideSet s = pool.back();
-      pool.pop_back();
-      c.side = s.side;
-      c.side2 = s.side2;
-      c.ev_fork = s.ev_fork;
-      c.ev_join = s.ev_join;
-      c.ev_join2 = s.ev_join2;
-      c.side_dev = dev;
-      return kOk;
-    }
-  }
-  SideSet s{};
-  MSC_HIP(hipStreamCreateWithFlags(&s.side, hipStreamNonBlocking));
-  MSC_HIP(hipStreamCreateWithFlags(&s.side2, hipStreamNonBlocking));
-  MSC_HIP(hipEventCreateWithFlags(&s.ev_fork, hipEventDisableTiming));
-  MSC_HIP(hipEventCreateWithFlags(&s.ev_join, hipEventDisableTiming));
-  MSC_HIP(hipEventCreateWithFlags(&s.ev_join2, hipEventDisableTiming));
-  c.side = s.side;
-  c.side2 = s.side2;
-  c.ev_fork = s.ev_fork;
-  c.ev_join = s.ev_join;
-  c.ev_join2 = s.ev_join2;
-  c.side_dev = dev;
-  return kOk;
-}
-
-static void release_side_set(Ctx& c) {
-  std::lock_guard<std::mutex> lock(g_side_mutex);
-  g_side_free[c.side_dev].push_back(SideSet{c.side, c.side2, c.ev_fork, c.ev_join, c.ev_join2});
-  c.side = c.side2 = nullptr;
-  c.ev_fork = c.ev_join = c.ev_join2 = nullptr;
-}
 
 static void carve_ctx(Carve& cv, Ctx& c, int64_t n, int64_t E) {
   const int64_t n1 = n + 1, E1 = E + 1;
@@ -1519,8 +1527,8 @@ static void carve_ctx(Carve& cv, Ctx& c, int64_t n, int64_t E) {
   h.hstart = cv.take<int64_t>(n1 + 1);
   c.hflag = cv.take<int32_t>(n1); c.hpos = cv.take<int32_t>(n1);
   c.hdeg = cv.take<int64_t>(n1); c.hdegoff = cv.take<int64_t>(n1);
-  c.mflag = cv.take<int32_t>(n1); c.mpos = cv.take<int32_t>(n1); c.mverts = cv.take<int32_t>(n1);
-  c.bflag = cv.take<int32_t>(n1); c.bpos = cv.take<int32_t>(n1); c.bverts = cv.take<int32_t>(n1);
+  c.hub_class = cv.take<int32_t>(n1); c.hub_flag = cv.take<int32_t>(n1); c.hub_pos = cv.take<int32_t>(n1);
+  c.hub_verts = cv.take<int32_t>(n1);
 }
 
 template <typename T>
@@ -1545,29 +1553,28 @@ static int graph_setup(Ctx& c, Graph& g) {
   Heavy& h = c.hv;
   h.nh = 0;
   h.eh = 0;
-  c.nm = 0;
-  c.nb = 0;
+  for (int i = 0; i < kHubClasses; ++i) c.n_hub[i] = 0;
   if (g.max_deg > kLightDeg) {
-    // community ids must fit the sort keys: below 2^22 for the mid tier, 2^20 for the big one
-    const int64_t mid_max = g.n < kMidMaxN ? kMidDeg : kLightDeg;
-    const int64_t big_max = g.n < kBigMaxN ? kBigDeg : mid_max;
-    k_heavy_flags<<<grid_for(g.n + 1), kThreads, 0, st>>>(g.indptr, g.n, mid_max, big_max, c.hflag, c.hdeg, c.mflag,
-                                                          c.bflag);
+    // the widest hub class whose community ids fit the sort key; above it, the sort path
+    int64_t hub_max = kLightDeg;
+    for (int i = 0; i < kHubClasses; ++i)
+      if (g.n < hub_class_max_n(i)) hub_max = int64_t(512) << i;
+    k_heavy_flags<<<grid_for(g.n + 1), kThreads, 0, st>>>(g.indptr, g.n, hub_max, c.hflag, c.hdeg, c.hub_class);
     MSC_LAUNCH_CHECK();
-    MSC_TRY(prims::exclusive_scan<int32_t>(c.mflag, c.mpos, g.n + 1, c.scan_ws, c.scan_ws_bytes, st));
-    MSC_TRY(prims::exclusive_scan<int32_t>(c.bflag, c.bpos, g.n + 1, c.scan_ws, c.scan_ws_bytes, st));
-    int32_t nm32, nb32;
-    MSC_TRY(read_scalar(c.mpos + g.n, &nm32, st));
-    MSC_TRY(read_scalar(c.bpos + g.n, &nb32, st));
-    c.nm = nm32;
-    c.nb = nb32;
-    if (c.nm > 0) {
-      k_heavy_compact<<<grid_for(g.n), kThreads, 0, st>>>(c.mflag, c.mpos, g.n, c.mverts);
+    int64_t off = 0;
+    for (int i = kHubClasses - 1; i >= 0; --i) {  // largest class first in hub_verts
+      if (int64_t(256) << i >= hub_max) continue;
+      k_class_flag<<<grid_for(g.n + 1), kThreads, 0, st>>>(c.hub_class, g.n, i, c.hub_flag);
       MSC_LAUNCH_CHECK();
-    }
-    if (c.nb > 0) {
-      k_heavy_compact<<<grid_for(g.n), kThreads, 0, st>>>(c.bflag, c.bpos, g.n, c.bverts);
-      MSC_LAUNCH_CHECK();
+      MSC_TRY(prims::exclusive_scan<int32_t>(c.hub_flag, c.hub_pos, g.n + 1, c.scan_ws, c.scan_ws_bytes, st));
+      int32_t cnt;
+      MSC_TRY(read_scalar(c.hub_pos + g.n, &cnt, st));
+      if (cnt > 0) {
+        k_heavy_compact<<<grid_for(g.n), kThreads, 0, st>>>(c.hub_flag, c.hub_pos, g.n, c.hub_verts + off);
+        MSC_LAUNCH_CHECK();
+      }
+      c.n_hub[i] = cnt;
+      off += cnt;
     }
     MSC_TRY(prims::exclusive_scan<int32_t>(c.hflag, c.hpos, g.n + 1, c.scan_ws, c.scan_ws_bytes, st));
     MSC_TRY(prims::exclusive_scan<int64_t>(c.hdeg, c.hdegoff, g.n + 1, c.scan_ws, c.scan_ws_bytes, st));
@@ -1652,37 +1659,18 @@ static int local_moving(Ctx& c, const Graph& g, int64_t seed, const int32_t* par
       k_refresh_cs<<<grid_for(g.n), kThreads, 0, st>>>(g.n, c.comm, part, c.sigtot, c.cs);
       MSC_LAUNCH_CHECK();
     }
-    // the three move kernels write target[] of disjoint vertex sets and only read the rest:
-    // the listed (mid / big) vertices run on the side stream concurrently with the light ones
-    const bool fork = c.nm > 0 || c.nb > 0;
-    if (fork && !c.side) MSC_TRY(acquire_side_set(c));  // most small graphs never list a vertex
-    if (fork) {
-      MSC_HIP(hipEventRecord(c.ev_fork, st));
-      if (c.nb > 0) MSC_HIP(hipStreamWaitEvent(c.side, c.ev_fork, 0));
-      if (c.nm > 0) MSC_HIP(hipStreamWaitEvent(c.side2, c.ev_fork, 0));
+    // one launch moves every vertex up to the widest hub class: the listed hub vertices (one
+    // block each) and the light ones; heavier vertices go through the sort path
+    HubLists hubs;
+    hubs.verts = c.hub_verts;
+    int64_t nblocks = ceil_div(ceil_div(g.n, kMoveGroup) * kWave, kThreads);
+    for (int i = 0; i < kHubClasses; ++i) {
+      hubs.count[i] = c.n_hub[i];
+      nblocks += c.n_hub[i];
     }
-    if (c.nb > 0) {
-      k_move_block<1024><<<static_cast<unsigned>(c.nb), 1024, 0, c.side>>>(
-          c.bverts, g.indptr, g.indices, g.wq, g.k, c.comm, part, c.sigtot, c.active, gp, c.target);
-      MSC_LAUNCH_CHECK();
-    }
-    if (c.nm > 0) {
-      k_move_block<256><<<static_cast<unsigned>(c.nm), 256, 0, c.side2>>>(
-          c.mverts, g.indptr, g.indices, g.wq, g.k, c.comm, part, c.sigtot, c.active, gp, c.target);
-      MSC_LAUNCH_CHECK();
-    }
-    // the listed vertices go first so that their large-LDS blocks find free CUs
-    k_move_light<<<grid_for(ceil_div(g.n, kMoveGroup) * kWave), kThreads, 0, st>>>(
-        g.indptr, g.indices, g.wq, g.k, g.n, c.comm, part, c.sigtot, c.cs, c.active, gp, c.target);
+    k_move<<<static_cast<unsigned>(nblocks), kThreads, 0, st>>>(hubs, g.indptr, g.indices, g.wq, g.k, g.n, c.comm,
+                                                               part, c.sigtot, c.cs, c.active, gp, c.target);
     MSC_LAUNCH_CHECK();
-    if (c.nb > 0) {
-      MSC_HIP(hipEventRecord(c.ev_join, c.side));
-      MSC_HIP(hipStreamWaitEvent(st, c.ev_join, 0));
-    }
-    if (c.nm > 0) {
-      MSC_HIP(hipEventRecord(c.ev_join2, c.side2));
-      MSC_HIP(hipStreamWaitEvent(st, c.ev_join2, 0));
-    }
     MSC_TRY(heavy_moves(c, g, c.comm, part, gp));
     k_commit<<<static_cast<unsigned>(nblk), kCommitThreads, 0, st>>>(
         c.comm, c.target, g.n, pass_key(key, p), c.thr, c.commit, c.active, c.pass_info + 2 * p, c.mv_v, c.mv_old,
@@ -1900,18 +1888,6 @@ static int ceil_log2(double x) {
   const double m = std::frexp(x, &e);
   return m == 0.5 ? e - 1 : e;
 }
-
-// returns the call's side set to the pool on every return path of an entry point, once the
-// side streams have drained (the main stream already waits on them through the join events)
-struct SideGuard {
-  Ctx& c;
-  ~SideGuard() {
-    if (!c.side) return;
-    (void)hipStreamSynchronize(c.side);
-    (void)hipStreamSynchronize(c.side2);
-    release_side_set(c);
-  }
-};
 
 // set-up shared by msc_leiden and msc_louvain: arguments, workspace, the fixed-point
 // exponent and the quantised level-0 graph g0
@@ -2281,7 +2257,6 @@ extern "C" int msc_leiden(const int64_t* indptr, const int32_t* indices, const f
                           const int32_t* init_labels, int32_t* out_labels, double* out_modularity,
                           msc_leiden_info* info, void* workspace, size_t workspace_bytes, void* stream) {
   Ctx c{};
-  SideGuard side_guard{c};
   Graph g0;
   MSC_TRY(begin_run(c, g0, indptr, indices, weights, n, nnz, resolution, commit_prob, out_labels, info, workspace,
                     workspace_bytes, stream));
@@ -2311,7 +2286,6 @@ extern "C" int msc_louvain(const int64_t* indptr, const int32_t* indices, const 
                            size_t workspace_bytes, void* stream) {
   MSC_REQUIRE(max_levels >= 1 && max_levels <= kMaxLevels, "louvain: max_levels must be in [1, %d]", kMaxLevels);
   Ctx c{};
-  SideGuard side_guard{c};
   Graph g0;
   MSC_TRY(begin_run(c, g0, indptr, indices, weights, n, nnz, resolution, commit_prob, out_labels, info, workspace,
                     workspace_bytes, stream));
@@ -2340,7 +2314,6 @@ extern "C" int msc_leiden_batched(const int64_t* indptr, const int32_t* indices,
                                   int n_iterations, double commit_prob, int32_t* out_labels, double* out_modularity,
                                   void* workspace, size_t workspace_bytes, void* stream) {
   Ctx c{};
-  SideGuard side_guard{c};
   BatchRun b;
   Graph g0;
   MSC_TRY(begin_batched(c, b, g0, indptr, indices, weights, n, nnz, offsets, n_samples, resolution, commit_prob,

@@ -52,14 +52,15 @@ def test_leiden_resolutions(c1_data, res):
 
 
 def test_leiden_heavy_vertices():
-    """Hubs of degree 90 (two neighbours per lane on the one-wave path), 300 (sixteen-per-lane
-    list path), 1500 (sixty-four-per-lane list path) and 5000 (the sort path over heavy
-    edges); the frontier rule applies with them present."""
+    """Hubs of degree 90 (two neighbours per lane on the one-wave path), 300, 700, 1500 and
+    3000 (the four block classes: 512, 1024, 2048 and 4096 sorted keys, 2 to 16 per thread)
+    and 5000 (the sort path over heavy edges); the frontier rule applies with them present."""
     rng = np.random.default_rng(5)
     n = 8000
     rows, cols = [], []
-    for hub, deg in [(0, 5000), (1, 1500), (2, 300), (3, 90)]:
-        nb = rng.choice(np.arange(4, n), deg, replace=False)
+    hubs = [(0, 5000), (1, 3000), (2, 1500), (3, 700), (4, 300), (5, 90)]
+    for hub, deg in hubs:
+        nb = rng.choice(np.arange(len(hubs), n), deg, replace=False)
         rows += [hub] * deg
         cols += nb.tolist()
     # planted blocks of 100
