@@ -105,26 +105,33 @@ __device__ __forceinline__ uint64_t shfl_up_u64(uint64_t x, int d) {
   return (static_cast<uint64_t>(hi) << 32) | lo;
 }
 
-// Value of lane l ^ J for a compile-time J, with all 64 lanes active.  J = 1, 2: DPP quad_perm;
-// J = 4, 8: DPP row_shl:J (lane i reads i + J) or row_shr:J (reads i - J) by lane bit J;
-// J = 16: ds_swizzle xor within 32 lanes; J = 32: ds_bpermute.  (The DPP forms are VALU
-// operands, the other two go through the LDS crossbar.)
+// Value of lane l ^ J for a compile-time J, with all 64 lanes active, in VALU operations only.
+// J = 1, 2: DPP quad_perm; J = 4: DPP row_shl:4 (lane i reads i + 4) or row_shr:4 (reads
+// i - 4) by lane bit 2; J = 8: DPP row_ror:8 (lane i of a 16-lane row reads (i + 8) mod 16);
+// J = 16, 32: v_permlane16_swap / v_permlane32_swap of the value with itself, which yield
+// (rows 0, 0, 2, 2 | rows 1, 1, 3, 3) and (lanes 0-31 twice | lanes 32-63 twice), and the
+// lane's half picks the other one.  The DPP moves take old = 0 with bound_ctrl, so that the
+// compiler can fold them into the instruction that uses them.
 template <int J>
 __device__ __forceinline__ uint32_t lane_xor(uint32_t v) {
   const int x = static_cast<int>(v);
   if constexpr (J == 1) {
-    return static_cast<uint32_t>(__builtin_amdgcn_update_dpp(x, x, 0xB1, 0xF, 0xF, false));
+    return static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, x, 0xB1, 0xF, 0xF, true));
   } else if constexpr (J == 2) {
-    return static_cast<uint32_t>(__builtin_amdgcn_update_dpp(x, x, 0x4E, 0xF, 0xF, false));
-  } else if constexpr (J == 4 || J == 8) {
-    const int up = __builtin_amdgcn_update_dpp(x, x, 0x100 | J, 0xF, 0xF, false);
-    const int dn = __builtin_amdgcn_update_dpp(x, x, 0x110 | J, 0xF, 0xF, false);
-    return static_cast<uint32_t>((__lane_id() & J) ? dn : up);
+    return static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, x, 0x4E, 0xF, 0xF, true));
+  } else if constexpr (J == 4) {
+    const int up = __builtin_amdgcn_update_dpp(0, x, 0x104, 0xF, 0xF, true);
+    const int dn = __builtin_amdgcn_update_dpp(0, x, 0x114, 0xF, 0xF, true);
+    return static_cast<uint32_t>((__lane_id() & 4) ? dn : up);
+  } else if constexpr (J == 8) {
+    return static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, x, 0x128, 0xF, 0xF, true));
   } else if constexpr (J == 16) {
-    return static_cast<uint32_t>(__builtin_amdgcn_ds_swizzle(x, (0x10 << 10) | 0x1F));
+    const auto p = __builtin_amdgcn_permlane16_swap(x, x, false, false);
+    return static_cast<uint32_t>((__lane_id() & 16) ? p[0] : p[1]);
   } else {
     static_assert(J == 32, "lane_xor: J must be a power of two below 64");
-    return static_cast<uint32_t>(__shfl_xor(x, 32, kWave));
+    const auto p = __builtin_amdgcn_permlane32_swap(x, x, false, false);
+    return static_cast<uint32_t>((__lane_id() & 32) ? p[0] : p[1]);
   }
 }
 template <int J>
