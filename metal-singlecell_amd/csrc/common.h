@@ -106,8 +106,9 @@ __device__ __forceinline__ uint64_t shfl_up_u64(uint64_t x, int d) {
 }
 
 // Value of lane l ^ J for a compile-time J, with all 64 lanes active, in VALU operations only.
-// J = 1, 2: DPP quad_perm; J = 4: DPP row_shl:4 (lane i reads i + 4) or row_shr:4 (reads
-// i - 4) by lane bit 2; J = 8: DPP row_ror:8 (lane i of a 16-lane row reads (i + 8) mod 16);
+// J = 1, 2: DPP quad_perm; J = 4: DPP row_half_mirror (lane i of an 8-lane half-row reads
+// i ^ 7) then quad_perm [3,2,1,0] (reads i ^ 3); J = 8: DPP row_ror:8 (lane i of a 16-lane
+// row reads (i + 8) mod 16);
 // J = 16, 32: v_permlane16_swap / v_permlane32_swap of the value with itself, which yield
 // (rows 0, 0, 2, 2 | rows 1, 1, 3, 3) and (lanes 0-31 twice | lanes 32-63 twice), and the
 // lane's half picks the other one.  The DPP moves take old = 0 with bound_ctrl, so that the
@@ -120,9 +121,8 @@ __device__ __forceinline__ uint32_t lane_xor(uint32_t v) {
   } else if constexpr (J == 2) {
     return static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, x, 0x4E, 0xF, 0xF, true));
   } else if constexpr (J == 4) {
-    const int up = __builtin_amdgcn_update_dpp(0, x, 0x104, 0xF, 0xF, true);
-    const int dn = __builtin_amdgcn_update_dpp(0, x, 0x114, 0xF, 0xF, true);
-    return static_cast<uint32_t>((__lane_id() & 4) ? dn : up);
+    const int m7 = __builtin_amdgcn_update_dpp(0, x, 0x141, 0xF, 0xF, true);
+    return static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, m7, 0x1B, 0xF, 0xF, true));
   } else if constexpr (J == 8) {
     return static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, x, 0x128, 0xF, 0xF, true));
   } else if constexpr (J == 16) {

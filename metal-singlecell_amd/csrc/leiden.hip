@@ -375,16 +375,38 @@ __device__ __forceinline__ int64_t sig_of(const int4& r) {
                               static_cast<uint32_t>(r.z));
 }
 
-// one butterfly step of the wave argmax over (gain, community), ties to the smaller community
+// one butterfly step of the wave maximum of a double
 template <int M>
-__device__ __forceinline__ void argmax_step(double& g, int32_t& c) {
-  const double og = __longlong_as_double(
+__device__ __forceinline__ double max_step(double g) {
+  const double o = __longlong_as_double(
       static_cast<long long>(lane_xor_u64<M>(static_cast<uint64_t>(__double_as_longlong(g)))));
-  const int32_t oc = static_cast<int32_t>(lane_xor<M>(static_cast<uint32_t>(c)));
-  if (better(og, oc, g, c)) {
-    g = og;
-    c = oc;
-  }
+  return o > g ? o : g;
+}
+template <int M>
+__device__ __forceinline__ uint32_t min_step(uint32_t c) {
+  const uint32_t o = lane_xor<M>(c);
+  return o < c ? o : c;
+}
+
+// Wave argmax over (gain, community), ties to the smaller community: the largest gain, then
+// the smallest community among the lanes that hold it (the lexicographic maximum of
+// (gain, -community), the order of better()).  Communities are non-negative.
+__device__ __forceinline__ void wave_argmax(double& g, int32_t& c) {
+  double m = max_step<1>(g);
+  m = max_step<2>(m);
+  m = max_step<4>(m);
+  m = max_step<8>(m);
+  m = max_step<16>(m);
+  m = max_step<32>(m);
+  uint32_t k = g == m ? static_cast<uint32_t>(c) : 0x7FFFFFFFu;
+  k = min_step<1>(k);
+  k = min_step<2>(k);
+  k = min_step<4>(k);
+  k = min_step<8>(k);
+  k = min_step<16>(k);
+  k = min_step<32>(k);
+  g = m;
+  c = static_cast<int32_t>(k);
 }
 
 // The decision for vertex v from its eligible neighbour slots e = i * 64 + lane: key[i] =
@@ -446,12 +468,7 @@ __device__ __forceinline__ void move_core(int64_t v, uint32_t (&key)[NI], int nv
       if (better(g, ci, best_g, best_c)) { best_g = g; best_c = ci; }
     }
   }
-  argmax_step<1>(best_g, best_c);
-  argmax_step<2>(best_g, best_c);
-  argmax_step<4>(best_g, best_c);
-  argmax_step<8>(best_g, best_c);
-  argmax_step<16>(best_g, best_c);
-  argmax_step<32>(best_g, best_c);
+  wave_argmax(best_g, best_c);
   const uint64_t sm = __ballot(has_stay);
   if (sm) {
     const int src = __ffsll(static_cast<long long>(sm)) - 1;
