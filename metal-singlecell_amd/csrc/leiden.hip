@@ -326,38 +326,80 @@ constexpr int kHubClasses = 4;
 __host__ __device__ constexpr int64_t hub_class_max_n(int i) { return int64_t(1) << (23 - i); }
 constexpr int kMoveGroup = 8;  // consecutive vertices per wave on the light path of k_move
 
-// Bitonic network over the 64 * NI slots e = i * 64 + l of a wave (keys in registers).  The
-// stage sequence is unrolled by template recursion so that every register index is a
-// compile-time constant for any NI.
-template <int NI, int K, int J>
-__device__ __forceinline__ void bitonic_stage(uint32_t (&key)[NI], int l) {
-  if constexpr (J >= kWave) {  // partner slot in another item of the same lane
+// Value of lane l ^ M for M = 2^j - 1 (the mirror image of l within its block of 2^j lanes):
+// M = 1, 3: DPP quad_perm; 7: row_half_mirror; 15: row_mirror; 31, 63: row_mirror followed
+// by the stride-16 (and stride-32) exchange.
+template <int M>
+__device__ __forceinline__ uint32_t lane_mirror(uint32_t v) {
+  const int x = static_cast<int>(v);
+  if constexpr (M == 1) {
+    return lane_xor<1>(v);
+  } else if constexpr (M == 3) {
+    return static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, x, 0x1B, 0xF, 0xF, true));
+  } else if constexpr (M == 7) {
+    return static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, x, 0x141, 0xF, 0xF, true));
+  } else if constexpr (M == 15) {
+    return static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, x, 0x140, 0xF, 0xF, true));
+  } else if constexpr (M == 31) {
+    return lane_xor<16>(lane_mirror<15>(v));
+  } else {
+    static_assert(M == 63, "lane_mirror: M must be 2^j - 1 below 64");
+    return lane_xor<32>(lane_xor<16>(lane_mirror<15>(v)));
+  }
+}
+
+// Bitonic sort of the 64 * NI slots e = i * 64 + l of a wave (keys in registers), ascending,
+// in the form whose comparators all point the same way: merge phase K first compares slot e
+// with its mirror e ^ (K - 1) inside the block of K slots, then e with e ^ J for J = K / 4 ..
+// 1; the lower slot of each pair keeps the minimum.  The choice between minimum and maximum
+// then depends on the stride only (six lane masks), not on the stage.  The stage sequence
+// is unrolled by template recursion so that every register index is a compile-time constant.
+template <int NI, int J>
+__device__ __forceinline__ void half_clean(uint32_t (&key)[NI], int l) {
+  if constexpr (J >= kWave) {  // partner slot in item i ^ (J / 64) of the same lane
 #pragma unroll
     for (int i = 0; i < NI; ++i) {
       const int ip = i ^ (J / kWave);
       if (ip > i) {
-        const bool asc = ((i * kWave + l) & K) == 0;
         const uint32_t a = key[i], b = key[ip];
-        const uint32_t lo = a < b ? a : b, hi = a < b ? b : a;
-        key[i] = asc ? lo : hi;
-        key[ip] = asc ? hi : lo;
+        key[i] = a < b ? a : b;
+        key[ip] = a < b ? b : a;
       }
     }
   } else {  // partner slot in lane l ^ J of the same item
+    const bool lower = (l & J) == 0;
 #pragma unroll
     for (int i = 0; i < NI; ++i) {
       const uint32_t p = lane_xor<J>(key[i]);
-      const bool asc = ((i * kWave + l) & K) == 0;
-      const bool lower = (l & J) == 0;
       const uint32_t lo = key[i] < p ? key[i] : p, hi = key[i] < p ? p : key[i];
-      key[i] = (lower == asc) ? lo : hi;
+      key[i] = lower ? lo : hi;
     }
   }
-  if constexpr (J > 1) bitonic_stage<NI, K, J / 2>(key, l);
+  if constexpr (J > 1) half_clean<NI, J / 2>(key, l);
 }
 template <int NI, int K>
 __device__ __forceinline__ void bitonic_sort_slots(uint32_t (&key)[NI], int l) {
-  bitonic_stage<NI, K, K / 2>(key, l);
+  if constexpr (K <= kWave) {  // mirror partner in lane l ^ (K - 1) of the same item
+    const bool lower = (l & (K / 2)) == 0;
+#pragma unroll
+    for (int i = 0; i < NI; ++i) {
+      const uint32_t p = lane_mirror<K - 1>(key[i]);
+      const uint32_t lo = key[i] < p ? key[i] : p, hi = key[i] < p ? p : key[i];
+      key[i] = lower ? lo : hi;
+    }
+  } else {  // mirror partner in item i ^ (K / 64 - 1), lane l ^ 63; item i < ip is the lower one
+#pragma unroll
+    for (int i = 0; i < NI; ++i) {
+      const int ip = i ^ (K / kWave - 1);
+      if (ip > i) {
+        const uint32_t t = lane_mirror<63>(key[ip]);
+        const uint32_t lo = key[i] < t ? key[i] : t, hi = key[i] < t ? t : key[i];
+        key[i] = lo;
+        key[ip] = lane_mirror<63>(hi);
+      }
+    }
+  }
+  if constexpr (K >= 4) half_clean<NI, K / 4>(key, l);
   if constexpr (K < NI * kWave) bitonic_sort_slots<NI, 2 * K>(key, l);
 }
 
