@@ -720,6 +720,24 @@ __global__ __launch_bounds__(kSelThreads, kSelBlocksPerCU) void select32_kernel(
   float tq = __uint_as_float(0x7F800000u);
   int cnt = 0;
   const int64_t ntiles = tm_n;
+  // With partition-pruned ranges the walk starts at the tile that holds the block's first query
+  // (its own partition, where its nearest candidates lie) and wraps around, so the admission
+  // threshold tightens early and fewer M-blocks take the hit path.  The tiles scanned and each
+  // candidate's place in its tile (hence its lane) do not change.
+  int32_t tstart = 0;
+  if (rng4 && qrows) {
+    const int32_t q0 = qrows[static_cast<int64_t>(blockIdx.x) * kSelQueries];
+    if (q0 >= tm_b0 && q0 < tm_e0) tstart = (q0 - tm_b0) / kTile;
+    else if (q0 >= tm_b1 && q0 < tm_e1) tstart = tm_t1 + (q0 - tm_b1) / kTile;
+    else if (q0 >= tm_b2 && q0 < tm_e2) tstart = tm_t2 + (q0 - tm_b2) / kTile;
+    else if (q0 >= tm_b3 && q0 < tm_e3) tstart = tm_t3 + (q0 - tm_b3) / kTile;
+  }
+  tstart = uniform_i32(tstart);
+  // the tile of the walk's step T (0 <= T < ntiles)
+  auto rot = [&](const int64_t T) __attribute__((always_inline)) -> int32_t {
+    const int32_t r = static_cast<int32_t>(T) + tstart;
+    return r >= static_cast<int32_t>(ntiles) ? r - static_cast<int32_t>(ntiles) : r;
+  };
 
   // this wave's loads of a tile: wave load u = wv + kSelWaves j (clamped), lane l fills LDS chunk
   // 64 u + l = (row, chunk) of the tile
@@ -733,7 +751,8 @@ __global__ __launch_bounds__(kSelThreads, kSelBlocksPerCU) void select32_kernel(
     src_off[j] = row * rowb + min(col, C::CH - 1) * 16;
   }
   auto load_tile = [&](char* buf, const int64_t T) __attribute__((always_inline)) {
-    const char* base = Xb + static_cast<int64_t>(MSC_TROW(T < ntiles ? T : ntiles - 1)) * rowb;
+    const int32_t tr = rot(T < ntiles ? T : ntiles - 1);
+    const char* base = Xb + static_cast<int64_t>(MSC_TROW(tr)) * rowb;
 #pragma unroll
     for (int j = 0; j < C::PERW; ++j)
       lds_load16(base + src_off[j], buf + lds_off[j]);
@@ -801,7 +820,10 @@ __global__ __launch_bounds__(kSelThreads, kSelBlocksPerCU) void select32_kernel(
       load_tile(Z, t + 2);
       read_mb(X, 1, af[1]);
       mfma_mb(af[0], acc0);
-      if (have1) select_mb(acc1, MSC_TROW(t - 1) + 32, MSC_TEND(t - 1));
+      if (have1) {
+        const int32_t tp = rot(t - 1);
+        select_mb(acc1, MSC_TROW(tp) + 32, MSC_TEND(tp));
+      }
       mfma_mb(af[1], acc1);
       // this wave's loads of tile t+1 are done; the barrier publishes everyone's.  A plain
       // barrier: __syncthreads' release fence would also wait for tile t+2's loads.
@@ -810,7 +832,8 @@ __global__ __launch_bounds__(kSelThreads, kSelBlocksPerCU) void select32_kernel(
       __builtin_amdgcn_s_barrier();
       asm volatile("" ::: "memory");
       read_mb(Y, 0, af[0]);
-      select_mb(acc0, MSC_TROW(t), MSC_TEND(t));
+      const int32_t tc = rot(t);
+      select_mb(acc0, MSC_TROW(tc), MSC_TEND(tc));
       have1 = true;
     };
     int64_t t = 0;
@@ -821,7 +844,8 @@ __global__ __launch_bounds__(kSelThreads, kSelBlocksPerCU) void select32_kernel(
     }
     if (t < ntiles) step(tA, tB, tC, t);
     if (t + 1 < ntiles) step(tB, tC, tA, t + 1);
-    select_mb(acc1, MSC_TROW(ntiles - 1) + 32, MSC_TEND(ntiles - 1));
+    const int32_t tl = rot(ntiles - 1);
+    select_mb(acc1, MSC_TROW(tl) + 32, MSC_TEND(tl));
   }
 
   flush();
