@@ -11,10 +11,12 @@ One step = one full pass from the device-resident embedding to device labels:
 ``value`` = cells / (mean step time) for that BF16+re-rank form.  The plain FP32 GPU form
 (FP32 MFMA ranking) is timed beside it on the same input and reported in ``fp32_form``;
 both forms return identical graphs and labels; ``fp32_form.full_scan`` is the FP32 form without
-the partition-pruned candidate ranges (every query scans all n rows).  ``roofline`` covers the dominant MFMA
-kernel (the kNN select kernel): algorithmic flops 2 d per scanned (query, candidate) pair
-per launch -- s n^2 pairs, s the scanned fraction of the partition-pruned layout, 1 without
-it -- over its HIP-event duration, against the 2.5 PFLOP/s dense BF16 peak.  ``cpu_baseline`` times the project's
+the partition-pruned candidate ranges (every query scans all n rows).  ``roofline`` covers the kernel
+with the most time per step, the Leiden move kernel (one launch per pass): algorithmic bytes
+8 E_l + 24 n_l per launch over its HIP-event duration, against 8 TB/s HBM.  ``roofline_select``
+covers the dominant MFMA kernel (the kNN select kernel): algorithmic flops 2 d per scanned
+(query, candidate) pair per launch -- s n^2 pairs, s the scanned fraction of the partition-pruned
+layout, 1 without it -- over its HIP-event duration, against the 2.5 PFLOP/s dense BF16 peak.  ``cpu_baseline`` times the project's
 CPU restatement (oracle/) of the same three stages on bounded samples on this host.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N] [--no-cpu]
@@ -72,6 +74,35 @@ def timed(Xd, k, precision, steps, warmup, prune=None):
     torch.cuda.synchronize()
     total = time.perf_counter() - t0
     return total / steps, recs, labels
+
+
+def move_roofline(Xd, k, select_ms):
+    """Roofline of the Leiden move kernel (k_move, one launch per pass), the kernel with the
+    most time per step: one more Leiden call on the step's graph with every k_move launch
+    bracketed by HIP events on the call's stream.  Algorithmic bytes per launch: 8 E_l + 24 n_l
+    (every edge and vertex of the level graph once per pass); HBM traffic per launch from the
+    committed FETCH_SIZE / WRITE_SIZE summary of the same kernel and workload."""
+    import torch
+    from metalsinglecell import _hip
+    idx, dist = _hip.knn_exact(Xd, k, precision="bf16")
+    (ip, ind, dat), _ = _hip.fuzzy_connectivities(idx, dist)
+    n = idx.shape[0]
+    _, _, info = _hip.leiden(ip, ind, dat, n, resolution=1.0, random_state=0, n_iterations=2, time_moves=True)
+    torch.cuda.synchronize()
+    launches = info["move_launches"]
+    avg_ms = info["move_ms"] / launches
+    bytes_avg = info["move_model_bytes"] / launches
+    achieved = bytes_avg / (avg_ms * 1e-3) / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "leiden_move_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("bytes_per_launch")
+    return {"bound": "hbm", "kernel": "k_move (Leiden best move, one launch per pass)", "achieved": achieved,
+            "peak": 8000.0, "unit": "GB/s", "frac": achieved / 8000.0, "traffic": traffic,
+            "bytes_per_launch": bytes_avg, "avg_launch_ms": avg_ms, "launches_per_step": launches,
+            "ms_per_step": info["move_ms"], "select_ms_per_step": select_ms,
+            "bytes_model": "8 E_l + 24 n_l per launch (every edge and vertex of the level graph once per pass)"}
 
 
 def cpu_baseline(X, k, threads, gpu_idx=None):
@@ -551,11 +582,12 @@ def main():
     if os.path.exists(tpath):
         with open(tpath) as f:
             traffic = json.load(f).get("bytes_per_launch")
-    roof = {"bound": "mfma", "kernel": "select32_kernel<7> (BF16, 32x32x16 MFMA)", "achieved": achieved,
-            "peak": PEAK["bf16"], "unit": "TFLOP/s", "frac": achieved / PEAK["bf16"],
-            "traffic": traffic, "flops_per_launch": flops, "avg_launch_ms": sel_ms,
-            "scanned_fraction": scanned,
-            "brute_force_equivalent_tflops": 2.0 * n * n * d / (sel_ms * 1e-3) / 1e12}
+    roof_sel = {"bound": "mfma", "kernel": "select32_kernel<7> (BF16, 32x32x16 MFMA)", "achieved": achieved,
+                "peak": PEAK["bf16"], "unit": "TFLOP/s", "frac": achieved / PEAK["bf16"],
+                "traffic": traffic, "flops_per_launch": flops, "avg_launch_ms": sel_ms,
+                "scanned_fraction": scanned,
+                "brute_force_equivalent_tflops": 2.0 * n * n * d / (sel_ms * 1e-3) / 1e12}
+    roof = move_roofline(Xd, a.k, sel_ms)
 
     from metalsinglecell import _hip
     gi, gd = _hip.knn_exact(Xd, a.k, precision="bf16")
@@ -616,7 +648,7 @@ def main():
         "stages_s": {"knn": last["knn_s"], "connectivities": last["conn_s"], "leiden": last["leiden_s"]},
         "knn_info": last["knn"], "modularity": last["q"], "n_clusters": last["n_clusters"],
         "fp32_form": fp32, "precision": prec, "end_to_end": e2e, "batched": batch, "serving": serve,
-        "extensions": ext, "roofline": roof,
+        "extensions": ext, "roofline": roof, "roofline_select": roof_sel,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -228,6 +228,14 @@ typedef struct {
   int scale_exponent;           /* s: weights quantised as rint(w * 2^s) */
   int64_t twom_fixed;           /* total directed weight in fixed point */
   msc_leiden_level levels[MSC_LEIDEN_MAX_RECORDS];
+  /* in: nonzero to time every launch of the move kernel (one per pass) with HIP events on
+   * the call's stream; off, the three outputs below are zero.  Timing adds two event records
+   * per pass and creates (and destroys) 16 events in the call. */
+  int32_t time_moves;
+  int32_t move_launches;        /* out: move-kernel launches timed */
+  double move_ms;               /* out: their summed durations */
+  double move_model_bytes;      /* out: sum over them of 8 E_l + 24 n_l (the level's edges and
+                                 * vertices, every edge counted once per pass) */
 } msc_leiden_info;
 
 int msc_leiden_workspace_size(int64_t n, int64_t nnz, size_t* bytes /*host*/);

@@ -1529,6 +1529,13 @@ struct Ctx {
   int32_t *hub_class, *hub_flag, *hub_pos, *hub_verts;  // listed hub vertices (k_move), by class
   int64_t n_hub[kHubClasses] = {0, 0, 0, 0};
   msc_leiden_info* info;
+  // move-kernel timing (info->time_moves): an event pair per pass in a ring, read at the
+  // host syncs of the pass loop
+  static constexpr int kTimeRing = 16;  // pairs; > kSyncEvery
+  bool time_moves = false;
+  hipEvent_t tev[2 * kTimeRing] = {};
+  int t_pending = 0, t_next = 0;
+  double t_bytes_pending[kTimeRing] = {};
   // batched run (samples laid out block-diagonally); null for a single graph
   int32_t* vsid = nullptr;          // sample of each vertex of the current level graph
   int64_t* d_vbase = nullptr;       // sample vertex offsets of the current level graph (S + 1)
@@ -1598,6 +1605,29 @@ static int read_scalar(const T* dev, T* host, hipStream_t st) {
 }
 
 // graph derived quantities: k, twom, max degree, heavy structure
+// destroys the timing events of a call on every return path
+struct TimingGuard {
+  Ctx& c;
+  ~TimingGuard() {
+    for (auto& e : c.tev)
+      if (e) (void)hipEventDestroy(e);
+  }
+};
+
+// adds the pending move-kernel timings to info (events complete: called after a host sync)
+static int collect_move_times(Ctx& c) {
+  for (int i = 0; i < c.t_pending; ++i) {
+    const int slot = (c.t_next - c.t_pending + i + Ctx::kTimeRing) % Ctx::kTimeRing;
+    float ms = 0.0f;
+    MSC_HIP(hipEventElapsedTime(&ms, c.tev[2 * slot], c.tev[2 * slot + 1]));
+    c.info->move_ms += ms;
+    c.info->move_model_bytes += c.t_bytes_pending[slot];
+    c.info->move_launches += 1;
+  }
+  c.t_pending = 0;
+  return kOk;
+}
+
 static int graph_setup(Ctx& c, Graph& g) {
   hipStream_t st = c.st;
   k_degrees<<<grid_for(g.n * kWave), kThreads, 0, st>>>(g.indptr, g.wq, g.n, g.k);
@@ -1727,9 +1757,23 @@ static int local_moving(Ctx& c, const Graph& g, int64_t seed, const int32_t* par
       hubs.count[i] = c.n_hub[i];
       nblocks += c.n_hub[i];
     }
+    const int tslot = c.t_next;
+    if (c.time_moves) {
+      if (c.t_pending == Ctx::kTimeRing) {  // ring full (no sync for a while): drain it
+        MSC_HIP(hipStreamSynchronize(st));
+        MSC_TRY(collect_move_times(c));
+      }
+      MSC_HIP(hipEventRecord(c.tev[2 * tslot], st));
+    }
     k_move<<<static_cast<unsigned>(nblocks), kThreads, 0, st>>>(hubs, g.indptr, g.indices, g.wq, g.k, g.n, c.comm,
                                                                part, c.sigtot, c.cs, c.active, gp, c.target);
     MSC_LAUNCH_CHECK();
+    if (c.time_moves) {
+      MSC_HIP(hipEventRecord(c.tev[2 * tslot + 1], st));
+      c.t_bytes_pending[tslot] = 8.0 * static_cast<double>(g.E) + 24.0 * static_cast<double>(g.n);
+      c.t_next = (tslot + 1) % Ctx::kTimeRing;
+      ++c.t_pending;
+    }
     MSC_TRY(heavy_moves(c, g, c.comm, part, gp));
     k_commit<<<static_cast<unsigned>(nblk), kCommitThreads, 0, st>>>(
         c.comm, c.target, g.n, pass_key(key, p), c.thr, c.commit, c.active, c.pass_info + 2 * p, c.mv_v, c.mv_old,
@@ -1752,12 +1796,17 @@ static int local_moving(Ctx& c, const Graph& g, int64_t seed, const int32_t* par
       int32_t info[2];  // (any vertex wanted to move, moved count)
       MSC_HIP(hipMemcpyAsync(info, c.pass_info + 2 * p, sizeof(info), hipMemcpyDeviceToHost, st));
       MSC_HIP(hipStreamSynchronize(st));
+      if (c.time_moves) MSC_TRY(collect_move_times(c));
       if (!info[0]) {
         ++p;
         break;
       }
       inc = can_inc && info[1] <= kDeltaSwitch;
     }
+  }
+  if (c.time_moves && c.t_pending > 0) {
+    MSC_HIP(hipStreamSynchronize(st));
+    MSC_TRY(collect_move_times(c));
   }
   *passes_out = std::min(p, kMaxPasses);
   if (delta_passes_out) *delta_passes_out = n_inc;
@@ -1963,7 +2012,15 @@ static int begin_run(Ctx& c, Graph& g0, const int64_t* indptr, const int32_t* in
   c.gamma = resolution;
   c.thr = static_cast<uint64_t>(std::min(std::floor(commit_prob * 4294967296.0), 4294967296.0));
   c.info = info;
-  if (info) info->n_levels = 0;
+  if (info) {
+    info->n_levels = 0;
+    info->move_launches = 0;
+    info->move_ms = 0.0;
+    info->move_model_bytes = 0.0;
+    c.time_moves = info->time_moves != 0;
+  }
+  if (c.time_moves)
+    for (auto& e : c.tev) MSC_HIP(hipEventCreate(&e));
   Carve cv(workspace, workspace_bytes);
   carve_ctx(cv, c, n, nnz);
   int64_t* wq0 = cv.take<int64_t>(nnz + 1);
@@ -2316,6 +2373,7 @@ extern "C" int msc_leiden(const int64_t* indptr, const int32_t* indices, const f
                           const int32_t* init_labels, int32_t* out_labels, double* out_modularity,
                           msc_leiden_info* info, void* workspace, size_t workspace_bytes, void* stream) {
   Ctx c{};
+  TimingGuard timing_guard{c};
   Graph g0;
   MSC_TRY(begin_run(c, g0, indptr, indices, weights, n, nnz, resolution, commit_prob, out_labels, info, workspace,
                     workspace_bytes, stream));
@@ -2345,6 +2403,7 @@ extern "C" int msc_louvain(const int64_t* indptr, const int32_t* indices, const 
                            size_t workspace_bytes, void* stream) {
   MSC_REQUIRE(max_levels >= 1 && max_levels <= kMaxLevels, "louvain: max_levels must be in [1, %d]", kMaxLevels);
   Ctx c{};
+  TimingGuard timing_guard{c};
   Graph g0;
   MSC_TRY(begin_run(c, g0, indptr, indices, weights, n, nnz, resolution, commit_prob, out_labels, info, workspace,
                     workspace_bytes, stream));

@@ -35,7 +35,8 @@ class LeidenLevel(ctypes.Structure):
 
 class LeidenInfo(ctypes.Structure):
     _fields_ = [("n_levels", c_int), ("scale_exponent", c_int), ("twom_fixed", c_i64),
-                ("levels", LeidenLevel * 64)]
+                ("levels", LeidenLevel * 64), ("time_moves", ctypes.c_int32), ("move_launches", ctypes.c_int32),
+                ("move_ms", c_dbl), ("move_model_bytes", c_dbl)]
 
 
 class SpatialInfo(ctypes.Structure):
@@ -357,8 +358,11 @@ def fuzzy_batched_workspace_bytes(n, k, n_groups):
 # (d) Leiden
 # ----------------------------------------------------------------------------------------
 def leiden(indptr, indices, data, n, resolution=1.0, random_state=0, n_iterations=2,
-           commit_prob=0.9, init_labels=None):
-    """Leiden on a device CSR: ``(labels int32 device (n,), modularity, info dict)``."""
+           commit_prob=0.9, init_labels=None, time_moves=False):
+    """Leiden on a device CSR: ``(labels int32 device (n,), modularity, info dict)``.  With
+    ``time_moves`` the info dict also holds the summed HIP-event time of the move-kernel
+    launches (``move_ms``, ``move_launches``) and their modelled bytes (``move_model_bytes``:
+    8 E_l + 24 n_l per launch)."""
     import torch
     lib = load_library()
     indptr = to_device(indptr, torch.int64)
@@ -372,14 +376,17 @@ def leiden(indptr, indices, data, n, resolution=1.0, random_state=0, n_iteration
     init = None if init_labels is None else to_device(init_labels, torch.int32)
     q = c_dbl(0.0)
     info = LeidenInfo()
+    info.time_moves = 1 if time_moves else 0
     check(lib.msc_leiden(ptr(indptr), ptr(indices), ptr(data), n, nnz, float(resolution),
                          int(random_state), int(n_iterations), float(commit_prob),
                          None if init is None else ptr(init), ptr(labels), ctypes.byref(q),
                          ctypes.byref(info), ptr(ws), ws.numel(), stream_ptr()))
     levels = [{f: getattr(info.levels[i], f) for f, _ in LeidenLevel._fields_}
               for i in range(info.n_levels)]
-    return labels, q.value, {"scale_exponent": info.scale_exponent,
-                             "twom_fixed": info.twom_fixed, "levels": levels}
+    out = {"scale_exponent": info.scale_exponent, "twom_fixed": info.twom_fixed, "levels": levels}
+    if time_moves:
+        out.update(move_ms=info.move_ms, move_launches=info.move_launches, move_model_bytes=info.move_model_bytes)
+    return labels, q.value, out
 
 
 def leiden_batched(indptr, indices, data, offsets, resolution=1.0, random_state=0, n_iterations=2,

@@ -14,7 +14,8 @@ Stages (sizes are the workloads' real sizes):
               second pass, fallback), per-stage event times and the layout's statistics
   conn-c3     UMAP connectivities of that kNN; bytes = 8 n k + 8 E + 8 (n + 1)
   leiden-c3   Leiden (res 1.0, seed 0, 2 iterations) on those connectivities;
-              bytes = sum over levels of (passes (8 E_l + 24 n_l) + 12 (E_l + E_l+1))
+              bytes = sum over levels of (passes (8 E_l + 24 n_l) + 12 (E_l + E_l+1)); then one
+              call with the move kernel's launches timed (move_kernel: 8 E_l + 24 n_l per launch)
   grid-2m     spatial kNN of 2M tissue cells (10 elliptical blobs), k = 6;
               bytes = n (24 + 8 k) + 4 (cells + 1), cells ~ n / max(2k, 8)
   moran-2m    Moran's I of 2M cells x 500 genes, kNN(6) weights, 100 device permutations;
@@ -148,8 +149,14 @@ def stage_leiden_c3(reps):
         nxt = levels[i + 1]["n_edges"] if i + 1 < len(levels) and levels[i + 1]["n"] < lv["n"] else 0
         b += 12.0 * (lv["n_edges"] + nxt)
     passes = sum(lv["move_passes"] + lv["refine_passes"] for lv in levels)
+    # one more call with every move-kernel launch timed by HIP events
+    _, _, ti = _hip.leiden(ip, ind, dat, n, resolution=1.0, random_state=0, n_iterations=2, time_moves=True)
+    mv = {"launches": ti["move_launches"], "ms": ti["move_ms"], "avg_launch_us": 1e3 * ti["move_ms"] / ti["move_launches"],
+          "model_bytes": ti["move_model_bytes"],
+          "gbps": ti["move_model_bytes"] / ti["move_ms"] / 1e6,
+          "frac": ti["move_model_bytes"] / (ti["move_ms"] * 1e-3) / HBM_PEAK}
     return {"ms": ms, "bytes": b, "gbps": b / ms / 1e6, "frac": b / (ms * 1e-3) / HBM_PEAK, "passes": passes,
-            "levels": len(levels), "modularity": out["r"][1]}
+            "levels": len(levels), "modularity": out["r"][1], "move_kernel": mv}
 
 
 def _tissue(n, genes=0):
