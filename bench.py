@@ -515,7 +515,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=8.0)
-    ap.add_argument("--serve-rates", default="20,30,40,60", help="offered job rates (jobs/s), comma-separated")
+    ap.add_argument("--serve-rates", default="20,40,60,80", help="offered job rates (jobs/s), comma-separated")
     ap.add_argument("--no-precision", action="store_true")
     ap.add_argument("--no-e2e", action="store_true")
     ap.add_argument("--no-batched", action="store_true")

@@ -10,9 +10,12 @@ Scheduling:
   jobs stay within ``hbm_budget`` and fewer than ``max_slots`` jobs are in flight; a smaller
   job may pass a waiting larger one unless that one has waited longer than ``max_bypass_s``.
   A job larger than the whole budget is rejected with an error result, never run into OOM.
-* Every admitted job runs in its own worker thread on its own HIP stream (default priority),
-  so jobs that arrive while others are on the GPU overlap with them on the device.  The ctypes
-  calls release the GIL, so host-side enqueueing of different jobs proceeds concurrently.
+* Every admitted job runs in its own worker thread on one of ``max_slots`` HIP streams (default
+  priority) that the server creates once and hands from job to job, so jobs that arrive while
+  others are on the GPU overlap with them on the device.  Reusing the streams also lets torch's
+  caching allocator, which keeps freed blocks per stream, serve a job from the blocks of an
+  earlier one instead of new device allocations.  The ctypes calls release the GIL, so host-side
+  enqueueing of different jobs proceeds concurrently.
 * Results are deterministic: a job's outputs are bit-identical alone and in any mix.
 
 Latency of a job = completion time - arrival (submit) time; ``Server.stats()`` reports
@@ -123,6 +126,7 @@ class Server:
         self._done = []
         self._stop = False
         self._threads = []
+        self._streams = [torch.cuda.Stream() for _ in range(self.max_slots)]  # free streams
         self._disp = threading.Thread(target=self._dispatch, daemon=True)
         self._disp.start()
 
@@ -174,13 +178,13 @@ class Server:
                 fut = self._waiting.pop(i)
                 self._in_use += fut.job.footprint()
                 self._active += 1
-            t = threading.Thread(target=self._work, args=(fut,), daemon=True)
+                stream = self._streams.pop()  # _active <= max_slots streams are taken
+            t = threading.Thread(target=self._work, args=(fut, stream), daemon=True)
             self._threads.append(t)
             t.start()
 
-    def _work(self, fut):
+    def _work(self, fut, stream):
         import torch
-        stream = torch.cuda.Stream()
         fut.t_start = time.perf_counter()
         try:
             with torch.cuda.stream(stream):
@@ -192,6 +196,7 @@ class Server:
         with self._lock:
             self._in_use -= fut.job.footprint()
             self._active -= 1
+            self._streams.append(stream)
             self._done.append(fut)
             self._lock.notify_all()
         fut._ev.set()
