@@ -40,13 +40,16 @@ class BatchGraph:
         self.n = int(self.offsets[-1])
         self.d, self.k, self.precision = int(d), int(k), precision
         dev = _hip.device()
-        self.X = torch.empty((self.n, self.d), dtype=torch.float32, device=dev)   # static input
+        # static input: `tune_on` when given, else zeros, so that the warm-up run below (and a
+        # result read before the first run()) never computes on uninitialised memory
+        self.X = torch.zeros((self.n, self.d), dtype=torch.float32, device=dev)
+        if tune_on is not None:
+            self.X.copy_(_hip.to_device(tune_on, torch.float32))
         if splits is None:
             # tile shape: the cached (splits, list_len) of this shape bucket, else tuned on
             # `tune_on`, else one split with the default list
             shape = autotune.lookup(self.offsets, d, k, precision)
             if shape is None and tune_on is not None:
-                self.X.copy_(_hip.to_device(tune_on, torch.float32))
                 shape, _ = autotune.tune_batched_tile(self.X, self.offsets, k, precision)
             if shape is not None:
                 splits, tuned_len = shape
