@@ -587,6 +587,73 @@ __device__ __forceinline__ void move_vertex(int64_t v, const int64_t* __restrict
   }
 }
 
+// The wave's light vertices of NI slots per lane (listed in `mask`, lanes of the wave's
+// kMoveGroup vertices), kMoveGroup / NI at a time: the neighbour ids and weights of the whole
+// batch are loaded, then every (comm, part, sigma) record, and only then does the first
+// decision start, so a wave keeps a batch's gathers in flight together.  Item j NI + s holds
+// slot s 64 + l of the batch's j-th vertex.  The same decisions as move_sorted<NI>.
+template <int NI>
+__device__ __forceinline__ void move_batched(uint64_t mask, int64_t v0, int64_t e0v, int64_t degv, int32_t currv,
+                                             int64_t kvv, int32_t pvv, const int32_t* __restrict__ indices,
+                                             const int64_t* __restrict__ wq, const CommSig* __restrict__ cs,
+                                             const int32_t* __restrict__ part, const GainParams& gp,
+                                             int32_t* __restrict__ target, int64_t (*wl)[kLightDeg]) {
+  constexpr int NB = kMoveGroup / NI;  // vertices per batch
+  constexpr int IB = NI == 1 ? 6 : NI == 2 ? 7 : 8;
+  const int l = lane_id();
+  while (mask) {
+    int vi[NB];  // wave-uniform lane of each vertex of the batch, -1 past the last
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+      vi[j] = mask ? __ffsll(static_cast<long long>(mask)) - 1 : -1;
+      mask &= mask ? mask - 1 : 0ull;
+    }
+    int32_t u[kMoveGroup];
+    int64_t wr[kMoveGroup];
+    int4 r[kMoveGroup];
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+      const int64_t e0 = vi[j] >= 0 ? readlane_i64(e0v, vi[j]) : 0;
+      const int deg = vi[j] >= 0 ? static_cast<int>(readlane_i64(degv, vi[j])) : 0;
+#pragma unroll
+      for (int s = 0; s < NI; ++s) {
+        const int it = j * NI + s, slot = s * kWave + l;
+        u[it] = -1;
+        wr[it] = 0;
+        if (slot < deg) {
+          u[it] = indices[e0 + slot];
+          wr[it] = wq[e0 + slot];
+        }
+      }
+    }
+    // Unconditional gathers (lanes without a neighbour read record 0, which `ok` below never
+    // uses): a conditional load lets the compiler move the first use of its value into the
+    // branch, and the wait it then needs there serializes the gathers.
+#pragma unroll
+    for (int it = 0; it < kMoveGroup; ++it) r[it] = reinterpret_cast<const int4*>(cs)[u[it] >= 0 ? u[it] : 0];
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+      if (vi[j] >= 0) {
+        const int64_t v = v0 + vi[j];
+        const int32_t pv = __builtin_amdgcn_readlane(pvv, vi[j]);
+        uint32_t key[NI];
+        int nvalid = 0;
+#pragma unroll
+        for (int s = 0; s < NI; ++s) {
+          const int it = j * NI + s, slot = s * kWave + l;
+          const bool ok = u[it] >= 0 && u[it] != static_cast<int32_t>(v) && (!part || r[it].y == pv);
+          key[s] = ok ? ((static_cast<uint32_t>(r[it].x) << IB) | static_cast<uint32_t>(slot)) : kNoKey;
+          wl[0][slot] = ok ? wr[it] : 0;
+          wl[1][slot] = ok ? sig_of(r[it]) : 0;
+          nvalid += __popcll(__ballot(ok));
+        }
+        move_core<NI>(v, key, nvalid, __builtin_amdgcn_readlane(currv, vi[j]), readlane_i64(kvv, vi[j]), gp,
+                      target, wl[0], wl[1], wl[2]);
+      }
+    }
+  }
+}
+
 // The light vertices of light block lb (degree <= 256): each wave owns kMoveGroup consecutive
 // vertices; inactive ones keep their community (one coalesced store), active ones are moved
 // one after another.  lds: 3 x kLightDeg int64 per wave.
@@ -616,46 +683,15 @@ __device__ __forceinline__ void move_light_block(
   uint64_t pending = __ballot(todo);
   int64_t(*wl)[kLightDeg] = lds[threadIdx.x / kWave];
   if (n < kSortedMaxN) {
-    // Vertices of degree <= 64 (one slot per lane): the gathers of all of them are issued
-    // before the first decision (neighbour ids and weights, then every (comm, part, sigma)
-    // record), so a wave keeps up to kMoveGroup vertices' loads in flight at once.
+    // degree <= 64, 65..128 and 129..256: batches of 8, 4 and 2 vertices (one, two and four
+    // slots per lane)
     const uint64_t small = __ballot(todo && degv <= kWave);
-    if (small) {
-      int32_t u[kMoveGroup];
-      int64_t wr[kMoveGroup];
-      int4 r[kMoveGroup];
-#pragma unroll
-      for (int i = 0; i < kMoveGroup; ++i) {
-        const int64_t e0 = readlane_i64(e0v, i);
-        const int deg = ((small >> i) & 1) ? static_cast<int>(readlane_i64(degv, i)) : 0;
-        u[i] = -1;
-        wr[i] = 0;
-        if (l < deg) {
-          u[i] = indices[e0 + l];
-          wr[i] = wq[e0 + l];
-        }
-      }
-      // Unconditional gathers (lanes without a neighbour read record 0, which `ok` below never
-      // uses): a conditional load lets the compiler move the first use of its value into the
-      // branch, and the wait it then needs there serializes the eight gathers.
-#pragma unroll
-      for (int i = 0; i < kMoveGroup; ++i) r[i] = reinterpret_cast<const int4*>(cs)[u[i] >= 0 ? u[i] : 0];
-#pragma unroll
-      for (int i = 0; i < kMoveGroup; ++i) {
-        if ((small >> i) & 1) {
-          const int64_t v = v0 + i;
-          const int32_t pv = __builtin_amdgcn_readlane(pvv, i);
-          const bool ok = u[i] >= 0 && u[i] != static_cast<int32_t>(v) && (!part || r[i].y == pv);
-          uint32_t key[1] = {ok ? ((static_cast<uint32_t>(r[i].x) << 6) | static_cast<uint32_t>(l)) : kNoKey};
-          wl[0][l] = ok ? wr[i] : 0;
-          wl[1][l] = ok ? sig_of(r[i]) : 0;
-          const int nvalid = __popcll(__ballot(ok));
-          move_core<1>(v, key, nvalid, __builtin_amdgcn_readlane(currv, i), readlane_i64(kvv, i), gp, target,
-                       wl[0], wl[1], wl[2]);
-        }
-      }
-      pending &= ~small;
-    }
+    const uint64_t mid2 = __ballot(todo && degv > kWave && degv <= 2 * kWave);
+    const uint64_t mid4 = __ballot(todo && degv > 2 * kWave);
+    move_batched<1>(small, v0, e0v, degv, currv, kvv, pvv, indices, wq, cs, part, gp, target, wl);
+    move_batched<2>(mid2, v0, e0v, degv, currv, kvv, pvv, indices, wq, cs, part, gp, target, wl);
+    move_batched<4>(mid4, v0, e0v, degv, currv, kvv, pvv, indices, wq, cs, part, gp, target, wl);
+    pending = 0;
   }
   while (pending) {
     const int64_t v = v0 + __ffsll(static_cast<long long>(pending)) - 1;
@@ -664,8 +700,8 @@ __device__ __forceinline__ void move_light_block(
   }
 }
 
-// Degree 257..1024 (ITEMS = 4) and 1025..4096 (ITEMS = 16): one 256-thread block per listed
-// vertex, ITEMS neighbour slots per thread (N = 256 ITEMS keys).  The same decision as
+// Degree 257..512, 513..1024, 1025..2048 and 2049..4096 (ITEMS = 2, 4, 8, 16): one 256-thread
+// block per listed vertex, ITEMS neighbour slots per thread (N = 256 ITEMS keys).  The same decision as
 // move_sorted, atomics-free: keys (community << IB | slot) sorted by a bitonic network whose
 // stages run where the partner lives -- inside the thread (stride < ITEMS), across the wave by
 // shuffles (stride < 64 ITEMS) or through LDS with barriers (the two or three widest strides)
