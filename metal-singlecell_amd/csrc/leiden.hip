@@ -323,6 +323,7 @@ constexpr int64_t kSortedMaxN = int64_t(1) << 24;  // NI <= 4
 // 2^(23 - i) (strictly below, so that no key equals kNoKey), so it is used for graphs of fewer
 // than hub_class_max_n(i) vertices.
 constexpr int kHubClasses = 4;
+constexpr int kHubMaxDeg = 512 << (kHubClasses - 1);  // 4096
 __host__ __device__ constexpr int64_t hub_class_max_n(int i) { return int64_t(1) << (23 - i); }
 constexpr int kMoveGroup = 8;  // consecutive vertices per wave on the light path of k_move
 
@@ -715,6 +716,7 @@ __device__ __forceinline__ void move_light_block(
 // a run head, x = the sum of its weights after its last head (all of them when f is false).
 // Returns, for thread t, the sum of the weights from the last run head before its first
 // position up to that position (exclusive): (f1, x1) . (f2, x2) = (f1 | f2, f2 ? x2 : x1 + x2).
+template <int NW>
 __device__ __forceinline__ int64_t block_excl_segsum_i64(bool f, int64_t x, int64_t* __restrict__ wsum,
                                                          int32_t* __restrict__ wflag) {
   const int l = lane_id(), wv = threadIdx.x / kWave;
@@ -768,26 +770,31 @@ __device__ __forceinline__ void stage_in_thread(uint32_t (&key)[ITEMS], int eb, 
   }
 }
 
-template <int ITEMS>
+template <int THREADS, int ITEMS>
 struct BlockMoveSmem {
-  uint32_t skey[kThreads * ITEMS];
-  int64_t wsum[kWavesPerBlock];
-  int32_t wflag[kWavesPerBlock];
-  double wbest_g[kWavesPerBlock];
-  int32_t wbest_c[kWavesPerBlock];
+  static constexpr int NW = THREADS / kWave;
+  uint32_t skey[THREADS * ITEMS];
+  int64_t wsum[NW];
+  int32_t wflag[NW];
+  double wbest_g[NW];
+  int32_t wbest_c[NW];
   double stay;
 };
 
-template <int ITEMS>
+__host__ __device__ constexpr int ilog2(int x) { return x <= 1 ? 0 : 1 + ilog2(x / 2); }
+
+template <int THREADS, int ITEMS>
 __device__ __forceinline__ void block_move(int64_t v, const int64_t* __restrict__ indptr,
                                            const int32_t* __restrict__ indices, const int64_t* __restrict__ wq,
                                            const int64_t* __restrict__ kdeg, const int32_t* __restrict__ comm,
                                            const int32_t* __restrict__ part, const int64_t* __restrict__ sigtot,
                                            const uint8_t* __restrict__ active, const GainParams& gp_all,
-                                           int32_t* __restrict__ target, BlockMoveSmem<ITEMS>& sm) {
+                                           int32_t* __restrict__ target, BlockMoveSmem<THREADS, ITEMS>& sm) {
   static_assert(ITEMS == 2 || ITEMS == 4 || ITEMS == 8 || ITEMS == 16, "ITEMS");
-  constexpr int N = kThreads * ITEMS;
-  constexpr int IB = ITEMS == 2 ? 9 : ITEMS == 4 ? 10 : ITEMS == 8 ? 11 : 12;  // log2 N
+  static_assert(THREADS == 256 || THREADS == 1024, "THREADS");
+  constexpr int NW = THREADS / kWave;
+  constexpr int N = THREADS * ITEMS;
+  constexpr int IB = ilog2(N);
   constexpr int kWaveSpan = ITEMS * kWave;  // positions held by one wave
   constexpr uint32_t kSlot = (1u << IB) - 1u;
   const int t = threadIdx.x, l = lane_id(), wv = t / kWave;
@@ -881,7 +888,7 @@ __device__ __forceinline__ void block_move(int64_t v, const int64_t* __restrict_
       }
     }
   }
-  const int64_t carry = block_excl_segsum_i64(seen_head, acc, sm.wsum, sm.wflag);
+  const int64_t carry = block_excl_segsum_i64<NW>(seen_head, acc, sm.wsum, sm.wflag);
   if (open_s >= 0) {
     uint32_t kk = key[0];
 #pragma unroll
@@ -907,7 +914,7 @@ __device__ __forceinline__ void block_move(int64_t v, const int64_t* __restrict_
   if (t == 0) {
     double bg = sm.wbest_g[0];
     int32_t bc = sm.wbest_c[0];
-    for (int w = 1; w < kWavesPerBlock; ++w)
+    for (int w = 1; w < NW; ++w)
       if (better(sm.wbest_g[w], sm.wbest_c[w], bg, bc)) {
         bg = sm.wbest_g[w];
         bc = sm.wbest_c[w];
@@ -917,7 +924,8 @@ __device__ __forceinline__ void block_move(int64_t v, const int64_t* __restrict_
 }
 
 constexpr size_t kLightSmem = sizeof(int64_t) * kWavesPerBlock * 3 * kLightDeg;
-constexpr size_t kMoveSmem = kLightSmem > sizeof(BlockMoveSmem<16>) ? kLightSmem : sizeof(BlockMoveSmem<16>);
+constexpr size_t kMoveSmem =
+    kLightSmem > sizeof(BlockMoveSmem<kThreads, 16>) ? kLightSmem : sizeof(BlockMoveSmem<kThreads, 16>);
 
 // The listed vertices of the four hub classes: one array, the largest class first.
 struct HubLists {
@@ -940,8 +948,8 @@ __global__ __launch_bounds__(kThreads) void k_move(
   const int32_t* verts = hubs.verts;
 #define MSC_HUB_CLASS(I, ITEMS)                                                                  \
   if (b < hubs.count[I]) {                                                                       \
-    block_move<ITEMS>(verts[b], indptr, indices, wq, kdeg, comm, part, sigtot, active, gp, target, \
-                      *reinterpret_cast<BlockMoveSmem<ITEMS>*>(smem));                           \
+    block_move<kThreads, ITEMS>(verts[b], indptr, indices, wq, kdeg, comm, part, sigtot, active, gp, target, \
+                                *reinterpret_cast<BlockMoveSmem<kThreads, ITEMS>*>(smem));             \
     return;                                                                                      \
   }                                                                                              \
   b -= hubs.count[I];                                                                            \
@@ -953,6 +961,22 @@ __global__ __launch_bounds__(kThreads) void k_move(
 #undef MSC_HUB_CLASS
   move_light_block(b, indptr, indices, wq, kdeg, n, comm, part, sigtot, cs, active, gp, target,
                    reinterpret_cast<int64_t(*)[3][kLightDeg]>(smem));
+}
+
+// Degree 4097..16384 (graphs below 2^18 vertices, so that (community << 14 | slot) fits a
+// key): one 1024-thread block per listed vertex, 16 slots per thread, the same decision as the
+// hub classes of k_move.  Launched after it on the same stream.
+constexpr int kHugeThreads = 1024;
+constexpr int kHugeDeg = kHugeThreads * 16;
+constexpr int64_t kHugeMaxN = int64_t(1) << 18;
+__global__ __launch_bounds__(kHugeThreads) void k_move_huge(
+    const int32_t* __restrict__ verts, const int64_t* __restrict__ indptr, const int32_t* __restrict__ indices,
+    const int64_t* __restrict__ wq, const int64_t* __restrict__ kdeg, const int32_t* __restrict__ comm,
+    const int32_t* __restrict__ part, const int64_t* __restrict__ sigtot, const uint8_t* __restrict__ active,
+    GainParams gp, int32_t* __restrict__ target) {
+  __shared__ BlockMoveSmem<kHugeThreads, 16> sm;
+  block_move<kHugeThreads, 16>(verts[blockIdx.x], indptr, indices, wq, kdeg, comm, part, sigtot, active, gp, target,
+                               sm);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -978,16 +1002,18 @@ struct Heavy {
 };
 
 // hub class of each vertex (kLightDeg < degree <= hub_max: class i for degrees in
-// (256 << i, 512 << i], else -1); heavy: degree above hub_max
-__global__ void k_heavy_flags(const int64_t* __restrict__ indptr, int64_t n, int64_t hub_max,
+// (256 << i, 512 << i]; hub_max < degree <= heavy_min: class 4, k_move_huge; else -1); heavy:
+// degree above heavy_min
+__global__ void k_heavy_flags(const int64_t* __restrict__ indptr, int64_t n, int64_t hub_max, int64_t heavy_min,
                               int32_t* __restrict__ flag, int64_t* __restrict__ deg_h, int32_t* __restrict__ hclass) {
   const int64_t v = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
   if (v < n) {
     const int64_t d = indptr[v + 1] - indptr[v];
-    flag[v] = d > hub_max ? 1 : 0;
-    deg_h[v] = d > hub_max ? d : 0;
+    flag[v] = d > heavy_min ? 1 : 0;
+    deg_h[v] = d > heavy_min ? d : 0;
     int cl = -1;
     if (d > kLightDeg && d <= hub_max) cl = d <= 512 ? 0 : d <= 1024 ? 1 : d <= 2048 ? 2 : 3;
+    else if (d > hub_max && d <= heavy_min) cl = 4;
     hclass[v] = cl;
   }
   if (v == n) {
@@ -1564,6 +1590,7 @@ struct Ctx {
   int64_t *hdeg, *hdegoff;
   int32_t *hub_class, *hub_flag, *hub_pos, *hub_verts;  // listed hub vertices (k_move), by class
   int64_t n_hub[kHubClasses] = {0, 0, 0, 0};
+  int64_t n_huge = 0, huge_off = 0;  // k_move_huge's vertices: hub_verts + huge_off
   msc_leiden_info* info;
   // move-kernel timing (info->time_moves): an event pair per pass in a ring, read at the
   // host syncs of the pass loop
@@ -1679,16 +1706,22 @@ static int graph_setup(Ctx& c, Graph& g) {
   h.nh = 0;
   h.eh = 0;
   for (int i = 0; i < kHubClasses; ++i) c.n_hub[i] = 0;
+  c.n_huge = 0;
   if (g.max_deg > kLightDeg) {
-    // the widest hub class whose community ids fit the sort key; above it, the sort path
+    // the widest hub class whose community ids fit the sort key; above it k_move_huge (when
+    // its keys fit too), then the sort path
     int64_t hub_max = kLightDeg;
     for (int i = 0; i < kHubClasses; ++i)
       if (g.n < hub_class_max_n(i)) hub_max = int64_t(512) << i;
-    k_heavy_flags<<<grid_for(g.n + 1), kThreads, 0, st>>>(g.indptr, g.n, hub_max, c.hflag, c.hdeg, c.hub_class);
+    const int64_t heavy_min = (hub_max == kHubMaxDeg && g.n < kHugeMaxN) ? kHugeDeg : hub_max;
+    k_heavy_flags<<<grid_for(g.n + 1), kThreads, 0, st>>>(g.indptr, g.n, hub_max, heavy_min, c.hflag, c.hdeg,
+                                                          c.hub_class);
     MSC_LAUNCH_CHECK();
     int64_t off = 0;
-    for (int i = kHubClasses - 1; i >= 0; --i) {  // largest class first in hub_verts
-      if (int64_t(256) << i >= hub_max) continue;
+    // hub_verts: class 3, 2, 1, 0 (k_move reads them in that order), then class 4 (k_move_huge)
+    for (int o = 0; o <= kHubClasses; ++o) {
+      const int i = o < kHubClasses ? kHubClasses - 1 - o : kHubClasses;
+      if (i == kHubClasses ? heavy_min == hub_max || g.max_deg <= hub_max : int64_t(256) << i >= hub_max) continue;
       k_class_flag<<<grid_for(g.n + 1), kThreads, 0, st>>>(c.hub_class, g.n, i, c.hub_flag);
       MSC_LAUNCH_CHECK();
       MSC_TRY(prims::exclusive_scan<int32_t>(c.hub_flag, c.hub_pos, g.n + 1, c.scan_ws, c.scan_ws_bytes, st));
@@ -1698,7 +1731,12 @@ static int graph_setup(Ctx& c, Graph& g) {
         k_heavy_compact<<<grid_for(g.n), kThreads, 0, st>>>(c.hub_flag, c.hub_pos, g.n, c.hub_verts + off);
         MSC_LAUNCH_CHECK();
       }
-      c.n_hub[i] = cnt;
+      if (i == kHubClasses) {
+        c.n_huge = cnt;
+        c.huge_off = off;
+      } else {
+        c.n_hub[i] = cnt;
+      }
       off += cnt;
     }
     MSC_TRY(prims::exclusive_scan<int32_t>(c.hflag, c.hpos, g.n + 1, c.scan_ws, c.scan_ws_bytes, st));
@@ -1809,6 +1847,11 @@ static int local_moving(Ctx& c, const Graph& g, int64_t seed, const int32_t* par
       c.t_bytes_pending[tslot] = 8.0 * static_cast<double>(g.E) + 24.0 * static_cast<double>(g.n);
       c.t_next = (tslot + 1) % Ctx::kTimeRing;
       ++c.t_pending;
+    }
+    if (c.n_huge > 0) {
+      k_move_huge<<<static_cast<unsigned>(c.n_huge), kHugeThreads, 0, st>>>(
+          c.hub_verts + c.huge_off, g.indptr, g.indices, g.wq, g.k, c.comm, part, c.sigtot, c.active, gp, c.target);
+      MSC_LAUNCH_CHECK();
     }
     MSC_TRY(heavy_moves(c, g, c.comm, part, gp));
     k_commit<<<static_cast<unsigned>(nblk), kCommitThreads, 0, st>>>(

@@ -53,12 +53,13 @@ def test_leiden_resolutions(c1_data, res):
 
 def test_leiden_heavy_vertices():
     """Hubs of degree 90 (two neighbours per lane on the one-wave path), 300, 700, 1500 and
-    3000 (the four block classes: 512, 1024, 2048 and 4096 sorted keys, 2 to 16 per thread)
-    and 5000 (the sort path over heavy edges); the frontier rule applies with them present."""
+    3000 (the four block classes: 512, 1024, 2048 and 4096 sorted keys, 2 to 16 per thread),
+    5000 (the 1024-thread class: 16384 keys) and 17000 (the sort path over heavy edges); the
+    frontier rule applies with them present."""
     rng = np.random.default_rng(5)
-    n = 8000
+    n = 20000
     rows, cols = [], []
-    hubs = [(0, 5000), (1, 3000), (2, 1500), (3, 700), (4, 300), (5, 90)]
+    hubs = [(0, 17000), (1, 5000), (2, 3000), (3, 1500), (4, 700), (5, 300), (6, 90)]
     for hub, deg in hubs:
         nb = rng.choice(np.arange(len(hubs), n), deg, replace=False)
         rows += [hub] * deg
@@ -75,7 +76,7 @@ def test_leiden_heavy_vertices():
     A = (A + A.T).tocsr()
     A.setdiag(0)
     A.eliminate_zeros()
-    assert np.diff(A.indptr).max() > 4096
+    assert np.diff(A.indptr).max() > 16384
     _check(A, random_state=1)
 
 
