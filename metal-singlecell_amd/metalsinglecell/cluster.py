@@ -16,12 +16,15 @@ from . import _hip
 
 
 def _canonical_csr(connectivities):
+    """float32 CSR with sorted, unique columns per row: the caller's arrays when they are that
+    already (only read), else a canonical copy (the caller's matrix is never modified)."""
     import scipy.sparse as sp
-    csr = sp.csr_matrix(connectivities, dtype=np.float32, copy=True)
-    csr.sum_duplicates()
-    csr.sort_indices()
+    csr = sp.csr_matrix(connectivities, dtype=np.float32, copy=False)
     if csr.shape[0] != csr.shape[1]:
         raise ValueError("connectivities must be square")
+    if not csr.has_canonical_format:
+        csr = csr.copy()
+        csr.sum_duplicates()   # sorts the columns of every row and sums repeated entries
     return csr
 
 
