@@ -170,6 +170,19 @@ def to_device(a, dtype):
     return torch.from_numpy(np.ascontiguousarray(a)).to(device=dev, dtype=dtype).contiguous()
 
 
+def to_host(t):
+    """Device tensor -> numpy array in ordinary host memory.  Large results go through a
+    page-locked staging buffer (one DMA at full link speed, then a host copy): 400 MB in ~40 ms
+    against ~100 ms for torch's pageable copy.  The result is not left in the page-locked
+    buffer, which the CPU reads an order of magnitude slower here."""
+    import torch
+    if not isinstance(t, torch.Tensor) or not t.is_cuda or t.numel() * t.element_size() < (1 << 20):
+        return t.cpu().numpy() if isinstance(t, torch.Tensor) else np.asarray(t)
+    h = torch.empty(t.shape, dtype=t.dtype, pin_memory=True)
+    h.copy_(t)
+    return h.numpy().copy()
+
+
 def workspace(nbytes):
     import torch
     return torch.empty(max(int(nbytes), 256), dtype=torch.uint8, device=device())

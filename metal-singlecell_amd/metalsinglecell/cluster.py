@@ -19,7 +19,10 @@ def _canonical_csr(connectivities):
     """float32 CSR with sorted, unique columns per row: the caller's arrays when they are that
     already (only read), else a canonical copy (the caller's matrix is never modified)."""
     import scipy.sparse as sp
-    csr = sp.csr_matrix(connectivities, dtype=np.float32, copy=False)
+    if sp.issparse(connectivities) and connectivities.format == "csr" and connectivities.dtype == np.float32:
+        csr = connectivities  # keeps scipy's cached format flags
+    else:
+        csr = sp.csr_matrix(connectivities, dtype=np.float32)
     if csr.shape[0] != csr.shape[1]:
         raise ValueError("connectivities must be square")
     if not csr.has_canonical_format:

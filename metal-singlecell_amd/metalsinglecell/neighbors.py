@@ -29,9 +29,12 @@ DEFAULT_PRECISION = "bf16"
 
 
 def _csr_to_scipy(indptr, indices, data, n):
+    """scipy CSR of a device CSR from the kernels here, whose rows hold sorted, distinct columns
+    (so the matrix is marked canonical and later users skip their O(nnz) check)."""
     import scipy.sparse as sp
-    return sp.csr_matrix((data.cpu().numpy(), indices.cpu().numpy(), indptr.cpu().numpy()),
-                         shape=(n, n))
+    m = sp.csr_matrix((_hip.to_host(data), _hip.to_host(indices), _hip.to_host(indptr)), shape=(n, n))
+    m.has_canonical_format = True
+    return m
 
 
 def knn_device(X, n_neighbors, precision=DEFAULT_PRECISION):

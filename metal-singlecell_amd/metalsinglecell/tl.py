@@ -31,9 +31,15 @@ def _natkey(s):
 
 
 def _categorical(labels):
+    """Categorical of the labels' strings with natural-sorted categories, built from the distinct
+    labels (one string each) and integer codes, not from a string per cell."""
     import pandas as pd
-    vals = np.asarray([str(x) for x in labels])
-    return pd.Categorical(values=vals, categories=sorted(set(vals.tolist()), key=_natkey))
+    uniq, inv = np.unique(np.asarray(labels), return_inverse=True)
+    names = [str(x) for x in uniq.tolist()]
+    cats = sorted(set(names), key=_natkey)
+    where = {c: i for i, c in enumerate(cats)}
+    code_of_uniq = np.fromiter((where[nm] for nm in names), dtype=np.int64, count=len(names))
+    return pd.Categorical.from_codes(code_of_uniq[np.asarray(inv).reshape(-1)], categories=cats)
 
 
 def leiden(adata, resolution: float = 1.0, key_added: str = "leiden", random_state: int = 0,
