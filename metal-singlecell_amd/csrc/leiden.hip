@@ -1195,15 +1195,19 @@ __global__ __launch_bounds__(kCommitThreads) void k_commit(
   if (threadIdx.x == 0) mv_cnt[blockIdx.x] = total;
 }
 
-// Incremental sigma_tot after a commit, atomics-free: each committed vertex v contributes
-// (previous community, -k_v) and (new community, +k_v).  The entries are taken kDeltaChunk
-// vertices at a time in vertex order, bitonic-sorted by community in LDS, and the first
-// entry of each equal-community run adds the run's exact int64 sum to sigtot.  Chunks run
-// one after another inside this single workgroup (barrier between them), so a community
-// is written by one thread at a time, and the integer result equals a full recomputation.
-// The moved total goes to *moved_out; apply = 0 stops there.
+// Incremental sigma_tot after a commit: each committed vertex v contributes (previous
+// community, -k_v) and (new community, +k_v).  The entries are taken kDeltaChunk vertices
+// at a time and summed per community in an LDS hash table (linear probing, slots claimed by
+// compare-and-swap, int64 adds; at most 2 * kDeltaChunk keys in kDeltaSlots slots, so every
+// probe ends); then each claimed slot adds its exact int64 total to sigtot.  Chunks run one
+// after another inside this single workgroup (barrier between them), so a community is
+// written by one thread at a time; integer sums do not depend on the order of the adds, so
+// the result equals a full recomputation.  The moved total goes to *moved_out; apply = 0
+// stops there.
 constexpr int kDeltaThreads = 1024;
-constexpr int kDeltaChunk = 1024;          // committed vertices per sort (2048 entries)
+constexpr int kDeltaChunk = 2048;          // committed vertices per table fill (4096 entries)
+constexpr int kDeltaSlotBits = 13;
+constexpr int kDeltaSlots = 1 << kDeltaSlotBits;
 constexpr int kDeltaMaxBlocks = 8192;      // commit blocks: graphs up to 8,388,608 vertices
 constexpr int64_t kDeltaSwitch = 8192;     // moved count at a sync below which passes go incremental
 __global__ __launch_bounds__(kDeltaThreads) void k_sigtot_delta(
@@ -1211,8 +1215,8 @@ __global__ __launch_bounds__(kDeltaThreads) void k_sigtot_delta(
     const int32_t* __restrict__ mv_old, const int32_t* __restrict__ comm, const int64_t* __restrict__ kdeg,
     int64_t* __restrict__ sigtot, int apply, int32_t* __restrict__ moved_out) {
   __shared__ int32_t pre[kDeltaMaxBlocks + 1];
-  __shared__ int32_t key[2 * kDeltaChunk];
-  __shared__ int64_t val[2 * kDeltaChunk];
+  __shared__ int32_t hkey[kDeltaSlots];
+  __shared__ unsigned long long hval[kDeltaSlots];
   __shared__ int32_t wsum[kDeltaThreads / kWave];
   const int t = threadIdx.x, w = t / kWave, l = lane_id();
   // exclusive prefix of the per-block counts; thread t owns a contiguous run of blocks
@@ -1244,54 +1248,45 @@ __global__ __launch_bounds__(kDeltaThreads) void k_sigtot_delta(
   }
   __syncthreads();
   if (!apply) return;
+  constexpr int32_t kEmpty = -1;  // community ids are >= 0
+  for (int i = t; i < kDeltaSlots; i += kDeltaThreads) {
+    hkey[i] = kEmpty;
+    hval[i] = 0;
+  }
+  __syncthreads();
+  auto add = [&](int32_t c, int64_t x) {
+    uint32_t h = (static_cast<uint32_t>(c) * 2654435761u) >> (32 - kDeltaSlotBits);
+    for (;;) {
+      const int32_t prev = atomicCAS(&hkey[h], kEmpty, c);
+      if (prev == kEmpty || prev == c) {
+        atomicAdd(&hval[h], static_cast<unsigned long long>(x));  // two's complement: exact
+        return;
+      }
+      h = (h + 1) & (kDeltaSlots - 1);
+    }
+  };
   for (int s0 = 0; s0 < total; s0 += kDeltaChunk) {
     const int cnt = min(kDeltaChunk, total - s0);
-    int n2 = 2;
-    while (n2 < 2 * cnt) n2 <<= 1;
-    if (t < n2 / 2) {
-      int32_t k0 = INT32_MAX, k1 = INT32_MAX;  // sentinels sort last (ids are < n <= INT32_MAX)
-      int64_t v0 = 0, v1 = 0;
-      if (t < cnt) {
-        const int e = s0 + t;
-        int lo = 0, hi = nblk - 1;  // last block with pre[b] <= e: it holds e
-        while (lo < hi) {
-          const int mid = (lo + hi + 1) >> 1;
-          if (pre[mid] <= e) lo = mid;
-          else hi = mid - 1;
-        }
-        const int64_t slot = static_cast<int64_t>(lo) * kCommitThreads + (e - pre[lo]);
-        const int32_t v = mv_v[slot];
-        const int64_t kv = kdeg[v];
-        k0 = mv_old[slot];
-        v0 = -kv;
-        k1 = comm[v];
-        v1 = kv;
+    for (int e = s0 + t; e < s0 + cnt; e += kDeltaThreads) {
+      int lo = 0, hi = nblk - 1;  // last block with pre[b] <= e: it holds e
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (pre[mid] <= e) lo = mid;
+        else hi = mid - 1;
       }
-      key[2 * t] = k0; val[2 * t] = v0;
-      key[2 * t + 1] = k1; val[2 * t + 1] = v1;
+      const int64_t slot = static_cast<int64_t>(lo) * kCommitThreads + (e - pre[lo]);
+      const int32_t v = mv_v[slot];
+      const int64_t kv = kdeg[v];
+      add(mv_old[slot], -kv);
+      add(comm[v], kv);
     }
     __syncthreads();
-    for (int size = 2; size <= n2; size <<= 1) {
-      for (int stride = size >> 1; stride > 0; stride >>= 1) {
-        if (t < n2 / 2) {
-          const int i = 2 * t - (t & (stride - 1));
-          const int j = i + stride;
-          const int32_t ki = key[i], kj = key[j];
-          if ((ki > kj) == ((i & size) == 0)) {
-            key[i] = kj; key[j] = ki;
-            const int64_t vi = val[i];
-            val[i] = val[j]; val[j] = vi;
-          }
-        }
-        __syncthreads();
-      }
-    }
-    for (int i = t; i < n2; i += kDeltaThreads) {
-      const int32_t ki = key[i];
-      if (ki != INT32_MAX && (i == 0 || key[i - 1] != ki)) {
-        int64_t sum = 0;
-        for (int j = i; j < n2 && key[j] == ki; ++j) sum += val[j];
-        sigtot[ki] += sum;
+    for (int i = t; i < kDeltaSlots; i += kDeltaThreads) {
+      const int32_t c = hkey[i];
+      if (c != kEmpty) {
+        sigtot[c] += static_cast<int64_t>(hval[i]);
+        hkey[i] = kEmpty;
+        hval[i] = 0;
       }
     }
     __syncthreads();
