@@ -218,8 +218,8 @@ typedef struct {
   int32_t move_passes, refine_passes;
   int64_t n_comms, n_refined;
   double modularity;
-  /* passes of each phase whose sigma_tot came from the previous pass's committed vertices
-   * alone (the rest rebuild it over all vertices); both forms give the same exact sums */
+  /* passes of each phase whose sigma_tot came from the previous passes' committed vertices
+   * alone (exact int64 updates); the first pass of a phase builds it over all vertices */
   int32_t move_delta_passes, refine_delta_passes;
 } msc_leiden_level;
 

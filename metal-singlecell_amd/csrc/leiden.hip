@@ -1147,144 +1147,167 @@ __global__ __launch_bounds__(kThreads) void k_heavy_argmax(
 // ------------------------------------------------------------------------------------------
 // commit, frontier, relabel, modularity, contraction
 // ------------------------------------------------------------------------------------------
+// sigma_tot is kept current across passes by the committed vertices alone: each one
+// contributes (previous community, -k_v) and (new community, +k_v).  Communities fall into
+// kBuckets buckets by a hash of their id; k_commit lays each block's entries out grouped by
+// bucket, and k_sigtot_apply gives every bucket one workgroup, which sums the bucket's entries
+// per community in an LDS hash table and adds each total to sigtot.  A community is written
+// by exactly one workgroup (plain read-modify-write, no global atomics) and integer sums do
+// not depend on the order of the adds, so sigtot equals a recomputation from comm exactly.
+constexpr int kCommitThreads = 1024;
+constexpr int kBucketBits = 8;
+constexpr int kBuckets = 1 << kBucketBits;
+constexpr int kApplyThreads = 1024;
+constexpr int kApplyChunk = 2048;          // entries per table fill
+constexpr int kApplySlotBits = 12;         // 4096 slots: load <= 1/2, so every probe ends
+constexpr int kApplySlots = 1 << kApplySlotBits;
+__device__ __forceinline__ int comm_bucket(int32_t c) {
+  return static_cast<int>((static_cast<uint32_t>(c) * 2654435761u) >> (32 - kBucketBits));
+}
+__device__ __forceinline__ uint32_t comm_slot(int32_t c) {
+  return (static_cast<uint32_t>(c) * 2246822519u) >> (32 - kApplySlotBits);
+}
+
 // Commit the coin-approved moves (louvain.py:355-362) and start the next active set
 // (louvain.py:364-372): commit | held = (wants & coin) | (wants & ~coin) = wants, so the
-// vertex's own part is its wants bit; k_frontier_push adds N(commit) afterwards.  Each
-// block also lists its committed vertices (id, previous community) in vertex order at
-// [blockIdx.x * kCommitThreads, + mv_cnt[blockIdx.x]) for k_sigtot_delta.
-constexpr int kCommitThreads = 1024;
+// vertex's own part is its wants bit; k_frontier_push adds N(commit) afterwards.  Block blk
+// writes its sigma_tot entries to [blk * 2 * kCommitThreads, + 2 * committed) grouped by
+// bucket, bucket j at offsets boff[blk * (kBuckets + 1) + j] (order inside a bucket is free).
 __global__ __launch_bounds__(kCommitThreads) void k_commit(
     int32_t* __restrict__ comm, const int32_t* __restrict__ target, int64_t n, uint64_t pkey, uint64_t thr,
     uint8_t* __restrict__ commit, uint8_t* __restrict__ active, int32_t* __restrict__ wants_flag,
-    int32_t* __restrict__ mv_v, int32_t* __restrict__ mv_old, int32_t* __restrict__ mv_cnt,
-    const int32_t* __restrict__ vsid, const int64_t* __restrict__ vbase) {
-  __shared__ int wcnt[kCommitThreads / kWave];
+    const int64_t* __restrict__ kdeg, int32_t* __restrict__ ekey, int64_t* __restrict__ eval,
+    int32_t* __restrict__ boff, const int32_t* __restrict__ vsid, const int64_t* __restrict__ vbase) {
+  __shared__ int hist[kBuckets];
+  __shared__ int wtot[kBuckets / kWave];
   const int64_t v = static_cast<int64_t>(blockIdx.x) * kCommitThreads + threadIdx.x;
+  if (threadIdx.x < kBuckets) hist[threadIdx.x] = 0;
+  __syncthreads();
   bool cm = false;
-  int32_t c = 0;
+  int32_t c = 0, t = 0;
+  int b0 = 0, b1 = 0, r0 = 0, r1 = 0;
   if (v < n) {
-    const int32_t t = target[v];
+    t = target[v];
     c = comm[v];
     const bool wants = t != c;
     // a batched run draws each sample's coins by the vertex's index within its sample
     const int64_t vl = vsid ? v - vbase[vsid[v]] : v;
     const bool coin = static_cast<uint64_t>(coin_u32(pkey, static_cast<uint64_t>(vl))) < thr;
     cm = wants && coin;
-    if (cm) comm[v] = t;
+    if (cm) {
+      comm[v] = t;
+      b0 = comm_bucket(c);
+      b1 = comm_bucket(t);
+      r0 = atomicAdd(&hist[b0], 1);  // LDS: a place inside the bucket
+      r1 = atomicAdd(&hist[b1], 1);
+    }
     commit[v] = cm;
     active[v] = wants;
     if (wants) *wants_flag = 1;  // every writer stores the same value
   }
-  const int w = threadIdx.x / kWave, l = lane_id();
-  const uint64_t b = __ballot(cm);
-  if (l == 0) wcnt[w] = __popcll(b);
   __syncthreads();
-  int base = 0, total = 0;
+  // exclusive scan of the bucket counts (the first kBuckets threads, one bucket each)
+  const int l = lane_id(), w = threadIdx.x / kWave;
+  int x = 0, incl = 0;
+  if (threadIdx.x < kBuckets) {
+    x = hist[threadIdx.x];
+    incl = x;
 #pragma unroll
-  for (int i = 0; i < kCommitThreads / kWave; ++i) {
-    const int x = wcnt[i];
-    base += (i < w) ? x : 0;
-    total += x;
+    for (int d = 1; d < kWave; d <<= 1) {
+      const int y = __shfl_up(incl, d, kWave);
+      if (l >= d) incl += y;
+    }
+    if (l == kWave - 1) wtot[w] = incl;
   }
+  __syncthreads();
+  int32_t* bo = boff + static_cast<int64_t>(blockIdx.x) * (kBuckets + 1);
+  if (threadIdx.x < kBuckets) {
+    int base = 0, total = 0;
+#pragma unroll
+    for (int i = 0; i < kBuckets / kWave; ++i) {
+      base += (i < w) ? wtot[i] : 0;
+      total += wtot[i];
+    }
+    const int ex = base + incl - x;
+    hist[threadIdx.x] = ex;  // each thread rewrites only its own bucket
+    bo[threadIdx.x] = ex;
+    if (threadIdx.x == 0) bo[kBuckets] = total;
+  }
+  __syncthreads();
   if (cm) {
-    const int64_t slot = static_cast<int64_t>(blockIdx.x) * kCommitThreads + base +
-                         __popcll(b & ((uint64_t(1) << l) - 1));
-    mv_v[slot] = static_cast<int32_t>(v);
-    mv_old[slot] = c;
+    const int64_t e0 = static_cast<int64_t>(blockIdx.x) * (2 * kCommitThreads);
+    const int64_t kv = kdeg[v];
+    ekey[e0 + hist[b0] + r0] = c;
+    eval[e0 + hist[b0] + r0] = -kv;
+    ekey[e0 + hist[b1] + r1] = t;
+    eval[e0 + hist[b1] + r1] = kv;
   }
-  if (threadIdx.x == 0) mv_cnt[blockIdx.x] = total;
 }
 
-// Incremental sigma_tot after a commit: each committed vertex v contributes (previous
-// community, -k_v) and (new community, +k_v).  The entries are taken kDeltaChunk vertices
-// at a time and summed per community in an LDS hash table (linear probing, slots claimed by
-// compare-and-swap, int64 adds; at most 2 * kDeltaChunk keys in kDeltaSlots slots, so every
-// probe ends); then each claimed slot adds its exact int64 total to sigtot.  Chunks run one
-// after another inside this single workgroup (barrier between them), so a community is
-// written by one thread at a time; integer sums do not depend on the order of the adds, so
-// the result equals a full recomputation.  The moved total goes to *moved_out; apply = 0
-// stops there.
-constexpr int kDeltaThreads = 1024;
-constexpr int kDeltaChunk = 2048;          // committed vertices per table fill (4096 entries)
-constexpr int kDeltaSlotBits = 13;
-constexpr int kDeltaSlots = 1 << kDeltaSlotBits;
-constexpr int kDeltaMaxBlocks = 8192;      // commit blocks: graphs up to 8,388,608 vertices
-constexpr int64_t kDeltaSwitch = 8192;     // moved count at a sync below which passes go incremental
-__global__ __launch_bounds__(kDeltaThreads) void k_sigtot_delta(
-    const int32_t* __restrict__ mv_cnt, int nblk, const int32_t* __restrict__ mv_v,
-    const int32_t* __restrict__ mv_old, const int32_t* __restrict__ comm, const int64_t* __restrict__ kdeg,
-    int64_t* __restrict__ sigtot, int apply, int32_t* __restrict__ moved_out) {
-  __shared__ int32_t pre[kDeltaMaxBlocks + 1];
-  __shared__ int32_t hkey[kDeltaSlots];
-  __shared__ unsigned long long hval[kDeltaSlots];
-  __shared__ int32_t wsum[kDeltaThreads / kWave];
-  const int t = threadIdx.x, w = t / kWave, l = lane_id();
-  // exclusive prefix of the per-block counts; thread t owns a contiguous run of blocks
-  const int per = (nblk + kDeltaThreads - 1) / kDeltaThreads;
-  const int b0 = min(t * per, nblk), b1 = min(b0 + per, nblk);
-  int s = 0;
-  for (int b = b0; b < b1; ++b) s += mv_cnt[b];
-  int x = s;
+// One workgroup per bucket: thread i takes commit blocks i, i + kApplyThreads, ...; the
+// bucket's entries are numbered by a block scan of the per-thread counts and summed
+// kApplyChunk at a time (at most kApplyChunk distinct keys in kApplySlots slots).
+__global__ __launch_bounds__(kApplyThreads) void k_sigtot_apply(const int32_t* __restrict__ ekey,
+                                                                const int64_t* __restrict__ eval,
+                                                                const int32_t* __restrict__ boff, int nblk,
+                                                                int64_t* __restrict__ sigtot) {
+  __shared__ int32_t hkey[kApplySlots];
+  __shared__ unsigned long long hval[kApplySlots];
+  __shared__ int wtot[kApplyThreads / kWave];
+  const int bk = blockIdx.x, t = threadIdx.x, l = lane_id(), w = t / kWave;
+  int mine = 0;
+  for (int blk = t; blk < nblk; blk += kApplyThreads) {
+    const int32_t* bo = boff + static_cast<int64_t>(blk) * (kBuckets + 1);
+    mine += bo[bk + 1] - bo[bk];
+  }
+  int incl = mine;
 #pragma unroll
   for (int d = 1; d < kWave; d <<= 1) {
-    const int y = __shfl_up(x, d, kWave);
-    if (l >= d) x += y;
+    const int y = __shfl_up(incl, d, kWave);
+    if (l >= d) incl += y;
   }
-  if (l == kWave - 1) wsum[w] = x;
-  __syncthreads();
-  int run = x - s, total = 0;
-#pragma unroll
-  for (int i = 0; i < kDeltaThreads / kWave; ++i) {
-    run += (i < w) ? wsum[i] : 0;
-    total += wsum[i];
-  }
-  for (int b = b0; b < b1; ++b) {
-    pre[b] = run;
-    run += mv_cnt[b];
-  }
-  if (t == 0) {
-    pre[nblk] = total;
-    *moved_out = total;
-  }
-  __syncthreads();
-  if (!apply) return;
+  if (l == kWave - 1) wtot[w] = incl;
   constexpr int32_t kEmpty = -1;  // community ids are >= 0
-  for (int i = t; i < kDeltaSlots; i += kDeltaThreads) {
+  for (int i = t; i < kApplySlots; i += kApplyThreads) {
     hkey[i] = kEmpty;
     hval[i] = 0;
   }
   __syncthreads();
-  auto add = [&](int32_t c, int64_t x) {
-    uint32_t h = (static_cast<uint32_t>(c) * 2654435761u) >> (32 - kDeltaSlotBits);
-    for (;;) {
-      const int32_t prev = atomicCAS(&hkey[h], kEmpty, c);
-      if (prev == kEmpty || prev == c) {
-        atomicAdd(&hval[h], static_cast<unsigned long long>(x));  // two's complement: exact
-        return;
+  int first = incl - mine, total = 0;
+#pragma unroll
+  for (int i = 0; i < kApplyThreads / kWave; ++i) {
+    first += (i < w) ? wtot[i] : 0;
+    total += wtot[i];
+  }
+  for (int s0 = 0; s0 < total; s0 += kApplyChunk) {
+    // this thread's entries numbered [first, first + mine); those in [s0, s0 + kApplyChunk)
+    if (first < s0 + kApplyChunk && first + mine > s0) {
+      int idx = first;
+      for (int blk = t; blk < nblk; blk += kApplyThreads) {
+        const int32_t* bo = boff + static_cast<int64_t>(blk) * (kBuckets + 1);
+        const int lo = bo[bk], hi = bo[bk + 1];
+        const int64_t e0 = static_cast<int64_t>(blk) * (2 * kCommitThreads);
+        for (int j = lo; j < hi; ++j, ++idx) {
+          if (idx < s0 || idx >= s0 + kApplyChunk) continue;
+          const int32_t ck = ekey[e0 + j];
+          const unsigned long long cv = static_cast<unsigned long long>(eval[e0 + j]);  // two's complement
+          uint32_t h = comm_slot(ck);
+          for (;;) {
+            const int32_t prev = atomicCAS(&hkey[h], kEmpty, ck);
+            if (prev == kEmpty || prev == ck) {
+              atomicAdd(&hval[h], cv);
+              break;
+            }
+            h = (h + 1) & (kApplySlots - 1);
+          }
+        }
       }
-      h = (h + 1) & (kDeltaSlots - 1);
-    }
-  };
-  for (int s0 = 0; s0 < total; s0 += kDeltaChunk) {
-    const int cnt = min(kDeltaChunk, total - s0);
-    for (int e = s0 + t; e < s0 + cnt; e += kDeltaThreads) {
-      int lo = 0, hi = nblk - 1;  // last block with pre[b] <= e: it holds e
-      while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (pre[mid] <= e) lo = mid;
-        else hi = mid - 1;
-      }
-      const int64_t slot = static_cast<int64_t>(lo) * kCommitThreads + (e - pre[lo]);
-      const int32_t v = mv_v[slot];
-      const int64_t kv = kdeg[v];
-      add(mv_old[slot], -kv);
-      add(comm[v], kv);
     }
     __syncthreads();
-    for (int i = t; i < kDeltaSlots; i += kDeltaThreads) {
-      const int32_t c = hkey[i];
-      if (c != kEmpty) {
-        sigtot[c] += static_cast<int64_t>(hval[i]);
+    for (int i = t; i < kApplySlots; i += kApplyThreads) {
+      const int32_t ck = hkey[i];
+      if (ck != kEmpty) {
+        sigtot[ck] += static_cast<int64_t>(hval[i]);
         hkey[i] = kEmpty;
         hval[i] = 0;
       }
@@ -1561,8 +1584,9 @@ struct Ctx {
   uint8_t *active, *commit;
   int64_t *sigtot, *iw, *tot, *inw, *kbuf[2];
   double* terms;
-  int32_t* pass_info;       // 2 * kMaxPasses: per pass (any vertex wanted to move, moved count)
-  int32_t *mv_v, *mv_old, *mv_cnt;  // committed vertices per commit block (k_commit)
+  int32_t* pass_info;       // 2 * kMaxPasses: per pass, [0] = any vertex wanted to move
+  int32_t *ekey, *boff;     // sigma_tot entries of the committed vertices, per commit block by bucket
+  int64_t* eval;
   CommSig* cs;              // (comm, part, sigtot[comm]) per vertex for the light path of k_move
   int64_t* red;             // small reduction outputs
   float* fred;
@@ -1613,8 +1637,11 @@ static void carve_ctx(Carve& cv, Ctx& c, int64_t n, int64_t E) {
   c.inw = cv.take<int64_t>(n1); c.kbuf[0] = cv.take<int64_t>(n1); c.kbuf[1] = cv.take<int64_t>(n1);
   c.terms = cv.take<double>(n1);
   c.pass_info = cv.take<int32_t>(2 * kMaxPasses);
-  c.mv_v = cv.take<int32_t>(n1); c.mv_old = cv.take<int32_t>(n1);
-  c.mv_cnt = cv.take<int32_t>(ceil_div(n1, kCommitThreads));
+  {
+    const int64_t nb = ceil_div(n1, kCommitThreads);
+    c.ekey = cv.take<int32_t>(2 * nb * kCommitThreads); c.eval = cv.take<int64_t>(2 * nb * kCommitThreads);
+    c.boff = cv.take<int32_t>(nb * (kBuckets + 1));
+  }
   c.cs = cv.take<CommSig>(n1);
   c.red = cv.take<int64_t>(8);
   c.fred = cv.take<float>(kMaxBlocks + 8);
@@ -1802,21 +1829,15 @@ static int local_moving(Ctx& c, const Graph& g, int64_t seed, const int32_t* par
   else k_fill_u8<<<grid_for(g.n), kThreads, 0, st>>>(c.active, g.n, 1);
   k_fill_i32<<<grid_for(2 * kMaxPasses), kThreads, 0, st>>>(c.pass_info, 2 * kMaxPasses, 0);
   MSC_LAUNCH_CHECK();
-  // sigma_tot is rebuilt from scratch (sort + segmented sums) at a pass start unless the
-  // previous pass updated it from its committed vertices (k_sigtot_delta).  The host picks
-  // the incremental form at each sync from the last moved count: the commit counts decay
-  // over the passes, and the late passes move few vertices.
+  // sigma_tot is built from comm once (sort + segmented sums); every commit then keeps it
+  // current (k_commit + k_sigtot_apply), and the frontier push rewrites each vertex's
+  // (comm, part, sigma_tot) record after it
   const int nblk = static_cast<int>(ceil_div(g.n, kCommitThreads));
-  const bool can_inc = nblk <= kDeltaMaxBlocks;
-  bool inc = false, sig_valid = false;
-  int n_inc = 0;
+  MSC_TRY(sigma_tot(c, g, c.comm));
+  k_refresh_cs<<<grid_for(g.n), kThreads, 0, st>>>(g.n, c.comm, part, c.sigtot, c.cs);
+  MSC_LAUNCH_CHECK();
   int p = 0;
   for (; p < kMaxPasses; ++p) {
-    if (!sig_valid) {
-      MSC_TRY(sigma_tot(c, g, c.comm));
-      k_refresh_cs<<<grid_for(g.n), kThreads, 0, st>>>(g.n, c.comm, part, c.sigtot, c.cs);
-      MSC_LAUNCH_CHECK();
-    }
     // one launch moves every vertex up to the widest hub class: the listed hub vertices (one
     // block each) and the light ones; heavier vertices go through the sort path
     HubLists hubs;
@@ -1850,32 +1871,21 @@ static int local_moving(Ctx& c, const Graph& g, int64_t seed, const int32_t* par
     }
     MSC_TRY(heavy_moves(c, g, c.comm, part, gp));
     k_commit<<<static_cast<unsigned>(nblk), kCommitThreads, 0, st>>>(
-        c.comm, c.target, g.n, pass_key(key, p), c.thr, c.commit, c.active, c.pass_info + 2 * p, c.mv_v, c.mv_old,
-        c.mv_cnt, c.vsid, c.d_vbase);
+        c.comm, c.target, g.n, pass_key(key, p), c.thr, c.commit, c.active, c.pass_info + 2 * p, g.k, c.ekey, c.eval,
+        c.boff, c.vsid, c.d_vbase);
+    k_sigtot_apply<<<kBuckets, kApplyThreads, 0, st>>>(c.ekey, c.eval, c.boff, nblk, c.sigtot);
+    k_frontier_push<<<grid_for(g.n), kThreads, 0, st>>>(g.indptr, g.indices, g.n, c.commit, c.active, 1, c.comm, part,
+                                                        c.sigtot, c.cs);
     MSC_LAUNCH_CHECK();
-    const bool sync = (p % kSyncEvery == kSyncEvery - 1) || p == kMaxPasses - 1;
-    // incremental passes update sigma_tot; a full-mode sync pass only counts the moves
-    if (inc || (sync && can_inc)) {
-      k_sigtot_delta<<<1, kDeltaThreads, 0, st>>>(c.mv_cnt, nblk, c.mv_v, c.mv_old, c.comm, g.k, c.sigtot,
-                                                  inc ? 1 : 0, c.pass_info + 2 * p + 1);
-      MSC_LAUNCH_CHECK();
-    }
-    sig_valid = inc;
-    n_inc += inc ? 1 : 0;
-    // after an incremental update the push also rewrites cs for the next pass
-    k_frontier_push<<<grid_for(g.n), kThreads, 0, st>>>(
-        g.indptr, g.indices, g.n, c.commit, c.active, inc ? 1 : 0, c.comm, part, c.sigtot, c.cs);
-    MSC_LAUNCH_CHECK();
-    if (sync) {
-      int32_t info[2];  // (any vertex wanted to move, moved count)
-      MSC_HIP(hipMemcpyAsync(info, c.pass_info + 2 * p, sizeof(info), hipMemcpyDeviceToHost, st));
+    if ((p % kSyncEvery == kSyncEvery - 1) || p == kMaxPasses - 1) {
+      int32_t wants;  // any vertex wanted to move
+      MSC_HIP(hipMemcpyAsync(&wants, c.pass_info + 2 * p, sizeof(wants), hipMemcpyDeviceToHost, st));
       MSC_HIP(hipStreamSynchronize(st));
       if (c.time_moves) MSC_TRY(collect_move_times(c));
-      if (!info[0]) {
+      if (!wants) {
         ++p;
         break;
       }
-      inc = can_inc && info[1] <= kDeltaSwitch;
     }
   }
   if (c.time_moves && c.t_pending > 0) {
@@ -1883,7 +1893,8 @@ static int local_moving(Ctx& c, const Graph& g, int64_t seed, const int32_t* par
     MSC_TRY(collect_move_times(c));
   }
   *passes_out = std::min(p, kMaxPasses);
-  if (delta_passes_out) *delta_passes_out = n_inc;
+  // every pass after the first took sigma_tot from the previous commits alone
+  if (delta_passes_out) *delta_passes_out = std::max(*passes_out - 1, 0);
   return kOk;
 }
 
